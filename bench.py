@@ -1,0 +1,158 @@
+"""Benchmark: cell-updates/s per RK stage of the reacting H2/O2 case on one MI355X.
+
+Workload: H2/O2-N2 (9 species, 28 reactions, the H/O subset of GRI-Mech 3.0), one periodic
+128^3 block on [0, 1 cm]^3, 1 atm, stoichiometric H2-air at rest, T 1000 K with eight seeded
+Gaussian hot spots clipped at 1500 K, MUSCL (Koren) + HLLC, coupled finite-rate chemistry,
+TVD-RK3, FP64, fixed dt = 1 ns. A step is one RK3 time step (three stages); value = interior
+cells x stages / wall time of the timed steps, with every array resident on the device.
+
+Alongside: the dominant kernel's achieved rate against its roofline (device events on the
+launch stream over the timed steps), and the CPU oracle timed on the host's cores on a bounded
+sample of the same case.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 128] [--cpu-n 32] [--no-cpu]
+"""
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+sys.path.insert(0, str(ROOT / "tests"))
+
+HBM_PEAK = 8000.0  # GB/s, MI355X HBM3E
+FP64_PEAK = 78.6  # TFLOP/s, MI355X FP64 vector
+DT = 1e-9
+L = (0.01, 0.01, 0.01)
+
+
+def case_config(support):
+    return support.config("h2o2", recon="muscl", primary="hllc", limiter="koren", chem=True, integrator="tvd-rk3", dt=DT)
+
+
+def algorithmic_bytes(name, ne, ng):
+    """Compulsory HBM bytes per item of one launch of an entry point (doubles x 8)."""
+    if name == "pga_flux":
+        # per face: Q and p, T, rhoinv of one cell (the stencil's others are neighbours' reads), S, F out
+        return 8 * (ne + 3 + 3 + ne)
+    if name == "pga_source":
+        # per cell: Q and T in, dQ out
+        return 8 * (ne + 1 + ne)
+    if name == "pga_applyflux":
+        # per cell: three low faces' F (the high faces are neighbours' lows), detJinv, dQ in and out
+        return 8 * (3 * ne + 1 + 2 * ne)
+    if name == "pga_combine":
+        return 8 * 3 * ne
+    if name == "pga_fromQ":
+        return 8 * (ne + 1 + 8)
+    return 0
+
+
+def cpu_baseline(support, n, stages):
+    """The oracle (C, OpenMP) on an n^3 block of the same case, all host cores."""
+    from oracle import oracle as O
+    from peregrine_amd.simulator import get_simulator
+
+    sim = get_simulator(case_config(support))
+    oc = O.OracleCase(sim.mixture, "muscl", "koren", "hllc")
+    hb = support.HostBlock((n,) * 3, sim.ng, sim.ne, 3, lengths=L)
+    q = support.h2_prims(hb.xcc, hb.ncell, lengths=L)
+    O.lib.pgo_fromprims(oc.ref, O.arr(support.to_dev(q)), O.arr(hb.regs[0]), O.arr(hb.state),
+                        O.box((0, 0, 0), hb.cshape))
+    b = support.oracle_block(hb, "tvd-rk3", True)
+    O.lib.pgo_stage(oc.ref, ctypes.byref(b), 0, DT)  # warm-up
+    times = []
+    for s in range(stages):
+        t0 = time.perf_counter()
+        O.lib.pgo_stage(oc.ref, ctypes.byref(b), s % 3, DT)
+        times.append(time.perf_counter() - t0)
+    med = float(np.median(times))
+    return dict(value=n**3 / med, unit="cell-updates/s", cores=O.threads(), kind="port",
+                sample=f"{n}^3 block of the same case, median of {stages} stages, C oracle with OpenMP")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=128)
+    ap.add_argument("--cpu-n", type=int, default=32)
+    ap.add_argument("--cpu-stages", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py runs on one GPU: this build has no multi-GPU path (the block sharding is design only)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import peregrine_amd as pg
+    import support
+
+    cfg = case_config(support)
+    backend = pg.get_backend("mi355x", cfg)
+    n = args.n
+    mesh = pg.mesher.CubeMesher(blkdims=[n + 1] * 3, lengths=L, periodic=[True] * 3)
+    mb = pg.get_solver(cfg, mesh, backend)
+    blk = mb.blocks[0]
+    mb.set_prims([support.h2_prims(blk.xcc.get(), blk.ncell, lengths=L)])
+    for _ in range(args.warmup):
+        mb.step(DT)
+    mb.time_kernels(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        mb.step(DT)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    nst = mb.integ.nstages
+    ncells = mb.ncells
+    value = ncells * nst * args.steps / elapsed
+    times = mb.kernel_times()
+    # the dominant entry point by total time over the timed steps
+    total = {k: c * ms for k, (c, ms) in times.items()}
+    dom = max(total, key=total.get)
+    count, ms = times[dom]
+    ne = mb.sim.ne
+    items = {"pga_flux": (n + 1) * n * n, "pga_source": n**3, "pga_applyflux": n**3, "pga_combine": n**3}.get(dom, n**3)
+    per = algorithmic_bytes(dom, ne, mb.ng)
+    achieved = per * items / (ms * 1e-3) / 1e9
+    roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK, unit="GB/s", frac=round(achieved / HBM_PEAK, 4),
+                traffic=None, kernel=dom, kernel_ms=round(ms, 4), launches=count,
+                bytes_per_item=per, items_per_launch=items)
+    breakdown = {k: {"launches": c, "mean_ms": round(t, 4), "share": round(c * t / sum(total.values()), 3)}
+                 for k, (c, t) in sorted(times.items(), key=lambda kv: -kv[1][0] * kv[1][1])}
+    cpu = None if args.no_cpu else cpu_baseline(support, args.cpu_n, args.cpu_stages)
+    line = {
+        "metric": "cell-updates/s per RK stage (FP64, H2-air 3D)",
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"h2o2-9sp-28rx {n}^3 periodic, muscl-koren + hllc, coupled chemistry, tvd-rk3",
+                   "cells": ncells, "stages_per_step": nst, "path": "per-kernel"},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "kernels": breakdown,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,145 @@
+/*
+ * peregrine_amd: the C ABI of an MI355X (gfx950) implementation of Peregrine's
+ * per-Runge-Kutta-stage right-hand side and stage update.
+ *
+ * Plain pointers and sizes only. Every array is device memory owned by the
+ * caller, in the reference's device layout: Fortran order [i, j, k, comp],
+ * i contiguous, each component a separate plane, ng halo layers on every side
+ * (peregrine/backend/device/backend.py:56-57, peregrine/launch/ranges.py:19-20).
+ * A face array along an axis has one plane more than the cells along it; face
+ * index f lies between cells f-1 and f (peregrine/launch/policy.py:76-93).
+ *
+ * Every entry point returns 0 on success or a negative PGA_E* code; the text
+ * of the last error on the calling thread is pga_last_error(). Launches are
+ * asynchronous on the HIP stream passed (NULL: the default stream).
+ *
+ * The reference binds one entry point per rendered kernel,
+ *   extern "C" __global__ void pg_<symbol>(const Functor k, const pg::Tiling t)
+ * (peregrine/backend/device/kernels/launch.mako:1-12); each function below
+ * names the template whose pg_<symbol> it replaces.
+ */
+#ifndef PEREGRINE_AMD_H
+#define PEREGRINE_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PGA_ABI_VERSION 1
+
+enum {
+  PGA_OK = 0,
+  PGA_EINVAL = -1,   /* an argument the kernels cannot take */
+  PGA_EUNSUP = -2,   /* a case this build has no kernel for (species count, mechanism) */
+  PGA_EHIP = -3,     /* the HIP runtime refused a launch */
+  PGA_ERANGE = -4,   /* a size beyond a stated maximum */
+};
+
+/* one array: base pointer at element (0,0,0,0), its j, k and component strides in elements, and
+ * its extents (i, j, k, components); every entry point checks its boxes and stencils against the
+ * extents before it launches */
+typedef struct {
+  double *data;
+  int64_t sj, sk, sc;
+  int32_t n[4];
+} pga_view;
+
+/* a box of array indices: [start, start + extent) per axis, halo cells included in the index */
+typedef struct {
+  int32_t start[3];
+  int32_t extent[3];
+} pga_box;
+
+/* the kernel variants of a case, from its config (peregrine/config.py:12-98) */
+typedef struct {
+  int32_t ns;        /* species; ne = 4 + ns conserved variables */
+  int32_t eos;       /* 0 cpg, 1 tpg */
+  int32_t recon;     /* 0 constant, 1 muscl */
+  int32_t limiter;   /* 0 none, 1 koren, 2 minmod, 3 mc, 4 superbee, 5 vanleer */
+  int32_t primary;   /* 0 kepaec, 1 hllc, 2 rusanov */
+  int32_t mechanism; /* -1 none, else the index of a compiled mechanism (pga_mechanism_find) */
+  int32_t reserved[10];
+} pga_case_desc;
+
+typedef struct pga_case pga_case;
+
+/* library and error reporting */
+int32_t pga_abi_version(void);
+const char *pga_last_error(void);
+
+/* The case constant table: per species MW, MWinv, Rsp, cppolyR[7], h298R, cp0, gpoly[8], h298, ...
+ * in one flat double array on the device. The offset and length of each named section for ns
+ * species; -1 when the name is unknown. Mirrors the constants the reference bakes into every unit
+ * (peregrine/simulator/mixture/mixture.py:54-70, peregrine/codegen/constants.py:7-65). */
+int32_t pga_const_offset(int32_t ns, const char *name, int32_t *count);
+int32_t pga_const_size(int32_t ns);
+
+/* compiled finite-rate mechanisms: count, the structure signature of one, and the length of its
+ * constant table; pga_mechanism_find returns the index whose signature equals the string, or -1 */
+int32_t pga_mechanism_count(void);
+const char *pga_mechanism_signature(int32_t index);
+const char *pga_mechanism_name(int32_t index);
+int32_t pga_mechanism_table_size(int32_t index);
+int32_t pga_mechanism_find(const char *signature);
+
+/* a case: the variant choice plus the device tables it reads (borrowed, not copied or freed) */
+int32_t pga_case_create(const pga_case_desc *desc, const double *dev_consts, const double *dev_chem,
+                        pga_case **out);
+int32_t pga_case_destroy(pga_case *c);
+
+/* EOS inversion Q -> state (p, T, rhoinv, rho, e, cp, gamma, c) over up to 8 boxes in one launch;
+ * state's T is read as the Newton start.
+ * Replaces pg_fromQ: peregrine/simulator/euler/kernels/state/fromQ.mako:5-19. */
+int32_t pga_fromQ(const pga_case *c, pga_view Q, pga_view state, const pga_box *boxes, int32_t nboxes,
+                  void *stream);
+
+/* primitives [p, u, v, w, T, Y_0..Y_{ns-2}] -> Q and state.
+ * Replaces pg_fromprims: peregrine/simulator/euler/kernels/state/fromprims.mako:4-13. */
+int32_t pga_fromprims(const pga_case *c, pga_view prims, pga_view Q, pga_view state, pga_box box, void *stream);
+
+/* the advective flux F (ne per face) through the faces of one axis in box (face indices).
+ * Replaces pg_flux: peregrine/simulator/euler/kernels/flux/single.mako:4-9 with recon/{constant,muscl}
+ * and function/{kepaec,hllc,rusanov}. */
+int32_t pga_flux(const pga_case *c, int32_t axis, pga_view Q, pga_view state, pga_view S, pga_view F,
+                 pga_box box, void *stream);
+
+/* dQ (= or +=) (iF_low + jF_low + kF_low - iF_high - jF_high - kF_high) * detJinv over the cells in box.
+ * Replaces pg_applyflux: peregrine/simulator/advec/kernels/applyflux.mako:2-11. */
+int32_t pga_applyflux(int32_t ne, pga_view dQ, pga_view iF, pga_view jF, pga_view kF, pga_view detJinv,
+                      int32_t append, pga_box box, void *stream);
+
+/* the coupled chemistry source: dQ[0..4] = 0, dQ[5+n] = omega_n (kg/m^3/s).
+ * Replaces pg_source: peregrine/simulator/chemint/kernels/source.mako:3-10. */
+int32_t pga_source(const pga_case *c, pga_view Q, pga_view state, pga_view dQ, pga_box box, void *stream);
+
+/* net production rates at n points: rho[n], Y[ns][n] (all species), T[n] -> omega[ns][n].
+ * The body of pg_source without the solution layout (finiterate.mako production). */
+int32_t pga_production(const pga_case *c, int32_t n, const double *rho, const double *Y, const double *T,
+                       double *omega, void *stream);
+
+/* out = [wout out] + sum_m w_m X_m, the last term times *dt when step (dt a device scalar).
+ * Replaces pg_combine: peregrine/integrators/kernels/combine.mako:1-13. */
+int32_t pga_combine(int32_t ne, pga_view out, int32_t own, double wout, int32_t nterms, const pga_view *X,
+                    const double *w, const double *dt, int32_t step, pga_box box, void *stream);
+
+/* the halo of a block periodic with itself along axis: each halo slab `depth` deep takes the
+ * interior planes next to the opposite interface; the other axes run over the interior only.
+ * Replaces pg_place on a self-periodic interface pair:
+ * peregrine/multiblock/collections/exchange/kernels/place.mako:11-37. */
+int32_t pga_fill_periodic(pga_view A, int32_t ncomp, const int32_t *ncell, int32_t ng, int32_t depth,
+                          int32_t axis, void *stream);
+
+/* a box of `extent` cells of every component copied from src (starting at src_start) to dst
+ * (at dst_start): the halo slab of an aligned block interface from its partner's interior planes.
+ * Replaces pg_place on an unrotated interface: place.mako:11-37 with
+ * peregrine/multiblock/collections/exchange/collection.py:78-96 (haloview / nbrview). */
+int32_t pga_copy_box(pga_view dst, const int32_t *dst_start, pga_view src, const int32_t *src_start,
+                     const int32_t *extent, int32_t ncomp, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

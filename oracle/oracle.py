@@ -1,0 +1,169 @@
+"""ctypes binding of the CPU oracle (libpgo.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and by
+bench.py's CPU-baseline leg, always as the checker. The product path
+(peregrine_amd) never imports it.
+
+Arrays are numpy views in the device layout: a block array of shape
+(ncomp, nk_full, nj_full, ni_full), C-contiguous, i.e. Fortran order
+[i, j, k, comp] as the reference's device backend lays it out.
+"""
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "libpgo.so"
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+
+
+class Arr(C.Structure):
+    _fields_ = [("d", _dp), ("sj", C.c_int64), ("sk", C.c_int64), ("sc", C.c_int64)]
+
+
+class Box(C.Structure):
+    _fields_ = [("start", C.c_int * 3), ("extent", C.c_int * 3)]
+
+
+class Case(C.Structure):
+    _fields_ = [
+        ("ns", C.c_int), ("ne", C.c_int), ("eos", C.c_int), ("recon", C.c_int), ("limiter", C.c_int),
+        ("primary", C.c_int), ("nt", C.c_int), ("ng_", C.c_int),
+        ("MW", _dp), ("MWinv", _dp), ("Rsp", _dp), ("cppolyR", _dp), ("h298R", _dp), ("cp0", _dp),
+        ("gpoly", _dp), ("h298", _dp),
+        ("nr", C.c_int), ("ntb", C.c_int),
+        ("kind", _ip), ("reversible", _ip), ("product", _ip), ("tb", _ip), ("troeidx", _ip),
+        ("nfwd", _ip), ("fwdsp", _ip), ("nreact", _ip), ("reactsp", _ip), ("nprod", _ip), ("prodsp", _ip),
+        ("nnet", _ip), ("netsp", _ip), ("ntbeff", _ip), ("tbeffsp", _ip),
+        ("logA", _dp), ("b", _dp), ("EaR", _dp), ("logprA", _dp), ("prb", _dp), ("prEaR", _dp), ("nusum", _dp),
+        ("fwdorder", _dp), ("reactnu", _dp), ("prodnu", _dp), ("netnu", _dp), ("netnuMW", _dp),
+        ("lfcpoly", _dp), ("tbdefeff", _dp), ("tbeffdev", _dp),
+        ("logprefRu", C.c_double), ("prefRu", C.c_double), ("Rupref", C.c_double), ("logcfloor", C.c_double),
+        ("expclamp", C.c_double), ("troeuc", C.c_double), ("troeuhinv", C.c_double),
+    ]
+
+
+class Block(C.Structure):
+    _fields_ = [
+        ("ncell", C.c_int * 3), ("ng", C.c_int), ("regs", Arr * 4), ("nregs", C.c_int),
+        ("state", Arr), ("iF", Arr), ("jF", Arr), ("kF", Arr), ("iS", Arr), ("jS", Arr), ("kS", Arr),
+        ("detJinv", Arr), ("integrator", C.c_int), ("chemistry", C.c_int),
+    ]
+
+
+def _load():
+    if not LIB.is_file():
+        raise RuntimeError(f"{LIB} is not built; run `make -C {HERE}`")
+    lib = C.CDLL(str(LIB))
+    lib.pgo_tlog.restype = C.c_double
+    lib.pgo_tlog.argtypes = [C.c_double]
+    lib.pgo_texp.restype = C.c_double
+    lib.pgo_texp.argtypes = [C.c_double]
+    lib.pgo_threads.restype = C.c_int
+    P = C.POINTER
+    lib.pgo_fromQ.argtypes = [P(Case), Arr, Arr, Box]
+    lib.pgo_fromprims.argtypes = [P(Case), Arr, Arr, Arr, Box]
+    lib.pgo_flux.argtypes = [P(Case), C.c_int, Arr, Arr, Arr, Arr, Box]
+    lib.pgo_applyflux.argtypes = [C.c_int, Arr, Arr, Arr, Arr, Arr, C.c_int, Box]
+    lib.pgo_production.argtypes = [P(Case), C.c_double, _dp, C.c_double, _dp]
+    lib.pgo_source.argtypes = [P(Case), Arr, Arr, Arr, Box]
+    lib.pgo_combine.argtypes = [C.c_int, Arr, C.c_int, C.c_double, C.c_int, P(Arr), _dp, C.c_double, C.c_int, Box]
+    lib.pgo_fill_periodic.argtypes = [Arr, C.c_int, P(C.c_int), C.c_int, C.c_int, C.c_int]
+    lib.pgo_step.argtypes = [P(Case), P(Block), C.c_double]
+    lib.pgo_stage.argtypes = [P(Case), P(Block), C.c_int, C.c_double]
+    lib.pgo_advance.argtypes = [P(Case), P(Block), C.c_int, C.c_double]
+    return lib
+
+
+lib = _load()
+tlog = lib.pgo_tlog
+texp = lib.pgo_texp
+
+EOS = {"cpg": 0, "tpg": 1}
+RECON = {"constant": 0, "muscl": 1}
+LIMITER = {None: 0, "koren": 1, "minmod": 2, "mc": 3, "superbee": 4, "vanleer": 5}
+PRIMARY = {"kepaec": 0, "hllc": 1, "rusanov": 2}
+INTEGRATOR = {"rk1": 0, "rk2": 1, "tvd-rk3": 2, "rk4": 3}
+
+
+def arr(a):
+    """A view of a (ncomp, nk, nj, ni) C-contiguous float64 array."""
+    if a.dtype != np.float64 or not a.flags.c_contiguous or a.ndim != 4:
+        raise ValueError("oracle arrays are C-contiguous float64 (ncomp, nk, nj, ni)")
+    nc, nk, nj, ni = a.shape
+    v = Arr(a.ctypes.data_as(_dp), ni, ni * nj, ni * nj * nk)
+    # the view holds the array, so a temporary lives as long as the call it is passed to
+    v._keep = a
+    return v
+
+
+def box(start, extent):
+    return Box((C.c_int * 3)(*start), (C.c_int * 3)(*extent))
+
+
+def ints(values):
+    return (C.c_int * len(values))(*values)
+
+
+class OracleCase:
+    """A pgo_case built from a mixture and the flux choices; keeps its arrays alive."""
+
+    def __init__(self, mixture, recon="constant", limiter=None, primary="kepaec"):
+        k = mixture.constants()
+        self.keep = []
+
+        def d(x):
+            a = np.ascontiguousarray(np.asarray(x, dtype=np.float64))
+            self.keep.append(a)
+            return a.ctypes.data_as(_dp)
+
+        def i(x):
+            a = np.ascontiguousarray(np.asarray(x, dtype=np.int32))
+            self.keep.append(a)
+            return a.ctypes.data_as(_ip)
+
+        ns = mixture.ns
+        c = Case()
+        c.ns, c.ne = ns, 4 + ns
+        c.eos = EOS[mixture.eos.name]
+        c.recon, c.limiter, c.primary = RECON[recon], LIMITER[limiter], PRIMARY[primary]
+        c.nt, c.ng_ = k["cppolyR"].shape[1], k["gpoly"].shape[1]
+        c.MW, c.MWinv, c.Rsp = d(k["MW"]), d(k["MWinv"]), d(k["Rsp"])
+        c.cppolyR, c.h298R, c.gpoly, c.h298 = d(k["cppolyR"]), d(k["h298R"]), d(k["gpoly"]), d(k["h298"])
+        c.cp0 = d(k.get("cp0", np.zeros(ns)))
+        if mixture.chemistry is not None:
+            records, sets, cols, sc = mixture.chemistry.tables(mixture.species)
+            c.nr, c.ntb = len(records), len(sets)
+            for name in ("kind", "reversible", "product", "tb", "troeidx", "nfwd", "fwdsp", "nreact", "reactsp",
+                         "nprod", "prodsp", "nnet", "netsp", "ntbeff", "tbeffsp"):
+                setattr(c, name, i(cols[name]))
+            for name in ("logA", "b", "EaR", "logprA", "prb", "prEaR", "nusum", "fwdorder", "reactnu", "prodnu",
+                         "netnu", "netnuMW", "lfcpoly", "tbdefeff", "tbeffdev"):
+                setattr(c, name, d(cols[name]))
+            for name in ("logprefRu", "prefRu", "Rupref", "logcfloor", "expclamp", "troeuc", "troeuhinv"):
+                setattr(c, name, sc[name])
+        self.c = c
+        self.ns, self.ne = ns, 4 + ns
+
+    @property
+    def ref(self):
+        return C.byref(self.c)
+
+    def production(self, rho, Y, T):
+        Y = np.ascontiguousarray(Y, dtype=np.float64)
+        out = np.zeros(self.ns)
+        lib.pgo_production(self.ref, rho, Y.ctypes.data_as(_dp), T, out.ctypes.data_as(_dp))
+        return out
+
+
+def threads():
+    return lib.pgo_threads()
+
+
+def set_threads(n):
+    os.environ["OMP_NUM_THREADS"] = str(n)

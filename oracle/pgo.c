@@ -1,0 +1,749 @@
+/*
+ * pgo: the CPU oracle of the per-stage right-hand side (see pgo.h).
+ *
+ * TEST INFRASTRUCTURE ONLY: tests, the smoke check and the benchmark's
+ * CPU-baseline leg call it as the checker; the product never does.
+ *
+ * Built with -ffp-contract=off so every expression is evaluated in the order
+ * written, one rounding per operation, as the reference's host backend does
+ * on a target without fused multiply-add.
+ */
+#include "pgo.h"
+
+#include <float.h>
+#include <math.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define MAXNS 16
+#define MAXNR 64
+#define MAXTB 16
+
+static inline double *at(pgo_arr a, int i, int j, int k, int c) {
+  return a.d + i + j * a.sj + k * a.sk + c * a.sc;
+}
+
+int pgo_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+/* ------------------------------------------------------------------------
+ * tlog / texp: backend/base/mathfns.py:1-22, backend/base/kernels/mathfns.mako:6-54
+ * log by a 128-bin table over the mantissa and a 5-term series; exp by a 64-bin
+ * table of 2^(j/64) and a 6-term series, the exponent added in the bits.
+ * ---------------------------------------------------------------------- */
+static double LOGTAB[128], RCPTAB[128], EXPTAB[64];
+static int tables_ready = 0;
+static const double LN2 = 0.6931471805599453;
+/* 64/ln 2 and ln 2/64 in two parts, as correctly rounded from a 40-digit ln 2 */
+static const double EXPK = 0x1.71547652b82fep+6;
+static const double LN2BINS_HI = 0x1.62e42fef00000p-7;
+static const double LN2BINS_LO = 0x1.473de6af278edp-40;
+
+static void make_tables(void) {
+  if (tables_ready) return;
+  for (int k = 0; k < 128; k++) {
+    double m = 0.5 + (k + 0.5) / (2 * 128);
+    LOGTAB[k] = log(m);
+    RCPTAB[k] = 1.0 / m;
+  }
+  for (int j = 0; j < 64; j++) EXPTAB[j] = pow(2.0, j / 64.0);
+  tables_ready = 1;
+}
+
+double pgo_tlog(double x) {
+  make_tables();
+  int64_t bits;
+  memcpy(&bits, &x, sizeof bits);
+  const int expo = (int)((bits >> 52) & 0x7ff) - 1022;
+  const int bin = (int)((bits >> (52 - 7)) & 127);
+  bits = (bits & 0x000fffffffffffffLL) | 0x3fe0000000000000LL;
+  double mant;
+  memcpy(&mant, &bits, sizeof mant);
+  const double u = mant * RCPTAB[bin] - 1.0;
+  double s = 0.2;
+  s = s * u - 0.25;
+  s = s * u + 1.0 / 3.0;
+  s = s * u - 0.5;
+  s = s * u + 1.0;
+  return expo * LN2 + LOGTAB[bin] + u * s;
+}
+
+double pgo_texp(double x) {
+  make_tables();
+  const double t = x * EXPK + 6755399441055744.0;
+  int64_t bits;
+  memcpy(&bits, &t, sizeof bits);
+  const int kexp = (int)bits, nexp = kexp >> 6;
+  const double kd = t - 6755399441055744.0;
+  const double r = (x - kd * LN2BINS_HI) - kd * LN2BINS_LO;
+  double s = 1.0 / 120.0;
+  s = s * r + 1.0 / 24.0;
+  s = s * r + 1.0 / 6.0;
+  s = s * r + 0.5;
+  s = s * r + 1.0;
+  s = s * r + 1.0;
+  const double m = EXPTAB[kexp & 63] * s;
+  memcpy(&bits, &m, sizeof bits);
+  bits += (int64_t)(nexp > -1022 ? nexp : -1022) << 52;
+  double y;
+  memcpy(&y, &bits, sizeof y);
+  return y;
+}
+
+/* ------------------------------------------------------------------------
+ * the equation of state
+ * ---------------------------------------------------------------------- */
+typedef struct {
+  double p, T, rhoinv, rho, e, cp, gamma, c;
+} eosstate;
+
+typedef struct {
+  double R, cp, h298, cpcoeff[8];
+} mixmom;
+
+/* eos/kernels/mixing.mako:3-30: the carried species, then the remainder */
+static void mix(const pgo_case *c, const double *Y, mixmom *m) {
+  memset(m, 0, sizeof *m);
+  double Ylast = 1.0;
+  for (int n = 0; n < c->ns; n++) {
+    double Yn;
+    if (n < c->ns - 1) {
+      Yn = Y[n];
+      Ylast -= Yn;
+    } else {
+      Yn = Ylast;
+    }
+    m->R += Yn * c->Rsp[n];
+    if (c->eos == 0) {
+      m->cp += Yn * c->cp0[n];
+    } else {
+      for (int t = 0; t < c->nt; t++) m->cpcoeff[t] += Yn * c->cppolyR[n * c->nt + t];
+    }
+    m->h298 += Yn * c->h298R[n];
+  }
+}
+
+/* caloric/tdepcp/kernels/caloric.mako:4-15 */
+static void ideal_cph(const pgo_case *c, const mixmom *m, double T, double *cp, double *h) {
+  const double u = log(T);
+  double cpv = 0.0, hv = 0.0, hc = 0.0;
+  for (int t = c->nt - 1; t >= 0; t--) {
+    cpv = cpv * u + m->cpcoeff[t];
+    hc = m->cpcoeff[t] - (t + 1) * hc;
+    hv = hv * u + hc;
+  }
+  *cp = cpv;
+  *h = hv * T + m->h298;
+}
+
+/* cpg eos/cpg/kernels/eos.mako:5-15; tpg eos/tpg/kernels/eos.mako:8-20 and newton.mako:3-11 */
+static void fromrhoE(const pgo_case *c, const mixmom *m, double rho, double e, double Tguess, eosstate *s) {
+  if (c->eos == 0) {
+    const double cv = m->cp - m->R;
+    s->rho = rho;
+    s->rhoinv = 1.0 / rho;
+    s->e = e;
+    s->T = (e - m->h298) / cv;
+    s->p = rho * m->R * s->T;
+    s->cp = m->cp;
+    s->gamma = m->cp / cv;
+    s->c = sqrt(s->gamma * m->R * s->T);
+    return;
+  }
+  double T = (Tguess < 1.0) ? 300.0 : Tguess, cp = 0.0, h = 0.0;
+  const double tol = 1e5 * DBL_EPSILON;
+  for (int k = 0; k < 12; k++) {
+    ideal_cph(c, m, T, &cp, &h);
+    const double dT = (e - (h - m->R * T)) / (cp - m->R);
+    T += dT;
+    if (k >= 2 && fabs(dT) <= tol * T) break;
+  }
+  s->rho = rho;
+  s->rhoinv = 1.0 / rho;
+  s->e = e;
+  s->T = T;
+  s->p = rho * m->R * T;
+  s->cp = cp;
+  s->gamma = cp / (cp - m->R);
+  s->c = sqrt(s->gamma * m->R * T);
+}
+
+/* cpg eos.mako:17-27; tpg eos.mako:22-33 */
+static void frompT(const pgo_case *c, const mixmom *m, double p, double T, eosstate *s) {
+  s->p = p;
+  s->T = T;
+  s->rho = p / (m->R * T);
+  s->rhoinv = 1.0 / s->rho;
+  if (c->eos == 0) {
+    const double cv = m->cp - m->R;
+    s->e = cv * T + m->h298;
+    s->cp = m->cp;
+    s->gamma = m->cp / cv;
+  } else {
+    double cp, h;
+    ideal_cph(c, m, T, &cp, &h);
+    s->e = h - m->R * T;
+    s->cp = cp;
+    s->gamma = cp / (cp - m->R);
+  }
+  s->c = sqrt(s->gamma * m->R * T);
+}
+
+static void store(pgo_arr st, int i, int j, int k, const eosstate *s) {
+  *at(st, i, j, k, 0) = s->p;
+  *at(st, i, j, k, 1) = s->T;
+  *at(st, i, j, k, 2) = s->rhoinv;
+  *at(st, i, j, k, 3) = s->rho;
+  *at(st, i, j, k, 4) = s->e;
+  *at(st, i, j, k, 5) = s->cp;
+  *at(st, i, j, k, 6) = s->gamma;
+  *at(st, i, j, k, 7) = s->c;
+}
+
+static void load(pgo_arr st, int i, int j, int k, eosstate *s) {
+  s->p = *at(st, i, j, k, 0);
+  s->T = *at(st, i, j, k, 1);
+  s->rhoinv = *at(st, i, j, k, 2);
+  s->rho = *at(st, i, j, k, 3);
+  s->e = *at(st, i, j, k, 4);
+  s->cp = *at(st, i, j, k, 5);
+  s->gamma = *at(st, i, j, k, 6);
+  s->c = *at(st, i, j, k, 7);
+}
+
+#define FOR_BOX(r)                                                     \
+  _Pragma("omp parallel for collapse(2) schedule(static)")             \
+  for (int k = (r).start[2]; k < (r).start[2] + (r).extent[2]; k++)    \
+    for (int j = (r).start[1]; j < (r).start[1] + (r).extent[1]; j++)  \
+      for (int i = (r).start[0]; i < (r).start[0] + (r).extent[0]; i++)
+
+/* euler/kernels/state/fromQ.mako:5-19 with efromQ, solution.mako:10-12 */
+void pgo_fromQ(const pgo_case *c, pgo_arr Q, pgo_arr state, pgo_box r) {
+  FOR_BOX(r) {
+    const double rho = *at(Q, i, j, k, 0), rhoinv = 1.0 / rho;
+    const double q1 = *at(Q, i, j, k, 1), q2 = *at(Q, i, j, k, 2), q3 = *at(Q, i, j, k, 3);
+    const double e = (*at(Q, i, j, k, 4) - 0.5 * ((q1) * (q1) + (q2) * (q2) + (q3) * (q3)) * rhoinv) * rhoinv;
+    double Y[MAXNS];
+    for (int n = 0; n < c->ns - 1; n++) Y[n] = *at(Q, i, j, k, 5 + n) * rhoinv;
+    mixmom m;
+    mix(c, Y, &m);
+    eosstate s;
+    fromrhoE(c, &m, rho, e, *at(state, i, j, k, 1), &s);
+    store(state, i, j, k, &s);
+  }
+}
+
+/* euler/kernels/state/fromprims.mako:4-13 with fromprims/fromthermo, solution.mako:2-23 */
+void pgo_fromprims(const pgo_case *c, pgo_arr prims, pgo_arr Q, pgo_arr state, pgo_box r) {
+  FOR_BOX(r) {
+    double Y[MAXNS], u[3];
+    for (int n = 0; n < c->ns - 1; n++) Y[n] = *at(prims, i, j, k, 5 + n);
+    for (int d = 0; d < 3; d++) u[d] = *at(prims, i, j, k, 1 + d);
+    mixmom m;
+    mix(c, Y, &m);
+    eosstate s;
+    frompT(c, &m, *at(prims, i, j, k, 0), *at(prims, i, j, k, 4), &s);
+    *at(Q, i, j, k, 0) = s.rho;
+    *at(Q, i, j, k, 1) = s.rho * u[0];
+    *at(Q, i, j, k, 2) = s.rho * u[1];
+    *at(Q, i, j, k, 3) = s.rho * u[2];
+    *at(Q, i, j, k, 4) = s.rho * (s.e + 0.5 * ((u[0]) * (u[0]) + (u[1]) * (u[1]) + (u[2]) * (u[2])));
+    for (int n = 0; n < c->ns - 1; n++) *at(Q, i, j, k, 5 + n) = s.rho * Y[n];
+    store(state, i, j, k, &s);
+  }
+}
+
+/* ------------------------------------------------------------------------
+ * the advective face flux: euler/kernels/flux/single.mako:4-9
+ * ---------------------------------------------------------------------- */
+typedef struct {
+  eosstate s;
+  double u, v, w;
+  double Y[MAXNS];
+} side;
+
+/* recon/{koren,minmod,mc,superbee,vanleer}.mako: b phi(a/b) for b > 0 */
+static double limit(int kind, double a, double b) {
+  switch (kind) {
+    case 1: return fmax(0.0, fmin(fmin(2.0 * a, (2.0 * b + a) * (1.0 / 3.0)), 2.0 * b));
+    case 2: return fmax(0.0, fmin(b, a));
+    case 3: return fmax(0.0, fmin(fmin(2.0 * a, (b + a) * 0.5), 2.0 * b));
+    case 4: return fmax(0.0, fmax(fmin(2.0 * a, b), fmin(a, 2.0 * b)));
+    default: return (a + fabs(a)) * b / (b + fabs(a));
+  }
+}
+
+/* recon/muscl.mako:5-13 */
+static double phi(int kind, double dm, double dp) {
+  double p = 0.0;
+  if (dp != 0.0) {
+    const double s = copysign(1.0, dp), a = s * dm, b = fabs(dp);
+    p = s * limit(kind, a, b);
+  }
+  return p;
+}
+
+static double lft(int kind, double vLL, double vL, double vR) { return vL + 0.5 * phi(kind, vL - vLL, vR - vL); }
+static double rgt(int kind, double vL, double vR, double vRR) { return vR - 0.5 * phi(kind, vRR - vR, vR - vL); }
+
+/* the face's two sides: recon/constant.mako:3-18 or recon/muscl.mako:15-51 */
+static void sides(const pgo_case *c, pgo_arr Q, pgo_arr st, const int *cl, const int *d, side *L, side *R) {
+  /* cl: the cell right of the face (R); d: the unit step along the axis */
+  int idx[4][3];
+  for (int o = 0; o < 4; o++)
+    for (int a = 0; a < 3; a++) idx[o][a] = cl[a] + (o - 2) * d[a];
+#define QQ(o, comp) (*at(Q, idx[o][0], idx[o][1], idx[o][2], comp))
+#define SS(o, f) (*at(st, idx[o][0], idx[o][1], idx[o][2], f))
+  const int ns = c->ns;
+  if (c->recon == 0) {
+    load(st, idx[1][0], idx[1][1], idx[1][2], &L->s);
+    load(st, idx[2][0], idx[2][1], idx[2][2], &R->s);
+    L->u = QQ(1, 1) / QQ(1, 0);
+    L->v = QQ(1, 2) / QQ(1, 0);
+    L->w = QQ(1, 3) / QQ(1, 0);
+    R->u = QQ(2, 1) / QQ(2, 0);
+    R->v = QQ(2, 2) / QQ(2, 0);
+    R->w = QQ(2, 3) / QQ(2, 0);
+    for (int n = 0; n < ns - 1; n++) {
+      L->Y[n] = QQ(1, 5 + n) / QQ(1, 0);
+      R->Y[n] = QQ(2, 5 + n) / QQ(2, 0);
+    }
+    return;
+  }
+  const int lim = c->limiter;
+  const double ri[4] = {SS(0, 2), SS(1, 2), SS(2, 2), SS(3, 2)};
+  const double pL = lft(lim, SS(0, 0), SS(1, 0), SS(2, 0)), pR = rgt(lim, SS(1, 0), SS(2, 0), SS(3, 0));
+  const double TL = lft(lim, SS(0, 1), SS(1, 1), SS(2, 1)), TR = rgt(lim, SS(1, 1), SS(2, 1), SS(3, 1));
+  for (int n = 0; n < ns - 1; n++) {
+    L->Y[n] = lft(lim, QQ(0, 5 + n) * ri[0], QQ(1, 5 + n) * ri[1], QQ(2, 5 + n) * ri[2]);
+    R->Y[n] = rgt(lim, QQ(1, 5 + n) * ri[1], QQ(2, 5 + n) * ri[2], QQ(3, 5 + n) * ri[3]);
+  }
+  mixmom m;
+  mix(c, L->Y, &m);
+  frompT(c, &m, pL, TL, &L->s);
+  L->u = lft(lim, QQ(0, 1) * ri[0], QQ(1, 1) * ri[1], QQ(2, 1) * ri[2]);
+  L->v = lft(lim, QQ(0, 2) * ri[0], QQ(1, 2) * ri[1], QQ(2, 2) * ri[2]);
+  L->w = lft(lim, QQ(0, 3) * ri[0], QQ(1, 3) * ri[1], QQ(2, 3) * ri[2]);
+  mix(c, R->Y, &m);
+  frompT(c, &m, pR, TR, &R->s);
+  R->u = rgt(lim, QQ(1, 1) * ri[1], QQ(2, 1) * ri[2], QQ(3, 1) * ri[3]);
+  R->v = rgt(lim, QQ(1, 2) * ri[1], QQ(2, 2) * ri[2], QQ(3, 2) * ri[3]);
+  R->w = rgt(lim, QQ(1, 3) * ri[1], QQ(2, 3) * ri[2], QQ(3, 3) * ri[3]);
+#undef QQ
+#undef SS
+}
+
+/* function/kepaec.mako:2-20 (Shima et al. 2021) */
+static void kepaec(int ns, const side *L, const side *R, const double *S, double *F) {
+  const double uf = 0.5 * (R->u + L->u), vf = 0.5 * (R->v + L->v), wf = 0.5 * (R->w + L->w);
+  const double U = ((S[0]) * (uf) + (S[1]) * (vf) + (S[2]) * (wf));
+  const double pf = 0.5 * (R->s.p + L->s.p);
+  const double rho = 0.5 * (R->s.rho + L->s.rho);
+  const double C = rho * U;
+  F[0] = C;
+  F[1] = C * uf + pf * S[0];
+  F[2] = C * vf + pf * S[1];
+  F[3] = C * wf + pf * S[2];
+  const double Kj = C * 0.5 * ((R->u) * (L->u) + (R->v) * (L->v) + (R->w) * (L->w));
+  const double Pj = 0.5 * (L->s.p * ((R->u) * (S[0]) + (R->v) * (S[1]) + (R->w) * (S[2])) +
+                           R->s.p * ((L->u) * (S[0]) + (L->v) * (S[1]) + (L->w) * (S[2])));
+  const double thL = L->s.cp / L->s.gamma * L->s.T, thR = R->s.cp / R->s.gamma * R->s.T;
+  const double ef = 2.0 * thL * thR / (thL + thR) + 0.5 * ((L->s.e - thL) + (R->s.e - thR));
+  F[4] = ef * C + Kj + Pj;
+  for (int n = 0; n < ns - 1; n++) F[5 + n] = 0.5 * (R->Y[n] + L->Y[n]) * C;
+}
+
+/* function/hllc.mako:2-43 */
+static void hllc(int ns, const side *L, const side *R, const double *S, double *F) {
+  const double area = sqrt(((S[0]) * (S[0]) + (S[1]) * (S[1]) + (S[2]) * (S[2])));
+  const double nx = S[0] / area, ny = S[1] / area, nz = S[2] / area;
+  const double unL = ((L->u) * (S[0]) + (L->v) * (S[1]) + (L->w) * (S[2])) / area;
+  const double unR = ((R->u) * (S[0]) + (R->v) * (S[1]) + (R->w) * (S[2])) / area;
+  const double SL = unL - L->s.c, SR = unR + R->s.c;
+  const double Sstar = (R->s.p - L->s.p + L->s.rho * unL * (SL - unL) - R->s.rho * unR * (SR - unR)) /
+                       (L->s.rho * (SL - unL) - R->s.rho * (SR - unR));
+  int left, star;
+  if (SL >= 0.0) {
+    left = 1, star = 0;
+  } else if (Sstar >= 0.0) {
+    left = 1, star = 1;
+  } else if (SR >= 0.0) {
+    left = 0, star = 1;
+  } else {
+    left = 0, star = 0;
+  }
+  const side *X = left ? L : R;
+  const double U = left ? unL : unR, SX = left ? SL : SR;
+  const double et = X->s.e + 0.5 * ((X->u) * (X->u) + (X->v) * (X->v) + (X->w) * (X->w));
+  const double mdot = U * X->s.rho * area;
+  F[0] = mdot;
+  F[1] = mdot * X->u + X->s.p * S[0];
+  F[2] = mdot * X->v + X->s.p * S[1];
+  F[3] = mdot * X->w + X->s.p * S[2];
+  F[4] = mdot * et + U * X->s.p * area;
+  for (int n = 0; n < ns - 1; n++) F[5 + n] = mdot * X->Y[n];
+  if (star) {
+    const double rhostar = X->s.rho * ((SX - U) / (SX - Sstar));
+    const double drho = rhostar - X->s.rho, dU = rhostar * (Sstar - U);
+    const double jump = SX * area;
+    F[0] += jump * drho;
+    F[1] += jump * (drho * X->u + dU * nx);
+    F[2] += jump * (drho * X->v + dU * ny);
+    F[3] += jump * (drho * X->w + dU * nz);
+    F[4] += jump * (drho * et + dU * (Sstar + X->s.p / (X->s.rho * (SX - U))));
+    for (int n = 0; n < ns - 1; n++) F[5 + n] += jump * drho * X->Y[n];
+  }
+}
+
+/* function/rusanov.mako:2-19 */
+static void rusanov(int ns, const side *L, const side *R, const double *S, double *F) {
+  const double area = sqrt(((S[0]) * (S[0]) + (S[1]) * (S[1]) + (S[2]) * (S[2])));
+  const double unL = ((L->u) * (S[0]) + (L->v) * (S[1]) + (L->w) * (S[2])) / area;
+  const double unR = ((R->u) * (S[0]) + (R->v) * (S[1]) + (R->w) * (S[2])) / area;
+  const double lam = fmax(fabs(unL) + L->s.c, fabs(unR) + R->s.c) * area;
+  const double UL = unL * area, UR = unR * area;
+  const double rL = L->s.rho, rR = R->s.rho;
+  F[0] = 0.5 * (UR * rR + UL * rL - lam * (rR - rL));
+  F[1] = 0.5 * (UR * rR * R->u + R->s.p * S[0] + UL * rL * L->u + L->s.p * S[0] - lam * (rR * R->u - rL * L->u));
+  F[2] = 0.5 * (UR * rR * R->v + R->s.p * S[1] + UL * rL * L->v + L->s.p * S[1] - lam * (rR * R->v - rL * L->v));
+  F[3] = 0.5 * (UR * rR * R->w + R->s.p * S[2] + UL * rL * L->w + L->s.p * S[2] - lam * (rR * R->w - rL * L->w));
+  const double EL = rL * (L->s.e + 0.5 * ((L->u) * (L->u) + (L->v) * (L->v) + (L->w) * (L->w)));
+  const double ER = rR * (R->s.e + 0.5 * ((R->u) * (R->u) + (R->v) * (R->v) + (R->w) * (R->w)));
+  F[4] = 0.5 * (UR * (ER + R->s.p) + UL * (EL + L->s.p) - lam * (ER - EL));
+  for (int n = 0; n < ns - 1; n++) {
+    const double rhoYL = rL * L->Y[n], rhoYR = rR * R->Y[n];
+    F[5 + n] = 0.5 * (rhoYR * UR + rhoYL * UL - lam * (rhoYR - rhoYL));
+  }
+}
+
+void pgo_flux(const pgo_case *c, int axis, pgo_arr Q, pgo_arr state, pgo_arr S, pgo_arr F, pgo_box r) {
+  int d[3] = {0, 0, 0};
+  d[axis] = 1;
+  FOR_BOX(r) {
+    side L, R;
+    const int cl[3] = {i, j, k};
+    sides(c, Q, state, cl, d, &L, &R);
+    const double Sv[3] = {*at(S, i, j, k, 0), *at(S, i, j, k, 1), *at(S, i, j, k, 2)};
+    double Fv[5 + MAXNS];
+    if (c->primary == 0)
+      kepaec(c->ns, &L, &R, Sv, Fv);
+    else if (c->primary == 1)
+      hllc(c->ns, &L, &R, Sv, Fv);
+    else
+      rusanov(c->ns, &L, &R, Sv, Fv);
+    for (int l = 0; l < c->ne; l++) *at(F, i, j, k, l) = Fv[l];
+  }
+}
+
+/* advec/kernels/applyflux.mako:2-11, summed low in, then high out */
+void pgo_applyflux(int ne, pgo_arr dQ, pgo_arr iF, pgo_arr jF, pgo_arr kF, pgo_arr detJinv, int append, pgo_box r) {
+  FOR_BOX(r) {
+    const double J = *at(detJinv, i, j, k, 0);
+    for (int l = 0; l < ne; l++) {
+      const double v = (*at(iF, i, j, k, l) + *at(jF, i, j, k, l) + *at(kF, i, j, k, l) - *at(iF, i + 1, j, k, l) -
+                        *at(jF, i, j + 1, k, l) - *at(kF, i, j, k + 1, l)) * J;
+      if (append)
+        *at(dQ, i, j, k, l) += v;
+      else
+        *at(dQ, i, j, k, l) = v;
+    }
+  }
+}
+
+/* ------------------------------------------------------------------------
+ * finite-rate production: chemistry/kernels/finiterate.mako, table-driven
+ * ---------------------------------------------------------------------- */
+static double troeblend(const pgo_case *c, int k, double logT, double logpr) {
+  const double log10e = 0.4342944819032518, ln10 = 2.302585092994046;
+  const double *P = c->lfcpoly + 13 * c->troeidx[k];
+  const double tf = (logT - c->troeuc) * c->troeuhinv;
+  double lfc = P[12];
+  for (int m = 11; m >= 0; m--) lfc = lfc * tf + P[m];
+  const double C = -0.4 - 0.67 * lfc, N = 0.75 - 1.27 * lfc;
+  const double A = logpr * log10e + C, B = N - 0.14 * A, B2 = B * B;
+  return lfc * B2 / (B2 + A * A) * ln10;
+}
+
+void pgo_production(const pgo_case *c, double rho, const double *Y, double T, double *omega) {
+  const int ns = c->ns;
+  /* ratestate, finiterate.mako:136-182 */
+  const double logT = pgo_tlog(T), Tinv = 1.0 / T;
+  const double logprefRuT = c->logprefRu - logT;
+  const double prefRuT = c->prefRu * Tinv, prefRuTinv = c->Rupref * T;
+  double ctot = 0.0, gRT[MAXNS], logc[MAXNS], cG[MAXNS], conc[MAXNS];
+  for (int n = 0; n < ns; n++) {
+    const double cn = rho * Y[n] * c->MWinv[n];
+    conc[n] = cn;
+    ctot += cn;
+    double g = 0.0;
+    for (int m = c->ng_ - 1; m >= 0; m--) g = g * logT + c->gpoly[n * c->ng_ + m];
+    gRT[n] = g + c->h298[n] * Tinv;
+    logc[n] = fmax(c->logcfloor, pgo_tlog(cn));
+    cG[n] = fmax(cn, 0.0) * pgo_texp(gRT[n]);
+  }
+  double ctb[MAXTB], logtb[MAXTB];
+  for (int j = 0; j < c->ntb; j++) {
+    double v = c->tbdefeff[j] * ctot;
+    for (int e = 0; e < c->ntbeff[j]; e++) {
+      const int sp = c->tbeffsp[j * ns + e];
+      v += c->tbeffdev[j * ns + e] * (rho * Y[sp] * c->MWinv[sp]);
+    }
+    ctb[j] = v;
+    logtb[j] = pgo_tlog(fmax(DBL_MIN, v));
+  }
+  (void)conc;
+  /* fold, finiterate.mako:185-226 */
+  const double expclamp = c->expclamp;
+  for (int n = 0; n < ns; n++) omega[n] = 0.0;
+  for (int k = 0; k < c->nr; k++) {
+    const int kind = c->kind[k], thirdbody = kind >= 1, falloff = kind >= 2, troe = kind == 3;
+    const int tb = c->tb[k];
+    double rp;
+    double lk = c->logA[k];
+    if (c->b[k] != 0.0) lk += c->b[k] * logT;
+    if (c->EaR[k] != 0.0) lk -= c->EaR[k] * Tinv;
+    if (c->product[k]) {
+      /* kform, finiterate.mako:69-125 */
+      double logpr = 0.0, pfac = 1.0;
+      if (falloff) {
+        logpr = logtb[tb] + c->logprA[k];
+        if (c->prb[k] != 0.0) logpr += c->prb[k] * logT;
+        if (c->prEaR[k] != 0.0) logpr -= c->prEaR[k] * Tinv;
+        const double pr = pgo_texp(fmin(logpr, expclamp));
+        pfac = pr / (1.0 + pr);
+      }
+      if (troe) lk += troeblend(c, k, logT, logpr);
+      double xk = lk;
+      for (int j = 0; j < c->nreact[k]; j++) xk -= c->reactnu[4 * k + j] * gRT[c->reactsp[4 * k + j]];
+      double K = pgo_texp(fmin(xk, expclamp));
+      if (thirdbody && !falloff) K *= ctb[tb];
+      if (falloff) K *= pfac;
+      double fwd = K;
+      for (int j = 0; j < c->nreact[k]; j++)
+        for (int t = 0; t < (int)c->reactnu[4 * k + j]; t++) fwd *= cG[c->reactsp[4 * k + j]];
+      double rev = 0.0;
+      if (c->reversible[k]) {
+        rev = K;
+        const double nus = c->nusum[k];
+        for (int t = 0; t < (int)fabs(nus); t++) rev *= nus > 0 ? prefRuTinv : prefRuT;
+        for (int j = 0; j < c->nprod[k]; j++)
+          for (int t = 0; t < (int)c->prodnu[4 * k + j]; t++) rev *= cG[c->prodsp[4 * k + j]];
+      }
+      rp = fwd - rev;
+    } else {
+      /* progress, finiterate.mako:7-48 */
+      if (thirdbody && !falloff) lk += logtb[tb];
+      double logpr = 0.0;
+      if (falloff) {
+        logpr = logtb[tb] + c->logprA[k];
+        if (c->prb[k] != 0.0) logpr += c->prb[k] * logT;
+        if (c->prEaR[k] != 0.0) logpr -= c->prEaR[k] * Tinv;
+        const double prexp = pgo_texp(-fabs(logpr));
+        lk += fmin(logpr, 0.0) - log1p(prexp);
+      }
+      if (troe) lk += troeblend(c, k, logT, logpr);
+      double lrp = lk;
+      for (int j = 0; j < c->nfwd[k]; j++) lrp += c->fwdorder[4 * k + j] * logc[c->fwdsp[4 * k + j]];
+      if (c->reversible[k]) {
+        double diff = c->nusum[k] != 0.0 ? c->nusum[k] * logprefRuT : 0.0;
+        for (int j = 0; j < c->nnet[k]; j++) {
+          const int sp = c->netsp[4 * k + j];
+          diff -= c->netnu[4 * k + j] * (gRT[sp] + logc[sp]);
+        }
+        const double fwd = pgo_texp(fmin(lrp, expclamp)), rev = pgo_texp(fmin(lrp - diff, expclamp));
+        rp = fwd - rev;
+      } else {
+        rp = pgo_texp(fmin(lrp, expclamp));
+      }
+    }
+    for (int j = 0; j < c->nnet[k]; j++) omega[c->netsp[4 * k + j]] += c->netnuMW[4 * k + j] * rp;
+  }
+}
+
+/* chemint/kernels/source.mako:3-10, cell.mako:5-20, massfrac solution.mako:25-35 */
+void pgo_source(const pgo_case *c, pgo_arr Q, pgo_arr state, pgo_arr dQ, pgo_box r) {
+  const int ns = c->ns;
+  FOR_BOX(r) {
+    const double rho = *at(Q, i, j, k, 0), rhoinv = 1.0 / rho;
+    double Ylast = 1.0, Ysum = 0.0, Y[MAXNS], omega[MAXNS];
+    for (int n = 0; n < ns - 1; n++) {
+      const double Yn = fmax(fmin(*at(Q, i, j, k, 5 + n) * rhoinv, 1.0), 0.0);
+      Ylast -= Yn;
+      Ysum += Yn;
+    }
+    const double Yscale = 1.0 / fmax(Ysum, 1.0);
+    Ylast = fmax(Ylast, 0.0);
+    for (int n = 0; n < ns - 1; n++) Y[n] = fmax(fmin(*at(Q, i, j, k, 5 + n) * rhoinv, 1.0), 0.0) * Yscale;
+    Y[ns - 1] = Ylast;
+    pgo_production(c, rho, Y, *at(state, i, j, k, 1), omega);
+    for (int l = 0; l < 5; l++) *at(dQ, i, j, k, l) = 0.0;
+    for (int n = 0; n < ns - 1; n++) *at(dQ, i, j, k, 5 + n) = omega[n];
+  }
+}
+
+/* integrators/kernels/combine.mako:1-13: out = [wout out] + w0 X0 + ... + w_last dt X_last */
+void pgo_combine(int ne, pgo_arr out, int own, double wout, int nterms, const pgo_arr *X, const double *w,
+                 double dt, int step, pgo_box r) {
+  FOR_BOX(r) {
+    for (int l = 0; l < ne; l++) {
+      double acc = 0.0;
+      int first = 1;
+      if (own) {
+        acc = wout * *at(out, i, j, k, l);
+        first = 0;
+      }
+      for (int m = 0; m < nterms; m++) {
+        const double term = (step && m == nterms - 1) ? w[m] * dt * *at(X[m], i, j, k, l) : w[m] * *at(X[m], i, j, k, l);
+        acc = first ? term : acc + term;
+        first = 0;
+      }
+      *at(out, i, j, k, l) = acc;
+    }
+  }
+}
+
+/* multiblock/collections/exchange/kernels/place.mako:11-37 on a block periodic with itself:
+ * each halo slab takes the interior planes next to the opposite interface; the other
+ * axes run over the interior only (edges and corners are never filled) */
+void pgo_fill_periodic(pgo_arr A, int ncomp, const int *ncell, int ng, int depth, int axis) {
+  const int n = ncell[axis];
+  int lo[3], hi[3];
+  for (int a = 0; a < 3; a++) {
+    lo[a] = ng;
+    hi[a] = ng + ncell[a];
+  }
+  lo[axis] = 0;
+  hi[axis] = depth;
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int l = 0; l < ncomp; l++)
+    for (int k = lo[2]; k < hi[2]; k++)
+      for (int j = lo[1]; j < hi[1]; j++)
+        for (int i = lo[0]; i < hi[0]; i++) {
+          int h[3] = {i, j, k};
+          /* layer h[axis] = 0..depth-1 counted outward */
+          const int layer = h[axis];
+          int dlo[3] = {i, j, k}, slo[3] = {i, j, k}, dhi[3] = {i, j, k}, shi[3] = {i, j, k};
+          dlo[axis] = ng - 1 - layer;
+          slo[axis] = ng - 1 - layer + n;
+          dhi[axis] = ng + n + layer;
+          shi[axis] = ng + layer;
+          *at(A, dlo[0], dlo[1], dlo[2], l) = *at(A, slo[0], slo[1], slo[2], l);
+          *at(A, dhi[0], dhi[1], dhi[2], l) = *at(A, shi[0], shi[1], shi[2], l);
+        }
+}
+
+/* ------------------------------------------------------------------------
+ * a whole stage and step: multiblock/solver.py:246-284,355-364,
+ * simulator/advec/simulator.py:57-87, integrators/rungekutta.py:9-95
+ * ---------------------------------------------------------------------- */
+static pgo_box interior(const pgo_block *b) {
+  pgo_box r;
+  for (int a = 0; a < 3; a++) {
+    r.start[a] = b->ng;
+    r.extent[a] = b->ncell[a];
+  }
+  return r;
+}
+
+static pgo_box facebox(const pgo_block *b, int axis) {
+  pgo_box r = interior(b);
+  r.extent[axis] += 1;
+  return r;
+}
+
+static void prerhs(const pgo_case *c, pgo_block *b, pgo_arr Q, pgo_arr dQ, int withdQ) {
+  for (int a = 0; a < 3; a++) pgo_fill_periodic(Q, c->ne, b->ncell, b->ng, b->ng, a);
+  for (int a = 0; a < 3; a++)
+    for (int side = 0; side < 2; side++) {
+      pgo_box r = interior(b);
+      r.start[a] = side ? b->ng + b->ncell[a] : 0;
+      r.extent[a] = b->ng;
+      pgo_fromQ(c, Q, b->state, r);
+    }
+  if (withdQ && b->chemistry) pgo_source(c, Q, b->state, dQ, interior(b));
+}
+
+static void rhs(const pgo_case *c, pgo_block *b, pgo_arr Q, pgo_arr dQ) {
+  pgo_arr F[3] = {b->iF, b->jF, b->kF}, S[3] = {b->iS, b->jS, b->kS};
+  for (int a = 0; a < 3; a++) pgo_flux(c, a, Q, b->state, S[a], F[a], facebox(b, a));
+  pgo_applyflux(c->ne, dQ, b->iF, b->jF, b->kF, b->detJinv, b->chemistry, interior(b));
+}
+
+/* the registers a stage reads and writes, and its combine weights */
+void pgo_advance(const pgo_case *c, pgo_block *b, int n, double dt) {
+  const pgo_box r = interior(b);
+  pgo_arr start = b->regs[0], stage = b->regs[1], dQ = b->regs[b->nregs - 1];
+  if (b->integrator == 3) {
+    pgo_arr total = b->regs[2];
+    static const double wdQ[4] = {0.5, 0.5, 1.0, -1.0}, wsum[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
+    const double w1[1] = {wsum[n]};
+    pgo_combine(c->ne, total, n != 0, 1.0, 1, &dQ, w1, dt, 1, r);
+    if (n == 3) {
+      const double w2[1] = {1.0};
+      pgo_combine(c->ne, start, 1, 1.0, 1, &total, w2, dt, 0, r);
+    } else {
+      pgo_arr X[2] = {start, dQ};
+      const double w2[2] = {1.0, wdQ[n]};
+      pgo_combine(c->ne, stage, 0, 0.0, 2, X, w2, dt, 1, r);
+    }
+    return;
+  }
+  /* (wQ0, wQ, wdQ) per stage; Qout = wQ Q + wQ0 Qstart + wdQ dt dQ */
+  static const double rk1[1][3] = {{0.0, 1.0, 1.0}};
+  static const double rk2[2][3] = {{0.0, 1.0, 1.0}, {0.5, 0.5, 0.5}};
+  static const double rk3[3][3] = {{0.0, 1.0, 1.0}, {0.75, 0.25, 0.25}, {1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0}};
+  const double *wt = b->integrator == 0 ? rk1[n] : b->integrator == 1 ? rk2[n] : rk3[n];
+  const int nst = b->integrator + 1, last = nst - 1;
+  pgo_arr Q = n == 0 ? start : stage, Qout = n == last ? start : stage;
+  const int Qis0 = n == 0, outis0 = n == last;
+  /* the weights per distinct register, Q first, then the start, then dQ */
+  double wQ = wt[1], w0 = wt[0];
+  if (Qis0) {
+    wQ += w0;
+    w0 = 0.0;
+  }
+  double own = 0.0;
+  pgo_arr X[3];
+  double w[3];
+  int nt = 0;
+  /* Q's weight */
+  if ((Qis0 && outis0) || (!Qis0 && !outis0)) {
+    own = wQ;
+  } else if (wQ != 0.0) {
+    X[nt] = Q;
+    w[nt++] = wQ;
+  }
+  if (!Qis0) {
+    if (outis0)
+      own = w0;
+    else if (w0 != 0.0) {
+      X[nt] = start;
+      w[nt++] = w0;
+    }
+  }
+  X[nt] = dQ;
+  w[nt++] = wt[2];
+  pgo_combine(c->ne, Qout, own != 0.0, own, nt, X, w, dt, 1, r);
+}
+
+void pgo_stage(const pgo_case *c, pgo_block *b, int n, double dt) {
+  const int nst = b->integrator == 3 ? 4 : b->integrator + 1, last = nst - 1;
+  pgo_arr Q = n == 0 ? b->regs[0] : b->regs[1], dQ = b->regs[b->nregs - 1];
+  pgo_arr Qout = n == last ? b->regs[0] : b->regs[1];
+  prerhs(c, b, Q, dQ, 1);
+  rhs(c, b, Q, dQ);
+  pgo_advance(c, b, n, dt);
+  pgo_fromQ(c, Qout, b->state, interior(b));
+}
+
+void pgo_step(const pgo_case *c, pgo_block *b, double dt) {
+  const int nst = b->integrator == 3 ? 4 : b->integrator + 1;
+  for (int n = 0; n < nst; n++) pgo_stage(c, b, n, dt);
+}

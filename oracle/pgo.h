@@ -1,0 +1,76 @@
+/*
+ * pgo: the CPU oracle of the per-stage right-hand side.
+ *
+ * TEST INFRASTRUCTURE ONLY. This is a plain C restatement of the reference's
+ * hot-path kernels, written loop by loop from the Mako templates they are
+ * rendered from (each function cites its template). It is read by the tests,
+ * by the smoke check and by the CPU-baseline leg of the benchmark, always as
+ * the checker; the product path never links, loads or calls it.
+ *
+ * Layout: every array is the device layout of the reference, Fortran order
+ * [i, j, k, comp] with i fastest and each component a separate plane, ng halo
+ * layers on every side; a view carries the j, k and component strides.
+ */
+#ifndef PGO_H
+#define PGO_H
+
+#include <stdint.h>
+
+typedef struct {
+  double *d;
+  int64_t sj, sk, sc;
+} pgo_arr;
+
+typedef struct {
+  int start[3];
+  int extent[3];
+} pgo_box;
+
+/* what the kernels read from the mixture, and which kernel variants a case takes */
+typedef struct {
+  int ns, ne;
+  int eos;     /* 0 cpg, 1 tpg */
+  int recon;   /* 0 constant, 1 muscl */
+  int limiter; /* 1 koren 2 minmod 3 mc 4 superbee 5 vanleer */
+  int primary; /* 0 kepaec, 1 hllc, 2 rusanov */
+  int nt, ng_; /* cppoly terms (7), gpoly terms (8) */
+  const double *MW, *MWinv, *Rsp, *cppolyR, *h298R, *cp0, *gpoly, *h298;
+  /* finite-rate tables, one row per reaction (width 4 for species lists) */
+  int nr, ntb;
+  const int32_t *kind, *reversible, *product, *tb, *troeidx;
+  const int32_t *nfwd, *fwdsp, *nreact, *reactsp, *nprod, *prodsp, *nnet, *netsp, *ntbeff, *tbeffsp;
+  const double *logA, *b, *EaR, *logprA, *prb, *prEaR, *nusum;
+  const double *fwdorder, *reactnu, *prodnu, *netnu, *netnuMW, *lfcpoly, *tbdefeff, *tbeffdev;
+  double logprefRu, prefRu, Rupref, logcfloor, expclamp, troeuc, troeuhinv;
+} pgo_case;
+
+double pgo_tlog(double x);
+double pgo_texp(double x);
+
+void pgo_fromQ(const pgo_case *c, pgo_arr Q, pgo_arr state, pgo_box r);
+void pgo_fromprims(const pgo_case *c, pgo_arr prims, pgo_arr Q, pgo_arr state, pgo_box r);
+void pgo_flux(const pgo_case *c, int axis, pgo_arr Q, pgo_arr state, pgo_arr S, pgo_arr F, pgo_box r);
+void pgo_applyflux(int ne, pgo_arr dQ, pgo_arr iF, pgo_arr jF, pgo_arr kF, pgo_arr detJinv, int append, pgo_box r);
+void pgo_production(const pgo_case *c, double rho, const double *Y, double T, double *omega);
+void pgo_source(const pgo_case *c, pgo_arr Q, pgo_arr state, pgo_arr dQ, pgo_box r);
+void pgo_combine(int ne, pgo_arr out, int own, double wout, int nterms, const pgo_arr *X, const double *w,
+                 double dt, int step, pgo_box r);
+void pgo_fill_periodic(pgo_arr A, int ncomp, const int *ncell, int ng, int depth, int axis);
+
+/* one whole step of a single periodic block: the stage sequence of the reference's solver */
+typedef struct {
+  int ncell[3], ng;
+  pgo_arr regs[4]; /* solution registers; the last is dQ */
+  int nregs;
+  pgo_arr state, iF, jF, kF, iS, jS, kS, detJinv;
+  int integrator; /* 0 rk1, 1 rk2, 2 tvd-rk3, 3 rk4 */
+  int chemistry;  /* 1: the coupled source begins dQ */
+} pgo_block;
+
+void pgo_step(const pgo_case *c, pgo_block *b, double dt);
+void pgo_stage(const pgo_case *c, pgo_block *b, int n, double dt);
+/* the RK combine(s) of stage n alone, over the interior */
+void pgo_advance(const pgo_case *c, pgo_block *b, int n, double dt);
+int pgo_threads(void);
+
+#endif

@@ -1,0 +1,208 @@
+// The advective face flux on the device: face sides by constant or MUSCL
+// reconstruction, then KEEP (kepaec), HLLC or Rusanov.
+#pragma once
+
+#include "pga_eos.hpp"
+
+namespace pga {
+
+enum { RECON_CONSTANT = 0, RECON_MUSCL = 1 };
+enum { FLUX_KEPAEC = 0, FLUX_HLLC = 1, FLUX_RUSANOV = 2 };
+
+template <int NS>
+struct FaceState {
+  EosState s;
+  double u, v, w;
+  double Y[NS > 1 ? NS - 1 : 1];
+};
+
+// b phi(a / b), b > 0, without the ratio (recon/{koren,minmod,mc,superbee,vanleer}.mako)
+__device__ __forceinline__ double limit(int kind, double a, double b) {
+  switch (kind) {
+    case 1: return fmax(0.0, fmin(fmin(2.0 * a, (2.0 * b + a) * (1.0 / 3.0)), 2.0 * b));
+    case 2: return fmax(0.0, fmin(b, a));
+    case 3: return fmax(0.0, fmin(fmin(2.0 * a, (b + a) * 0.5), 2.0 * b));
+    case 4: return fmax(0.0, fmax(fmin(2.0 * a, b), fmin(a, 2.0 * b)));
+    default: return (a + fabs(a)) * b / (b + fabs(a));
+  }
+}
+
+// recon/muscl.mako:5-13: the limited slope across the face carried to dp's sign and back
+__device__ __forceinline__ double phi(int kind, double dm, double dp) {
+  double p = 0.0;
+  if (dp != 0.0) {
+    const double s = copysign(1.0, dp);
+    p = s * limit(kind, s * dm, fabs(dp));
+  }
+  return p;
+}
+
+__device__ __forceinline__ double left_of(int kind, double vLL, double vL, double vR) {
+  return vL + 0.5 * phi(kind, vL - vLL, vR - vL);
+}
+
+__device__ __forceinline__ double right_of(int kind, double vL, double vR, double vRR) {
+  return vR - 0.5 * phi(kind, vRR - vR, vR - vL);
+}
+
+// the face's sides: recon/constant.mako:3-18 (the two cells as they are) or
+// recon/muscl.mako:15-51 (p, T, Y and velocity limited, each side closed by the eos from p, T, Y);
+// o is the offset of the face's right cell R, d the stride along the axis
+template <int NS, int EOS, int RECON>
+__device__ __forceinline__ void face_sides(const Gas<NS> &g, const View &Q, const View &st, int64_t o, int64_t d,
+                                           int lim, FaceState<NS> &L, FaceState<NS> &R) {
+  if constexpr (RECON == RECON_CONSTANT) {
+    const int64_t oL = o - d;
+    load_state(st, oL, L.s);
+    load_state(st, o, R.s);
+    const double rL = Q.at(oL, 0), rR = Q.at(o, 0);
+    L.u = Q.at(oL, 1) / rL;
+    L.v = Q.at(oL, 2) / rL;
+    L.w = Q.at(oL, 3) / rL;
+    R.u = Q.at(o, 1) / rR;
+    R.v = Q.at(o, 2) / rR;
+    R.w = Q.at(o, 3) / rR;
+#pragma unroll
+    for (int n = 0; n < NS - 1; n++) {
+      L.Y[n] = Q.at(oL, 5 + n) / rL;
+      R.Y[n] = Q.at(o, 5 + n) / rR;
+    }
+  } else {
+    const int64_t oLL = o - 2 * d, oL = o - d, oRR = o + d;
+    const double ri0 = st.at(oLL, 2), ri1 = st.at(oL, 2), ri2 = st.at(o, 2), ri3 = st.at(oRR, 2);
+    const double p0 = st.at(oLL, 0), p1 = st.at(oL, 0), p2 = st.at(o, 0), p3 = st.at(oRR, 0);
+    const double T0 = st.at(oLL, 1), T1 = st.at(oL, 1), T2 = st.at(o, 1), T3 = st.at(oRR, 1);
+    const double pL = left_of(lim, p0, p1, p2), pR = right_of(lim, p1, p2, p3);
+    const double TL = left_of(lim, T0, T1, T2), TR = right_of(lim, T1, T2, T3);
+#pragma unroll
+    for (int n = 0; n < NS - 1; n++) {
+      const double y0 = Q.at(oLL, 5 + n) * ri0, y1 = Q.at(oL, 5 + n) * ri1;
+      const double y2 = Q.at(o, 5 + n) * ri2, y3 = Q.at(oRR, 5 + n) * ri3;
+      L.Y[n] = left_of(lim, y0, y1, y2);
+      R.Y[n] = right_of(lim, y1, y2, y3);
+    }
+    {
+      MixMom<NS, EOS> m;
+      mixmom<NS, EOS>(g, [&](int n) { return L.Y[n]; }, m);
+      frompT<NS, EOS>(m, pL, TL, L.s);
+    }
+    double vel[3][4];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      vel[c][0] = Q.at(oLL, 1 + c) * ri0;
+      vel[c][1] = Q.at(oL, 1 + c) * ri1;
+      vel[c][2] = Q.at(o, 1 + c) * ri2;
+      vel[c][3] = Q.at(oRR, 1 + c) * ri3;
+    }
+    L.u = left_of(lim, vel[0][0], vel[0][1], vel[0][2]);
+    L.v = left_of(lim, vel[1][0], vel[1][1], vel[1][2]);
+    L.w = left_of(lim, vel[2][0], vel[2][1], vel[2][2]);
+    {
+      MixMom<NS, EOS> m;
+      mixmom<NS, EOS>(g, [&](int n) { return R.Y[n]; }, m);
+      frompT<NS, EOS>(m, pR, TR, R.s);
+    }
+    R.u = right_of(lim, vel[0][1], vel[0][2], vel[0][3]);
+    R.v = right_of(lim, vel[1][1], vel[1][2], vel[1][3]);
+    R.w = right_of(lim, vel[2][1], vel[2][2], vel[2][3]);
+  }
+}
+
+// function/kepaec.mako:2-20 (Shima et al. 2021): arithmetic means, the harmonic mean of cv T
+template <int NS>
+__device__ __forceinline__ void kepaec(const FaceState<NS> &L, const FaceState<NS> &R, const double S[3],
+                                       double F[NS + 4]) {
+  const double uf = 0.5 * (R.u + L.u), vf = 0.5 * (R.v + L.v), wf = 0.5 * (R.w + L.w);
+  const double U = S[0] * uf + S[1] * vf + S[2] * wf;
+  const double pf = 0.5 * (R.s.p + L.s.p);
+  const double rho = 0.5 * (R.s.rho + L.s.rho);
+  const double C = rho * U;
+  F[0] = C;
+  F[1] = C * uf + pf * S[0];
+  F[2] = C * vf + pf * S[1];
+  F[3] = C * wf + pf * S[2];
+  const double Kj = C * 0.5 * (R.u * L.u + R.v * L.v + R.w * L.w);
+  const double Pj = 0.5 * (L.s.p * (R.u * S[0] + R.v * S[1] + R.w * S[2]) + R.s.p * (L.u * S[0] + L.v * S[1] + L.w * S[2]));
+  const double thL = L.s.cp / L.s.gamma * L.s.T, thR = R.s.cp / R.s.gamma * R.s.T;
+  const double ef = 2.0 * thL * thR / (thL + thR) + 0.5 * ((L.s.e - thL) + (R.s.e - thR));
+  F[4] = ef * C + Kj + Pj;
+#pragma unroll
+  for (int n = 0; n < NS - 1; n++) F[5 + n] = 0.5 * (R.Y[n] + L.Y[n]) * C;
+}
+
+// function/hllc.mako:2-43: Davis wave speeds, the star jump through drho and dU
+template <int NS>
+__device__ __forceinline__ void hllc(const FaceState<NS> &L, const FaceState<NS> &R, const double S[3],
+                                     double F[NS + 4]) {
+  const double area = sqrt(S[0] * S[0] + S[1] * S[1] + S[2] * S[2]);
+  const double ainv = 1.0 / area;
+  const double nx = S[0] * ainv, ny = S[1] * ainv, nz = S[2] * ainv;
+  const double unL = (L.u * S[0] + L.v * S[1] + L.w * S[2]) * ainv;
+  const double unR = (R.u * S[0] + R.v * S[1] + R.w * S[2]) * ainv;
+  const double SL = unL - L.s.c, SR = unR + R.s.c;
+  const double Sstar = (R.s.p - L.s.p + L.s.rho * unL * (SL - unL) - R.s.rho * unR * (SR - unR)) /
+                       (L.s.rho * (SL - unL) - R.s.rho * (SR - unR));
+  const bool left = SL >= 0.0 || Sstar >= 0.0;
+  const bool star = !(SL >= 0.0) && (Sstar >= 0.0 || SR >= 0.0);
+  const FaceState<NS> &X = left ? L : R;
+  const double U = left ? unL : unR, SX = left ? SL : SR;
+  const double et = X.s.e + 0.5 * (X.u * X.u + X.v * X.v + X.w * X.w);
+  const double mdot = U * X.s.rho * area;
+  F[0] = mdot;
+  F[1] = mdot * X.u + X.s.p * S[0];
+  F[2] = mdot * X.v + X.s.p * S[1];
+  F[3] = mdot * X.w + X.s.p * S[2];
+  F[4] = mdot * et + U * X.s.p * area;
+#pragma unroll
+  for (int n = 0; n < NS - 1; n++) F[5 + n] = mdot * X.Y[n];
+  if (star) {
+    const double rhostar = X.s.rho * ((SX - U) / (SX - Sstar));
+    const double drho = rhostar - X.s.rho, dU = rhostar * (Sstar - U);
+    const double jump = SX * area;
+    F[0] += jump * drho;
+    F[1] += jump * (drho * X.u + dU * nx);
+    F[2] += jump * (drho * X.v + dU * ny);
+    F[3] += jump * (drho * X.w + dU * nz);
+    F[4] += jump * (drho * et + dU * (Sstar + X.s.p / (X.s.rho * (SX - U))));
+#pragma unroll
+    for (int n = 0; n < NS - 1; n++) F[5 + n] += jump * drho * X.Y[n];
+  }
+}
+
+// function/rusanov.mako:2-19: the central flux less the largest Davis wave speed's jump
+template <int NS>
+__device__ __forceinline__ void rusanov(const FaceState<NS> &L, const FaceState<NS> &R, const double S[3],
+                                        double F[NS + 4]) {
+  const double area = sqrt(S[0] * S[0] + S[1] * S[1] + S[2] * S[2]);
+  const double ainv = 1.0 / area;
+  const double unL = (L.u * S[0] + L.v * S[1] + L.w * S[2]) * ainv;
+  const double unR = (R.u * S[0] + R.v * S[1] + R.w * S[2]) * ainv;
+  const double lam = fmax(fabs(unL) + L.s.c, fabs(unR) + R.s.c) * area;
+  const double UL = unL * area, UR = unR * area;
+  const double rL = L.s.rho, rR = R.s.rho;
+  F[0] = 0.5 * (UR * rR + UL * rL - lam * (rR - rL));
+  F[1] = 0.5 * (UR * rR * R.u + R.s.p * S[0] + UL * rL * L.u + L.s.p * S[0] - lam * (rR * R.u - rL * L.u));
+  F[2] = 0.5 * (UR * rR * R.v + R.s.p * S[1] + UL * rL * L.v + L.s.p * S[1] - lam * (rR * R.v - rL * L.v));
+  F[3] = 0.5 * (UR * rR * R.w + R.s.p * S[2] + UL * rL * L.w + L.s.p * S[2] - lam * (rR * R.w - rL * L.w));
+  const double EL = rL * (L.s.e + 0.5 * (L.u * L.u + L.v * L.v + L.w * L.w));
+  const double ER = rR * (R.s.e + 0.5 * (R.u * R.u + R.v * R.v + R.w * R.w));
+  F[4] = 0.5 * (UR * (ER + R.s.p) + UL * (EL + L.s.p) - lam * (ER - EL));
+#pragma unroll
+  for (int n = 0; n < NS - 1; n++) {
+    const double rhoYL = rL * L.Y[n], rhoYR = rR * R.Y[n];
+    F[5 + n] = 0.5 * (rhoYR * UR + rhoYL * UL - lam * (rhoYR - rhoYL));
+  }
+}
+
+template <int NS, int PRIMARY>
+__device__ __forceinline__ void flux_function(const FaceState<NS> &L, const FaceState<NS> &R, const double S[3],
+                                              double F[NS + 4]) {
+  if constexpr (PRIMARY == FLUX_KEPAEC)
+    kepaec<NS>(L, R, S, F);
+  else if constexpr (PRIMARY == FLUX_HLLC)
+    hllc<NS>(L, R, S, F);
+  else
+    rusanov<NS>(L, R, S, F);
+}
+
+}  // namespace pga

@@ -1,0 +1,88 @@
+// The advective face flux of the per-kernel path: one launch per axis over the
+// faces of a box, one face per thread, lanes along the contiguous i index so the
+// stencil reads of every axis are coalesced.
+#include "pga_flux.hpp"
+#include "pga_internal.hpp"
+
+namespace pga {
+
+template <int NS, int EOS, int RECON, int PRIMARY>
+__global__ __launch_bounds__(256) void k_flux(const double *__restrict__ K, int axis, int lim, View Q, View st, View S,
+                                              View F, Box3 b) {
+  int i, j, k;
+  if (!cell_of(b, i, j, k)) return;
+  const Gas<NS> g{K};
+  const int64_t o = Q.off(i, j, k);
+  const int64_t d = axis == 0 ? 1 : axis == 1 ? Q.sj : Q.sk;
+  FaceState<NS> L, R;
+  face_sides<NS, EOS, RECON>(g, Q, st, o, d, lim, L, R);
+  const int64_t of = S.off(i, j, k);
+  const double Sv[3] = {S.at(of, 0), S.at(of, 1), S.at(of, 2)};
+  double Fv[NS + 4];
+  flux_function<NS, PRIMARY>(L, R, Sv, Fv);
+  const int64_t oF = F.off(i, j, k);
+#pragma unroll
+  for (int l = 0; l < NS + 4; l++) F.at(oF, l) = Fv[l];
+}
+
+template <int NS, int EOS>
+static void launch_flux(const pga_case *c, int axis, pga_view Q, pga_view st, pga_view S, pga_view F, Box3 b,
+                        hipStream_t s) {
+  const dim3 grid = grid_of(b), block(TI, TJ);
+  const int lim = c->desc.limiter, recon = c->desc.recon, prim = c->desc.primary;
+#define L(RECON, PRIM)                                                                                            \
+  if (recon == RECON && prim == PRIM)                                                                             \
+    hipLaunchKernelGGL((k_flux<NS, EOS, RECON, PRIM>), grid, block, 0, s, c->K, axis, lim, view(Q), view(st), view(S), \
+                       view(F), b);
+  L(RECON_CONSTANT, FLUX_KEPAEC)
+  L(RECON_CONSTANT, FLUX_HLLC)
+  L(RECON_CONSTANT, FLUX_RUSANOV)
+  L(RECON_MUSCL, FLUX_KEPAEC)
+  L(RECON_MUSCL, FLUX_HLLC)
+  L(RECON_MUSCL, FLUX_RUSANOV)
+#undef L
+}
+
+}  // namespace pga
+
+using namespace pga;
+
+extern "C" int32_t pga_flux(const pga_case *c, int32_t axis, pga_view Q, pga_view state, pga_view S, pga_view F,
+                            pga_box box, void *stream) {
+  if (!c || axis < 0 || axis > 2 || !box_ok(box)) {
+    set_error("pga_flux: null case, bad axis or box");
+    return PGA_EINVAL;
+  }
+  const pga_case_desc &d = c->desc;
+  if (d.recon < 0 || d.recon > 1 || d.primary < 0 || d.primary > 2 || (d.recon == 1 && (d.limiter < 1 || d.limiter > 5))) {
+    set_error("pga_flux: no reconstruction %d / flux function %d / limiter %d", d.recon, d.primary, d.limiter);
+    return PGA_EUNSUP;
+  }
+  if (empty(box)) return PGA_OK;
+  {
+    // a face's stencil: LL, L, R, RR along the axis under MUSCL, L and R otherwise
+    int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    lo[axis] = d.recon == 1 ? 2 : 1;
+    hi[axis] = d.recon == 1 ? 1 : 0;
+    PGA_REQUIRE(covers(Q, box, lo, hi, d.ns + 4) && covers(state, box, lo, hi, 8),
+                "pga_flux: the faces' stencil reaches outside Q or state");
+    PGA_REQUIRE(covers(S, box, 3) && covers(F, box, d.ns + 4), "pga_flux: the faces reach outside S or F");
+  }
+  const Box3 b = box3(box);
+  bool done = false;
+#define X(N)                                                                          \
+  if (d.ns == N) {                                                                    \
+    if (d.eos == EOS_CPG)                                                             \
+      launch_flux<N, EOS_CPG>(c, axis, Q, state, S, F, b, stream_of(stream));         \
+    else                                                                              \
+      launch_flux<N, EOS_TPG>(c, axis, Q, state, S, F, b, stream_of(stream));         \
+    done = true;                                                                      \
+  }
+  PGA_NS_CASES(X)
+#undef X
+  if (!done) {
+    set_error("pga_flux: no kernel for %d species", d.ns);
+    return PGA_EUNSUP;
+  }
+  return launched("pga_flux");
+}

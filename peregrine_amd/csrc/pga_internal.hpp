@@ -1,0 +1,68 @@
+// Host-side internals shared by the library's translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+
+#include "../../include/peregrine_amd.h"
+
+struct pga_case {
+  pga_case_desc desc;
+  const double *K;     // the species constant table (device)
+  const double *chem;  // the mechanism's constant table (device), or null
+};
+
+namespace pga {
+
+void set_error(const char *fmt, ...);
+
+inline hipStream_t stream_of(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+// the status of the launch just made on this thread
+inline int32_t launched(const char *what) {
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    set_error("%s: %s", what, hipGetErrorString(e));
+    return PGA_EHIP;
+  }
+  return PGA_OK;
+}
+
+inline bool box_ok(const pga_box &b) {
+  for (int a = 0; a < 3; a++)
+    if (b.start[a] < 0 || b.extent[a] < 0) return false;
+  return b.extent[2] <= 65535;
+}
+
+inline bool empty(const pga_box &b) { return b.extent[0] == 0 || b.extent[1] == 0 || b.extent[2] == 0; }
+
+// whether the cells [start - lo, start + extent + hi) along each axis lie inside the view,
+// and it holds at least ncomp components
+inline bool covers(const pga_view &v, const pga_box &b, const int lo[3], const int hi[3], int ncomp) {
+  if (!v.data || v.n[3] < ncomp) return false;
+  for (int a = 0; a < 3; a++) {
+    if (b.extent[a] == 0) continue;
+    if (b.start[a] - lo[a] < 0 || b.start[a] + b.extent[a] + hi[a] > v.n[a]) return false;
+  }
+  // the strides must not let one plane overlap the next
+  return v.sj >= v.n[0] && v.sk >= v.sj * v.n[1] && v.sc >= v.sk * v.n[2];
+}
+
+inline bool covers(const pga_view &v, const pga_box &b, int ncomp) {
+  const int z[3] = {0, 0, 0};
+  return covers(v, b, z, z, ncomp);
+}
+
+#define PGA_REQUIRE(cond, ...)      \
+  do {                              \
+    if (!(cond)) {                  \
+      set_error(__VA_ARGS__);       \
+      return PGA_ERANGE;            \
+    }                               \
+  } while (0)
+
+#define PGA_NS_CASES(X) X(1) X(2) X(3) X(9)
+
+}  // namespace pga

@@ -1,0 +1,119 @@
+"""ctypes binding of libpga.so, the HIP library (include/peregrine_amd.h).
+
+Loading needs only the library file and the HIP runtime library; calling an
+entry point that launches needs a GPU. There is no fallback: every compute call
+goes to the HIP kernels, and a failure raises.
+"""
+
+import ctypes as C
+from pathlib import Path
+
+HERE = Path(__file__).resolve().parent
+LIBPATH = HERE / "libpga.so"
+
+_i32 = C.c_int32
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+
+
+class View(C.Structure):
+    _fields_ = [("data", C.c_void_p), ("sj", C.c_int64), ("sk", C.c_int64), ("sc", C.c_int64), ("n", _i32 * 4)]
+
+
+def view(ptr, ni, nj, nk, nc):
+    """The ABI view of a contiguous array with extents (ni, nj, nk, nc), i fastest."""
+    return View(ptr, ni, ni * nj, ni * nj * nk, (_i32 * 4)(ni, nj, nk, nc))
+
+
+class Box(C.Structure):
+    _fields_ = [("start", _i32 * 3), ("extent", _i32 * 3)]
+
+
+class CaseDesc(C.Structure):
+    _fields_ = [
+        ("ns", _i32), ("eos", _i32), ("recon", _i32), ("limiter", _i32), ("primary", _i32), ("mechanism", _i32),
+        ("reserved", _i32 * 10),
+    ]
+
+
+ERRORS = {-1: "invalid argument", -2: "unsupported case", -3: "HIP launch failure", -4: "size out of range"}
+
+# name -> (restype, argtypes); every launching entry point takes the stream last
+SIGNATURES = {
+    "pga_abi_version": (_i32, []),
+    "pga_last_error": (C.c_char_p, []),
+    "pga_const_offset": (_i32, [_i32, C.c_char_p, _ip]),
+    "pga_const_size": (_i32, [_i32]),
+    "pga_mechanism_count": (_i32, []),
+    "pga_mechanism_signature": (C.c_char_p, [_i32]),
+    "pga_mechanism_name": (C.c_char_p, [_i32]),
+    "pga_mechanism_table_size": (_i32, [_i32]),
+    "pga_mechanism_find": (_i32, [C.c_char_p]),
+    "pga_case_create": (_i32, [C.POINTER(CaseDesc), C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p)]),
+    "pga_case_destroy": (_i32, [C.c_void_p]),
+    "pga_fromQ": (_i32, [C.c_void_p, View, View, C.POINTER(Box), _i32, C.c_void_p]),
+    "pga_fromprims": (_i32, [C.c_void_p, View, View, View, Box, C.c_void_p]),
+    "pga_flux": (_i32, [C.c_void_p, _i32, View, View, View, View, Box, C.c_void_p]),
+    "pga_applyflux": (_i32, [_i32, View, View, View, View, View, _i32, Box, C.c_void_p]),
+    "pga_source": (_i32, [C.c_void_p, View, View, View, Box, C.c_void_p]),
+    "pga_production": (_i32, [C.c_void_p, _i32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "pga_combine": (_i32, [_i32, View, _i32, C.c_double, _i32, C.POINTER(View), _dp, C.c_void_p, _i32, Box,
+                           C.c_void_p]),
+    "pga_fill_periodic": (_i32, [View, _i32, _ip, _i32, _i32, _i32, C.c_void_p]),
+    "pga_copy_box": (_i32, [View, _ip, View, _ip, _ip, _i32, C.c_void_p]),
+}
+
+
+class NativeError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    """The loaded library; raises when it is not built."""
+    global _lib
+    if _lib is None:
+        if not LIBPATH.is_file():
+            raise NativeError(f"{LIBPATH} is not built: run peregrine_amd.build (or __graft_entry__.build())")
+        handle = C.CDLL(str(LIBPATH))
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.restype, fn.argtypes = res, args
+        _lib = handle
+    return _lib
+
+
+def check(rc, what):
+    if rc != 0:
+        msg = lib().pga_last_error().decode(errors="replace")
+        raise NativeError(f"{what}: {ERRORS.get(rc, rc)}: {msg}")
+
+
+def box(start, extent):
+    return Box((_i32 * 3)(*start), (_i32 * 3)(*extent))
+
+
+def ints(values):
+    return (_i32 * len(values))(*values)
+
+
+def mechanisms():
+    L = lib()
+    return [
+        (L.pga_mechanism_name(i).decode(), L.pga_mechanism_signature(i).decode(), L.pga_mechanism_table_size(i))
+        for i in range(L.pga_mechanism_count())
+    ]
+
+
+def const_layout(ns):
+    """{section: (offset, count)} of the species constant table, and its size."""
+    L = lib()
+    names = ("MW", "MWinv", "Rsp", "cppolyR", "h298R", "cp0", "gpoly", "h298", "mu0", "kappa0", "lewis")
+    out = {}
+    for name in names:
+        count = _i32(0)
+        off = L.pga_const_offset(ns, name.encode(), C.byref(count))
+        out[name] = (off, count.value)
+    return out, L.pga_const_size(ns)

@@ -1,0 +1,165 @@
+"""Simulators: what a case solves (Euler), its conserved vector and its kernel choices.
+
+Mirrors ``peregrine/simulator/euler/simulator.py:19-161``:
+Q = [rho, rho u, rho v, rho w, rho E, rho Y_0 .. rho Y_{ns-2}], the primitives
+[p, u, v, w, T, Y_0 .. Y_{ns-2}], the last species what the others leave; the
+advective flux is chosen by ``adv-flux`` (recon, limiter, primary); the
+chemistry integration by ``chem-integration.kind``.
+"""
+
+import numpy as np
+
+from peregrine_amd import mechgen, native
+from peregrine_amd.config import ConfigError
+from peregrine_amd.misc import find_subclass
+from peregrine_amd.mixture import get_mixture
+
+EOS = {"cpg": 0, "tpg": 1}
+RECON = {"constant": 0, "muscl": 1}
+LIMITER = {None: 0, "koren": 1, "minmod": 2, "mc": 3, "superbee": 4, "vanleer": 5}
+PRIMARY = {"kepaec": 0, "hllc": 1, "rusanov": 2}
+
+
+class IoVars:
+    def __init__(self, sim):
+        self.sim = sim
+        sp = sim.mixture.spnames
+        self.privars = ["p", "u", "v", "w", "T", *sp[:-1]]
+        self.exported = ["rho", *self.privars, sp[-1]]
+
+    def data(self, blk, names):
+        """Named fields of a block on the host, from Q and the state (as the reference's export)."""
+        Q, state = blk.Q.get(), blk.state.get()
+        rho = Q[..., 0]
+        sp = self.sim.mixture.spnames
+        with np.errstate(divide="ignore", invalid="ignore"):
+            rhoinv = 1.0 / rho
+            values = {"p": state[..., 0], "T": state[..., 1], "rho": rho}
+            for n, name in enumerate("uvw"):
+                values[name] = Q[..., 1 + n] * rhoinv
+            for n, name in enumerate(sp[:-1]):
+                values[name] = Q[..., 5 + n] * rhoinv
+            values[sp[-1]] = np.ones_like(rho) - sum((values[n] for n in sp[:-1]), np.zeros_like(rho))
+        return {name: values[name] for name in names}
+
+
+class BaseSimulator:
+    name = None
+
+    def __init__(self, cfg):
+        self.cfg = cfg
+
+
+class EulerSimulator(BaseSimulator):
+    name = "euler"
+    viscous = False
+
+    def __init__(self, cfg):
+        super().__init__(cfg)
+        self.mixture = get_mixture(cfg)
+        adv = cfg["adv-flux"]
+        if adv["formulation"] != "single" or adv["secondary"] or adv["correction"] or adv["switch"]:
+            raise ConfigError("adv-flux", "this build has the single formulation with no switch or correction")
+        if adv["recon"] not in RECON:
+            raise ConfigError("adv-flux.recon", f"one of {sorted(RECON)}")
+        if adv["primary"] not in PRIMARY:
+            raise ConfigError("adv-flux.primary", f"one of {sorted(PRIMARY)}")
+        if adv["recon"] == "muscl" and adv["limiter"] not in LIMITER or (adv["recon"] == "muscl" and not adv["limiter"]):
+            raise ConfigError("adv-flux.limiter", f"muscl needs one of {sorted(k for k in LIMITER if k)}")
+        if adv["recon"] == "muscl" and adv["primary"] == "kepaec":
+            raise ConfigError("adv-flux", "kepaec is the scheme of the two cells as they are; it takes recon constant")
+        self.recon, self.limiter, self.primary = adv["recon"], adv["limiter"], adv["primary"]
+        kind = cfg["chem-integration"]["kind"]
+        if kind not in (None, "coupled"):
+            raise ConfigError("chem-integration.kind", "this build has coupled chemistry")
+        if kind and self.mixture.chemistry is None:
+            raise ConfigError("chem-integration.kind", "coupled chemistry needs mixture.chemistry finite-rate")
+        self.chemistry = bool(kind)
+        self.iovars = IoVars(self)
+
+    @property
+    def ns(self):
+        return self.mixture.ns
+
+    @property
+    def ne(self):
+        return 4 + self.mixture.ns
+
+    @property
+    def ng(self):
+        # the stencil's reach: MUSCL reads two cells each side of a face
+        return 2 if self.recon == "muscl" else 1
+
+    def ics(self):
+        ic = self.cfg["ics"]
+        Y, names = ic["Y"], self.mixture.spnames
+        unknown = sorted(set(Y) - set(names))
+        if unknown:
+            raise ConfigError("ics.Y", f"names {unknown}, not species of {names}")
+        if sum(Y.values()) > 1.0 + 1e-12:
+            raise ConfigError("ics.Y", "sums to more than one")
+        return [ic[k] for k in ("p", "u", "v", "w", "T")] + [Y.get(n, 0.0) for n in names[:-1]]
+
+    def report(self):
+        return (
+            f"  Simulator: {self.name}\n  Species: {self.mixture.spnames}\n"
+            f"  Mixture: {self.mixture.described()}\n"
+            f"  Advective flux: {self.primary} / {self.recon}{' ' + self.limiter if self.limiter else ''}\n"
+        )
+
+
+def get_simulator(cfg):
+    name = cfg["simulation"]["simulator"]
+    try:
+        return find_subclass(BaseSimulator, name=name)(cfg)
+    except KeyError:
+        raise ConfigError("simulation.simulator", f"{name!r}: this build has euler") from None
+
+
+class NativeCase:
+    """The constant tables of a case on the device and the library's case handle."""
+
+    def __init__(self, sim, backend):
+        import ctypes as C
+
+        mix = sim.mixture
+        ns = mix.ns
+        layout, size = native.const_layout(ns)
+        if size < 0:
+            raise native.NativeError(f"no constant layout for {ns} species")
+        k = mix.constants()
+        table = np.zeros(size)
+        for name, (off, count) in layout.items():
+            if name in k:
+                table[off : off + count] = np.asarray(k[name], float).ravel()[:count]
+        self.consts = backend.tensor(table)
+        self.chem = None
+        mechanism = -1
+        if mix.chemistry is not None:
+            records, scalars = mix.chemistry.records(mix.species)
+            sets = mix.chemistry.thirdbodies(mix.species)
+            sig = mechgen.signature(records, sets, ns)
+            mechanism = native.lib().pga_mechanism_find(sig.encode())
+            if mechanism < 0:
+                have = ", ".join(f"{n} ({s})" for n, s, _ in native.mechanisms())
+                raise native.NativeError(
+                    f"no compiled mechanism has this case's structure {sig}; compiled: {have}. "
+                    "Add the mechanism and its T-range to peregrine_amd/data/mechanisms/compiled.yaml and rebuild."
+                )
+            self.chem = backend.tensor(mechgen.pack(records, sets, scalars, ns))
+        desc = native.CaseDesc()
+        desc.ns, desc.eos = ns, EOS[mix.eos.name]
+        desc.recon, desc.limiter, desc.primary = RECON[sim.recon], LIMITER[sim.limiter], PRIMARY[sim.primary]
+        desc.mechanism = mechanism
+        handle = C.c_void_p()
+        chem_ptr = self.chem.data_ptr() if self.chem is not None else None
+        native.check(native.lib().pga_case_create(C.byref(desc), self.consts.data_ptr(), chem_ptr, C.byref(handle)),
+                     "pga_case_create")
+        self.handle = handle
+        self.desc = desc
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value:
+            native.lib().pga_case_destroy(h)
+            self.handle = None

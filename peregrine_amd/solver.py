@@ -1,0 +1,305 @@
+"""The multiblock solver: blocks, their arrays, and the slots of an RK stage.
+
+Mirrors ``peregrine/multiblock/solver.py:23-391``. A stage is
+  prerhs   halo exchange of Q, the EOS over the halos, the chemistry source into dQ
+  rhs      the advective flux through every face, its divergence into dQ
+  advance  the RK combine into the stage's output register
+  postrhs  the EOS over the interior of the output register
+(``simulator/advec/simulator.py:57-87``, ``integrators/rungekutta.py:33-39``),
+each slot a sequence of launches of the HIP library on one stream.
+"""
+
+import ctypes
+
+import numpy as np
+
+from peregrine_amd import grid, native
+from peregrine_amd.backend import HostScalar
+from peregrine_amd.integrators import get_integrator
+from peregrine_amd.mesher import Interface
+from peregrine_amd.simulator import NativeCase, get_simulator
+
+NSTATE = 8
+
+
+class Block:
+    def __init__(self, nblki, mb):
+        self.nblki = nblki
+        self.mb = mb
+        self.faces = {n: Interface(n) for n in range(1, 7)}
+        self.interior_nodes = None
+
+    def intface(self, nface):
+        return self.faces[nface]
+
+    def set_extents(self, ni, nj, nk):
+        self.ni, self.nj, self.nk = int(ni), int(nj), int(nk)
+
+    @property
+    def extents(self):
+        return (self.ni, self.nj, self.nk)
+
+    @property
+    def ncell(self):
+        return (self.ni - 1, self.nj - 1, self.nk - 1)
+
+    @property
+    def ncells(self):
+        return int(np.prod(self.ncell))
+
+    @property
+    def ng(self):
+        return self.mb.ng
+
+    @property
+    def cellshape(self):
+        return tuple(n + 2 * self.ng for n in self.ncell)
+
+    def faceshape(self, axis):
+        s = list(self.cellshape)
+        s[axis] += 1
+        return tuple(s)
+
+    @property
+    def interior(self):
+        ng = self.ng
+        return np.s_[ng:-ng, ng:-ng, ng:-ng]
+
+    def interior_box(self):
+        return native.box((self.ng,) * 3, self.ncell)
+
+    def face_box(self, axis):
+        ext = list(self.ncell)
+        ext[axis] += 1
+        return native.box((self.ng,) * 3, ext)
+
+    def halo_boxes(self):
+        """Each interface's halo slab, ng deep, over the interior of the other axes."""
+        boxes = []
+        for nface, f in self.faces.items():
+            if f.nbr is None:
+                continue
+            start, ext = [self.ng] * 3, list(self.ncell)
+            start[f.axis] = 0 if f.is_low else self.ng + self.ncell[f.axis]
+            ext[f.axis] = self.ng
+            boxes.append(native.box(start, ext))
+        return boxes
+
+    def alloc(self, backend, sim, nregs):
+        ng, cs = self.ng, self.cellshape
+        ne = sim.ne
+        self.regs = [backend.alloc(cs + (ne,), name=f"Q{r}") for r in range(nregs)]
+        self.state = backend.alloc(cs + (NSTATE,), name="state")
+        self.iF, self.jF, self.kF = (backend.alloc(self.faceshape(a) + (ne,), name=f"{'ijk'[a]}F") for a in range(3))
+        self.iS, self.jS, self.kS = (backend.alloc(self.faceshape(a) + (3,), name=f"{'ijk'[a]}S") for a in range(3))
+        self.detJinv = backend.alloc(cs + (1,), name="detJinv")
+        self.xcc = backend.alloc(cs + (3,), name="xcc")
+        self.q = backend.alloc(cs + (ne,), name="q")
+
+    @property
+    def Q(self):
+        return self.regs[0]
+
+    def make_metrics(self, periodic):
+        ng = self.ng
+        nodes = grid.halo_nodes(self.interior_nodes, ng, periodic, self.period)
+        self.nodes = nodes
+        self.xcc.set(grid.cellcenters(nodes))
+        for a, S in enumerate((self.iS, self.jS, self.kS)):
+            S.set(grid.areavecs(nodes, a))
+        self.detJinv.set(grid.detJinv(nodes, ng)[..., None])
+
+
+class Solver:
+    def __init__(self, cfg, mesh, backend, sim, integ, state=None):
+        self.cfg, self.mesh, self.backend, self.sim, self.integ = cfg, mesh, backend, sim, integ
+        self.nrt = 0
+        self.tme = HostScalar(0.0)
+        self.dt = backend.scalar("dt", cfg["simulation"]["dt"])
+        self.ng = sim.ng
+        self.blocks = []
+        mesh.fill(self)
+        self._check_bcs()
+        for blk in self.blocks:
+            blk.alloc(backend, sim, integ.nregs)
+            periodic = [blk.faces[2 * a + 1].periodic and blk.faces[2 * a + 1].nbr == blk.nblki for a in range(3)]
+            blk.make_metrics(periodic)
+        self.case = NativeCase(sim, backend)
+        self.plugins = self._plugins()
+        self._set_state(state)
+        for p in self.plugins:
+            p.start(self)
+
+    # -- construction -------------------------------------------------------
+    def add_blk(self):
+        blk = Block(len(self.blocks), self)
+        self.blocks.append(blk)
+        return blk
+
+    @property
+    def nblocks(self):
+        return len(self.blocks)
+
+    def _check_bcs(self):
+        for blk in self.blocks:
+            for f in blk.faces.values():
+                if f.nbr is None:
+                    raise ValueError(
+                        f"block {blk.nblki} interface {f.nface} is a boundary ({f.bcname!r}); this build runs "
+                        "periodic and block-to-block interfaces only"
+                    )
+            if min(blk.ncell) < self.ng:
+                raise ValueError(f"block {blk.nblki}: {blk.ncell} cells, thinner than the {self.ng}-deep halo")
+
+    def _plugins(self):
+        from peregrine_amd.plugins import claim
+
+        return claim(self.cfg["plugins"], self)
+
+    @property
+    def regs(self):
+        return tuple(range(self.integ.nregs))
+
+    @property
+    def ncells(self):
+        return sum(b.ncells for b in self.blocks)
+
+    # -- state ---------------------------------------------------------------
+    def _set_state(self, state):
+        values = self.sim.ics()
+        for blk in self.blocks:
+            q = np.empty(blk.cellshape + (self.sim.ne,))
+            q[...] = values
+            blk.q.set(q)
+        self.from_prims()
+
+    def set_prims(self, prims):
+        for blk, values in zip(self.blocks, prims):
+            blk.q.set(values)
+        self.from_prims()
+
+    def from_prims(self):
+        for blk in self.blocks:
+            full = native.box((0, 0, 0), blk.cellshape)
+            self._call("pga_fromprims", self.case.handle, blk.q.view(), blk.Q.view(), blk.state.view(), full)
+        self.postrhs(0)
+
+    # -- launches --------------------------------------------------------------
+    timers = None
+
+    def time_kernels(self, on=True):
+        """Bracket every launch with device events on the launch stream, by entry point."""
+        self.timers = {} if on else None
+
+    def _call(self, name, *args):
+        """Launch one entry point of the library on the backend's stream; raise on failure."""
+        fn = getattr(native.lib(), name)
+        if self.timers is None:
+            native.check(fn(*args, self.backend.stream), name)
+            return
+        torch = self.backend.torch
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        native.check(fn(*args, self.backend.stream), name)
+        e1.record()
+        self.timers.setdefault(name, []).append((e0, e1))
+
+    def kernel_times(self):
+        """{entry point: (launches, mean ms)} of the launches timed so far."""
+        self.synchronize()
+        return {k: (len(v), sum(a.elapsed_time(b) for a, b in v) / len(v)) for k, v in (self.timers or {}).items()}
+
+    # -- the slots -----------------------------------------------------------
+    def exchange(self, reg):
+        """Each interface's halo slab from its partner's interior planes (place, per interface)."""
+        ng = self.ng
+        for blk in self.blocks:
+            for f in blk.faces.values():
+                nbr = self.blocks[f.nbr]
+                a = f.axis
+                ext = list(blk.ncell)
+                ext[a] = ng
+                dst = [ng] * 3
+                src = [ng] * 3
+                if f.is_low:
+                    dst[a] = 0
+                    src[a] = ng + nbr.ncell[a] - ng
+                else:
+                    dst[a] = ng + blk.ncell[a]
+                    src[a] = ng
+                if list(nbr.ncell[m] for m in range(3) if m != a) != list(blk.ncell[m] for m in range(3) if m != a):
+                    raise ValueError(f"blocks {blk.nblki} and {nbr.nblki} do not match across interface {f.nface}")
+                self._call("pga_copy_box", blk.regs[reg].view(), native.ints(dst), nbr.regs[reg].view(),
+                           native.ints(src), native.ints(ext), self.sim.ne)
+
+    def prerhs(self, Q, dQ=None):
+        self.exchange(Q)
+        for blk in self.blocks:
+            boxes = blk.halo_boxes()
+            arr = (native.Box * len(boxes))(*boxes)
+            self._call("pga_fromQ", self.case.handle, blk.regs[Q].view(), blk.state.view(), arr, len(boxes))
+            if dQ is not None and self.sim.chemistry:
+                self._call("pga_source", self.case.handle, blk.regs[Q].view(), blk.state.view(), blk.regs[dQ].view(),
+                           blk.interior_box())
+
+    def rhs(self, *, Q, dQ):
+        self.prerhs(Q, dQ)
+        for blk in self.blocks:
+            Qv, st = blk.regs[Q].view(), blk.state.view()
+            for a, (S, F) in enumerate(((blk.iS, blk.iF), (blk.jS, blk.jF), (blk.kS, blk.kF))):
+                self._call("pga_flux", self.case.handle, a, Qv, st, S.view(), F.view(), blk.face_box(a))
+            self._call("pga_applyflux", self.sim.ne, blk.regs[dQ].view(), blk.iF.view(), blk.jF.view(), blk.kF.view(),
+                       blk.detJinv.view(), int(self.sim.chemistry), blk.interior_box())
+
+    def advance(self, n):
+        Qout = self.integ.stageregs(self.regs)[n].Qout
+        for c in self.integ.stagekerns(n, self.regs):
+            for blk in self.blocks:
+                X = (native.View * max(len(c.X), 1))(*[blk.regs[r].view() for r in c.X])
+                w = (ctypes.c_double * max(len(c.w), 1))(*c.w)
+                self._call("pga_combine", self.sim.ne, blk.regs[c.out].view(), int(c.own), c.wout, len(c.X), X, w,
+                           self.dt.ptr, int(c.step), blk.interior_box())
+        self.postrhs(Qout, prerhs=False)
+
+    def postrhs(self, Q=0, prerhs=True):
+        for blk in self.blocks:
+            b = blk.interior_box()
+            self._call("pga_fromQ", self.case.handle, blk.regs[Q].view(), blk.state.view(), (native.Box * 1)(b), 1)
+        if prerhs:
+            self.prerhs(Q)
+
+    def stage(self, n, dt=None):
+        regs = self.integ.stageregs(self.regs)[n]
+        self.rhs(Q=regs.Q, dQ=regs.dQ)
+        self.advance(n)
+
+    def step(self, dt):
+        start = self.tme.get()
+        self.dt.set(dt)
+        times = self.integ.stagetimes(start, dt)
+        for n in range(self.integ.nstages):
+            self.stage(n)
+            self.tme.set(times[n])
+        self.nrt += 1
+        self.tme.set(start + dt)
+
+    def run(self):
+        try:
+            for _ in range(self.cfg["simulation"]["niter"]):
+                self.step(self.integ.controller.stepsize(self))
+                for p in self.plugins:
+                    if p.due(self):
+                        p(self)
+        finally:
+            for p in self.plugins:
+                p.finalize(self)
+
+    def synchronize(self):
+        self.backend.synchronize()
+
+    def __repr__(self):
+        return f"  Blocks: {self.nblocks}\n" + self.sim.report() + f"  Time Integrator: {self.integ.name}\n"
+
+
+def get_solver(cfg, mesh, backend, state=None, cls=Solver):
+    return cls(cfg, mesh, backend, get_simulator(cfg), get_integrator(cfg), state)

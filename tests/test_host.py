@@ -1,0 +1,167 @@
+"""Host-side logic: config, the mixture constant pipeline, the mechanism generator, the C ABI's exports.
+
+None of these launch anything; the library is loaded (it needs only the HIP runtime library)
+and queried for its layout and compiled mechanisms.
+"""
+
+import ctypes
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import support as sp
+from peregrine_amd import build, mechgen, native
+from peregrine_amd.config import Config
+from peregrine_amd.mixture import fits
+from peregrine_amd.mixture.constants import Ru, Tref
+from peregrine_amd.mixture.fits import Nasa7
+from peregrine_amd.simulator import get_simulator
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def test_config_refuses_an_unknown_key():
+    c = Config()
+    with pytest.raises(KeyError):
+        c["mixture"]["specis"] = "x"
+    with pytest.raises(KeyError):
+        c["nonsense"] = {}
+    c["mixture"]["species"] = "h2o2.yaml"
+
+
+def test_mechanism_parse():
+    sim = get_simulator(sp.config("h2o2", chem=True))
+    rx = sim.mixture.reactions
+    assert len(rx) == 28
+    kinds = [r["type"] for r in rx]
+    assert kinds.count("elementary") == 22 and kinds.count("three-body") == 5 and kinds.count("troe") == 1
+    # O + H2 <=> H + OH: A = 3.87e4 cm^3/mol/s = 38.7 m^3/kmol/s, Ea = 6260 cal/mol
+    r = next(r for r in rx if r["equation"] == "O + H2 <=> H + OH")
+    A, b, EaR = r["rate"]
+    assert abs(A - 38.7) < 1e-12 and b == 2.7 and abs(EaR - 6260 * 4184.0 / Ru) < 1e-9
+    # a three-body rate carries one more concentration: cm^6/mol^2/s -> m^6/kmol^2/s
+    r = next(r for r in rx if r["equation"] == "2 O + M <=> O2 + M")
+    assert abs(r["rate"][0] - 1.2e17 * 1e-6) < 1e-3
+    assert r["efficiencies"] == {"H2": 2.4, "H2O": 15.4}
+    troe = next(r for r in rx if r["type"] == "troe")
+    assert troe["troe"] == [0.7346, 94.0, 1756.0, 5182.0]
+
+
+@pytest.mark.parametrize("gas", ["air-tpg", "h2o2"])
+def test_the_refits_follow_nasa7(gas):
+    sim = get_simulator(sp.config(gas))
+    mix = sim.mixture
+    k = mix.constants()
+    T = np.linspace(300.0, 3500.0, 97)
+    u = np.log(T)
+    for n, s in enumerate(mix.species.values()):
+        n7 = s["NASA7"]
+        # at the fit's own samples: within the refit tolerance, or the error of the highest degree allowed
+        Ts, cps = mix.eos.caloric._sample(s)
+        assert np.abs(np.polyval(np.flip(s["cppoly"]), np.log(Ts)) / cps - 1).max() <= max(1e-3, s["cpfiterr"]) * (1 + 1e-9)
+        fit = np.polyval(np.flip(s["cppoly"]), u)
+        # h's polynomial is cp's e^u integral: h(T2) - h(T1) = integral of cp dT, here by quadrature
+        hR = T * np.polyval(np.flip(s["hpoly"]), u) + s["h298"]
+        dh = np.diff(hR)
+        quad = 0.5 * (fit[1:] + fit[:-1]) * np.diff(T)
+        assert np.abs(dh - quad).max() < 1e-3 * np.abs(quad).max()
+        # the reference enthalpy lands on NASA-7's at 298.15 K
+        hRref = Tref * np.polyval(np.flip(s["hpoly"]), np.log(Tref)) + s["h298"]
+        assert abs(hRref - Nasa7.hRT(n7, Tref) * Tref) < 1e-9 * max(1.0, abs(hRref))
+        # g/RT from the fits against NASA-7's own, within the cp fit's error integrated
+        g = mix.eos.gibbs(mix.species, T)[n]
+        gn = np.array([Nasa7.hRT(n7, t) - Nasa7.sR(n7, t) for t in T])
+        assert np.abs(g - gn).max() < 0.05
+    assert k["cppolyR"].shape == (mix.ns, 7) and k["gpoly"].shape == (mix.ns, 8)
+
+
+def test_expintegral_is_the_exponential_integral():
+    P = np.array([1.5, -0.3, 0.07, 0.002])
+    Q = fits.expintegral(P)
+    # d/du (e^u Q) = e^u (Q + Q') = e^u P
+    Qd = np.flip(np.polyder(np.flip(Q)))
+    assert np.allclose(Q[: len(Qd)] + Qd, P[: len(Qd)]) and np.isclose(Q[-1], P[-1])
+
+
+def test_the_library_exports_every_declared_symbol():
+    header = (ROOT / "include" / "peregrine_amd.h").read_text()
+    names = re.findall(r"^\s*(?:int32_t|const char \*)\s*(pga_\w+)\(", header, re.M)
+    assert len(names) >= 20
+    lib = ctypes.CDLL(str(native.LIBPATH))
+    for name in names:
+        assert hasattr(lib, name), name
+    assert native.lib().pga_abi_version() == 1
+
+
+def test_the_constant_layout():
+    layout, size = native.const_layout(9)
+    assert layout["MW"] == (0, 9) and layout["cppolyR"][1] == 63 and layout["gpoly"][1] == 72
+    spans = sorted(layout.values())
+    for (o1, c1), (o2, _) in zip(spans, spans[1:]):
+        assert o1 + c1 == o2
+    assert spans[-1][0] + spans[-1][1] == size
+    assert native.lib().pga_const_offset(9, b"nonsense", None) == -1
+
+
+def test_the_compiled_mechanism_is_the_case_mechanism():
+    sim = get_simulator(sp.config("h2o2", chem=True))
+    mix = sim.mixture
+    records, scalars = mix.chemistry.records(mix.species)
+    sets = mix.chemistry.thirdbodies(mix.species)
+    sig = mechgen.signature(records, sets, mix.ns)
+    (name, libsig, size), = native.mechanisms()
+    assert name == "h2o2" and libsig == sig
+    assert size == len(mechgen.pack(records, sets, scalars, mix.ns))
+    # the generated header in the tree is what the generator makes now
+    text = (build.GEN / "mech_h2o2.hpp").read_text()
+    assert text == mechgen.render("h2o2", records, sets, mix.ns)
+
+
+def test_a_case_without_a_compiled_mechanism_is_refused_by_name():
+    sim = get_simulator(sp.config("h2o2", chem=True))
+    mix = sim.mixture
+    records, _ = mix.chemistry.records(mix.species)
+    sets = mix.chemistry.thirdbodies(mix.species)
+    records[0].b = 0.0 if records[0].b else 1.0
+    sig = mechgen.signature(records, sets, mix.ns)
+    assert native.lib().pga_mechanism_find(sig.encode()) == -1
+
+
+def test_case_create_checks_its_arguments():
+    lib = native.lib()
+    desc = native.CaseDesc()
+    desc.ns, desc.eos, desc.mechanism = 0, 1, -1
+    h = ctypes.c_void_p()
+    # no device memory is touched: the pointers are only recorded
+    assert lib.pga_case_create(ctypes.byref(desc), 8, None, ctypes.byref(h)) == -1
+    desc.ns, desc.mechanism = 2, 0
+    assert lib.pga_case_create(ctypes.byref(desc), 8, 8, ctypes.byref(h)) == -1
+    desc.ns, desc.mechanism = 9, 0
+    assert lib.pga_case_create(ctypes.byref(desc), 8, 8, ctypes.byref(h)) == 0
+    lib.pga_case_destroy(h)
+    assert b"species" in lib.pga_last_error() or lib.pga_last_error() is not None
+
+
+def test_integrator_weights_give_the_amplification_polynomials():
+    from peregrine_amd.integrators import RK1, RK2, RK3, RK4, RK34, FixedController
+
+    coeffs = {RK1: [1, 1], RK2: [1, 1, 1 / 2], RK3: [1, 1, 1 / 2, 1 / 6], RK34: [1, 1, 1 / 2, 1 / 6, 1 / 48],
+              RK4: [1, 1, 1 / 2, 1 / 6, 1 / 24]}
+    for cls, want in coeffs.items():
+        integ = cls(FixedController({"simulation": {"dt": 1.0}}))
+        regs = tuple(range(integ.nregs))
+        for z in (0.3, -0.7, 1.1):
+            v = np.zeros(integ.nregs)
+            v[0] = 1.0
+            # dQ = z Q at unit dt, so the step's dt factor is one
+            for n in range(integ.nstages):
+                Q = integ.stageregs(regs)[n].Q
+                v[regs[-1]] = z * v[Q]
+                for c in integ.stagekerns(n, regs):
+                    acc = c.wout * v[c.out] if c.own else 0.0
+                    for r, w in zip(c.X, c.w):
+                        acc += w * v[r]
+                    v[c.out] = acc
+            assert abs(v[0] - np.polyval(np.flip(want), z)) < 1e-14
