@@ -64,6 +64,8 @@ def _load():
     lib.pgo_tlog.argtypes = [C.c_double]
     lib.pgo_texp.restype = C.c_double
     lib.pgo_texp.argtypes = [C.c_double]
+    lib.pgo_elog.restype = C.c_double
+    lib.pgo_elog.argtypes = [C.c_double]
     lib.pgo_threads.restype = C.c_int
     P = C.POINTER
     lib.pgo_fromQ.argtypes = [P(Case), Arr, Arr, Box]
@@ -83,6 +85,7 @@ def _load():
 lib = _load()
 tlog = lib.pgo_tlog
 texp = lib.pgo_texp
+elog = lib.pgo_elog
 
 EOS = {"cpg": 0, "tpg": 1}
 RECON = {"constant": 0, "muscl": 1}

@@ -10,6 +10,9 @@
  */
 #include "pgo.h"
 
+#define PGA_ELOG_TABLE static const
+#include "pga_elog_tables.h"
+
 #include <float.h>
 #include <math.h>
 #include <string.h>
@@ -98,6 +101,44 @@ double pgo_texp(double x) {
 }
 
 /* ------------------------------------------------------------------------
+ * ln T of the equation of state. The reference takes it from the platform's log
+ * (caloric/tdepcp/kernels/caloric.mako:5); libm logs differ in the last bit between
+ * platforms, and the degree-6 ln T polynomials turn a last-bit change into 1e-11 of h.
+ * This build fixes the method on every platform: x = m 2^e, m in [1, 2); r_k a
+ * 9-bit reciprocal of m's bin centre (top 7 bits); ln x = e ln 2 - ln r_k + ln(1 + r),
+ * r = m r_k - 1 by one fma, ln(1 + r) by its series to r^7; -ln r_k and ln 2 in two
+ * parts. Tables: pga_elog_tables.h, written once by the build.
+ * ---------------------------------------------------------------------- */
+double pgo_elog(double x) {
+  if (!(x >= DBL_MIN) || x > DBL_MAX) return log(x);
+  if (fabs(x - 1.0) < 0.03125) {
+    /* near 1 the table's terms would cancel: ln(1 + r) by its series, r = x - 1 exact */
+    const double r = x - 1.0;
+    double q = -1.0 / 14.0;
+    for (int n = 13; n >= 2; n--) q = q * r + ((n & 1) ? 1.0 : -1.0) / n;
+    return r + (r * r) * q;
+  }
+  int64_t bits;
+  memcpy(&bits, &x, sizeof bits);
+  const int e = (int)(bits >> 52) - 1023;
+  const int k = (int)((bits >> 45) & 127);
+  int64_t mb = (bits & 0x000fffffffffffffLL) | 0x3ff0000000000000LL;
+  double m;
+  memcpy(&m, &mb, sizeof m);
+  const double r = fma(m, ELOG_RCP[k], -1.0);
+  double q = 1.0 / 7.0;
+  q = q * r - 1.0 / 6.0;
+  q = q * r + 0.2;
+  q = q * r - 0.25;
+  q = q * r + 1.0 / 3.0;
+  q = q * r - 0.5;
+  const double p = (r * r) * q;
+  const double hi = e * PGA_ELOG_LN2HI + ELOG_LHI[k];
+  const double lo = e * PGA_ELOG_LN2LO + ELOG_LLO[k];
+  return hi + (r + (p + lo));
+}
+
+/* ------------------------------------------------------------------------
  * the equation of state
  * ---------------------------------------------------------------------- */
 typedef struct {
@@ -132,7 +173,7 @@ static void mix(const pgo_case *c, const double *Y, mixmom *m) {
 
 /* caloric/tdepcp/kernels/caloric.mako:4-15 */
 static void ideal_cph(const pgo_case *c, const mixmom *m, double T, double *cp, double *h) {
-  const double u = log(T);
+  const double u = pgo_elog(T);
   double cpv = 0.0, hv = 0.0, hc = 0.0;
   for (int t = c->nt - 1; t >= 0; t--) {
     cpv = cpv * u + m->cpcoeff[t];

@@ -45,6 +45,7 @@ typedef struct {
 } pgo_case;
 
 double pgo_tlog(double x);
+double pgo_elog(double x);
 double pgo_texp(double x);
 
 void pgo_fromQ(const pgo_case *c, pgo_arr Q, pgo_arr state, pgo_box r);

@@ -21,6 +21,7 @@ __global__ __launch_bounds__(256) void k_fromQ(const double *__restrict__ K, Vie
 // euler/kernels/state/fromprims.mako:4-13 with solution.mako:2-23
 template <int NS, int EOS>
 __global__ __launch_bounds__(256) void k_fromprims(const double *__restrict__ K, View P, View Q, View st, Box3 b) {
+#pragma clang fp contract(off)
   int i, j, k;
   if (!cell_of(b, i, j, k)) return;
   const Gas<NS> g{K};
@@ -46,6 +47,7 @@ __global__ __launch_bounds__(256) void k_fromprims(const double *__restrict__ K,
 // advec/kernels/applyflux.mako:2-11, the sum in the reference's order
 __global__ __launch_bounds__(256) void k_applyflux(int ne, View dQ, View iF, View jF, View kF, View J, int append,
                                                    Box3 b) {
+#pragma clang fp contract(off)
   int i, j, k;
   if (!cell_of(b, i, j, k)) return;
   const int64_t oi = iF.off(i, j, k), oj = jF.off(i, j, k), ok = kF.off(i, j, k), oq = dQ.off(i, j, k);
@@ -68,6 +70,7 @@ struct CombineArgs {
 // integrators/kernels/combine.mako:1-13
 __global__ __launch_bounds__(256) void k_combine(int ne, View out, int own, double wout, int nterms, CombineArgs a,
                                                  const double *__restrict__ dt, int step, Box3 b) {
+#pragma clang fp contract(off)
   int i, j, k;
   if (!cell_of(b, i, j, k)) return;
   const double h = step ? *dt : 1.0;

@@ -7,6 +7,8 @@
 
 #include "../../include/peregrine_amd.h"
 #include "generated/pga_tables.hpp"
+#define PGA_ELOG_TABLE static __constant__ const
+#include "generated/pga_elog_tables.h"
 
 namespace pga {
 
@@ -98,6 +100,7 @@ __device__ __forceinline__ bool cell_of(const Box3 &b, int &i, int &j, int &k) {
 // methods, backend/base/mathfns.py:1-22 and kernels/mathfns.mako:6-54)
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ double tlog(double x) {
+#pragma clang fp contract(off)
   long long bits = __double_as_longlong(x);
   const int expo = (int)((bits >> 52) & 0x7ff) - 1022;
   const int bin = (int)((bits >> (52 - LOGBITS)) & (LOGBINS - 1));
@@ -113,6 +116,7 @@ __device__ __forceinline__ double tlog(double x) {
 }
 
 __device__ __forceinline__ double texp(double x) {
+#pragma clang fp contract(off)
   const double t = x * TAB_EXPK + 6755399441055744.0;
   long long bits = __double_as_longlong(t);
   const int kexp = (int)bits, nexp = kexp >> EXPBITS;
@@ -128,6 +132,43 @@ __device__ __forceinline__ double texp(double x) {
   bits = __double_as_longlong(m);
   bits += (long long)(nexp > -1022 ? nexp : -1022) << 52;
   return __longlong_as_double(bits);
+}
+
+// ---------------------------------------------------------------------------
+// elog: the EOS's ln T. ln x = e ln 2 - ln r_k + ln(1 + (m r_k - 1)) for x = m 2^e, m in [1, 2),
+// r_k a 9-bit reciprocal of the centre of m's bin by its top 7 bits; m r_k - 1 by one fused
+// multiply-add, ln(1 + r) by its series to r^7, -ln r_k and ln 2 in two parts. IEEE operations
+// and explicit fma only, so every platform gives the same bits (within an ulp of ln x): the ln T
+// polynomials of the tpg EOS turn a last-bit change of ln T into 1e-11 of h.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double elog(double x) {
+#pragma clang fp contract(off)
+  if (!(x >= 2.2250738585072014e-308) || x > 1.7976931348623157e308) {
+    // zero, negative, NaN, subnormal or infinite: the library's answer (never reached by a gas state)
+    return log(x);
+  }
+  if (fabs(x - 1.0) < 0.03125) {
+    // near 1 the table's terms would cancel: ln(1 + r) by its series, r = x - 1 exact
+    const double r = x - 1.0;
+    double q = -1.0 / 14.0;
+    for (int n = 13; n >= 2; n--) q = q * r + ((n & 1) ? 1.0 : -1.0) / n;
+    return r + (r * r) * q;
+  }
+  const long long bits = __double_as_longlong(x);
+  const int e = (int)(bits >> 52) - 1023;
+  const int k = (int)((bits >> 45) & 127);
+  const double m = __longlong_as_double((bits & 0x000fffffffffffffLL) | 0x3ff0000000000000LL);
+  const double r = fma(m, ELOG_RCP[k], -1.0);
+  double q = 1.0 / 7.0;
+  q = q * r - 1.0 / 6.0;
+  q = q * r + 0.2;
+  q = q * r - 0.25;
+  q = q * r + 1.0 / 3.0;
+  q = q * r - 0.5;
+  const double p = (r * r) * q;
+  const double hi = e * PGA_ELOG_LN2HI + ELOG_LHI[k];
+  const double lo = e * PGA_ELOG_LN2LO + ELOG_LLO[k];
+  return hi + (r + (p + lo));
 }
 
 }  // namespace pga

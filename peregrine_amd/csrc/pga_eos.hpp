@@ -28,8 +28,13 @@ struct MixMom<NS, EOS_CPG> {
 };
 
 // eos/kernels/mixing.mako:3-30: Y(n) gives the carried ns - 1; the last is what they leave
+// The ln T polynomials of a tpg gas carry coefficients of 1e6 whose terms cancel to cp ~ 1e3:
+// one rounding more or less per step moves h by 1e-11 of itself. The EOS is therefore evaluated
+// with one rounding per written operation (no contracted multiply-adds), the order the
+// reference's host build evaluates it in.
 template <int NS, int EOS, class YF>
 __device__ __forceinline__ void mixmom(const Gas<NS> &g, const YF &Y, MixMom<NS, EOS> &m) {
+#pragma clang fp contract(off)
   m.R = 0.0;
   m.h298 = 0.0;
   if constexpr (EOS == EOS_TPG) {
@@ -63,7 +68,8 @@ __device__ __forceinline__ void mixmom(const Gas<NS> &g, const YF &Y, MixMom<NS,
 // built in the same descent as hcoeff[m] = cpcoeff[m] - (m + 1) hcoeff[m + 1]
 template <int NS>
 __device__ __forceinline__ void ideal_cph(const MixMom<NS, EOS_TPG> &m, double T, double &cp, double &h) {
-  const double u = log(T);
+#pragma clang fp contract(off)
+  const double u = elog(T);
   double cpv = 0.0, hv = 0.0, hc = 0.0;
 #pragma unroll
   for (int t = NT - 1; t >= 0; t--) {
@@ -81,6 +87,7 @@ __device__ __forceinline__ void ideal_cph(const MixMom<NS, EOS_TPG> &m, double T
 template <int NS, int EOS>
 __device__ __forceinline__ void fromrhoE(const MixMom<NS, EOS> &m, double rho, double e, double Tguess,
                                          EosState &s) {
+#pragma clang fp contract(off)
   if constexpr (EOS == EOS_CPG) {
     const double cv = m.cp - m.R;
     s.rho = rho;
@@ -114,6 +121,7 @@ __device__ __forceinline__ void fromrhoE(const MixMom<NS, EOS> &m, double rho, d
 // (p, T) -> state: cpg eos.mako:17-27, tpg eos.mako:22-33
 template <int NS, int EOS>
 __device__ __forceinline__ void frompT(const MixMom<NS, EOS> &m, double p, double T, EosState &s) {
+#pragma clang fp contract(off)
   s.p = p;
   s.T = T;
   s.rho = p / (m.R * T);
@@ -158,6 +166,7 @@ __device__ __forceinline__ void load_state(const View &st, int64_t o, EosState &
 // euler/kernels/state/fromQ.mako:5-19 for the cell at o: Q -> state, T from state as the start
 template <int NS, int EOS>
 __device__ __forceinline__ void cell_fromQ(const Gas<NS> &g, const View &Q, const View &st, int64_t o) {
+#pragma clang fp contract(off)
   const double rho = Q.at(o, 0), rhoinv = 1.0 / rho;
   const double q1 = Q.at(o, 1), q2 = Q.at(o, 2), q3 = Q.at(o, 3);
   const double e = (Q.at(o, 4) - 0.5 * (q1 * q1 + q2 * q2 + q3 * q3) * rhoinv) * rhoinv;

@@ -170,3 +170,32 @@ def close(got, want, within, floor=0.0):
     scale = np.maximum(scale, floor)
     err = np.abs(got - want).reshape(want.shape[0], -1).max(axis=1)
     return bool((err <= within * scale).all()), err / np.where(scale > 0, scale, 1.0)
+
+
+def halo_eos(oc, hb, ne):
+    """The oracle's halo exchange of Q (self-periodic) and its EOS over the halo slabs."""
+    from oracle import oracle as O
+
+    ng = hb.ng
+    for a in range(3):
+        O.lib.pgo_fill_periodic(O.arr(hb.regs[0]), ne, O.ints(hb.ncell), ng, ng, a)
+    for a in range(3):
+        for lo in (True, False):
+            start, ext = [ng] * 3, list(hb.ncell)
+            start[a] = 0 if lo else ng + hb.ncell[a]
+            ext[a] = ng
+            O.lib.pgo_fromQ(oc.ref, O.arr(hb.regs[0]), O.arr(hb.state), O.box(start, ext))
+
+
+def reacting_flow(x, n, L):
+    """Hot spots in H2-air with radicals, plus a swirling velocity and a pressure wave, so every
+    conserved row, momentum included, carries more than rounding."""
+    q = sp.h2_prims(x, (n,) * 3, lengths=L)
+    q[..., 6] = 1e-4
+    k = 2 * np.pi / L[0]
+    X, Y, Z = x[..., 0] * k, x[..., 1] * k, x[..., 2] * k
+    q[..., 0] *= 1.0 + 0.05 * np.sin(X) * np.cos(Z)
+    q[..., 1] = 40.0 * np.sin(X) * np.cos(Y)
+    q[..., 2] = -40.0 * np.cos(X) * np.sin(Y)
+    q[..., 3] = 20.0 * np.sin(Z)
+    return q

@@ -41,7 +41,10 @@ class Dev:
     @staticmethod
     def view(t):
         nc, nk, nj, ni = t.shape
-        return native.view(t.data_ptr(), ni, nj, nk, nc)
+        v = native.view(t.data_ptr(), ni, nj, nk, nc)
+        # the view holds its tensor: a temporary stays allocated until the launch it is passed to
+        v._keep = t
+        return v
 
     def get(self, t):
         self.torch.cuda.synchronize()
@@ -106,9 +109,10 @@ def test_fromprims_and_fromQ(dev, gas):
         O.lib.pgo_fromQ(oc.ref, O.arr(hb.regs[0]), O.arr(st), O.box(*b))
     arr = (native.Box * 2)(*[native.box(*b) for b in boxes])
     native.check(L.pga_fromQ(nc.handle, dev.view(dev.put(hb.regs[0])), dev.view(dS), arr, 2, None), "pga_fromQ")
-    # the tpg Newton lands within its rounding floor of the root, 1e-11 of T for these ln T fits
-    tol = 1e-13 if gas in ("air-cpg", "binary") else 1e-10
-    assert _rowerr(dev.get(dS), st) < tol
+    # the same Newton from the same guesses: ln T is taken by one method on both sides (elog)
+    err = _rowerr(dev.get(dS), st)
+    print(f"fromQ {gas}: {err:.3e}")
+    assert err < 1e-13
 
 
 FLUXES = [("kepaec", "constant", None), ("hllc", "constant", None), ("rusanov", "constant", None),
@@ -135,8 +139,9 @@ def test_face_flux_and_divergence(dev, gas, primary, recon, limiter):
         F = dev.put(np.zeros_like(hb.F[a]))
         native.check(L.pga_flux(nc.handle, a, dev.view(dQv), dev.view(dSt), dev.view(dev.put(hb.S[a])), dev.view(F),
                                 native.box((ng,) * 3, ext), None), "pga_flux")
-        # measured below 1e-14 of each row's largest face flux
-        assert _rowerr(dev.get(F), hb.F[a]) < 1e-12
+        err = _rowerr(dev.get(F), hb.F[a])
+        print(f"flux {gas} {primary} {recon} {limiter} axis {a}: {err:.3e}")
+        assert err < 1e-12
         dF.append(F)
     # the divergence on identical face fluxes is bitwise: one order of sums, one product
     for append in (0, 1):
@@ -172,7 +177,9 @@ def test_production_at_random_states(dev):
     # per point, against that point's largest rate: the forward and reverse rates round apart
     # at 1e-14 and near equilibrium cancel
     scale = np.abs(want).max(axis=0)
-    assert (np.abs(got - want).max(axis=0) <= 1e-12 * scale).all()
+    err = (np.abs(got - want).max(axis=0) / scale).max()
+    print(f"production at {n} states: {err:.3e} of each point's largest rate")
+    assert err <= 1e-12
 
 
 def test_source_on_a_hot_block(dev):
@@ -190,7 +197,9 @@ def test_source_on_a_hot_block(dev):
                                          dev.view(got), native.box((2, 2, 2), hb.ncell), None), "pga_source")
     g = dev.get(got)
     assert np.array_equal(g[:5], want[:5])
-    assert _rowerr(g[5:], want[5:]) < 1e-12
+    err = _rowerr(g[5:], want[5:])
+    print(f"source on a hot block: {err:.3e}")
+    assert err < 1e-12
 
 
 def test_combine(dev):
@@ -212,8 +221,8 @@ def test_combine(dev):
         Xv = (native.View * 3)(*[dev.view(x) for x in dX])
         native.check(L.pga_combine(6, dev.view(got), own, 1.0 / 3.0, nterms, Xv, (ctypes.c_double * 3)(*w),
                                    dt.data_ptr(), step, native.box((1, 1, 1), (68, 7, 5)), None), "pga_combine")
-        # a contracted multiply-add rounds once where the oracle rounds twice
-        assert np.abs(dev.get(got) - want).max() <= 4e-16 * np.abs(want).max()
+        # one rounding per operation in the same order: the same bits
+        assert np.array_equal(dev.get(got), want)
 
 
 def test_halo_copies(dev):
@@ -259,6 +268,10 @@ def _run_pair(gas, recon, primary, limiter, chem, integrator, ncell, steps, dt, 
     mb.set_prims([q])
     full = O.box((0, 0, 0), hb.cshape)
     O.lib.pgo_fromprims(oc.ref, O.arr(sp.to_dev(q)), O.arr(hb.regs[0]), O.arr(hb.state), full)
+    # the solver's set_prims ends as the reference's from_prims does (multiblock/solver.py:298-305):
+    # the EOS over the interior, then the halo exchange and the EOS over the halos
+    O.lib.pgo_fromQ(oc.ref, O.arr(hb.regs[0]), O.arr(hb.state), O.box((hb.ng,) * 3, hb.ncell))
+    sp.halo_eos(oc, hb, sim.ne)
     b = sp.oracle_block(hb, integrator, chem)
     for _ in range(steps):
         mb.step(dt)
@@ -266,17 +279,46 @@ def _run_pair(gas, recon, primary, limiter, chem, integrator, ncell, steps, dt, 
     return mb, hb
 
 
+def test_reacting_rhs_matches(dev):
+    """One right-hand side of the reacting case (prerhs + rhs: halo exchange, halo EOS, source,
+    three flux sweeps, divergence) against the oracle's: 1e-12 per conserved row."""
+    import peregrine_amd as pg
+
+    n, L = 24, (0.01,) * 3
+    cfg = sp.config("h2o2", recon="muscl", primary="hllc", limiter="koren", chem=True, dt=1e-8)
+    mb = pg.get_solver(cfg, pg.mesher.CubeMesher(blkdims=[n + 1] * 3, lengths=L, periodic=[True] * 3),
+                       pg.get_backend("mi355x", cfg))
+    sim = mb.sim
+    oc = O.OracleCase(sim.mixture, "muscl", "koren", "hllc")
+    hb = sp.HostBlock((n,) * 3, sim.ng, sim.ne, 3, lengths=L)
+    q = sp.reacting_flow(hb.xcc, n, L)
+    mb.set_prims([q])
+    O.lib.pgo_fromprims(oc.ref, O.arr(sp.to_dev(q)), O.arr(hb.regs[0]), O.arr(hb.state), O.box((0, 0, 0), hb.cshape))
+    O.lib.pgo_fromQ(oc.ref, O.arr(hb.regs[0]), O.arr(hb.state), O.box((hb.ng,) * 3, hb.ncell))
+    sp.halo_eos(oc, hb, sim.ne)
+    regs = mb.integ.stageregs(mb.regs)[0]
+    mb.rhs(Q=regs.Q, dQ=regs.dQ)
+    sp.halo_eos(oc, hb, sim.ne)
+    ng = hb.ng
+    dQ = hb.regs[2]
+    O.lib.pgo_source(oc.ref, O.arr(hb.regs[0]), O.arr(hb.state), O.arr(dQ), O.box((ng,) * 3, hb.ncell))
+    for a in range(3):
+        ext = list(hb.ncell)
+        ext[a] += 1
+        O.lib.pgo_flux(oc.ref, a, O.arr(hb.regs[0]), O.arr(hb.state), O.arr(hb.S[a]), O.arr(hb.F[a]), O.box((ng,) * 3, ext))
+    O.lib.pgo_applyflux(sim.ne, O.arr(dQ), *(O.arr(f) for f in hb.F), O.arr(hb.detJinv), 1, O.box((ng,) * 3, hb.ncell))
+    got = mb.blocks[0].regs[regs.dQ].get()[ng:-ng, ng:-ng, ng:-ng].transpose(3, 2, 1, 0)
+    err = _rowerr(got, dQ[hb.interior()])
+    print(f"reacting rhs: max row-normalised difference {err:.3e}")
+    assert err < 1e-12
+
+
 def test_reacting_steps_match(dev):
     """H2/O2-N2, MUSCL + HLLC, coupled chemistry, tvd-rk3: the reacting case at 24^3, 10 steps."""
     n = 24
     L = (0.01,) * 3
-
-    def prims(x):
-        q = sp.h2_prims(x, (n,) * 3, lengths=L)
-        q[..., 6] = 1e-4
-        return q
-
-    mb, hb = _run_pair("h2o2", "muscl", "hllc", "koren", True, "tvd-rk3", (n,) * 3, 10, 1e-8, prims, L)
+    mb, hb = _run_pair("h2o2", "muscl", "hllc", "koren", True, "tvd-rk3", (n,) * 3, 10, 1e-8,
+                       lambda x: sp.reacting_flow(x, n, L), L)
     g = hb.ng
     got = mb.blocks[0].Q.get()[g:-g, g:-g, g:-g].transpose(3, 2, 1, 0)
     want = hb.regs[0][hb.interior()]
@@ -285,8 +327,10 @@ def test_reacting_steps_match(dev):
     assert err < 1e-10
 
 
-def test_tgv_rk4_100_steps_match(dev):
-    """Taylor-Green, air (cpg), KEEP, RK4, 100 steps at 16^3."""
+@pytest.mark.parametrize("gas", ["air-tpg", "air-cpg"])
+def test_tgv_rk4_100_steps_match(dev, gas):
+    """Taylor-Green at Mach 0.1 (air, thermally perfect as the 32^3 case states, and calorically
+    perfect), KEEP, RK4, 100 steps at 16^3."""
     n = 16
     L = (2 * np.pi,) * 3
     R, cp = 8314.46261815324 / 28.96, 1005.0
@@ -305,11 +349,11 @@ def test_tgv_rk4_100_steps_match(dev):
         return q
 
     dt = 0.2 * (2 * np.pi / n) / (c0 + V0)
-    mb, hb = _run_pair("air-cpg", "constant", "kepaec", None, False, "rk4", (n,) * 3, 100, dt, prims, L)
+    mb, hb = _run_pair(gas, "constant", "kepaec", None, False, "rk4", (n,) * 3, 100, dt, prims, L)
     g = hb.ng
     got = mb.blocks[0].Q.get()[g:-g, g:-g, g:-g].transpose(3, 2, 1, 0)
     err = _rowerr(got, hb.regs[0][hb.interior()])
-    print(f"tgv rk4 100 steps: max row-normalised difference {err:.3e}")
+    print(f"tgv {gas} rk4 100 steps: max row-normalised difference {err:.3e}")
     assert err < 1e-10
 
 

@@ -39,6 +39,15 @@ def test_table_log_and_exp_are_within_their_stated_accuracy():
     assert abs(O.tlog(1.0)) < 3e-14
 
 
+def test_the_eos_log_is_within_an_ulp():
+    rng = np.random.default_rng(4)
+    x = np.concatenate([rng.uniform(50.0, 20000.0, 20000), np.exp(rng.uniform(-700, 700, 5000)),
+                        1.0 + rng.uniform(-0.05, 0.05, 5000)])
+    got = np.array([O.elog(v) for v in x])
+    want = np.log(x)
+    assert (np.abs(got - want) <= np.spacing(np.abs(want))).all()
+
+
 @pytest.fixture(scope="module")
 def h2():
     sim = get_simulator(sp.config("h2o2", recon="muscl", primary="hllc", limiter="koren", chem=True))
