@@ -37,8 +37,21 @@ def case_config(support):
     return support.config("h2o2", recon="muscl", primary="hllc", limiter="koren", chem=True, integrator="tvd-rk3", dt=DT)
 
 
+def b_alg(ne, rbar=5.0 / 3.0, wbar=1.0, tguess=2, G=10):
+    """Compulsory bytes per cell per stage of a fused pass (tvd-rk3, Euler): the stage input read
+    and the step's start read in two of three stages (rbar ne), the result written (wbar ne), T read
+    and written as the EOS's start, and the geometry read (detJinv and one face's area vector per
+    axis)."""
+    return 8.0 * (rbar * ne + wbar * ne + tguess + G)
+
+
 def algorithmic_bytes(name, ne, ng):
     """Compulsory HBM bytes per item of one launch of an entry point (doubles x 8)."""
+    if name == "pga_stage":
+        return b_alg(ne)
+    if name == "pga_rates":
+        # per cell: Q and T in, the carried species' rates out
+        return 8 * (ne + 1 + ne - 5)
     if name == "pga_flux":
         # per face: Q and p, T, rhoinv of one cell (the stencil's others are neighbours' reads), S, F out
         return 8 * (ne + 3 + 3 + ne)
@@ -87,6 +100,8 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=32)
     ap.add_argument("--cpu-stages", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--per-kernel", action="store_true", help="the reference's launch sequence instead of the fused stage")
+    ap.add_argument("--k-chunk", type=int, default=None)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU: this build has no multi-GPU path (the block sharding is design only)",
@@ -99,6 +114,9 @@ def main():
     import support
 
     cfg = case_config(support)
+    cfg["backend-mi355x"]["fused-stage"] = not args.per_kernel
+    if args.k_chunk:
+        cfg["backend-mi355x"]["k-chunk"] = args.k_chunk
     backend = pg.get_backend("mi355x", cfg)
     n = args.n
     mesh = pg.mesher.CubeMesher(blkdims=[n + 1] * 3, lengths=L, periodic=[True] * 3)
@@ -123,7 +141,7 @@ def main():
     dom = max(total, key=total.get)
     count, ms = times[dom]
     ne = mb.sim.ne
-    items = {"pga_flux": (n + 1) * n * n, "pga_source": n**3, "pga_applyflux": n**3, "pga_combine": n**3}.get(dom, n**3)
+    items = {"pga_flux": (n + 1) * n * n}.get(dom, n**3)
     per = algorithmic_bytes(dom, ne, mb.ng)
     achieved = per * items / (ms * 1e-3) / 1e9
     roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK, unit="GB/s", frac=round(achieved / HBM_PEAK, 4),
@@ -146,7 +164,8 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": f"h2o2-9sp-28rx {n}^3 periodic, muscl-koren + hllc, coupled chemistry, tvd-rk3",
-                   "cells": ncells, "stages_per_step": nst, "path": "per-kernel"},
+                   "cells": ncells, "stages_per_step": nst, "path": "fused" if mb.fused else "per-kernel",
+                   "k_chunk": cfg["backend-mi355x"]["k-chunk"]},
         "roofline": roof,
         "cpu_baseline": cpu,
         "kernels": breakdown,

@@ -138,6 +138,39 @@ int32_t pga_fill_periodic(pga_view A, int32_t ncomp, const int32_t *ncell, int32
 int32_t pga_copy_box(pga_view dst, const int32_t *dst_start, pga_view src, const int32_t *src_start,
                      const int32_t *extent, int32_t ncomp, void *stream);
 
+/* 1 when a fused stage is compiled for the case's variants, else 0 (use the per-kernel entry points) */
+int32_t pga_stage_supported(const pga_case *c);
+
+/* One RK stage of a block's interior in one pass: the coupled source (a reacting case's rates,
+ * from pga_rates, ns - 1 components per cell), the advective fluxes of all faces, their divergence, the RK combine and the EOS of
+ * the result. Qin and stin hold the stage's input with its halo filled (pga_halo); Q0 the step's
+ * start; the result goes to Qout and stout, which must not be Qin and stin. The combine is
+ * out = sum_t w[t] x_t over nterms terms in order, x_t = Qin (src 0), Q0 (1) or the stage's dQ (2),
+ * a term with withdt[t] taking w[t] * (*dt) * x_t. kchunk: the k-planes one workgroup marches.
+ * Replaces, for one stage, the reference's launches of pg_source
+ * (chemint/kernels/source.mako:3-10), pg_flux x3 (euler/kernels/flux/single.mako:4-9),
+ * pg_applyflux (advec/kernels/applyflux.mako:2-11), pg_combine (integrators/kernels/combine.mako)
+ * and the interior pg_fromQ (euler/kernels/state/fromQ.mako:5-19), as sequenced by
+ * multiblock/solver.py:246-284. */
+int32_t pga_stage(const pga_case *c, pga_view Qin, pga_view stin, pga_view Q0, pga_view Qout, pga_view stout,
+                  pga_view iS, pga_view jS, pga_view kS, pga_view detJinv, pga_view rates, int32_t nterms,
+                  const int32_t *src,
+                  const double *w, const int32_t *withdt, const double *dt, pga_box box, int32_t kchunk,
+                  void *stream);
+
+/* The carried species' production rates (kg/m^3/s) of every cell in box into rates (ns - 1
+ * components): the coupled source of a reacting stage. The body of pg_source
+ * (peregrine/simulator/chemint/kernels/source.mako:3-10) writing only the species rows. */
+int32_t pga_rates(const pga_case *c, pga_view Q, pga_view state, pga_view rates, pga_box box, void *stream);
+
+/* The halo pass of a stage: for up to 6 slabs, Q copied from src (at src_start) to dst (at
+ * dst_start), extent cells each, then the EOS over the slab into stdst, its Newton started from
+ * stguess's T at the same cells. Replaces the prerhs slot's pg_place and the halo pg_fromQ
+ * (simulator/advec/simulator.py:57-64). */
+int32_t pga_halo(const pga_case *c, int32_t ntask, const pga_view *Qdst, const pga_view *Qsrc,
+                 const int32_t *dst_start, const int32_t *src_start, const int32_t *extent, pga_view stdst,
+                 pga_view stguess, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

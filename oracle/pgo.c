@@ -115,7 +115,7 @@ double pgo_elog(double x) {
     /* near 1 the table's terms would cancel: ln(1 + r) by its series, r = x - 1 exact */
     const double r = x - 1.0;
     double q = -1.0 / 14.0;
-    for (int n = 13; n >= 2; n--) q = q * r + ((n & 1) ? 1.0 : -1.0) / n;
+    for (int n = 13; n >= 2; n--) q = fma(q, r, ((n & 1) ? 1.0 : -1.0) / n);
     return r + (r * r) * q;
   }
   int64_t bits;
@@ -127,11 +127,11 @@ double pgo_elog(double x) {
   memcpy(&m, &mb, sizeof m);
   const double r = fma(m, ELOG_RCP[k], -1.0);
   double q = 1.0 / 7.0;
-  q = q * r - 1.0 / 6.0;
-  q = q * r + 0.2;
-  q = q * r - 0.25;
-  q = q * r + 1.0 / 3.0;
-  q = q * r - 0.5;
+  q = fma(q, r, -1.0 / 6.0);
+  q = fma(q, r, 0.2);
+  q = fma(q, r, -0.25);
+  q = fma(q, r, 1.0 / 3.0);
+  q = fma(q, r, -0.5);
   const double p = (r * r) * q;
   const double hi = e * PGA_ELOG_LN2HI + ELOG_LHI[k];
   const double lo = e * PGA_ELOG_LN2LO + ELOG_LLO[k];
@@ -149,7 +149,10 @@ typedef struct {
   double R, cp, h298, cpcoeff[8];
 } mixmom;
 
-/* eos/kernels/mixing.mako:3-30: the carried species, then the remainder */
+/* eos/kernels/mixing.mako:3-30: the carried species, then the remainder. The sums, and the
+ * Horner steps of the ln T polynomials below, are taken as explicit fused multiply-adds: the
+ * polynomials' terms cancel by six orders, so their rounding sequence is fixed by the code (the
+ * device evaluates the same sequence) rather than left to a compiler's contraction. */
 static void mix(const pgo_case *c, const double *Y, mixmom *m) {
   memset(m, 0, sizeof *m);
   double Ylast = 1.0;
@@ -161,13 +164,13 @@ static void mix(const pgo_case *c, const double *Y, mixmom *m) {
     } else {
       Yn = Ylast;
     }
-    m->R += Yn * c->Rsp[n];
+    m->R = fma(Yn, c->Rsp[n], m->R);
     if (c->eos == 0) {
-      m->cp += Yn * c->cp0[n];
+      m->cp = fma(Yn, c->cp0[n], m->cp);
     } else {
-      for (int t = 0; t < c->nt; t++) m->cpcoeff[t] += Yn * c->cppolyR[n * c->nt + t];
+      for (int t = 0; t < c->nt; t++) m->cpcoeff[t] = fma(Yn, c->cppolyR[n * c->nt + t], m->cpcoeff[t]);
     }
-    m->h298 += Yn * c->h298R[n];
+    m->h298 = fma(Yn, c->h298R[n], m->h298);
   }
 }
 
@@ -176,12 +179,12 @@ static void ideal_cph(const pgo_case *c, const mixmom *m, double T, double *cp, 
   const double u = pgo_elog(T);
   double cpv = 0.0, hv = 0.0, hc = 0.0;
   for (int t = c->nt - 1; t >= 0; t--) {
-    cpv = cpv * u + m->cpcoeff[t];
-    hc = m->cpcoeff[t] - (t + 1) * hc;
-    hv = hv * u + hc;
+    cpv = fma(cpv, u, m->cpcoeff[t]);
+    hc = fma(-(double)(t + 1), hc, m->cpcoeff[t]);
+    hv = fma(hv, u, hc);
   }
   *cp = cpv;
-  *h = hv * T + m->h298;
+  *h = fma(hv, T, m->h298);
 }
 
 /* cpg eos/cpg/kernels/eos.mako:5-15; tpg eos/tpg/kernels/eos.mako:8-20 and newton.mako:3-11 */
@@ -524,8 +527,8 @@ void pgo_production(const pgo_case *c, double rho, const double *Y, double T, do
     conc[n] = cn;
     ctot += cn;
     double g = 0.0;
-    for (int m = c->ng_ - 1; m >= 0; m--) g = g * logT + c->gpoly[n * c->ng_ + m];
-    gRT[n] = g + c->h298[n] * Tinv;
+    for (int m = c->ng_ - 1; m >= 0; m--) g = fma(g, logT, c->gpoly[n * c->ng_ + m]);
+    gRT[n] = fma(c->h298[n], Tinv, g);
     logc[n] = fmax(c->logcfloor, pgo_tlog(cn));
     cG[n] = fmax(cn, 0.0) * pgo_texp(gRT[n]);
   }

@@ -73,6 +73,7 @@ _DEFAULTS = {
         "tile-i": 64,
         "tile-j": 4,
         "fused-stage": True,
+        "k-chunk": 8,
         "graphs": True,
     },
     "halo-exchange": {"kind": "hoststaged"},

@@ -8,14 +8,15 @@
 namespace pga {
 
 // g/(Ru T) of species n: its Gibbs polynomial in ln T plus h298/T (eos/kernels/species.mako:16-21),
-// one rounding per operation: the polynomial's terms cancel by six orders
+// Horner steps as explicit fused multiply-adds (the oracle evaluates the same sequence): the
+// polynomial's terms cancel by six orders, so the rounding sequence is fixed, not left to the compiler
 template <int NS>
 __device__ __forceinline__ double gibbs_rt(const Gas<NS> &g, int n, double logT, double Tinv) {
 #pragma clang fp contract(off)
   double gr = 0.0;
 #pragma unroll
-  for (int m = NG - 1; m >= 0; m--) gr = gr * logT + g.gpoly(n, m);
-  return gr + g.h298(n) * Tinv;
+  for (int m = NG - 1; m >= 0; m--) gr = fma(gr, logT, g.gpoly(n, m));
+  return fma(g.h298(n), Tinv, gr);
 }
 
 // euler/kernels/solution.mako:25-35: Y clamped to [0, 1], rescaled when their sum

@@ -151,7 +151,7 @@ __device__ __forceinline__ double elog(double x) {
     // near 1 the table's terms would cancel: ln(1 + r) by its series, r = x - 1 exact
     const double r = x - 1.0;
     double q = -1.0 / 14.0;
-    for (int n = 13; n >= 2; n--) q = q * r + ((n & 1) ? 1.0 : -1.0) / n;
+    for (int n = 13; n >= 2; n--) q = fma(q, r, ((n & 1) ? 1.0 : -1.0) / n);
     return r + (r * r) * q;
   }
   const long long bits = __double_as_longlong(x);
@@ -160,11 +160,11 @@ __device__ __forceinline__ double elog(double x) {
   const double m = __longlong_as_double((bits & 0x000fffffffffffffLL) | 0x3ff0000000000000LL);
   const double r = fma(m, ELOG_RCP[k], -1.0);
   double q = 1.0 / 7.0;
-  q = q * r - 1.0 / 6.0;
-  q = q * r + 0.2;
-  q = q * r - 0.25;
-  q = q * r + 1.0 / 3.0;
-  q = q * r - 0.5;
+  q = fma(q, r, -1.0 / 6.0);
+  q = fma(q, r, 0.2);
+  q = fma(q, r, -0.25);
+  q = fma(q, r, 1.0 / 3.0);
+  q = fma(q, r, -0.5);
   const double p = (r * r) * q;
   const double hi = e * PGA_ELOG_LN2HI + ELOG_LHI[k];
   const double lo = e * PGA_ELOG_LN2LO + ELOG_LLO[k];

@@ -30,8 +30,9 @@ struct MixMom<NS, EOS_CPG> {
 // eos/kernels/mixing.mako:3-30: Y(n) gives the carried ns - 1; the last is what they leave
 // The ln T polynomials of a tpg gas carry coefficients of 1e6 whose terms cancel to cp ~ 1e3:
 // one rounding more or less per step moves h by 1e-11 of itself. The EOS is therefore evaluated
-// with one rounding per written operation (no contracted multiply-adds), the order the
-// reference's host build evaluates it in.
+// with a fixed rounding sequence: the composition sums and the Horner steps as explicit fused
+// multiply-adds, everything else one rounding per written operation, in the reference's order.
+// The oracle evaluates the same sequence, so the two agree to the bit.
 template <int NS, int EOS, class YF>
 __device__ __forceinline__ void mixmom(const Gas<NS> &g, const YF &Y, MixMom<NS, EOS> &m) {
 #pragma clang fp contract(off)
@@ -53,14 +54,14 @@ __device__ __forceinline__ void mixmom(const Gas<NS> &g, const YF &Y, MixMom<NS,
     } else {
       Yn = Ylast;
     }
-    m.R += Yn * g.Rsp(n);
+    m.R = fma(Yn, g.Rsp(n), m.R);
     if constexpr (EOS == EOS_TPG) {
 #pragma unroll
-      for (int t = 0; t < NT; t++) m.cpcoeff[t] += Yn * g.cppolyR(n, t);
+      for (int t = 0; t < NT; t++) m.cpcoeff[t] = fma(Yn, g.cppolyR(n, t), m.cpcoeff[t]);
     } else {
-      m.cp += Yn * g.cp0(n);
+      m.cp = fma(Yn, g.cp0(n), m.cp);
     }
-    m.h298 += Yn * g.h298R(n);
+    m.h298 = fma(Yn, g.h298R(n), m.h298);
   }
 }
 
@@ -73,12 +74,12 @@ __device__ __forceinline__ void ideal_cph(const MixMom<NS, EOS_TPG> &m, double T
   double cpv = 0.0, hv = 0.0, hc = 0.0;
 #pragma unroll
   for (int t = NT - 1; t >= 0; t--) {
-    cpv = cpv * u + m.cpcoeff[t];
-    hc = m.cpcoeff[t] - (t + 1) * hc;
-    hv = hv * u + hc;
+    cpv = fma(cpv, u, m.cpcoeff[t]);
+    hc = fma(-(double)(t + 1), hc, m.cpcoeff[t]);
+    hv = fma(hv, u, hc);
   }
   cp = cpv;
-  h = hv * T + m.h298;
+  h = fma(hv, T, m.h298);
 }
 
 // (rho, e) -> state: cpg closed form (eos/cpg/kernels/eos.mako:5-15), tpg Newton on

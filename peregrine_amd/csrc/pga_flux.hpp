@@ -21,37 +21,41 @@ struct FaceState {
   double Y[NS > 1 ? NS - 1 : 1];
 };
 
+enum { LIM_NONE = 0, LIM_KOREN = 1, LIM_MINMOD = 2, LIM_MC = 3, LIM_SUPERBEE = 4, LIM_VANLEER = 5 };
+
 // b phi(a / b), b > 0, without the ratio (recon/{koren,minmod,mc,superbee,vanleer}.mako)
-__device__ __forceinline__ double limit(int kind, double a, double b) {
+template <int LIM>
+__device__ __forceinline__ double limit(double a, double b) {
 #pragma clang fp contract(off)
-  switch (kind) {
-    case 1: return fmax(0.0, fmin(fmin(2.0 * a, (2.0 * b + a) * (1.0 / 3.0)), 2.0 * b));
-    case 2: return fmax(0.0, fmin(b, a));
-    case 3: return fmax(0.0, fmin(fmin(2.0 * a, (b + a) * 0.5), 2.0 * b));
-    case 4: return fmax(0.0, fmax(fmin(2.0 * a, b), fmin(a, 2.0 * b)));
-    default: return (a + fabs(a)) * b / (b + fabs(a));
-  }
+  if constexpr (LIM == LIM_KOREN) return fmax(0.0, fmin(fmin(2.0 * a, (2.0 * b + a) * (1.0 / 3.0)), 2.0 * b));
+  if constexpr (LIM == LIM_MINMOD) return fmax(0.0, fmin(b, a));
+  if constexpr (LIM == LIM_MC) return fmax(0.0, fmin(fmin(2.0 * a, (b + a) * 0.5), 2.0 * b));
+  if constexpr (LIM == LIM_SUPERBEE) return fmax(0.0, fmax(fmin(2.0 * a, b), fmin(a, 2.0 * b)));
+  return (a + fabs(a)) * b / (b + fabs(a));
 }
 
 // recon/muscl.mako:5-13: the limited slope across the face carried to dp's sign and back
-__device__ __forceinline__ double phi(int kind, double dm, double dp) {
+template <int LIM>
+__device__ __forceinline__ double phi(double dm, double dp) {
 #pragma clang fp contract(off)
   double p = 0.0;
   if (dp != 0.0) {
     const double s = copysign(1.0, dp);
-    p = s * limit(kind, s * dm, fabs(dp));
+    p = s * limit<LIM>(s * dm, fabs(dp));
   }
   return p;
 }
 
-__device__ __forceinline__ double left_of(int kind, double vLL, double vL, double vR) {
+template <int LIM>
+__device__ __forceinline__ double left_of(double vLL, double vL, double vR) {
 #pragma clang fp contract(off)
-  return vL + 0.5 * phi(kind, vL - vLL, vR - vL);
+  return vL + 0.5 * phi<LIM>(vL - vLL, vR - vL);
 }
 
-__device__ __forceinline__ double right_of(int kind, double vL, double vR, double vRR) {
+template <int LIM>
+__device__ __forceinline__ double right_of(double vL, double vR, double vRR) {
 #pragma clang fp contract(off)
-  return vR - 0.5 * phi(kind, vRR - vR, vR - vL);
+  return vR - 0.5 * phi<LIM>(vRR - vR, vR - vL);
 }
 
 // the face's sides: recon/constant.mako:3-18 (the two cells as they are) or
@@ -59,9 +63,9 @@ __device__ __forceinline__ double right_of(int kind, double vL, double vR, doubl
 // o is the offset of the face's right cell R, d the stride along the axis
 // evaluated with one rounding per operation: the sides' p and T feed the EOS's ill-conditioned
 // ln T polynomials, where a last-bit change in T moves e by 1e-11 of itself
-template <int NS, int EOS, int RECON>
+template <int NS, int EOS, int RECON, int LIM>
 __device__ __forceinline__ void face_sides(const Gas<NS> &g, const View &Q, const View &st, int64_t o, int64_t d,
-                                           int lim, FaceState<NS> &L, FaceState<NS> &R) {
+                                           FaceState<NS> &L, FaceState<NS> &R) {
 #pragma clang fp contract(off)
   if constexpr (RECON == RECON_CONSTANT) {
     const int64_t oL = o - d;
@@ -84,14 +88,14 @@ __device__ __forceinline__ void face_sides(const Gas<NS> &g, const View &Q, cons
     const double ri0 = st.at(oLL, 2), ri1 = st.at(oL, 2), ri2 = st.at(o, 2), ri3 = st.at(oRR, 2);
     const double p0 = st.at(oLL, 0), p1 = st.at(oL, 0), p2 = st.at(o, 0), p3 = st.at(oRR, 0);
     const double T0 = st.at(oLL, 1), T1 = st.at(oL, 1), T2 = st.at(o, 1), T3 = st.at(oRR, 1);
-    const double pL = left_of(lim, p0, p1, p2), pR = right_of(lim, p1, p2, p3);
-    const double TL = left_of(lim, T0, T1, T2), TR = right_of(lim, T1, T2, T3);
+    const double pL = left_of<LIM>(p0, p1, p2), pR = right_of<LIM>(p1, p2, p3);
+    const double TL = left_of<LIM>(T0, T1, T2), TR = right_of<LIM>(T1, T2, T3);
 #pragma unroll
     for (int n = 0; n < NS - 1; n++) {
       const double y0 = Q.at(oLL, 5 + n) * ri0, y1 = Q.at(oL, 5 + n) * ri1;
       const double y2 = Q.at(o, 5 + n) * ri2, y3 = Q.at(oRR, 5 + n) * ri3;
-      L.Y[n] = left_of(lim, y0, y1, y2);
-      R.Y[n] = right_of(lim, y1, y2, y3);
+      L.Y[n] = left_of<LIM>(y0, y1, y2);
+      R.Y[n] = right_of<LIM>(y1, y2, y3);
     }
     {
       MixMom<NS, EOS> m;
@@ -106,17 +110,17 @@ __device__ __forceinline__ void face_sides(const Gas<NS> &g, const View &Q, cons
       vel[c][2] = Q.at(o, 1 + c) * ri2;
       vel[c][3] = Q.at(oRR, 1 + c) * ri3;
     }
-    L.u = left_of(lim, vel[0][0], vel[0][1], vel[0][2]);
-    L.v = left_of(lim, vel[1][0], vel[1][1], vel[1][2]);
-    L.w = left_of(lim, vel[2][0], vel[2][1], vel[2][2]);
+    L.u = left_of<LIM>(vel[0][0], vel[0][1], vel[0][2]);
+    L.v = left_of<LIM>(vel[1][0], vel[1][1], vel[1][2]);
+    L.w = left_of<LIM>(vel[2][0], vel[2][1], vel[2][2]);
     {
       MixMom<NS, EOS> m;
       mixmom<NS, EOS>(g, [&](int n) { return R.Y[n]; }, m);
       frompT<NS, EOS>(m, pR, TR, R.s);
     }
-    R.u = right_of(lim, vel[0][1], vel[0][2], vel[0][3]);
-    R.v = right_of(lim, vel[1][1], vel[1][2], vel[1][3]);
-    R.w = right_of(lim, vel[2][1], vel[2][2], vel[2][3]);
+    R.u = right_of<LIM>(vel[0][1], vel[0][2], vel[0][3]);
+    R.v = right_of<LIM>(vel[1][1], vel[1][2], vel[1][3]);
+    R.w = right_of<LIM>(vel[2][1], vel[2][2], vel[2][3]);
   }
 }
 
@@ -157,28 +161,30 @@ __device__ __forceinline__ void hllc(const FaceState<NS> &L, const FaceState<NS>
                        (L.s.rho * (SL - unL) - R.s.rho * (SR - unR));
   const bool left = SL >= 0.0 || Sstar >= 0.0;
   const bool star = !(SL >= 0.0) && (Sstar >= 0.0 || SR >= 0.0);
-  const FaceState<NS> &X = left ? L : R;
+  // the upwind side's values, selected one by one (a reference to one side would put both in memory)
+  const double Xrho = left ? L.s.rho : R.s.rho, Xp = left ? L.s.p : R.s.p, Xe = left ? L.s.e : R.s.e;
+  const double Xu = left ? L.u : R.u, Xv = left ? L.v : R.v, Xw = left ? L.w : R.w;
   const double U = left ? unL : unR, SX = left ? SL : SR;
-  const double et = X.s.e + 0.5 * (X.u * X.u + X.v * X.v + X.w * X.w);
-  const double mdot = U * X.s.rho * area;
+  const double et = Xe + 0.5 * (Xu * Xu + Xv * Xv + Xw * Xw);
+  const double mdot = U * Xrho * area;
   F[0] = mdot;
-  F[1] = mdot * X.u + X.s.p * S[0];
-  F[2] = mdot * X.v + X.s.p * S[1];
-  F[3] = mdot * X.w + X.s.p * S[2];
-  F[4] = mdot * et + U * X.s.p * area;
+  F[1] = mdot * Xu + Xp * S[0];
+  F[2] = mdot * Xv + Xp * S[1];
+  F[3] = mdot * Xw + Xp * S[2];
+  F[4] = mdot * et + U * Xp * area;
 #pragma unroll
-  for (int n = 0; n < NS - 1; n++) F[5 + n] = mdot * X.Y[n];
+  for (int n = 0; n < NS - 1; n++) F[5 + n] = mdot * (left ? L.Y[n] : R.Y[n]);
   if (star) {
-    const double rhostar = X.s.rho * ((SX - U) / (SX - Sstar));
-    const double drho = rhostar - X.s.rho, dU = rhostar * (Sstar - U);
+    const double rhostar = Xrho * ((SX - U) / (SX - Sstar));
+    const double drho = rhostar - Xrho, dU = rhostar * (Sstar - U);
     const double jump = SX * area;
     F[0] += jump * drho;
-    F[1] += jump * (drho * X.u + dU * nx);
-    F[2] += jump * (drho * X.v + dU * ny);
-    F[3] += jump * (drho * X.w + dU * nz);
-    F[4] += jump * (drho * et + dU * (Sstar + X.s.p / (X.s.rho * (SX - U))));
+    F[1] += jump * (drho * Xu + dU * nx);
+    F[2] += jump * (drho * Xv + dU * ny);
+    F[3] += jump * (drho * Xw + dU * nz);
+    F[4] += jump * (drho * et + dU * (Sstar + Xp / (Xrho * (SX - U))));
 #pragma unroll
-    for (int n = 0; n < NS - 1; n++) F[5 + n] += jump * drho * X.Y[n];
+    for (int n = 0; n < NS - 1; n++) F[5 + n] += jump * drho * (left ? L.Y[n] : R.Y[n]);
   }
 }
 

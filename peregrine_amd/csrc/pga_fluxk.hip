@@ -6,8 +6,8 @@
 
 namespace pga {
 
-template <int NS, int EOS, int RECON, int PRIMARY>
-__global__ __launch_bounds__(256) void k_flux(const double *__restrict__ K, int axis, int lim, View Q, View st, View S,
+template <int NS, int EOS, int RECON, int PRIMARY, int LIM>
+__global__ __launch_bounds__(256) void k_flux(const double *__restrict__ K, int axis, View Q, View st, View S,
                                               View F, Box3 b) {
   int i, j, k;
   if (!cell_of(b, i, j, k)) return;
@@ -15,7 +15,7 @@ __global__ __launch_bounds__(256) void k_flux(const double *__restrict__ K, int 
   const int64_t o = Q.off(i, j, k);
   const int64_t d = axis == 0 ? 1 : axis == 1 ? Q.sj : Q.sk;
   FaceState<NS> L, R;
-  face_sides<NS, EOS, RECON>(g, Q, st, o, d, lim, L, R);
+  face_sides<NS, EOS, RECON, LIM>(g, Q, st, o, d, L, R);
   const int64_t of = S.off(i, j, k);
   const double Sv[3] = {S.at(of, 0), S.at(of, 1), S.at(of, 2)};
   double Fv[NS + 4];
@@ -30,16 +30,19 @@ static void launch_flux(const pga_case *c, int axis, pga_view Q, pga_view st, pg
                         hipStream_t s) {
   const dim3 grid = grid_of(b), block(TI, TJ);
   const int lim = c->desc.limiter, recon = c->desc.recon, prim = c->desc.primary;
-#define L(RECON, PRIM)                                                                                            \
-  if (recon == RECON && prim == PRIM)                                                                             \
-    hipLaunchKernelGGL((k_flux<NS, EOS, RECON, PRIM>), grid, block, 0, s, c->K, axis, lim, view(Q), view(st), view(S), \
-                       view(F), b);
-  L(RECON_CONSTANT, FLUX_KEPAEC)
-  L(RECON_CONSTANT, FLUX_HLLC)
-  L(RECON_CONSTANT, FLUX_RUSANOV)
-  L(RECON_MUSCL, FLUX_KEPAEC)
-  L(RECON_MUSCL, FLUX_HLLC)
-  L(RECON_MUSCL, FLUX_RUSANOV)
+#define L(RECON, PRIM, LIM)                                                                                    \
+  if (recon == RECON && prim == PRIM && (RECON == RECON_CONSTANT || lim == LIM))                              \
+    hipLaunchKernelGGL((k_flux<NS, EOS, RECON, PRIM, LIM>), grid, block, 0, s, c->K, axis, view(Q), view(st),   \
+                       view(S), view(F), b);
+  L(RECON_CONSTANT, FLUX_KEPAEC, LIM_NONE)
+  L(RECON_CONSTANT, FLUX_HLLC, LIM_NONE)
+  L(RECON_CONSTANT, FLUX_RUSANOV, LIM_NONE)
+#define M(PRIM) L(RECON_MUSCL, PRIM, LIM_KOREN) L(RECON_MUSCL, PRIM, LIM_MINMOD) L(RECON_MUSCL, PRIM, LIM_MC) \
+  L(RECON_MUSCL, PRIM, LIM_SUPERBEE) L(RECON_MUSCL, PRIM, LIM_VANLEER)
+  M(FLUX_HLLC)
+  M(FLUX_RUSANOV)
+  M(FLUX_KEPAEC)
+#undef M
 #undef L
 }
 

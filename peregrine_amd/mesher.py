@@ -81,16 +81,18 @@ class CubeMesher(BaseMesher):
         self.origin = [float(o) for o in origin]
         self.lengths = [float(L) for L in lengths]
         self.periodic = [bool(p) for p in periodic]
-        self.edges = [np.linspace(o, o + L, n + 1) for o, L, n in zip(self.origin, self.lengths, self.mbdims)]
+        # the whole lattice's nodes along each axis, sliced per block: a node on an interface has the
+        # same coordinate in both blocks and in a single block spanning the lattice
+        self.lines = [np.linspace(o, o + L, m * (n - 1) + 1)
+                      for o, L, m, n in zip(self.origin, self.lengths, self.mbdims, self.blkdims)]
 
     @property
     def periodicaxes(self):
         return tuple(self.periodic)
 
     def shape_blk(self, blk, i, j, k):
-        corner = [e[m] for e, m in zip(self.edges, (i, j, k))]
-        span = [e[m + 1] - e[m] for e, m in zip(self.edges, (i, j, k))]
-        x, y, z = (np.linspace(o, o + L, n) for o, L, n in zip(corner, span, self.blkdims))
+        x, y, z = (line[m * (n - 1):(m + 1) * (n - 1) + 1]
+                   for line, m, n in zip(self.lines, (i, j, k), self.blkdims))
         blk.set_extents(*self.blkdims)
         blk.interior_nodes = np.stack(np.meshgrid(x, y, z, indexing="ij"), axis=-1)
         # the whole lattice's period, for a periodic interface's halo nodes

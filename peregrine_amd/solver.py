@@ -114,6 +114,8 @@ class Solver:
     def __init__(self, cfg, mesh, backend, sim, integ, state=None):
         self.cfg, self.mesh, self.backend, self.sim, self.integ = cfg, mesh, backend, sim, integ
         self.nrt = 0
+        # register -> the derivative register its prerhs began (None: halos ready, no derivative)
+        self.readied = {}
         self.tme = HostScalar(0.0)
         self.dt = backend.scalar("dt", cfg["simulation"]["dt"])
         self.ng = sim.ng
@@ -125,6 +127,11 @@ class Solver:
             periodic = [blk.faces[2 * a + 1].periodic and blk.faces[2 * a + 1].nbr == blk.nblki for a in range(3)]
             blk.make_metrics(periodic)
         self.case = NativeCase(sim, backend)
+        self.fused = self._fused_supported()
+        if self.fused:
+            for blk in self.blocks:
+                blk.state2 = backend.alloc(blk.cellshape + (NSTATE,), name="state2")
+                blk.rates = backend.alloc(blk.cellshape + (max(sim.ns - 1, 1),), name="rates") if sim.chemistry else None
         self.plugins = self._plugins()
         self._set_state(state)
         for p in self.plugins:
@@ -156,6 +163,14 @@ class Solver:
 
         return claim(self.cfg["plugins"], self)
 
+    def _fused_supported(self):
+        """The fused stage runs when the config asks for it, the integrator takes one combine per
+        stage, and the library has the case's variant compiled."""
+        sect = self.cfg["backend-mi355x"]
+        if not sect["fused-stage"] or self.integ.name not in ("rk1", "rk2", "tvd-rk3"):
+            return False
+        return bool(native.lib().pga_stage_supported(self.case.handle))
+
     @property
     def regs(self):
         return tuple(range(self.integ.nregs))
@@ -182,7 +197,12 @@ class Solver:
         for blk in self.blocks:
             full = native.box((0, 0, 0), blk.cellshape)
             self._call("pga_fromprims", self.case.handle, blk.q.view(), blk.Q.view(), blk.state.view(), full)
+        self.readied.clear()
         self.postrhs(0)
+        if self.fused:
+            # the fused path's second state starts as the first: its halo is the next halo EOS's guess
+            for blk in self.blocks:
+                blk.state2.t.copy_(blk.state.t)
 
     # -- launches --------------------------------------------------------------
     timers = None
@@ -238,18 +258,43 @@ class Solver:
             boxes = blk.halo_boxes()
             arr = (native.Box * len(boxes))(*boxes)
             self._call("pga_fromQ", self.case.handle, blk.regs[Q].view(), blk.state.view(), arr, len(boxes))
-            if dQ is not None and self.sim.chemistry:
+        if dQ is not None:
+            self.begin_dQ(Q, dQ)
+        self.readied[Q] = dQ
+
+    def begin_dQ(self, Q, dQ):
+        """The derivative's first term, the coupled source, before the fluxes append to it."""
+        if self.sim.chemistry:
+            for blk in self.blocks:
                 self._call("pga_source", self.case.handle, blk.regs[Q].view(), blk.state.view(), blk.regs[dQ].view(),
                            blk.interior_box())
+        self.readied[Q] = dQ
+
+    def _ready(self, Q, dQ):
+        """Halos of Q exchanged and their EOS taken once per solution: a register whose halos are
+        ready (after from_prims or a postrhs with its prerhs) only begins its derivative
+        (peregrine/multiblock/solver.py:251-268). A second halo EOS from the first's T would move
+        T's last bits, and a block's halo would no longer hold its partner cells' state."""
+        standing = self.readied.get(Q, False)
+        if standing == dQ:
+            return
+        if standing is None:
+            self.begin_dQ(Q, dQ)
+        else:
+            self.prerhs(Q, dQ)
+
+    def _spent(self, dQ):
+        self.readied = {q: d for q, d in self.readied.items() if d != dQ}
 
     def rhs(self, *, Q, dQ):
-        self.prerhs(Q, dQ)
+        self._ready(Q, dQ)
         for blk in self.blocks:
             Qv, st = blk.regs[Q].view(), blk.state.view()
             for a, (S, F) in enumerate(((blk.iS, blk.iF), (blk.jS, blk.jF), (blk.kS, blk.kF))):
                 self._call("pga_flux", self.case.handle, a, Qv, st, S.view(), F.view(), blk.face_box(a))
             self._call("pga_applyflux", self.sim.ne, blk.regs[dQ].view(), blk.iF.view(), blk.jF.view(), blk.kF.view(),
                        blk.detJinv.view(), int(self.sim.chemistry), blk.interior_box())
+        self._spent(dQ)
 
     def advance(self, n):
         Qout = self.integ.stageregs(self.regs)[n].Qout
@@ -259,6 +304,7 @@ class Solver:
                 w = (ctypes.c_double * max(len(c.w), 1))(*c.w)
                 self._call("pga_combine", self.sim.ne, blk.regs[c.out].view(), int(c.own), c.wout, len(c.X), X, w,
                            self.dt.ptr, int(c.step), blk.interior_box())
+        self.readied.pop(Qout, None)
         self.postrhs(Qout, prerhs=False)
 
     def postrhs(self, Q=0, prerhs=True):
@@ -273,13 +319,84 @@ class Solver:
         self.rhs(Q=regs.Q, dQ=regs.dQ)
         self.advance(n)
 
+    # -- the fused stage ------------------------------------------------------
+    def _halo_pass(self, qin):
+        """Each block's halo slabs from its partners' interior planes, and the EOS over them."""
+        ng = self.ng
+        for blk in self.blocks:
+            dsts, srcs, ds, ss, es = [], [], [], [], []
+            for f in blk.faces.values():
+                nbr = self.blocks[f.nbr]
+                a = f.axis
+                ext = list(blk.ncell)
+                ext[a] = ng
+                dst, src = [ng] * 3, [ng] * 3
+                dst[a] = 0 if f.is_low else ng + blk.ncell[a]
+                src[a] = nbr.ncell[a] if f.is_low else ng
+                dsts.append(blk.regs[qin].view())
+                srcs.append(nbr.regs[qin].view())
+                ds += dst
+                ss += src
+                es += ext
+            n = len(dsts)
+            self._call("pga_halo", self.case.handle, n, (native.View * n)(*dsts), (native.View * n)(*srcs),
+                       native.ints(ds), native.ints(ss), native.ints(es), blk.state.view(), blk.state2.view())
+
+    def _combine_terms(self, n):
+        """The stage's combine as (source, weight, times dt) terms in the reference's order: its
+        registers read as the step's start (Q0), the stage's input (Qin) or its derivative."""
+        (c,) = self.integ.stagekerns(n, self.regs)
+        dQ = self.regs[-1]
+
+        def src(r):
+            return 2 if r == dQ else (1 if r == 0 else 0)
+
+        terms = [(src(c.out), c.wout, 0)] if c.own else []
+        for m, (r, w) in enumerate(zip(c.X, c.w)):
+            terms.append((src(r), w, int(c.step and m == len(c.X) - 1)))
+        return terms
+
+    def _fused_step(self):
+        nst = self.integ.nstages
+        kc = int(self.cfg["backend-mi355x"]["k-chunk"])
+        qin = 0
+        for n in range(nst):
+            last = n == nst - 1
+            out = 0 if (last and qin != 0) else (2 if qin == 1 else 1)
+            if qin not in self.readied:
+                self._halo_pass(qin)
+            self.readied.clear()
+            terms = self._combine_terms(n)
+            src = native.ints([t[0] for t in terms])
+            w = (ctypes.c_double * len(terms))(*[t[1] for t in terms])
+            withdt = native.ints([t[2] for t in terms])
+            for blk in self.blocks:
+                box = blk.interior_box()
+                Qin, st = blk.regs[qin].view(), blk.state.view()
+                if self.sim.chemistry:
+                    self._call("pga_rates", self.case.handle, Qin, st, blk.rates.view(), box)
+                rates = blk.rates.view() if self.sim.chemistry else native.View()
+                self._call("pga_stage", self.case.handle, Qin, st, blk.regs[0].view(), blk.regs[out].view(),
+                           blk.state2.view(), blk.iS.view(), blk.jS.view(), blk.kS.view(), blk.detJinv.view(), rates,
+                           len(terms), src, w, withdt, self.dt.ptr, box, kc)
+            for blk in self.blocks:
+                blk.state, blk.state2 = blk.state2, blk.state
+            if last and qin == 0:
+                # one stage in all: it could not write its own input, so its output becomes the start
+                for blk in self.blocks:
+                    blk.regs[0], blk.regs[1] = blk.regs[1], blk.regs[0]
+            qin = out
+
     def step(self, dt):
         start = self.tme.get()
         self.dt.set(dt)
         times = self.integ.stagetimes(start, dt)
-        for n in range(self.integ.nstages):
-            self.stage(n)
-            self.tme.set(times[n])
+        if self.fused:
+            self._fused_step()
+        else:
+            for n in range(self.integ.nstages):
+                self.stage(n)
+                self.tme.set(times[n])
         self.nrt += 1
         self.tme.set(start + dt)
 

@@ -190,7 +190,7 @@ def halo_eos(oc, hb, ne):
 def reacting_flow(x, n, L):
     """Hot spots in H2-air with radicals, plus a swirling velocity and a pressure wave, so every
     conserved row, momentum included, carries more than rounding."""
-    q = sp.h2_prims(x, (n,) * 3, lengths=L)
+    q = h2_prims(x, (n,) * 3, lengths=L)
     q[..., 6] = 1e-4
     k = 2 * np.pi / L[0]
     X, Y, Z = x[..., 0] * k, x[..., 1] * k, x[..., 2] * k

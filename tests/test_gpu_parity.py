@@ -252,11 +252,12 @@ def test_halo_copies(dev):
     assert np.array_equal(dev.get(dB), want)
 
 
-def _run_pair(gas, recon, primary, limiter, chem, integrator, ncell, steps, dt, prims, lengths):
+def _run_pair(gas, recon, primary, limiter, chem, integrator, ncell, steps, dt, prims, lengths, fused=True):
     """The same case stepped by the solver on the device and by the oracle on the host."""
     import peregrine_amd as pg
 
     cfg = sp.config(gas, recon=recon, primary=primary, limiter=limiter, chem=chem, integrator=integrator, dt=dt)
+    cfg["backend-mi355x"]["fused-stage"] = fused
     backend = pg.get_backend("mi355x", cfg)
     mesh = pg.mesher.CubeMesher(blkdims=[n + 1 for n in ncell], lengths=lengths, periodic=[True] * 3)
     mb = pg.get_solver(cfg, mesh, backend)
@@ -269,9 +270,9 @@ def _run_pair(gas, recon, primary, limiter, chem, integrator, ncell, steps, dt, 
     full = O.box((0, 0, 0), hb.cshape)
     O.lib.pgo_fromprims(oc.ref, O.arr(sp.to_dev(q)), O.arr(hb.regs[0]), O.arr(hb.state), full)
     # the solver's set_prims ends as the reference's from_prims does (multiblock/solver.py:298-305):
-    # the EOS over the interior, then the halo exchange and the EOS over the halos
+    # the EOS over the interior, then the halo exchange and the EOS over the halos, which its first
+    # stage then takes as ready; the oracle's first stage exchanges and takes the halo EOS itself
     O.lib.pgo_fromQ(oc.ref, O.arr(hb.regs[0]), O.arr(hb.state), O.box((hb.ng,) * 3, hb.ncell))
-    sp.halo_eos(oc, hb, sim.ne)
     b = sp.oracle_block(hb, integrator, chem)
     for _ in range(steps):
         mb.step(dt)
@@ -296,9 +297,9 @@ def test_reacting_rhs_matches(dev):
     O.lib.pgo_fromprims(oc.ref, O.arr(sp.to_dev(q)), O.arr(hb.regs[0]), O.arr(hb.state), O.box((0, 0, 0), hb.cshape))
     O.lib.pgo_fromQ(oc.ref, O.arr(hb.regs[0]), O.arr(hb.state), O.box((hb.ng,) * 3, hb.ncell))
     sp.halo_eos(oc, hb, sim.ne)
+    # the halos' EOS ran once, in set_prims: the rhs begins with the source
     regs = mb.integ.stageregs(mb.regs)[0]
     mb.rhs(Q=regs.Q, dQ=regs.dQ)
-    sp.halo_eos(oc, hb, sim.ne)
     ng = hb.ng
     dQ = hb.regs[2]
     O.lib.pgo_source(oc.ref, O.arr(hb.regs[0]), O.arr(hb.state), O.arr(dQ), O.box((ng,) * 3, hb.ncell))
@@ -313,18 +314,46 @@ def test_reacting_rhs_matches(dev):
     assert err < 1e-12
 
 
-def test_reacting_steps_match(dev):
-    """H2/O2-N2, MUSCL + HLLC, coupled chemistry, tvd-rk3: the reacting case at 24^3, 10 steps."""
+@pytest.mark.parametrize("fused", [True, False], ids=["fused", "per-kernel"])
+def test_reacting_steps_match(dev, fused):
+    """H2/O2-N2, MUSCL + HLLC, coupled chemistry, tvd-rk3: the reacting case at 70 x 24 x 24 (tiles
+    that overhang the block), 10 steps, by the fused stage and by the per-kernel sequence."""
     n = 24
     L = (0.01,) * 3
-    mb, hb = _run_pair("h2o2", "muscl", "hllc", "koren", True, "tvd-rk3", (n,) * 3, 10, 1e-8,
-                       lambda x: sp.reacting_flow(x, n, L), L)
+    mb, hb = _run_pair("h2o2", "muscl", "hllc", "koren", True, "tvd-rk3", (70, n, n), 10, 1e-8,
+                       lambda x: sp.reacting_flow(x, n, L), (0.01 * 70 / 24, 0.01, 0.01), fused=fused)
+    assert mb.fused == fused
     g = hb.ng
     got = mb.blocks[0].Q.get()[g:-g, g:-g, g:-g].transpose(3, 2, 1, 0)
     want = hb.regs[0][hb.interior()]
     err = _rowerr(got, want)
-    print(f"reacting 10 steps: max row-normalised difference {err:.3e}")
+    print(f"reacting 10 steps ({'fused' if fused else 'per-kernel'}): max row-normalised difference {err:.3e}")
     assert err < 1e-10
+
+
+def test_fused_and_per_kernel_stages_agree_to_the_bit(dev):
+    """The fused stage against the per-kernel sequence on the reacting case, RK3 and RK2: the same
+    device functions in the same order give the same bits."""
+    import peregrine_amd as pg
+
+    n, L = 20, (0.01,) * 3
+    for integ in ("tvd-rk3", "rk2", "rk1"):
+        out = []
+        for fused in (True, False):
+            cfg = sp.config("h2o2", recon="muscl", primary="hllc", limiter="koren", chem=True, integrator=integ, dt=1e-8)
+            cfg["backend-mi355x"]["fused-stage"] = fused
+            cfg["backend-mi355x"]["k-chunk"] = 3
+            mb = pg.get_solver(cfg, pg.mesher.CubeMesher(blkdims=[n + 1] * 3, lengths=L, periodic=[True] * 3),
+                               pg.get_backend("mi355x", cfg))
+            assert mb.fused == fused
+            mb.set_prims([sp.reacting_flow(mb.blocks[0].xcc.get(), n, L)])
+            for _ in range(3):
+                mb.step(1e-8)
+            out.append((mb.blocks[0].Q.get(), mb.blocks[0].state.get()))
+        g = mb.ng
+        inner = np.s_[g:-g, g:-g, g:-g]
+        assert np.array_equal(out[0][0][inner], out[1][0][inner]), integ
+        assert np.array_equal(out[0][1][inner], out[1][1][inner]), integ
 
 
 @pytest.mark.parametrize("gas", ["air-tpg", "air-cpg"])
@@ -357,29 +386,45 @@ def test_tgv_rk4_100_steps_match(dev, gas):
     assert err < 1e-10
 
 
-def test_blocks_of_a_lattice_step_as_one_block(dev):
+@pytest.mark.parametrize("fused", [True, False], ids=["fused", "per-kernel"])
+def test_blocks_of_a_lattice_step_as_one_block(dev, fused):
     """2 x 2 x 1 blocks against one: the halo exchange between blocks is the periodic fill's equal."""
     import peregrine_amd as pg
 
     n = 16
     L = (1.0,) * 3
     results = []
+    field = None
     for mbdims in ((1, 1, 1), (2, 2, 1)):
         cfg = sp.config("air-tpg", recon="muscl", primary="hllc", limiter="koren", dt=2e-6)
+        cfg["backend-mi355x"]["fused-stage"] = fused
         backend = pg.get_backend("mi355x", cfg)
         blkdims = [n // m + 1 for m in mbdims]
         mesh = pg.mesher.CubeMesher(mbdims=mbdims, blkdims=blkdims, lengths=L, periodic=[True] * 3)
         mb = pg.get_solver(cfg, mesh, backend)
-        mb.set_prims([sp.smooth_prims(blk.xcc.get(), mb.sim.ne) for blk in mb.blocks])
+        g = mb.ng
+        if field is None:
+            # the whole lattice's primitives with its periodic halo: each block's halo starts as its
+            # partners' cells, so every halo EOS starts from the same T guess in both runs (the
+            # Newton iterate's last bits depend on its start)
+            inner = sp.smooth_prims(mb.blocks[0].xcc.get(), mb.sim.ne)[g:-g, g:-g, g:-g]
+            field = np.pad(inner, [(g, g)] * 3 + [(0, 0)], mode="wrap")
+        prims = []
+        for blk in mb.blocks:
+            i, j = blk.nblki % mbdims[0], (blk.nblki // mbdims[0]) % mbdims[1]
+            ni, nj, nk = blk.ncell
+            prims.append(field[i * ni:(i + 1) * ni + 2 * g, j * nj:(j + 1) * nj + 2 * g, :].copy())
+        mb.set_prims(prims)
         for _ in range(3):
             mb.step(2e-6)
-        g = mb.ng
         whole = np.zeros((n, n, n, mb.sim.ne))
         for blk in mb.blocks:
             i, j = blk.nblki % mbdims[0], (blk.nblki // mbdims[0]) % mbdims[1]
             ni, nj, nk = blk.ncell
             whole[i * ni:(i + 1) * ni, j * nj:(j + 1) * nj, :] = blk.Q.get()[g:-g, g:-g, g:-g]
         results.append(whole)
-    # each cell's arithmetic is the same whichever block holds it; the halo's EOS starts from
-    # another T guess, within the Newton's rounding floor
-    assert _rowerr(results[1].transpose(3, 2, 1, 0), results[0].transpose(3, 2, 1, 0)) < 1e-10
+    diff = np.abs(results[1] - results[0]).max(axis=(0, 1, 2)) / np.abs(results[0]).max(axis=(0, 1, 2))
+    print(f"lattice 2x2x1 vs one block: max row-normalised difference {diff.max():.3e}")
+    # each cell's arithmetic is the same whichever block holds it, and so is each halo EOS's
+    # start: the same bits
+    assert np.array_equal(results[1], results[0])

@@ -70,10 +70,12 @@ def test_rates_are_mass_action(h2):
         rho = density(mix, Y, T, p)
         got = oc.production(rho, Y, T)
         # with ln T taken as the kernels take it (the table log), only rounding separates the forms:
-        # measured 1.0e-14 of the largest rate
+        # the Gibbs polynomials' terms cancel by six orders, and the oracle's fused Horner steps
+        # round apart from numpy's plain ones: measured 8.1e-11 of the largest rate (1.0e-14 when
+        # both take plain steps), the polynomials' own rounding floor carried into K
         want = production(mix, rho, Y, T, u=O.tlog(T))
         scale = np.abs(want).max()
-        assert np.abs(got - want).max() <= 1e-12 * scale
+        assert np.abs(got - want).max() <= 2e-10 * scale
         # with the library log the table log's error shows through the Gibbs polynomials (measured 5.7e-11)
         want = production(mix, rho, Y, T)
         assert np.abs(got - want).max() <= 1e-9 * np.abs(want).max()
