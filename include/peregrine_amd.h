@@ -60,7 +60,8 @@ typedef struct {
   int32_t limiter;   /* 0 none, 1 koren, 2 minmod, 3 mc, 4 superbee, 5 vanleer */
   int32_t primary;   /* 0 kepaec, 1 hllc, 2 rusanov */
   int32_t mechanism; /* -1 none, else the index of a compiled mechanism (pga_mechanism_find) */
-  int32_t reserved[10];
+  int32_t trans;     /* -1 none (Euler), 0 constant, 1 kinetic theory; Wilke mixing, Lewis diffusion */
+  int32_t reserved[9];
 } pga_case_desc;
 
 typedef struct pga_case pga_case;
@@ -138,25 +139,25 @@ int32_t pga_fill_periodic(pga_view A, int32_t ncomp, const int32_t *ncell, int32
 int32_t pga_copy_box(pga_view dst, const int32_t *dst_start, pga_view src, const int32_t *src_start,
                      const int32_t *extent, int32_t ncomp, void *stream);
 
-/* 1 when a fused stage is compiled for the case's variants, else 0 (use the per-kernel entry points) */
+/* 1 when the fused stage is compiled for the case's species count, else 0 (use the per-kernel
+ * entry points) */
 int32_t pga_stage_supported(const pga_case *c);
 
-/* One RK stage of a block's interior in one pass: the coupled source (a reacting case's rates,
- * from pga_rates, ns - 1 components per cell), the advective fluxes of all faces, their divergence, the RK combine and the EOS of
- * the result. Qin and stin hold the stage's input with its halo filled (pga_halo); Q0 the step's
- * start; the result goes to Qout and stout, which must not be Qin and stin. The combine is
- * out = sum_t w[t] x_t over nterms terms in order, x_t = Qin (src 0), Q0 (1) or the stage's dQ (2),
- * a term with withdt[t] taking w[t] * (*dt) * x_t. kchunk: the k-planes one workgroup marches.
+/* One RK stage of a block's interior in one pass after its three flux sweeps: the divergence of
+ * the six face fluxes (iF, jF, kF from pga_flux on Qin), the coupled source (a reacting case's
+ * rates from pga_rates, ns - 1 components per cell), the RK combine and the EOS of the result,
+ * its Newton started from stin's T. Q0 is the step's start. The combine is
+ * out = sum_t w[t] x_t over nterms terms in order, x_t = Qin (src 0), Q0 (1) or the stage's
+ * dQ (2), a term with withdt[t] taking w[t] * (*dt) * x_t; dQ is never stored. Each cell reads
+ * only itself, so Qout may be Qin and stout may be stin. kchunk: the k-planes one thread walks.
  * Replaces, for one stage, the reference's launches of pg_source
- * (chemint/kernels/source.mako:3-10), pg_flux x3 (euler/kernels/flux/single.mako:4-9),
- * pg_applyflux (advec/kernels/applyflux.mako:2-11), pg_combine (integrators/kernels/combine.mako)
- * and the interior pg_fromQ (euler/kernels/state/fromQ.mako:5-19), as sequenced by
- * multiblock/solver.py:246-284. */
+ * (chemint/kernels/source.mako:3-10), pg_applyflux (advec/kernels/applyflux.mako:2-11),
+ * pg_combine (integrators/kernels/combine.mako:1-13) and the interior pg_fromQ
+ * (euler/kernels/state/fromQ.mako:5-19), as sequenced by multiblock/solver.py:246-284. */
 int32_t pga_stage(const pga_case *c, pga_view Qin, pga_view stin, pga_view Q0, pga_view Qout, pga_view stout,
-                  pga_view iS, pga_view jS, pga_view kS, pga_view detJinv, pga_view rates, int32_t nterms,
-                  const int32_t *src,
-                  const double *w, const int32_t *withdt, const double *dt, pga_box box, int32_t kchunk,
-                  void *stream);
+                  pga_view iF, pga_view jF, pga_view kF, pga_view detJinv, pga_view rates, int32_t nterms,
+                  const int32_t *src, const double *w, const int32_t *withdt, const double *dt, pga_box box,
+                  int32_t kchunk, void *stream);
 
 /* The carried species' production rates (kg/m^3/s) of every cell in box into rates (ns - 1
  * components): the coupled source of a reacting stage. The body of pg_source
@@ -170,6 +171,30 @@ int32_t pga_rates(const pga_case *c, pga_view Q, pga_view state, pga_view rates,
 int32_t pga_halo(const pga_case *c, int32_t ntask, const pga_view *Qdst, const pga_view *Qsrc,
                  const int32_t *dst_start, const int32_t *src_start, const int32_t *extent, pga_view stdst,
                  pga_view stguess, void *stream);
+
+/* --- the viscous terms (Navier-Stokes) ---------------------------------------------------- */
+
+/* Per cell of up to 8 boxes: trstate = [mu, kappa, D_0 .. D_{ns-1}] by the case's transport model
+ * (constant species values or kinetic-theory fits, Wilke's rule, the conductivity mean, Lewis
+ * diffusivities) and hsp = the species enthalpies h_n(T) (J/kg).
+ * Replaces pg_transport: peregrine/simulator/navstokes/kernels/transport.mako:5-35. */
+int32_t pga_transport(const pga_case *c, pga_view Q, pga_view state, pga_view trstate, pga_view hsp,
+                      const pga_box *boxes, int32_t nboxes, void *stream);
+
+/* Per cell of box: grads[3 l + e] = d(row l)/dx_e for u, v, w, T and the carried Y, by central
+ * differences in index space mapped by Jinv[3 r + e] = d xi_r / d x_e (one halo layer read).
+ * Replaces pg_dqdxyz: peregrine/simulator/navstokes/kernels/dqdxyz.mako:16-41. */
+int32_t pga_grads(const pga_case *c, pga_view Q, pga_view state, pga_view Jinv, pga_view grads, pga_box box,
+                  void *stream);
+
+/* The alpha-damped viscous flux of the faces of one axis in box (face indices), added to F:
+ * Newtonian stress, Fourier heat, Fick species fluxes with the correction velocity, their
+ * enthalpy flux and the stress work; xcc the cell centres, xf the face centres.
+ * Replaces pg_flux of navier-stokes diff-flux alpha-damping:
+ * peregrine/simulator/navstokes/kernels/flux/alpha-damping.mako:3-99. */
+int32_t pga_diffflux(const pga_case *c, int32_t axis, double alpha, pga_view Q, pga_view state, pga_view trstate,
+                     pga_view hsp, pga_view grads, pga_view xcc, pga_view xf, pga_view S, pga_view F, pga_box box,
+                     void *stream);
 
 #ifdef __cplusplus
 }

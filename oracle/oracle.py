@@ -45,6 +45,9 @@ class Case(C.Structure):
         ("lfcpoly", _dp), ("tbdefeff", _dp), ("tbeffdev", _dp),
         ("logprefRu", C.c_double), ("prefRu", C.c_double), ("Rupref", C.c_double), ("logcfloor", C.c_double),
         ("expclamp", C.c_double), ("troeuc", C.c_double), ("troeuhinv", C.c_double),
+        ("trans", C.c_int), ("nmu", C.c_int), ("nkappa", C.c_int),
+        ("hpoly", _dp), ("mu0", _dp), ("kappa0", _dp), ("lewis", _dp), ("MWqinv", _dp), ("wilkepair", _dp),
+        ("mupoly", _dp), ("kappapoly", _dp), ("Ru", C.c_double),
     ]
 
 
@@ -53,6 +56,8 @@ class Block(C.Structure):
         ("ncell", C.c_int * 3), ("ng", C.c_int), ("regs", Arr * 4), ("nregs", C.c_int),
         ("state", Arr), ("iF", Arr), ("jF", Arr), ("kF", Arr), ("iS", Arr), ("jS", Arr), ("kS", Arr),
         ("detJinv", Arr), ("integrator", C.c_int), ("chemistry", C.c_int),
+        ("viscous", C.c_int), ("alpha", C.c_double),
+        ("trstate", Arr), ("hsp", Arr), ("grads", Arr), ("Jinv", Arr), ("xcc", Arr), ("xf", Arr * 3),
     ]
 
 
@@ -76,6 +81,9 @@ def _load():
     lib.pgo_source.argtypes = [P(Case), Arr, Arr, Arr, Box]
     lib.pgo_combine.argtypes = [C.c_int, Arr, C.c_int, C.c_double, C.c_int, P(Arr), _dp, C.c_double, C.c_int, Box]
     lib.pgo_fill_periodic.argtypes = [Arr, C.c_int, P(C.c_int), C.c_int, C.c_int, C.c_int]
+    lib.pgo_transport.argtypes = [P(Case), Arr, Arr, Arr, Arr, Box]
+    lib.pgo_grads.argtypes = [P(Case), Arr, Arr, Arr, Arr, Box]
+    lib.pgo_diffflux.argtypes = [P(Case), C.c_int, C.c_double, Arr, Arr, Arr, Arr, Arr, Arr, Arr, Arr, Arr, Box]
     lib.pgo_step.argtypes = [P(Case), P(Block), C.c_double]
     lib.pgo_stage.argtypes = [P(Case), P(Block), C.c_int, C.c_double]
     lib.pgo_advance.argtypes = [P(Case), P(Block), C.c_int, C.c_double]
@@ -139,6 +147,14 @@ class OracleCase:
         c.MW, c.MWinv, c.Rsp = d(k["MW"]), d(k["MWinv"]), d(k["Rsp"])
         c.cppolyR, c.h298R, c.gpoly, c.h298 = d(k["cppolyR"]), d(k["h298R"]), d(k["gpoly"]), d(k["h298"])
         c.cp0 = d(k.get("cp0", np.zeros(ns)))
+        c.hpoly, c.Ru = d(k["hpoly"]), k["Ru"]
+        c.trans = {None: -1, "constant": 0, "kinetic-theory": 1}[mixture.trans.name if mixture.trans else None]
+        if mixture.trans is not None:
+            c.lewis, c.MWqinv, c.wilkepair = d(k["lewis"]), d(k["MWqinv"]), d(k["wilkepair"])
+            c.mu0, c.kappa0 = d(k.get("mu0", np.zeros(ns))), d(k.get("kappa0", np.zeros(ns)))
+            mup, kap = k.get("mupoly", np.zeros((ns, 1))), k.get("kappapoly", np.zeros((ns, 1)))
+            c.nmu, c.nkappa = mup.shape[1], kap.shape[1]
+            c.mupoly, c.kappapoly = d(mup), d(kap)
         if mixture.chemistry is not None:
             records, sets, cols, sc = mixture.chemistry.tables(mixture.species)
             c.nr, c.ntb = len(records), len(sets)

@@ -687,6 +687,227 @@ void pgo_fill_periodic(pgo_arr A, int ncomp, const int *ncell, int ng, int depth
  * a whole stage and step: multiblock/solver.py:246-284,355-364,
  * simulator/advec/simulator.py:57-87, integrators/rungekutta.py:9-95
  * ---------------------------------------------------------------------- */
+/* ------------------------------------------------------------------------
+ * the viscous terms
+ * ---------------------------------------------------------------------- */
+
+/* mixrule/kernels/wilke.mako:3-17: phi_nm = (1 + s_n / s_m)^2 wilkepair(n, m), with
+ * s_n = sqrt(mu_n) MW_n^(-1/4) */
+static double viscmix(const pgo_case *c, const double *X, const double *sqrtmu) {
+  const int ns = c->ns;
+  double sinv[MAXNS];
+  for (int n = 0; n < ns; n++) sinv[n] = 1.0 / (sqrtmu[n] * c->MWqinv[n]);
+  double mu = 0.0;
+  for (int n = 0; n < ns; n++) {
+    const double sn = sqrtmu[n] * c->MWqinv[n];
+    double phi = 0.0;
+    for (int m = 0; m < ns; m++) {
+      const double r = 1.0 + sn * sinv[m];
+      phi += r * r * c->wilkepair[n * ns + m] * X[m];
+    }
+    mu += sqrtmu[n] * sqrtmu[n] * X[n] / phi;
+  }
+  return mu;
+}
+
+/* transport/kernels/mixing.mako:3-10: the mean of the series and parallel sums */
+static double kappamix(const pgo_case *c, const double *X, const double *kappasp) {
+  double series = 0.0, parallel = 0.0;
+  for (int n = 0; n < c->ns; n++) {
+    series += X[n] * kappasp[n];
+    parallel += X[n] / kappasp[n];
+  }
+  return 0.5 * (series + 1.0 / parallel);
+}
+
+/* navstokes/kernels/transport.mako:5-35 with transport/kernels/{constant,kinetic,dilute}.mako,
+ * diffusion/kernels/lewis.mako:3-7 and the ideal hspecies (eos/kernels/species.mako:29-36).
+ * ln T is taken by the EOS's log (pgo_elog), as everywhere in this path. */
+void pgo_transport(const pgo_case *c, pgo_arr Q, pgo_arr state, pgo_arr trstate, pgo_arr hsp, pgo_box r) {
+  const int ns = c->ns;
+  FOR_BOX(r) {
+    const double rho = *at(Q, i, j, k, 0), rhoinv = 1.0 / rho;
+    double Y[MAXNS];
+    double Ylast = 1.0;
+    for (int n = 0; n < ns - 1; n++) {
+      Y[n] = *at(Q, i, j, k, 5 + n) * rhoinv;
+      Ylast -= *at(Q, i, j, k, 5 + n) * rhoinv;
+    }
+    Y[ns - 1] = fmax(Ylast, 0.0);
+    /* eos/kernels/mixing.mako:34-39 molmass */
+    double denom = 0.0;
+    for (int n = 0; n < ns; n++) denom += Y[n] * c->MWinv[n];
+    const double MWmix = 1.0 / denom;
+    double X[MAXNS];
+    for (int n = 0; n < ns; n++) X[n] = Y[n] * c->MWinv[n] * MWmix;
+    const double T = *at(state, i, j, k, 1), lnT = pgo_elog(T);
+    double sqrtmu[MAXNS], kappasp[MAXNS];
+    if (c->trans == 0) {
+      for (int n = 0; n < ns; n++) sqrtmu[n] = sqrt(c->mu0[n]), kappasp[n] = c->kappa0[n];
+    } else {
+      /* dilute.mako:5-26: sqrt(mu) = fit(ln T) T^(1/4), kappa = fit(ln T) sqrt(T) */
+      const double sqrtT = sqrt(T), sqrtsqrtT = sqrt(sqrtT);
+      for (int n = 0; n < ns; n++) {
+        double fit = 0.0;
+        for (int m = c->nmu - 1; m >= 0; m--) fit = fit * lnT + c->mupoly[n * c->nmu + m];
+        sqrtmu[n] = fit * sqrtsqrtT;
+      }
+      for (int n = 0; n < ns; n++) {
+        double fit = 0.0;
+        for (int m = c->nkappa - 1; m >= 0; m--) fit = fit * lnT + c->kappapoly[n * c->nkappa + m];
+        kappasp[n] = fit * sqrtT;
+      }
+    }
+    const double mu = viscmix(c, X, sqrtmu);
+    const double kappa = kappamix(c, X, kappasp);
+    *at(trstate, i, j, k, 0) = mu;
+    *at(trstate, i, j, k, 1) = kappa;
+    const double kappaoverrhocp = kappa * rhoinv / *at(state, i, j, k, 5);
+    for (int n = 0; n < ns; n++) *at(trstate, i, j, k, 2 + n) = kappaoverrhocp / c->lewis[n];
+    const double Tinv = 1.0 / T;
+    for (int n = 0; n < ns; n++) {
+      double hRT = 0.0;
+      for (int m = c->nt - 1; m >= 0; m--) hRT = fma(hRT, lnT, c->hpoly[n * c->nt + m]);
+      hRT += c->h298[n] * Tinv;
+      *at(hsp, i, j, k, n) = hRT * T * c->Ru * c->MWinv[n];
+    }
+  }
+}
+
+/* navstokes/kernels/dqdxyz.mako:16-41: central differences in index space mapped by Jinv;
+ * a velocity or mass fraction as the difference of two quotients taken as one division */
+void pgo_grads(const pgo_case *c, pgo_arr Q, pgo_arr state, pgo_arr Jinv, pgo_arr grads, pgo_box r) {
+  const int ngrad = c->ne - 1;
+  FOR_BOX(r) {
+    const int off[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    double rp[3], rm[3], inv[3];
+    for (int a = 0; a < 3; a++) {
+      rp[a] = *at(Q, i + off[a][0], j + off[a][1], k + off[a][2], 0);
+      rm[a] = *at(Q, i - off[a][0], j - off[a][1], k - off[a][2], 0);
+      inv[a] = 1.0 / (rp[a] * rm[a]);
+    }
+    double J[3][3];
+    for (int a = 0; a < 3; a++)
+      for (int e = 0; e < 3; e++) J[a][e] = *at(Jinv, i, j, k, 3 * a + e);
+    for (int l = 0; l < ngrad; l++) {
+      double d[3];
+      for (int a = 0; a < 3; a++) {
+        const int ip = i + off[a][0], jp = j + off[a][1], kp = k + off[a][2];
+        const int im = i - off[a][0], jm = j - off[a][1], km = k - off[a][2];
+        if (l == 3) {
+          d[a] = 0.5 * (*at(state, ip, jp, kp, 1) - *at(state, im, jm, km, 1));
+        } else {
+          const int m = l + 1;
+          d[a] = 0.5 * (*at(Q, ip, jp, kp, m) * rm[a] - *at(Q, im, jm, km, m) * rp[a]) * inv[a];
+        }
+      }
+      for (int e = 0; e < 3; e++) *at(grads, i, j, k, 3 * l + e) = d[0] * J[0][e] + d[1] * J[1][e] + d[2] * J[2][e];
+    }
+  }
+}
+
+/* navstokes/kernels/flux/alpha-damping.mako:3-99 (Nishikawa's alpha damping): each face
+ * gradient is the sides' mean damped toward the jump between the values each side extrapolates
+ * to the face; Newtonian stress (lambda = -2 mu / 3), Fourier heat, Fick species fluxes with the
+ * correction velocity, their enthalpy flux and the stress work, added to F */
+void pgo_diffflux(const pgo_case *c, int axis, double alpha, pgo_arr Q, pgo_arr state, pgo_arr trstate,
+                  pgo_arr hsp, pgo_arr grads, pgo_arr xcc, pgo_arr xf, pgo_arr S, pgo_arr F, pgo_box r) {
+  const int ns = c->ns;
+  const int dl[3] = {axis == 0, axis == 1, axis == 2};
+  FOR_BOX(r) {
+    const int iL = i - dl[0], jL = j - dl[1], kL = k - dl[2];
+    const double bulkvisc = 0.0;
+    const double Sx = *at(S, i, j, k, 0), Sy = *at(S, i, j, k, 1), Sz = *at(S, i, j, k, 2);
+    const double Sabs = sqrt(Sx * Sx + Sy * Sy + Sz * Sz), Sinv = 1.0 / Sabs;
+    const double nx = Sx * Sinv, ny = Sy * Sinv, nz = Sz * Sinv;
+    const double mu = 0.5 * (*at(trstate, i, j, k, 0) + *at(trstate, iL, jL, kL, 0));
+    const double kappa = 0.5 * (*at(trstate, i, j, k, 1) + *at(trstate, iL, jL, kL, 1));
+    const double lambda = bulkvisc - 2.0 / 3.0 * mu;
+    const double rhoinvL = *at(state, iL, jL, kL, 2), rhoinvR = *at(state, i, j, k, 2);
+    const double uL = *at(Q, iL, jL, kL, 1) * rhoinvL, vL = *at(Q, iL, jL, kL, 2) * rhoinvL,
+                 wL = *at(Q, iL, jL, kL, 3) * rhoinvL;
+    const double uR = *at(Q, i, j, k, 1) * rhoinvR, vR = *at(Q, i, j, k, 2) * rhoinvR, wR = *at(Q, i, j, k, 3) * rhoinvR;
+    const double ex = *at(xcc, i, j, k, 0) - *at(xcc, iL, jL, kL, 0);
+    const double ey = *at(xcc, i, j, k, 1) - *at(xcc, iL, jL, kL, 1);
+    const double ez = *at(xcc, i, j, k, 2) - *at(xcc, iL, jL, kL, 2);
+    const double en0 = ex * nx, en1 = ey * ny, en2 = ez * nz;
+    const double dampinv = alpha / sqrt(en0 * en0 + en1 * en1 + en2 * en2);
+    const double dampx = dampinv * nx, dampy = dampinv * ny, dampz = dampinv * nz;
+    const double xLx = *at(xf, i, j, k, 0) - *at(xcc, iL, jL, kL, 0), xLy = *at(xf, i, j, k, 1) - *at(xcc, iL, jL, kL, 1),
+                 xLz = *at(xf, i, j, k, 2) - *at(xcc, iL, jL, kL, 2);
+    const double xRx = *at(xf, i, j, k, 0) - *at(xcc, i, j, k, 0), xRy = *at(xf, i, j, k, 1) - *at(xcc, i, j, k, 1),
+                 xRz = *at(xf, i, j, k, 2) - *at(xcc, i, j, k, 2);
+#define FACEGRAD(l, phiL, phiR, gx, gy, gz)                                                                  \
+  do {                                                                                                       \
+    const double gLx = *at(grads, iL, jL, kL, 3 * (l)), gLy = *at(grads, iL, jL, kL, 3 * (l) + 1),           \
+                 gLz = *at(grads, iL, jL, kL, 3 * (l) + 2);                                                  \
+    const double gRx = *at(grads, i, j, k, 3 * (l)), gRy = *at(grads, i, j, k, 3 * (l) + 1),                 \
+                 gRz = *at(grads, i, j, k, 3 * (l) + 2);                                                     \
+    const double wl = (phiL) + gLx * xLx + gLy * xLy + gLz * xLz;                                            \
+    const double wr = (phiR) + gRx * xRx + gRy * xRy + gRz * xRz;                                            \
+    const double jump = wr - wl;                                                                             \
+    gx = 0.5 * (gRx + gLx) + dampx * jump;                                                                   \
+    gy = 0.5 * (gRy + gLy) + dampy * jump;                                                                   \
+    gz = 0.5 * (gRz + gLz) + dampz * jump;                                                                   \
+  } while (0)
+    double dudx, dudy, dudz, dvdx, dvdy, dvdz, dwdx, dwdy, dwdz;
+    FACEGRAD(0, uL, uR, dudx, dudy, dudz);
+    FACEGRAD(1, vL, vR, dvdx, dvdy, dvdz);
+    FACEGRAD(2, wL, wR, dwdx, dwdy, dwdz);
+    const double div = dudx + dvdy + dwdz;
+    const double txx = -2.0 * mu * dudx - lambda * div;
+    const double txy = -mu * (dvdx + dudy);
+    const double txz = -mu * (dwdx + dudz);
+    const double tyy = -2.0 * mu * dvdy - lambda * div;
+    const double tyz = -mu * (dwdy + dvdz);
+    const double tzz = -2.0 * mu * dwdz - lambda * div;
+    *at(F, i, j, k, 1) += txx * Sx + txy * Sy + txz * Sz;
+    *at(F, i, j, k, 2) += txy * Sx + tyy * Sy + tyz * Sz;
+    *at(F, i, j, k, 3) += txz * Sx + tyz * Sy + tzz * Sz;
+    double dTx, dTy, dTz;
+    FACEGRAD(3, *at(state, iL, jL, kL, 1), *at(state, i, j, k, 1), dTx, dTy, dTz);
+    const double heatflux = -kappa * (dTx * Sx + dTy * Sy + dTz * Sz);
+    const double uf = 0.5 * (uR + uL), vf = 0.5 * (vR + vL), wf = 0.5 * (wR + wL);
+    double f4 = (uf * txx + vf * txy + wf * txz) * Sx + (uf * txy + vf * tyy + wf * tyz) * Sy +
+                (uf * txz + vf * tyz + wf * tzz) * Sz + heatflux;
+    double YL[MAXNS], YR[MAXNS], YlastL = 1.0, YlastR = 1.0;
+    for (int n = 0; n < ns - 1; n++) {
+      YL[n] = *at(Q, iL, jL, kL, 5 + n) * rhoinvL;
+      YR[n] = *at(Q, i, j, k, 5 + n) * rhoinvR;
+      YlastL -= *at(Q, iL, jL, kL, 5 + n) * rhoinvL;
+      YlastR -= *at(Q, i, j, k, 5 + n) * rhoinvR;
+    }
+    (void)YlastL;
+    (void)YlastR;
+    double Dk, Vc = 0.0, gradYns = 0.0, sumYh = 0.0, Yns = 1.0;
+    const double rho = 0.5 * (*at(Q, i, j, k, 0) + *at(Q, iL, jL, kL, 0));
+    for (int n = 0; n < ns - 1; n++) {
+      Dk = 0.5 * (*at(trstate, i, j, k, 2 + n) + *at(trstate, iL, jL, kL, 2 + n));
+      double dYx, dYy, dYz;
+      FACEGRAD(4 + n, YL[n], YR[n], dYx, dYy, dYz);
+      const double gradYk = dYx * Sx + dYy * Sy + dYz * Sz;
+      gradYns -= gradYk;
+      Vc += Dk * gradYk;
+      const double Jk = -rho * Dk * gradYk;
+      const double hk = 0.5 * (*at(hsp, i, j, k, n) + *at(hsp, iL, jL, kL, n));
+      const double Yk = 0.5 * (YR[n] + YL[n]);
+      Yns -= Yk;
+      sumYh += Yk * hk;
+      *at(F, i, j, k, 5 + n) += Jk;
+      f4 += Jk * hk;
+    }
+#undef FACEGRAD
+    Dk = 0.5 * (*at(trstate, i, j, k, 2 + ns - 1) + *at(trstate, iL, jL, kL, 2 + ns - 1));
+    Vc += Dk * gradYns;
+    const double corr = rho * Vc;
+    for (int n = 0; n < ns - 1; n++) *at(F, i, j, k, 5 + n) += 0.5 * (YR[n] + YL[n]) * corr;
+    f4 += corr * sumYh;
+    const double hns = 0.5 * (*at(hsp, i, j, k, ns - 1) + *at(hsp, iL, jL, kL, ns - 1));
+    f4 += (-rho * Dk * gradYns + Yns * corr) * hns;
+    *at(F, i, j, k, 4) += f4;
+  }
+}
+
 static pgo_box interior(const pgo_block *b) {
   pgo_box r;
   for (int a = 0; a < 3; a++) {
@@ -702,21 +923,43 @@ static pgo_box facebox(const pgo_block *b, int axis) {
   return r;
 }
 
+/* the halo slab of one interface, depth planes deep, over the interior of the other axes */
+static pgo_box haloslab(const pgo_block *b, int a, int side, int depth) {
+  pgo_box r = interior(b);
+  r.start[a] = side ? b->ng + b->ncell[a] : b->ng - depth;
+  r.extent[a] = depth;
+  return r;
+}
+
+/* simulator/advec/simulator.py:57-75: the exchange, the halo EOS, the source; for a viscous
+ * case the transport over the interior and the first halo layer ("shallow",
+ * launch/ranges.py:52-53; advecdiff/simulator.py postbc) */
 static void prerhs(const pgo_case *c, pgo_block *b, pgo_arr Q, pgo_arr dQ, int withdQ) {
   for (int a = 0; a < 3; a++) pgo_fill_periodic(Q, c->ne, b->ncell, b->ng, b->ng, a);
   for (int a = 0; a < 3; a++)
-    for (int side = 0; side < 2; side++) {
-      pgo_box r = interior(b);
-      r.start[a] = side ? b->ng + b->ncell[a] : 0;
-      r.extent[a] = b->ng;
-      pgo_fromQ(c, Q, b->state, r);
-    }
+    for (int side = 0; side < 2; side++) pgo_fromQ(c, Q, b->state, haloslab(b, a, side, b->ng));
   if (withdQ && b->chemistry) pgo_source(c, Q, b->state, dQ, interior(b));
+  if (b->viscous) {
+    pgo_transport(c, Q, b->state, b->trstate, b->hsp, interior(b));
+    for (int a = 0; a < 3; a++)
+      for (int side = 0; side < 2; side++) pgo_transport(c, Q, b->state, b->trstate, b->hsp, haloslab(b, a, side, 1));
+  }
 }
 
+/* advec/simulator.py:76-80; advecdiff/simulator.py:20-35: gradients over the interior and their
+ * one-plane exchange, the advective flux, the viscous flux added to it, the divergence */
 static void rhs(const pgo_case *c, pgo_block *b, pgo_arr Q, pgo_arr dQ) {
   pgo_arr F[3] = {b->iF, b->jF, b->kF}, S[3] = {b->iS, b->jS, b->kS};
-  for (int a = 0; a < 3; a++) pgo_flux(c, a, Q, b->state, S[a], F[a], facebox(b, a));
+  if (b->viscous) {
+    pgo_grads(c, Q, b->state, b->Jinv, b->grads, interior(b));
+    for (int a = 0; a < 3; a++) pgo_fill_periodic(b->grads, 3 * (c->ne - 1), b->ncell, b->ng, 1, a);
+  }
+  for (int a = 0; a < 3; a++) {
+    pgo_flux(c, a, Q, b->state, S[a], F[a], facebox(b, a));
+    if (b->viscous)
+      pgo_diffflux(c, a, b->alpha, Q, b->state, b->trstate, b->hsp, b->grads, b->xcc, b->xf[a], S[a], F[a],
+                   facebox(b, a));
+  }
   pgo_applyflux(c->ne, dQ, b->iF, b->jF, b->kF, b->detJinv, b->chemistry, interior(b));
 }
 

@@ -42,6 +42,10 @@ typedef struct {
   const double *logA, *b, *EaR, *logprA, *prb, *prEaR, *nusum;
   const double *fwdorder, *reactnu, *prodnu, *netnu, *netnuMW, *lfcpoly, *tbdefeff, *tbeffdev;
   double logprefRu, prefRu, Rupref, logcfloor, expclamp, troeuc, troeuhinv;
+  /* transport: -1 none (Euler), 0 constant, 1 kinetic theory; Wilke mixing, Lewis diffusion */
+  int trans, nmu, nkappa;
+  const double *hpoly, *mu0, *kappa0, *lewis, *MWqinv, *wilkepair, *mupoly, *kappapoly;
+  double Ru;
 } pgo_case;
 
 double pgo_tlog(double x);
@@ -58,6 +62,13 @@ void pgo_combine(int ne, pgo_arr out, int own, double wout, int nterms, const pg
                  double dt, int step, pgo_box r);
 void pgo_fill_periodic(pgo_arr A, int ncomp, const int *ncell, int ng, int depth, int axis);
 
+/* the viscous terms: transport state (mu, kappa, D_n) and species enthalpies per cell; the
+ * cell gradients of u, v, w, T and the carried Y; the alpha-damped viscous face flux, added to F */
+void pgo_transport(const pgo_case *c, pgo_arr Q, pgo_arr state, pgo_arr trstate, pgo_arr hsp, pgo_box r);
+void pgo_grads(const pgo_case *c, pgo_arr Q, pgo_arr state, pgo_arr Jinv, pgo_arr grads, pgo_box r);
+void pgo_diffflux(const pgo_case *c, int axis, double alpha, pgo_arr Q, pgo_arr state, pgo_arr trstate,
+                  pgo_arr hsp, pgo_arr grads, pgo_arr xcc, pgo_arr xf, pgo_arr S, pgo_arr F, pgo_box r);
+
 /* one whole step of a single periodic block: the stage sequence of the reference's solver */
 typedef struct {
   int ncell[3], ng;
@@ -66,6 +77,10 @@ typedef struct {
   pgo_arr state, iF, jF, kF, iS, jS, kS, detJinv;
   int integrator; /* 0 rk1, 1 rk2, 2 tvd-rk3, 3 rk4 */
   int chemistry;  /* 1: the coupled source begins dQ */
+  /* Navier-Stokes: the viscous arrays and the damping coefficient */
+  int viscous;
+  double alpha;
+  pgo_arr trstate, hsp, grads, Jinv, xcc, xf[3];
 } pgo_block;
 
 void pgo_step(const pgo_case *c, pgo_block *b, double dt);

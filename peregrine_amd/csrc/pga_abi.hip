@@ -48,6 +48,8 @@ extern "C" int32_t pga_const_offset(int32_t ns, const char *name, int32_t *count
       {"MW", o.MW, ns},           {"MWinv", o.MWinv, ns},     {"Rsp", o.Rsp, ns},   {"cppolyR", o.cppolyR, ns * NT},
       {"h298R", o.h298R, ns},     {"cp0", o.cp0, ns},         {"gpoly", o.gpoly, ns * NG}, {"h298", o.h298, ns},
       {"mu0", o.mu0, ns},         {"kappa0", o.kappa0, ns},   {"lewis", o.lewis, ns},
+      {"hpoly", o.hpoly, ns * NT}, {"MWqinv", o.MWqinv, ns},  {"wilkepair", o.wilkepair, ns * ns},
+      {"mupoly", o.mupoly, ns * NT}, {"kappapoly", o.kappapoly, ns * NT},
   };
   for (auto &r : rows)
     if (!strcmp(r.n, name)) {
@@ -86,6 +88,10 @@ extern "C" int32_t pga_case_create(const pga_case_desc *desc, const double *dev_
   }
   if (desc->eos < 0 || desc->eos > 1) {
     set_error("pga_case_create: no eos %d (0 cpg, 1 tpg)", desc->eos);
+    return PGA_EUNSUP;
+  }
+  if (desc->trans < -1 || desc->trans > 1) {
+    set_error("pga_case_create: no transport model %d (-1 none, 0 constant, 1 kinetic theory)", desc->trans);
     return PGA_EUNSUP;
   }
   if (desc->mechanism >= 0) {

@@ -16,12 +16,15 @@ namespace pga {
 constexpr int NT = 7;
 constexpr int NG = 8;
 constexpr int MAXNS = 16;
+// J/kmol/K, the universal gas constant as the mixture tables state it
+constexpr double Ru = 8314.46261815324;
 
 // ---------------------------------------------------------------------------
 // the case constant table: one flat array of doubles, sections per species
 // ---------------------------------------------------------------------------
 struct Offsets {
-  int MW, MWinv, Rsp, cppolyR, h298R, cp0, gpoly, h298, mu0, kappa0, lewis, size;
+  int MW, MWinv, Rsp, cppolyR, h298R, cp0, gpoly, h298, mu0, kappa0, lewis, hpoly, MWqinv, wilkepair, mupoly,
+      kappapoly, size;
 };
 
 __host__ __device__ constexpr Offsets offsets(int ns) {
@@ -37,7 +40,12 @@ __host__ __device__ constexpr Offsets offsets(int ns) {
   o.mu0 = o.h298 + ns;
   o.kappa0 = o.mu0 + ns;
   o.lewis = o.kappa0 + ns;
-  o.size = o.lewis + ns;
+  o.hpoly = o.lewis + ns;
+  o.MWqinv = o.hpoly + ns * NT;
+  o.wilkepair = o.MWqinv + ns;
+  o.mupoly = o.wilkepair + ns * ns;
+  o.kappapoly = o.mupoly + ns * NT;
+  o.size = o.kappapoly + ns * NT;
   return o;
 }
 
@@ -55,6 +63,16 @@ struct Gas {
   __device__ __forceinline__ double cp0(int n) const { return K[O.cp0 + n]; }
   __device__ __forceinline__ double gpoly(int n, int m) const { return K[O.gpoly + n * NG + m]; }
   __device__ __forceinline__ double h298(int n) const { return K[O.h298 + n]; }
+  // transport: constant mu and kappa, Lewis numbers, Wilke's MW^-1/4 and pair factors, the
+  // kinetic-theory fits of sqrt(mu) / T^1/4 and kappa / sqrt(T) in ln T; the enthalpy fits
+  __device__ __forceinline__ double mu0(int n) const { return K[O.mu0 + n]; }
+  __device__ __forceinline__ double kappa0(int n) const { return K[O.kappa0 + n]; }
+  __device__ __forceinline__ double lewis(int n) const { return K[O.lewis + n]; }
+  __device__ __forceinline__ double hpoly(int n, int m) const { return K[O.hpoly + n * NT + m]; }
+  __device__ __forceinline__ double MWqinv(int n) const { return K[O.MWqinv + n]; }
+  __device__ __forceinline__ double wilkepair(int n, int m) const { return K[O.wilkepair + n * NS + m]; }
+  __device__ __forceinline__ double mupoly(int n, int m) const { return K[O.mupoly + n * NT + m]; }
+  __device__ __forceinline__ double kappapoly(int n, int m) const { return K[O.kappapoly + n * NT + m]; }
 };
 
 // ---------------------------------------------------------------------------

@@ -1,0 +1,368 @@
+// The viscous terms of a Navier-Stokes stage: the transport state and species enthalpies of
+// each cell, the cell gradients of u, v, w, T and the carried mass fractions, and the
+// alpha-damped viscous face flux added to the advective one.
+//
+// Layout: trstate [mu, kappa, D_0 .. D_{ns-1}], hsp [h_0 .. h_{ns-1}] (J/kg), grads
+// [3 * l + e] = d(row l)/dx_e for the rows u, v, w, T, Y_0 .. Y_{ns-2}, Jinv [3 * r + e] =
+// d xi_r / d x_e, xcc and the face centres [x, y, z].
+//
+// Arithmetic follows the oracle operation for operation (contraction off, ln T by elog), so a
+// face's flux is the oracle's to the bit.
+#include "pga_eos.hpp"
+#include "pga_internal.hpp"
+
+namespace pga {
+
+enum { TRANS_CONSTANT = 0, TRANS_KINETIC = 1 };
+
+// navstokes/kernels/transport.mako:5-35: mu and kappa by Wilke's rule and the conductivity mean
+// (mixrule/kernels/wilke.mako:3-17, transport/kernels/mixing.mako:3-10) from constant species
+// values (transport/kernels/constant.mako:5-16) or the kinetic-theory fits
+// (transport/kernels/dilute.mako:5-26); D_n by the Lewis numbers (diffusion/kernels/lewis.mako:3-7);
+// h_n ideal (eos/kernels/species.mako:29-36)
+template <int NS, int TRANS>
+__device__ __forceinline__ void transport_cell(const Gas<NS> &g, const View &Q, const View &st, int64_t o,
+                                               const View &tr, const View &hsp, int64_t ot, int64_t oh) {
+#pragma clang fp contract(off)
+  const double rho = Q.at(o, 0), rhoinv = 1.0 / rho;
+  double Y[NS];
+  double Ylast = 1.0;
+#pragma unroll
+  for (int n = 0; n < NS - 1; n++) {
+    const double q = Q.at(o, 5 + n);
+    Y[n] = q * rhoinv;
+    Ylast -= q * rhoinv;
+  }
+  Y[NS - 1] = fmax(Ylast, 0.0);
+  double denom = 0.0;
+#pragma unroll
+  for (int n = 0; n < NS; n++) denom += Y[n] * g.MWinv(n);
+  const double MWmix = 1.0 / denom;
+  double X[NS];
+#pragma unroll
+  for (int n = 0; n < NS; n++) X[n] = Y[n] * g.MWinv(n) * MWmix;
+  const double T = st.at(o, 1), cp = st.at(o, 5), lnT = elog(T);
+  double sqrtmu[NS], kappasp[NS];
+  if constexpr (TRANS == TRANS_CONSTANT) {
+#pragma unroll
+    for (int n = 0; n < NS; n++) sqrtmu[n] = sqrt(g.mu0(n)), kappasp[n] = g.kappa0(n);
+  } else {
+    const double sqrtT = sqrt(T), sqrtsqrtT = sqrt(sqrtT);
+#pragma unroll
+    for (int n = 0; n < NS; n++) {
+      double fit = 0.0;
+#pragma unroll
+      for (int m = NT - 1; m >= 0; m--) fit = fit * lnT + g.mupoly(n, m);
+      sqrtmu[n] = fit * sqrtsqrtT;
+    }
+#pragma unroll
+    for (int n = 0; n < NS; n++) {
+      double fit = 0.0;
+#pragma unroll
+      for (int m = NT - 1; m >= 0; m--) fit = fit * lnT + g.kappapoly(n, m);
+      kappasp[n] = fit * sqrtT;
+    }
+  }
+  double sinv[NS];
+#pragma unroll
+  for (int n = 0; n < NS; n++) sinv[n] = 1.0 / (sqrtmu[n] * g.MWqinv(n));
+  double mu = 0.0;
+#pragma unroll
+  for (int n = 0; n < NS; n++) {
+    const double sn = sqrtmu[n] * g.MWqinv(n);
+    double phi = 0.0;
+#pragma unroll
+    for (int m = 0; m < NS; m++) {
+      const double r = 1.0 + sn * sinv[m];
+      phi += r * r * g.wilkepair(n, m) * X[m];
+    }
+    mu += sqrtmu[n] * sqrtmu[n] * X[n] / phi;
+  }
+  double series = 0.0, parallel = 0.0;
+#pragma unroll
+  for (int n = 0; n < NS; n++) {
+    series += X[n] * kappasp[n];
+    parallel += X[n] / kappasp[n];
+  }
+  const double kappa = 0.5 * (series + 1.0 / parallel);
+  tr.at(ot, 0) = mu;
+  tr.at(ot, 1) = kappa;
+  const double kappaoverrhocp = kappa * rhoinv / cp;
+#pragma unroll
+  for (int n = 0; n < NS; n++) tr.at(ot, 2 + n) = kappaoverrhocp / g.lewis(n);
+  const double Tinv = 1.0 / T;
+#pragma unroll
+  for (int n = 0; n < NS; n++) {
+    double hRT = 0.0;
+#pragma unroll
+    for (int m = NT - 1; m >= 0; m--) hRT = fma(hRT, lnT, g.hpoly(n, m));
+    hRT += g.h298(n) * Tinv;
+    hsp.at(oh, n) = hRT * T * Ru * g.MWinv(n);
+  }
+}
+
+template <int NS, int TRANS>
+__global__ __launch_bounds__(256) void k_transport(const double *K, View Q, View st, View tr, View hsp, Box3 b) {
+  int i, j, k;
+  if (!cell_of(b, i, j, k)) return;
+  const Gas<NS> g{K};
+  transport_cell<NS, TRANS>(g, Q, st, Q.off(i, j, k), tr, hsp, tr.off(i, j, k), hsp.off(i, j, k));
+}
+
+// navstokes/kernels/dqdxyz.mako:16-41
+template <int NS>
+__global__ __launch_bounds__(256) void k_grads(View Q, View st, View Jinv, View gr, Box3 b) {
+#pragma clang fp contract(off)
+  constexpr int NGRAD = NS + 3;
+  int i, j, k;
+  if (!cell_of(b, i, j, k)) return;
+  const int64_t o = Q.off(i, j, k);
+  const int64_t d[3] = {1, Q.sj, Q.sk};
+  double rp[3], rm[3], inv[3];
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    rp[a] = Q.at(o + d[a], 0);
+    rm[a] = Q.at(o - d[a], 0);
+    inv[a] = 1.0 / (rp[a] * rm[a]);
+  }
+  const int64_t oj = Jinv.off(i, j, k), og = gr.off(i, j, k);
+  double J[3][3];
+#pragma unroll
+  for (int a = 0; a < 3; a++)
+#pragma unroll
+    for (int e = 0; e < 3; e++) J[a][e] = Jinv.at(oj, 3 * a + e);
+#pragma unroll
+  for (int l = 0; l < NGRAD; l++) {
+    double dd[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      if (l == 3) {
+        dd[a] = 0.5 * (st.at(o + d[a], 1) - st.at(o - d[a], 1));
+      } else {
+        const int m = l + 1;
+        dd[a] = 0.5 * (Q.at(o + d[a], m) * rm[a] - Q.at(o - d[a], m) * rp[a]) * inv[a];
+      }
+    }
+#pragma unroll
+    for (int e = 0; e < 3; e++) gr.at(og, 3 * l + e) = dd[0] * J[0][e] + dd[1] * J[1][e] + dd[2] * J[2][e];
+  }
+}
+
+struct DiffArgs {
+  View Q, st, tr, hsp, gr, xcc, xf, S, F;
+  double alpha;
+  Box3 b;
+};
+
+// navstokes/kernels/flux/alpha-damping.mako:3-99, added to F
+template <int NS, int AXIS>
+__global__ __launch_bounds__(256) void k_diffflux(DiffArgs a) {
+#pragma clang fp contract(off)
+  int i, j, k;
+  if (!cell_of(a.b, i, j, k)) return;
+  const View Q = a.Q, st = a.st, tr = a.tr, hsp = a.hsp, gr = a.gr, xcc = a.xcc, xf = a.xf, S = a.S, F = a.F;
+  const int64_t oR = Q.off(i, j, k), oL = oR - (AXIS == 0 ? 1 : AXIS == 1 ? Q.sj : Q.sk);
+  const int64_t ofa = S.off(i, j, k), off_ = F.off(i, j, k), oxf = xf.off(i, j, k);
+  const double alpha = a.alpha, bulkvisc = 0.0;
+  const double Sx = S.at(ofa, 0), Sy = S.at(ofa, 1), Sz = S.at(ofa, 2);
+  const double Sabs = sqrt(Sx * Sx + Sy * Sy + Sz * Sz), Sinv = 1.0 / Sabs;
+  const double nx = Sx * Sinv, ny = Sy * Sinv, nz = Sz * Sinv;
+  const double mu = 0.5 * (tr.at(oR, 0) + tr.at(oL, 0));
+  const double kappa = 0.5 * (tr.at(oR, 1) + tr.at(oL, 1));
+  const double lambda = bulkvisc - 2.0 / 3.0 * mu;
+  const double rhoinvL = st.at(oL, 2), rhoinvR = st.at(oR, 2);
+  const double uL = Q.at(oL, 1) * rhoinvL, vL = Q.at(oL, 2) * rhoinvL, wL = Q.at(oL, 3) * rhoinvL;
+  const double uR = Q.at(oR, 1) * rhoinvR, vR = Q.at(oR, 2) * rhoinvR, wR = Q.at(oR, 3) * rhoinvR;
+  const double ex = xcc.at(oR, 0) - xcc.at(oL, 0), ey = xcc.at(oR, 1) - xcc.at(oL, 1), ez = xcc.at(oR, 2) - xcc.at(oL, 2);
+  const double en0 = ex * nx, en1 = ey * ny, en2 = ez * nz;
+  const double dampinv = alpha / sqrt(en0 * en0 + en1 * en1 + en2 * en2);
+  const double dampx = dampinv * nx, dampy = dampinv * ny, dampz = dampinv * nz;
+  const double xLx = xf.at(oxf, 0) - xcc.at(oL, 0), xLy = xf.at(oxf, 1) - xcc.at(oL, 1), xLz = xf.at(oxf, 2) - xcc.at(oL, 2);
+  const double xRx = xf.at(oxf, 0) - xcc.at(oR, 0), xRy = xf.at(oxf, 1) - xcc.at(oR, 1), xRz = xf.at(oxf, 2) - xcc.at(oR, 2);
+  const auto facegrad = [&](int l, double phiL, double phiR, double &gx, double &gy, double &gz) {
+    const double gLx = gr.at(oL, 3 * l), gLy = gr.at(oL, 3 * l + 1), gLz = gr.at(oL, 3 * l + 2);
+    const double gRx = gr.at(oR, 3 * l), gRy = gr.at(oR, 3 * l + 1), gRz = gr.at(oR, 3 * l + 2);
+    const double wl = phiL + gLx * xLx + gLy * xLy + gLz * xLz;
+    const double wr = phiR + gRx * xRx + gRy * xRy + gRz * xRz;
+    const double jump = wr - wl;
+    gx = 0.5 * (gRx + gLx) + dampx * jump;
+    gy = 0.5 * (gRy + gLy) + dampy * jump;
+    gz = 0.5 * (gRz + gLz) + dampz * jump;
+  };
+  double dudx, dudy, dudz, dvdx, dvdy, dvdz, dwdx, dwdy, dwdz;
+  facegrad(0, uL, uR, dudx, dudy, dudz);
+  facegrad(1, vL, vR, dvdx, dvdy, dvdz);
+  facegrad(2, wL, wR, dwdx, dwdy, dwdz);
+  const double div = dudx + dvdy + dwdz;
+  const double txx = -2.0 * mu * dudx - lambda * div;
+  const double txy = -mu * (dvdx + dudy);
+  const double txz = -mu * (dwdx + dudz);
+  const double tyy = -2.0 * mu * dvdy - lambda * div;
+  const double tyz = -mu * (dwdy + dvdz);
+  const double tzz = -2.0 * mu * dwdz - lambda * div;
+  F.at(off_, 1) += txx * Sx + txy * Sy + txz * Sz;
+  F.at(off_, 2) += txy * Sx + tyy * Sy + tyz * Sz;
+  F.at(off_, 3) += txz * Sx + tyz * Sy + tzz * Sz;
+  double dTx, dTy, dTz;
+  facegrad(3, st.at(oL, 1), st.at(oR, 1), dTx, dTy, dTz);
+  const double heatflux = -kappa * (dTx * Sx + dTy * Sy + dTz * Sz);
+  const double uf = 0.5 * (uR + uL), vf = 0.5 * (vR + vL), wf = 0.5 * (wR + wL);
+  double f4 = (uf * txx + vf * txy + wf * txz) * Sx + (uf * txy + vf * tyy + wf * tyz) * Sy +
+              (uf * txz + vf * tyz + wf * tzz) * Sz + heatflux;
+  double YL[NS], YR[NS];
+#pragma unroll
+  for (int n = 0; n < NS - 1; n++) {
+    YL[n] = Q.at(oL, 5 + n) * rhoinvL;
+    YR[n] = Q.at(oR, 5 + n) * rhoinvR;
+  }
+  double Dk, Vc = 0.0, gradYns = 0.0, sumYh = 0.0, Yns = 1.0;
+  const double rho = 0.5 * (Q.at(oR, 0) + Q.at(oL, 0));
+  double Fsp[NS];
+#pragma unroll
+  for (int n = 0; n < NS - 1; n++) {
+    Dk = 0.5 * (tr.at(oR, 2 + n) + tr.at(oL, 2 + n));
+    double dYx, dYy, dYz;
+    facegrad(4 + n, YL[n], YR[n], dYx, dYy, dYz);
+    const double gradYk = dYx * Sx + dYy * Sy + dYz * Sz;
+    gradYns -= gradYk;
+    Vc += Dk * gradYk;
+    const double Jk = -rho * Dk * gradYk;
+    const double hk = 0.5 * (hsp.at(oR, n) + hsp.at(oL, n));
+    const double Yk = 0.5 * (YR[n] + YL[n]);
+    Yns -= Yk;
+    sumYh += Yk * hk;
+    Fsp[n] = F.at(off_, 5 + n) + Jk;
+    f4 += Jk * hk;
+  }
+  Dk = 0.5 * (tr.at(oR, 2 + NS - 1) + tr.at(oL, 2 + NS - 1));
+  Vc += Dk * gradYns;
+  const double corr = rho * Vc;
+#pragma unroll
+  for (int n = 0; n < NS - 1; n++) F.at(off_, 5 + n) = Fsp[n] + 0.5 * (YR[n] + YL[n]) * corr;
+  f4 += corr * sumYh;
+  const double hns = 0.5 * (hsp.at(oR, NS - 1) + hsp.at(oL, NS - 1));
+  f4 += (-rho * Dk * gradYns + Yns * corr) * hns;
+  F.at(off_, 4) += f4;
+}
+
+}  // namespace pga
+
+using namespace pga;
+
+extern "C" int32_t pga_transport(const pga_case *c, pga_view Q, pga_view state, pga_view trstate, pga_view hsp,
+                                 const pga_box *boxes, int32_t nboxes, void *stream) {
+  if (!c || nboxes < 0 || nboxes > 8 || (nboxes && !boxes)) {
+    set_error("pga_transport: null argument or more than 8 boxes");
+    return PGA_EINVAL;
+  }
+  const pga_case_desc &d = c->desc;
+  if (d.trans < 0) {
+    set_error("pga_transport: the case has no transport model");
+    return PGA_EUNSUP;
+  }
+  const int ne = d.ns + 4;
+  for (int n = 0; n < nboxes; n++) {
+    PGA_REQUIRE(box_ok(boxes[n]), "pga_transport: box %d has a negative extent", n);
+    PGA_REQUIRE(covers(Q, boxes[n], ne) && covers(state, boxes[n], 8) && covers(trstate, boxes[n], 2 + d.ns) &&
+                    covers(hsp, boxes[n], d.ns),
+                "pga_transport: box %d reaches outside its arrays", n);
+  }
+  bool known = false;
+#define X(N) known = known || d.ns == N;
+  PGA_NS_CASES(X)
+#undef X
+  if (!known) {
+    set_error("pga_transport: no kernel for %d species", d.ns);
+    return PGA_EUNSUP;
+  }
+  for (int n = 0; n < nboxes; n++) {
+    if (empty(boxes[n])) continue;
+    const Box3 b = box3(boxes[n]);
+#define X(N)                                                                                                   \
+  if (d.ns == N) {                                                                                             \
+    if (d.trans == TRANS_CONSTANT)                                                                             \
+      hipLaunchKernelGGL((k_transport<N, TRANS_CONSTANT>), grid_of(b), dim3(TI, TJ), 0, stream_of(stream),     \
+                         c->K, view(Q), view(state), view(trstate), view(hsp), b);                             \
+    else                                                                                                       \
+      hipLaunchKernelGGL((k_transport<N, TRANS_KINETIC>), grid_of(b), dim3(TI, TJ), 0, stream_of(stream),      \
+                         c->K, view(Q), view(state), view(trstate), view(hsp), b);                             \
+  }
+    PGA_NS_CASES(X)
+#undef X
+  }
+  return launched("pga_transport");
+}
+
+extern "C" int32_t pga_grads(const pga_case *c, pga_view Q, pga_view state, pga_view Jinv, pga_view grads,
+                             pga_box box, void *stream) {
+  if (!c || !box_ok(box)) {
+    set_error("pga_grads: null case or bad box");
+    return PGA_EINVAL;
+  }
+  if (empty(box)) return PGA_OK;
+  const pga_case_desc &d = c->desc;
+  const int ne = d.ns + 4;
+  const int one[3] = {1, 1, 1};
+  PGA_REQUIRE(covers(Q, box, one, one, ne) && covers(state, box, one, one, 8),
+              "pga_grads: the stencil reaches outside Q or the state (one halo layer needed)");
+  PGA_REQUIRE(covers(Jinv, box, 9) && covers(grads, box, 3 * (ne - 1)), "pga_grads: the box reaches outside Jinv or grads");
+  PGA_REQUIRE(Q.sj == state.sj && Q.sk == state.sk, "pga_grads: Q and state must share their layout");
+  const Box3 b = box3(box);
+  bool done = false;
+#define X(N)                                                                                                    \
+  if (d.ns == N) {                                                                                              \
+    hipLaunchKernelGGL((k_grads<N>), grid_of(b), dim3(TI, TJ), 0, stream_of(stream), view(Q), view(state),      \
+                       view(Jinv), view(grads), b);                                                             \
+    done = true;                                                                                                \
+  }
+  PGA_NS_CASES(X)
+#undef X
+  if (!done) {
+    set_error("pga_grads: no kernel for %d species", d.ns);
+    return PGA_EUNSUP;
+  }
+  return launched("pga_grads");
+}
+
+extern "C" int32_t pga_diffflux(const pga_case *c, int32_t axis, double alpha, pga_view Q, pga_view state,
+                                pga_view trstate, pga_view hsp, pga_view grads, pga_view xcc, pga_view xf, pga_view S,
+                                pga_view F, pga_box box, void *stream) {
+  if (!c || axis < 0 || axis > 2 || !box_ok(box)) {
+    set_error("pga_diffflux: null case, an axis other than 0..2, or a bad box");
+    return PGA_EINVAL;
+  }
+  if (empty(box)) return PGA_OK;
+  const pga_case_desc &d = c->desc;
+  const int ne = d.ns + 4;
+  int lo[3] = {0, 0, 0};
+  const int hi[3] = {0, 0, 0};
+  lo[axis] = 1;
+  PGA_REQUIRE(covers(Q, box, lo, hi, ne) && covers(state, box, lo, hi, 8) && covers(trstate, box, lo, hi, 2 + d.ns) &&
+                  covers(hsp, box, lo, hi, d.ns) && covers(grads, box, lo, hi, 3 * (ne - 1)) &&
+                  covers(xcc, box, lo, hi, 3),
+              "pga_diffflux: the face's cells reach outside a cell array");
+  PGA_REQUIRE(covers(xf, box, 3) && covers(S, box, 3) && covers(F, box, ne), "pga_diffflux: the box reaches outside xf, S or F");
+  PGA_REQUIRE(Q.sj == state.sj && Q.sk == state.sk && Q.sj == trstate.sj && Q.sk == trstate.sk && Q.sj == hsp.sj &&
+                  Q.sk == hsp.sk && Q.sj == grads.sj && Q.sk == grads.sk && Q.sj == xcc.sj && Q.sk == xcc.sk,
+              "pga_diffflux: the cell arrays must share their layout");
+  DiffArgs a{view(Q), view(state), view(trstate), view(hsp), view(grads), view(xcc), view(xf), view(S), view(F),
+             alpha, box3(box)};
+  bool done = false;
+#define X(N)                                                                                                     \
+  if (d.ns == N) {                                                                                               \
+    if (axis == 0)                                                                                               \
+      hipLaunchKernelGGL((k_diffflux<N, 0>), grid_of(a.b), dim3(TI, TJ), 0, stream_of(stream), a);              \
+    else if (axis == 1)                                                                                          \
+      hipLaunchKernelGGL((k_diffflux<N, 1>), grid_of(a.b), dim3(TI, TJ), 0, stream_of(stream), a);              \
+    else                                                                                                         \
+      hipLaunchKernelGGL((k_diffflux<N, 2>), grid_of(a.b), dim3(TI, TJ), 0, stream_of(stream), a);              \
+    done = true;                                                                                                 \
+  }
+  PGA_NS_CASES(X)
+#undef X
+  if (!done) {
+    set_error("pga_diffflux: no kernel for %d species", d.ns);
+    return PGA_EUNSUP;
+  }
+  return launched("pga_diffflux");
+}

@@ -32,7 +32,7 @@ class Box(C.Structure):
 class CaseDesc(C.Structure):
     _fields_ = [
         ("ns", _i32), ("eos", _i32), ("recon", _i32), ("limiter", _i32), ("primary", _i32), ("mechanism", _i32),
-        ("reserved", _i32 * 10),
+        ("trans", _i32), ("reserved", _i32 * 9),
     ]
 
 
@@ -66,6 +66,10 @@ SIGNATURES = {
                          C.c_void_p, Box, _i32, C.c_void_p]),
     "pga_rates": (_i32, [C.c_void_p, View, View, View, Box, C.c_void_p]),
     "pga_halo": (_i32, [C.c_void_p, _i32, C.POINTER(View), C.POINTER(View), _ip, _ip, _ip, View, View, C.c_void_p]),
+    "pga_transport": (_i32, [C.c_void_p, View, View, View, View, C.POINTER(Box), _i32, C.c_void_p]),
+    "pga_grads": (_i32, [C.c_void_p, View, View, View, View, Box, C.c_void_p]),
+    "pga_diffflux": (_i32, [C.c_void_p, _i32, C.c_double, View, View, View, View, View, View, View, View, View, Box,
+                            C.c_void_p]),
 }
 
 
@@ -115,7 +119,8 @@ def mechanisms():
 def const_layout(ns):
     """{section: (offset, count)} of the species constant table, and its size."""
     L = lib()
-    names = ("MW", "MWinv", "Rsp", "cppolyR", "h298R", "cp0", "gpoly", "h298", "mu0", "kappa0", "lewis")
+    names = ("MW", "MWinv", "Rsp", "cppolyR", "h298R", "cp0", "gpoly", "h298", "mu0", "kappa0", "lewis", "hpoly",
+             "MWqinv", "wilkepair", "mupoly", "kappapoly")
     out = {}
     for name in names:
         count = _i32(0)

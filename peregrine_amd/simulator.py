@@ -18,6 +18,7 @@ EOS = {"cpg": 0, "tpg": 1}
 RECON = {"constant": 0, "muscl": 1}
 LIMITER = {None: 0, "koren": 1, "minmod": 2, "mc": 3, "superbee": 4, "vanleer": 5}
 PRIMARY = {"kepaec": 0, "hllc": 1, "rusanov": 2}
+TRANS = {None: -1, "constant": 0, "kinetic-theory": 1}
 
 
 class IoVars:
@@ -108,12 +109,38 @@ class EulerSimulator(BaseSimulator):
         )
 
 
+class NavierStokesSimulator(EulerSimulator):
+    """``peregrine/simulator/navstokes/simulator.py:10-60``: the Euler terms plus the viscous
+    flux (``diff-flux.formula``: alpha-damping) from the transport state (mu, kappa, a
+    diffusivity per species), the species enthalpies and the gradients of every primitive but
+    the pressure (u, v, w, T, the carried Y: ngrad = ne - 1)."""
+
+    name = "navier-stokes"
+    viscous = True
+
+    def __init__(self, cfg):
+        super().__init__(cfg)
+        if self.mixture.trans is None:
+            raise ConfigError("mixture.trans", "a viscous case names a transport model")
+        diff = cfg["diff-flux"]
+        if diff["formula"] != "alpha-damping":
+            raise ConfigError("diff-flux.formula", "this build has alpha-damping")
+        self.alpha = float(dict(diff["values"] or {}).get("alpha", 1.0))
+
+    @property
+    def ngrad(self):
+        return self.ne - 1
+
+    def report(self):
+        return super().report() + f"  Viscous flux: alpha-damping (alpha {self.alpha})\n"
+
+
 def get_simulator(cfg):
     name = cfg["simulation"]["simulator"]
     try:
         return find_subclass(BaseSimulator, name=name)(cfg)
     except KeyError:
-        raise ConfigError("simulation.simulator", f"{name!r}: this build has euler") from None
+        raise ConfigError("simulation.simulator", f"{name!r}: this build has euler and navier-stokes") from None
 
 
 class NativeCase:
@@ -151,6 +178,7 @@ class NativeCase:
         desc.ns, desc.eos = ns, EOS[mix.eos.name]
         desc.recon, desc.limiter, desc.primary = RECON[sim.recon], LIMITER[sim.limiter], PRIMARY[sim.primary]
         desc.mechanism = mechanism
+        desc.trans = TRANS[mix.trans.name if mix.trans is not None else None]
         handle = C.c_void_p()
         chem_ptr = self.chem.data_ptr() if self.chem is not None else None
         native.check(native.lib().pga_case_create(C.byref(desc), self.consts.data_ptr(), chem_ptr, C.byref(handle)),

@@ -73,15 +73,17 @@ class Block:
         ext[axis] += 1
         return native.box((self.ng,) * 3, ext)
 
-    def halo_boxes(self):
-        """Each interface's halo slab, ng deep, over the interior of the other axes."""
+    def halo_boxes(self, depth=None):
+        """Each interface's halo slab, depth (ng) deep next to the interior, over the interior of
+        the other axes."""
+        depth = self.ng if depth is None else depth
         boxes = []
         for nface, f in self.faces.items():
             if f.nbr is None:
                 continue
             start, ext = [self.ng] * 3, list(self.ncell)
-            start[f.axis] = 0 if f.is_low else self.ng + self.ncell[f.axis]
-            ext[f.axis] = self.ng
+            start[f.axis] = self.ng - depth if f.is_low else self.ng + self.ncell[f.axis]
+            ext[f.axis] = depth
             boxes.append(native.box(start, ext))
         return boxes
 
@@ -95,6 +97,16 @@ class Block:
         self.detJinv = backend.alloc(cs + (1,), name="detJinv")
         self.xcc = backend.alloc(cs + (3,), name="xcc")
         self.q = backend.alloc(cs + (ne,), name="q")
+        self.viscous = sim.viscous
+        if sim.viscous:
+            ns = sim.ns
+            # navstokes/simulator.py:30-36, advecdiff/simulator.py:12-15: the transport state, the
+            # species enthalpies, the gradients; the metrics the gradients and viscous flux read
+            self.trstate = backend.alloc(cs + (2 + ns,), name="trstate")
+            self.hsp = backend.alloc(cs + (ns,), name="hsp")
+            self.grads = backend.alloc(cs + (3 * sim.ngrad,), name="grads")
+            self.Jinv = backend.alloc(cs + (9,), name="Jinv")
+            self.xf = [backend.alloc(self.faceshape(a) + (3,), name=f"{'ijk'[a]}xc") for a in range(3)]
 
     @property
     def Q(self):
@@ -108,6 +120,12 @@ class Block:
         for a, S in enumerate((self.iS, self.jS, self.kS)):
             S.set(grid.areavecs(nodes, a))
         self.detJinv.set(grid.detJinv(nodes, ng)[..., None])
+        if self.viscous:
+            with np.errstate(divide="ignore", invalid="ignore"):
+                Jinv = grid.Jinv(nodes)
+            self.Jinv.set(Jinv.reshape(self.cellshape + (9,)))
+            for a, xf in enumerate(self.xf):
+                xf.set(grid.facecenters(nodes, a))
 
 
 class Solver:
@@ -130,7 +148,6 @@ class Solver:
         self.fused = self._fused_supported()
         if self.fused:
             for blk in self.blocks:
-                blk.state2 = backend.alloc(blk.cellshape + (NSTATE,), name="state2")
                 blk.rates = backend.alloc(blk.cellshape + (max(sim.ns - 1, 1),), name="rates") if sim.chemistry else None
         self.plugins = self._plugins()
         self._set_state(state)
@@ -199,10 +216,6 @@ class Solver:
             self._call("pga_fromprims", self.case.handle, blk.q.view(), blk.Q.view(), blk.state.view(), full)
         self.readied.clear()
         self.postrhs(0)
-        if self.fused:
-            # the fused path's second state starts as the first: its halo is the next halo EOS's guess
-            for blk in self.blocks:
-                blk.state2.t.copy_(blk.state.t)
 
     # -- launches --------------------------------------------------------------
     timers = None
@@ -231,26 +244,55 @@ class Solver:
 
     # -- the slots -----------------------------------------------------------
     def exchange(self, reg):
-        """Each interface's halo slab from its partner's interior planes (place, per interface)."""
+        """Each interface's halo slab of Q from its partner's interior planes (place, per interface)."""
+        self._exchange(lambda b: b.regs[reg], self.sim.ne, self.ng)
+
+    def _exchange(self, array, ncomp, depth):
         ng = self.ng
         for blk in self.blocks:
             for f in blk.faces.values():
                 nbr = self.blocks[f.nbr]
                 a = f.axis
                 ext = list(blk.ncell)
-                ext[a] = ng
+                ext[a] = depth
                 dst = [ng] * 3
                 src = [ng] * 3
                 if f.is_low:
-                    dst[a] = 0
-                    src[a] = ng + nbr.ncell[a] - ng
+                    dst[a] = ng - depth
+                    src[a] = ng + nbr.ncell[a] - depth
                 else:
                     dst[a] = ng + blk.ncell[a]
                     src[a] = ng
                 if list(nbr.ncell[m] for m in range(3) if m != a) != list(blk.ncell[m] for m in range(3) if m != a):
                     raise ValueError(f"blocks {blk.nblki} and {nbr.nblki} do not match across interface {f.nface}")
-                self._call("pga_copy_box", blk.regs[reg].view(), native.ints(dst), nbr.regs[reg].view(),
-                           native.ints(src), native.ints(ext), self.sim.ne)
+                self._call("pga_copy_box", array(blk).view(), native.ints(dst), array(nbr).view(),
+                           native.ints(src), native.ints(ext), ncomp)
+
+    def transport(self, Q):
+        """The transport state and species enthalpies over the interior and the first halo layer
+        ("shallow", launch/ranges.py:52-53), after the halo EOS (advec/simulator.py:68-75)."""
+        for blk in self.blocks:
+            boxes = [blk.interior_box()] + blk.halo_boxes(1)
+            arr = (native.Box * len(boxes))(*boxes)
+            self._call("pga_transport", self.case.handle, blk.regs[Q].view(), blk.state.view(), blk.trstate.view(),
+                       blk.hsp.view(), arr, len(boxes))
+
+    def gradients(self, Q):
+        """The cell gradients over the interior and their one-plane exchange
+        (advecdiff/simulator.py:20-27)."""
+        for blk in self.blocks:
+            self._call("pga_grads", self.case.handle, blk.regs[Q].view(), blk.state.view(), blk.Jinv.view(),
+                       blk.grads.view(), blk.interior_box())
+        self._exchange(lambda b: b.grads, 3 * self.sim.ngrad, 1)
+
+    def fluxes(self, blk, Qv, st):
+        """The advective flux of each axis, with the viscous flux added for a viscous case."""
+        for a, (S, F) in enumerate(((blk.iS, blk.iF), (blk.jS, blk.jF), (blk.kS, blk.kF))):
+            self._call("pga_flux", self.case.handle, a, Qv, st, S.view(), F.view(), blk.face_box(a))
+            if self.sim.viscous:
+                self._call("pga_diffflux", self.case.handle, a, self.sim.alpha, Qv, st, blk.trstate.view(),
+                           blk.hsp.view(), blk.grads.view(), blk.xcc.view(), blk.xf[a].view(), S.view(), F.view(),
+                           blk.face_box(a))
 
     def prerhs(self, Q, dQ=None):
         self.exchange(Q)
@@ -260,6 +302,8 @@ class Solver:
             self._call("pga_fromQ", self.case.handle, blk.regs[Q].view(), blk.state.view(), arr, len(boxes))
         if dQ is not None:
             self.begin_dQ(Q, dQ)
+        if self.sim.viscous:
+            self.transport(Q)
         self.readied[Q] = dQ
 
     def begin_dQ(self, Q, dQ):
@@ -288,10 +332,11 @@ class Solver:
 
     def rhs(self, *, Q, dQ):
         self._ready(Q, dQ)
+        if self.sim.viscous:
+            self.gradients(Q)
         for blk in self.blocks:
             Qv, st = blk.regs[Q].view(), blk.state.view()
-            for a, (S, F) in enumerate(((blk.iS, blk.iF), (blk.jS, blk.jF), (blk.kS, blk.kF))):
-                self._call("pga_flux", self.case.handle, a, Qv, st, S.view(), F.view(), blk.face_box(a))
+            self.fluxes(blk, Qv, st)
             self._call("pga_applyflux", self.sim.ne, blk.regs[dQ].view(), blk.iF.view(), blk.jF.view(), blk.kF.view(),
                        blk.detJinv.view(), int(self.sim.chemistry), blk.interior_box())
         self._spent(dQ)
@@ -340,7 +385,7 @@ class Solver:
                 es += ext
             n = len(dsts)
             self._call("pga_halo", self.case.handle, n, (native.View * n)(*dsts), (native.View * n)(*srcs),
-                       native.ints(ds), native.ints(ss), native.ints(es), blk.state.view(), blk.state2.view())
+                       native.ints(ds), native.ints(ss), native.ints(es), blk.state.view(), blk.state.view())
 
     def _combine_terms(self, n):
         """The stage's combine as (source, weight, times dt) terms in the reference's order: its
@@ -357,35 +402,32 @@ class Solver:
         return terms
 
     def _fused_step(self):
-        nst = self.integ.nstages
+        """A step by stages of five launches per block: the halo pass (exchange and halo EOS, unless
+        from_prims left the halos ready), the chemistry rates, the three flux sweeps, and the fused
+        divergence + source + combine + EOS, in place on the reference's stage registers."""
         kc = int(self.cfg["backend-mi355x"]["k-chunk"])
-        qin = 0
-        for n in range(nst):
-            last = n == nst - 1
-            out = 0 if (last and qin != 0) else (2 if qin == 1 else 1)
-            if qin not in self.readied:
-                self._halo_pass(qin)
+        for n, regs in enumerate(self.integ.stageregs(self.regs)):
+            if regs.Q not in self.readied:
+                self._halo_pass(regs.Q)
+                if self.sim.viscous:
+                    self.transport(regs.Q)
             self.readied.clear()
+            if self.sim.viscous:
+                self.gradients(regs.Q)
             terms = self._combine_terms(n)
             src = native.ints([t[0] for t in terms])
             w = (ctypes.c_double * len(terms))(*[t[1] for t in terms])
             withdt = native.ints([t[2] for t in terms])
             for blk in self.blocks:
                 box = blk.interior_box()
-                Qin, st = blk.regs[qin].view(), blk.state.view()
+                Qin, st = blk.regs[regs.Q].view(), blk.state.view()
                 if self.sim.chemistry:
                     self._call("pga_rates", self.case.handle, Qin, st, blk.rates.view(), box)
                 rates = blk.rates.view() if self.sim.chemistry else native.View()
-                self._call("pga_stage", self.case.handle, Qin, st, blk.regs[0].view(), blk.regs[out].view(),
-                           blk.state2.view(), blk.iS.view(), blk.jS.view(), blk.kS.view(), blk.detJinv.view(), rates,
-                           len(terms), src, w, withdt, self.dt.ptr, box, kc)
-            for blk in self.blocks:
-                blk.state, blk.state2 = blk.state2, blk.state
-            if last and qin == 0:
-                # one stage in all: it could not write its own input, so its output becomes the start
-                for blk in self.blocks:
-                    blk.regs[0], blk.regs[1] = blk.regs[1], blk.regs[0]
-            qin = out
+                self.fluxes(blk, Qin, st)
+                self._call("pga_stage", self.case.handle, Qin, st, blk.regs[0].view(), blk.regs[regs.Qout].view(),
+                           st, blk.iF.view(), blk.jF.view(), blk.kF.view(), blk.detJinv.view(), rates, len(terms),
+                           src, w, withdt, self.dt.ptr, box, kc)
 
     def step(self, dt):
         start = self.tme.get()

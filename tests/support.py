@@ -18,6 +18,11 @@ from peregrine_amd.config import Config  # noqa: E402
 
 # constants only: a calorically perfect air and a thermally perfect one (NASA-7 fit of air)
 AIR_CPG = {"Air": {"MW": 28.96, "cp0": 1005.0, "mu0": 1.8591191080521142e-05, "kappa0": 0.02625394405190068}}
+# constant transport of the Taylor-Green case at Re 1600 (mu = rho0 V0 L / 1600, L = 1 m, V0 Mach 0.1
+# at 300 K and 1 atm; kappa = mu cp / 0.71)
+_R_AIR = 8314.46261815324 / 28.96
+TGV_MU = (101325.0 / (_R_AIR * 300.0)) * 0.1 * np.sqrt(1005.0 / (1005.0 - _R_AIR) * _R_AIR * 300.0) / 1600.0
+TGV_KAPPA = TGV_MU * 1005.0 / 0.71
 AIR_NASA7 = {
     # 0.79 N2 + 0.21 O2 by mole: the molar average of the two GRI-Mech 3.0 NASA-7 fits
     "Air": {
@@ -27,21 +32,28 @@ AIR_NASA7 = {
             "low": [3.40027067, 0.000483195742, -1.06301196e-06, 2.42372488e-09, -1.2502517e-12, -1029.93907, 3.88890578],
             "high": [3.00137855, 0.00148695006, -6.0826904e-07, 1.23755417e-10, -9.88622096e-15, -957.586304, 5.86979569],
         },
+        "mu0": 1.8591191080521142e-05,
+        "kappa0": 0.02625394405190068,
     }
 }
 # two ideal species with different weights and heats: exercises the mixture sums
 BINARY_CPG = {
-    "A": {"MW": 28.0, "cp0": 1040.0},
-    "B": {"MW": 44.0, "cp0": 846.0},
+    "A": {"MW": 28.0, "cp0": 1040.0, "mu0": 1.8e-5, "kappa0": 0.026, "lewis": 1.0},
+    "B": {"MW": 44.0, "cp0": 846.0, "mu0": 1.5e-5, "kappa0": 0.017, "lewis": 1.3},
 }
 
 H2_AIR_Y = {"H2": 0.02852, "O2": 0.22636}  # stoichiometric in air, N2 the rest
 
 
-def config(gas="air-cpg", recon="constant", primary="kepaec", limiter=None, chem=False, integrator="tvd-rk3", dt=1e-7):
+def config(gas="air-cpg", recon="constant", primary="kepaec", limiter=None, chem=False, integrator="tvd-rk3", dt=1e-7,
+           trans=None, alpha=None):
+    """A case config; trans (a transport model) makes it Navier-Stokes with alpha damping."""
     c = Config()
-    c["simulation"]["simulator"] = "euler"
+    c["simulation"]["simulator"] = "navier-stokes" if trans else "euler"
     m = c["mixture"]
+    m["trans"] = trans
+    if alpha is not None:
+        c["diff-flux"]["values"] = {"alpha": alpha}
     if gas == "air-cpg":
         m["species"], m["eos"] = AIR_CPG, "cpg"
     elif gas == "air-tpg":
@@ -96,13 +108,21 @@ class HostBlock:
         self.F = [np.zeros((ne,) + s.shape[1:]) for s in self.S]
         self.detJinv = to_dev(grid.detJinv(self.nodes, ng)[..., None])
         self.xcc = grid.cellcenters(self.nodes)
+        # the viscous terms' geometry and arrays (Navier-Stokes)
+        self.xccd = to_dev(self.xcc)
+        self.Jinv = to_dev(grid.Jinv(self.nodes).reshape(self.cshape + (9,)))
+        self.xf = [to_dev(grid.facecenters(self.nodes, a)) for a in range(3)]
+        ns = ne - 4
+        self.trstate = np.zeros((2 + ns,) + self.cshape[::-1])
+        self.hsp = np.zeros((ns,) + self.cshape[::-1])
+        self.grads = np.zeros((3 * (ne - 1),) + self.cshape[::-1])
 
     def interior(self):
         g = self.ng
         return np.s_[:, g:-g, g:-g, g:-g]
 
 
-def oracle_block(hb, integrator, chemistry):
+def oracle_block(hb, integrator, chemistry, viscous=False, alpha=1.0):
     from oracle import oracle as O
 
     b = O.Block()
@@ -117,6 +137,11 @@ def oracle_block(hb, integrator, chemistry):
     b.detJinv = O.arr(hb.detJinv)
     b.integrator = O.INTEGRATOR[integrator]
     b.chemistry = int(chemistry)
+    b.viscous, b.alpha = int(viscous), alpha
+    b.trstate, b.hsp, b.grads = O.arr(hb.trstate), O.arr(hb.hsp), O.arr(hb.grads)
+    b.Jinv, b.xcc = O.arr(hb.Jinv), O.arr(hb.xccd)
+    for a in range(3):
+        b.xf[a] = O.arr(hb.xf[a])
     return b
 
 

@@ -252,11 +252,15 @@ def test_halo_copies(dev):
     assert np.array_equal(dev.get(dB), want)
 
 
-def _run_pair(gas, recon, primary, limiter, chem, integrator, ncell, steps, dt, prims, lengths, fused=True):
+def _run_pair(gas, recon, primary, limiter, chem, integrator, ncell, steps, dt, prims, lengths, fused=True,
+              trans=None, species=None):
     """The same case stepped by the solver on the device and by the oracle on the host."""
     import peregrine_amd as pg
 
-    cfg = sp.config(gas, recon=recon, primary=primary, limiter=limiter, chem=chem, integrator=integrator, dt=dt)
+    cfg = sp.config(gas, recon=recon, primary=primary, limiter=limiter, chem=chem, integrator=integrator, dt=dt,
+                    trans=trans)
+    if species is not None:
+        cfg["mixture"]["species"] = species
     cfg["backend-mi355x"]["fused-stage"] = fused
     backend = pg.get_backend("mi355x", cfg)
     mesh = pg.mesher.CubeMesher(blkdims=[n + 1 for n in ncell], lengths=lengths, periodic=[True] * 3)
@@ -273,7 +277,7 @@ def _run_pair(gas, recon, primary, limiter, chem, integrator, ncell, steps, dt, 
     # the EOS over the interior, then the halo exchange and the EOS over the halos, which its first
     # stage then takes as ready; the oracle's first stage exchanges and takes the halo EOS itself
     O.lib.pgo_fromQ(oc.ref, O.arr(hb.regs[0]), O.arr(hb.state), O.box((hb.ng,) * 3, hb.ncell))
-    b = sp.oracle_block(hb, integrator, chem)
+    b = sp.oracle_block(hb, integrator, chem, viscous=sim.viscous, alpha=getattr(sim, "alpha", 1.0))
     for _ in range(steps):
         mb.step(dt)
         O.lib.pgo_step(oc.ref, ctypes.byref(b), dt)
@@ -428,3 +432,103 @@ def test_blocks_of_a_lattice_step_as_one_block(dev, fused):
     # each cell's arithmetic is the same whichever block holds it, and so is each halo EOS's
     # start: the same bits
     assert np.array_equal(results[1], results[0])
+
+
+# --- the viscous terms -----------------------------------------------------------------------
+
+
+@pytest.mark.parametrize("gas", ["air-tpg", "binary"])
+def test_transport_gradients_and_viscous_flux(dev, gas):
+    """pga_transport, pga_grads and pga_diffflux against the oracle on a jostled block: the same
+    operations in the same order, so the same bits."""
+    cfg = sp.config(gas, trans="constant")
+    sim = get_simulator(cfg)
+    nc = NativeCase(sim, Mi355xBackend(cfg))
+    oc = O.OracleCase(sim.mixture)
+    hb = sp.HostBlock((67, 9, 6), 1, sim.ne, 2, jostle=0.3, seed=5)
+    q = sp.smooth_prims(hb.xcc, sim.ne, ns=sim.ns)
+    _prepared(sim, oc, hb, q)
+    L = native.lib()
+    full = O.box((0, 0, 0), hb.cshape)
+    O.lib.pgo_transport(oc.ref, O.arr(hb.regs[0]), O.arr(hb.state), O.arr(hb.trstate), O.arr(hb.hsp), full)
+    dQv, dSt = dev.put(hb.regs[0]), dev.put(hb.state)
+    dTr, dH = dev.put(np.zeros_like(hb.trstate)), dev.put(np.zeros_like(hb.hsp))
+    boxes = (native.Box * 1)(native.box((0, 0, 0), hb.cshape))
+    native.check(L.pga_transport(nc.handle, dev.view(dQv), dev.view(dSt), dev.view(dTr), dev.view(dH), boxes, 1, None),
+                 "pga_transport")
+    assert np.array_equal(dev.get(dTr), hb.trstate)
+    assert np.array_equal(dev.get(dH), hb.hsp)
+    inner = O.box((1, 1, 1), tuple(n - 2 for n in hb.cshape))
+    O.lib.pgo_grads(oc.ref, O.arr(hb.regs[0]), O.arr(hb.state), O.arr(hb.Jinv), O.arr(hb.grads), inner)
+    dG = dev.put(np.zeros_like(hb.grads))
+    native.check(L.pga_grads(nc.handle, dev.view(dQv), dev.view(dSt), dev.view(dev.put(hb.Jinv)), dev.view(dG),
+                             native.box((1, 1, 1), tuple(n - 2 for n in hb.cshape)), None), "pga_grads")
+    assert np.array_equal(dev.get(dG), hb.grads)
+    dXc = dev.put(hb.xccd)
+    for a in range(3):
+        # faces whose two cells both hold gradients
+        start, ext = [2, 2, 2], [n - 4 for n in hb.cshape]
+        ext[a] += 1
+        F0 = np.random.default_rng(a).random(hb.F[a].shape)
+        want = F0.copy()
+        O.lib.pgo_diffflux(oc.ref, a, 1.0, O.arr(hb.regs[0]), O.arr(hb.state), O.arr(hb.trstate), O.arr(hb.hsp),
+                           O.arr(hb.grads), O.arr(hb.xccd), O.arr(hb.xf[a]), O.arr(hb.S[a]), O.arr(want),
+                           O.box(start, ext))
+        dF = dev.put(F0)
+        native.check(L.pga_diffflux(nc.handle, a, 1.0, dev.view(dQv), dev.view(dSt), dev.view(dTr), dev.view(dH),
+                                    dev.view(dG), dev.view(dXc), dev.view(dev.put(hb.xf[a])),
+                                    dev.view(dev.put(hb.S[a])), dev.view(dF), native.box(start, ext), None),
+                     "pga_diffflux")
+        got = dev.get(dF)
+        err = _rowerr(got, want)
+        print(f"viscous flux {gas} axis {a}: {err:.3e}")
+        assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("gas", ["air-tpg", "air-cpg"])
+def test_tgv_navier_stokes_rk4_100_steps_match(dev, gas):
+    """The Taylor-Green case as stated for it: Navier-Stokes at Re 1600 (constant mu, Pr 0.71),
+    Mach 0.1, KEEP, alpha damping, RK4, 100 steps at 32^3; 1e-10 per conserved row."""
+    n = 32
+    L = (2 * np.pi,) * 3
+    R, cp = 8314.46261815324 / 28.96, 1005.0
+    gamma = cp / (cp - R)
+    c0 = np.sqrt(gamma * R * 300.0)
+    V0 = 0.1 * c0
+    rho0 = 101325.0 / (R * 300.0)
+    base = sp.AIR_CPG if gas == "air-cpg" else sp.AIR_NASA7
+    species = {k: dict(v, mu0=sp.TGV_MU, kappa0=sp.TGV_KAPPA) for k, v in base.items()}
+
+    def prims(x):
+        X, Y, Z = x[..., 0], x[..., 1], x[..., 2]
+        q = np.zeros(x.shape[:3] + (5,))
+        q[..., 0] = 101325.0 + rho0 * V0**2 / 16.0 * (np.cos(2 * X) + np.cos(2 * Y)) * (np.cos(2 * Z) + 2.0)
+        q[..., 1] = V0 * np.sin(X) * np.cos(Y) * np.cos(Z)
+        q[..., 2] = -V0 * np.cos(X) * np.sin(Y) * np.cos(Z)
+        q[..., 4] = q[..., 0] / (rho0 * R)
+        return q
+
+    dt = 0.2 * (2 * np.pi / n) / (c0 + V0)
+    mb, hb = _run_pair(gas, "constant", "kepaec", None, False, "rk4", (n,) * 3, 100, dt, prims, L, trans="constant",
+                       species=species)
+    g = hb.ng
+    got = mb.blocks[0].Q.get()[g:-g, g:-g, g:-g].transpose(3, 2, 1, 0)
+    err = _rowerr(got, hb.regs[0][hb.interior()])
+    print(f"tgv navier-stokes {gas} rk4 100 steps: max row-normalised difference {err:.3e}")
+    assert err < 1e-10
+
+
+@pytest.mark.parametrize("fused", [True, False], ids=["fused", "per-kernel"])
+def test_viscous_binary_muscl_steps_match(dev, fused):
+    """Two species with unequal Lewis numbers, MUSCL + HLLC and the viscous flux, tvd-rk3, 10 steps
+    on a 40 x 20 x 12 block, by the fused stage and by the per-kernel sequence."""
+    ncell = (40, 20, 12)
+    L = (1.0, 0.5, 0.3)
+    mb, hb = _run_pair("binary", "muscl", "hllc", "koren", False, "tvd-rk3", ncell, 10, 2e-6,
+                       lambda x: sp.smooth_prims(x, 6, ns=2), L, fused=fused, trans="constant")
+    assert mb.fused == fused
+    g = hb.ng
+    got = mb.blocks[0].Q.get()[g:-g, g:-g, g:-g].transpose(3, 2, 1, 0)
+    err = _rowerr(got, hb.regs[0][hb.interior()])
+    print(f"viscous binary 10 steps ({'fused' if fused else 'per-kernel'}): {err:.3e}")
+    assert err < 1e-10
