@@ -48,6 +48,11 @@ class Case(C.Structure):
         ("trans", C.c_int), ("nmu", C.c_int), ("nkappa", C.c_int),
         ("hpoly", _dp), ("mu0", _dp), ("kappa0", _dp), ("lewis", _dp), ("MWqinv", _dp), ("wilkepair", _dp),
         ("mupoly", _dp), ("kappapoly", _dp), ("Ru", C.c_double),
+        ("cubic", C.c_double * 4),
+        ("cubicsqrtac", _dp), ("cubicb", _dp), ("cubicda", _dp), ("cubicmTc", _dp), ("cubiconepm", _dp),
+        ("cubicsqrtacMW", _dp), ("cubicbMW", _dp), ("vshift", _dp), ("vshiftMW", _dp), ("cubicdaMW", _dp),
+        ("acentric", _dp),
+        ("cppoly", _dp), ("chungsigma", _dp), ("chungepsk", _dp), ("chungvisc", _dp), ("chungcond", _dp),
     ]
 
 
@@ -95,7 +100,8 @@ tlog = lib.pgo_tlog
 texp = lib.pgo_texp
 elog = lib.pgo_elog
 
-EOS = {"cpg": 0, "tpg": 1}
+EOS = {"cpg": 0, "tpg": 1, "real-gas": 2, "peng-robinson": 2}
+TRANS = {None: -1, "constant": 0, "kinetic-theory": 1, "chung": 2}
 RECON = {"constant": 0, "muscl": 1}
 LIMITER = {None: 0, "koren": 1, "minmod": 2, "mc": 3, "superbee": 4, "vanleer": 5}
 PRIMARY = {"kepaec": 0, "hllc": 1, "rusanov": 2}
@@ -148,7 +154,16 @@ class OracleCase:
         c.cppolyR, c.h298R, c.gpoly, c.h298 = d(k["cppolyR"]), d(k["h298R"]), d(k["gpoly"]), d(k["h298"])
         c.cp0 = d(k.get("cp0", np.zeros(ns)))
         c.hpoly, c.Ru = d(k["hpoly"]), k["Ru"]
-        c.trans = {None: -1, "constant": 0, "kinetic-theory": 1}[mixture.trans.name if mixture.trans else None]
+        c.trans = TRANS[mixture.trans.name if mixture.trans else None]
+        c.cppoly = d(k["cppoly"])
+        if c.eos == 2:
+            c.cubic = (C.c_double * 4)(*k["cubic"])
+            for name in ("cubicsqrtac", "cubicb", "cubicda", "cubicmTc", "cubiconepm", "cubicsqrtacMW", "cubicbMW",
+                         "vshift", "vshiftMW", "cubicdaMW", "acentric"):
+                setattr(c, name, d(k[name]))
+        if c.trans == 2:
+            for name in ("chungsigma", "chungepsk", "chungvisc", "chungcond"):
+                setattr(c, name, d(k[name]))
         if mixture.trans is not None:
             c.lewis, c.MWqinv, c.wilkepair = d(k["lewis"]), d(k["MWqinv"]), d(k["wilkepair"])
             c.mu0, c.kappa0 = d(k.get("mu0", np.zeros(ns))), d(k.get("kappa0", np.zeros(ns)))

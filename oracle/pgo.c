@@ -142,11 +142,11 @@ double pgo_elog(double x) {
  * the equation of state
  * ---------------------------------------------------------------------- */
 typedef struct {
-  double p, T, rhoinv, rho, e, cp, gamma, c;
+  double p, T, rhoinv, rho, e, cp, gamma, c, Z;
 } eosstate;
 
 typedef struct {
-  double R, cp, h298, cpcoeff[8];
+  double R, cp, h298, cpcoeff[8], b, c, da;
 } mixmom;
 
 /* eos/kernels/mixing.mako:3-30: the carried species, then the remainder. The sums, and the
@@ -171,6 +171,11 @@ static void mix(const pgo_case *c, const double *Y, mixmom *m) {
       for (int t = 0; t < c->nt; t++) m->cpcoeff[t] = fma(Yn, c->cppolyR[n * c->nt + t], m->cpcoeff[t]);
     }
     m->h298 = fma(Yn, c->h298R[n], m->h298);
+    if (c->eos == 2) {
+      m->b = fma(Yn, c->cubicbMW[n], m->b);
+      m->c = fma(Yn, c->vshiftMW[n], m->c);
+      m->da = fma(Yn, c->cubicdaMW[n], m->da);
+    }
   }
 }
 
@@ -187,8 +192,144 @@ static void ideal_cph(const pgo_case *c, const mixmom *m, double T, double *cp, 
   *h = fma(hv, T, m->h298);
 }
 
-/* cpg eos/cpg/kernels/eos.mako:5-15; tpg eos/tpg/kernels/eos.mako:8-20 and newton.mako:3-11 */
-static void fromrhoE(const pgo_case *c, const mixmom *m, double rho, double e, double Tguess, eosstate *s) {
+/* ------------------------------------------------------------------------
+ * the cubic real gas: eos/realgas/kernels/eos.mako, newton.mako. Logarithms are taken by the
+ * EOS's log (pgo_elog), as everywhere in this path.
+ * ---------------------------------------------------------------------- */
+typedef struct {
+  double MWmix, bm, cm, S1, S2, am, dam, Astar, Bstar, Cstar, invRuT, Z, z1, z2, invslope, dZdT, v, I, logratio, q, qd;
+} cubicpt;
+
+#define CU (c->cubic[0])
+#define CW (c->cubic[1])
+#define CSIG (c->cubic[2])
+#define CEPS (c->cubic[3])
+
+/* Soave's |1 + m (1 - sqrt(T / Tc))| as |(1 + m) - sqrt(T) m / sqrt(Tc)| */
+static double modulus(const pgo_case *c, int n, double sqrtT) { return fabs(fma(-sqrtT, c->cubicmTc[n], c->cubiconepm[n])); }
+
+/* eos.mako:13-18 */
+static void cubicmix(const pgo_case *c, const mixmom *m, cubicpt *q) {
+  q->MWmix = c->Ru / m->R;
+  q->bm = m->b * q->MWmix;
+  q->cm = m->c * q->MWmix;
+  q->S2 = m->da * q->MWmix;
+}
+
+/* eos.mako:21-32: k_ij = 0, am = S1^2, dam/dT = -S1 S2 / sqrt(T) */
+static void attraction(const pgo_case *c, cubicpt *q, const double *Y, double T) {
+  const double sqrtT = sqrt(T);
+  double s1 = 0.0, Ylast = 1.0;
+  for (int n = 0; n < c->ns - 1; n++) {
+    const double Yn = Y[n];
+    Ylast -= Yn;
+    s1 = fma(Yn, c->cubicsqrtacMW[n] * modulus(c, n, sqrtT), s1);
+  }
+  s1 = fma(Ylast, c->cubicsqrtacMW[c->ns - 1] * modulus(c, c->ns - 1, sqrtT), s1);
+  q->S1 = s1 * q->MWmix;
+  q->am = q->S1 * q->S1;
+  q->dam = -q->S1 * q->S2 / sqrtT;
+}
+
+/* eos.mako:34-43 */
+static void dimless(const pgo_case *c, cubicpt *q, double p, double T) {
+  q->invRuT = 1.0 / (c->Ru * T);
+  const double pRT = p * q->invRuT;
+  q->Astar = q->am * pRT * q->invRuT;
+  q->Bstar = q->bm * pRT;
+  q->Cstar = q->cm * pRT;
+  const double B = q->Bstar, B2 = B * B;
+  q->z1 = q->Astar + CW * B2 - CU * B - CU * B2;
+  q->z2 = -(1.0 + B - CU * B);
+}
+
+/* eos.mako:86-89 */
+static void rootslope(cubicpt *q) { q->invslope = 1.0 / (3.0 * q->Z * q->Z + 2.0 * q->Z * q->z2 + q->z1); }
+
+/* eos.mako:46-83: Cardano; of the roots above B*, the least Gibbs energy departure */
+static void zsolve(const pgo_case *c, cubicpt *q, double p, double T) {
+  dimless(c, q, p, T);
+  const double B = q->Bstar, B2 = B * B, z1 = q->z1, z2 = q->z2;
+  const double z0 = -(q->Astar * B + CW * B2 * (1.0 + B));
+  const double Q = (z2 * z2 - 3.0 * z1) * (1.0 / 9.0);
+  const double R = (2.0 * z2 * z2 * z2 - 9.0 * z2 * z1 + 27.0 * z0) * (1.0 / 54.0);
+  const double M = R * R - Q * Q * Q;
+  const double z2o3 = z2 * (1.0 / 3.0);
+  if (M > 0.0) {
+    const double S = -copysign(cbrt(fabs(R) + sqrt(M)), R);
+    q->Z = S + Q / S - z2o3;
+  } else {
+    const double sqQ = sqrt(Q), twopio3 = 2.0943951023931954923;
+    const double theta3 = acos(R / (Q * sqQ)) * (1.0 / 3.0);
+    const double roots[3] = {-2.0 * sqQ * cos(theta3) - z2o3, -2.0 * sqQ * cos(theta3 + twopio3) - z2o3,
+                             -2.0 * sqQ * cos(theta3 - twopio3) - z2o3};
+    double Z = fmax(roots[0], fmax(roots[1], roots[2]));
+    int above = 0;
+    for (int k = 0; k < 3; k++) above += roots[k] > B;
+    if (above > 1) {
+      const double AoBse = q->Astar / (B * (CSIG - CEPS));
+      double gmin = 0.0;
+      int found = 0;
+      for (int k = 0; k < 3; k++) {
+        const double z = roots[k];
+        if (z <= B) continue;
+        const double g = z - 1.0 - pgo_elog(z - B) + AoBse * pgo_elog((z + CEPS * B) / (z + CSIG * B));
+        if (!found || g < gmin) {
+          gmin = g;
+          Z = z;
+          found = 1;
+        }
+      }
+    }
+    q->Z = Z;
+  }
+  rootslope(q);
+}
+
+/* eos.mako:91-97: dZ for given dA*, dB* */
+static double dZ(const pgo_case *c, const cubicpt *q, double dA, double dB) {
+  const double B = q->Bstar, Z = q->Z, Z2 = Z * Z;
+  const double dz0 = -(B * dA + (q->Astar + (2.0 * B + 3.0 * B * B) * CW) * dB);
+  const double dz1 = dA + (2.0 * B * (CW - CU) - CU) * dB;
+  const double dz2 = -(1.0 - CU) * dB;
+  return -(dz2 * Z2 + dz1 * Z + dz0) * q->invslope;
+}
+
+/* eos.mako:100-115 */
+static void departures(const pgo_case *c, cubicpt *q, double T, double *hdep, double *cpdep) {
+  const double Tinv = c->Ru * q->invRuT;
+  const double dlnamdT = q->dam / q->am;
+  q->dZdT = dZ(c, q, -2.0 * q->Astar * Tinv * (1.0 - 0.5 * T * dlnamdT), -q->Bstar * Tinv);
+  const double x = q->Z / q->Bstar, xm1 = x - 1.0;
+  const double D = x * x + CU * x + CW;
+  q->logratio = pgo_elog((x + CEPS) / (x + CSIG));
+  const double invBmRu = 1.0 / (q->bm * c->Ru);
+  const double hres = invBmRu / (CSIG - CEPS) * (q->am * Tinv - q->dam) * q->logratio;
+  q->qd = q->dam * invBmRu;
+  q->q = q->am * invBmRu * Tinv;
+  const double num = D - q->qd * xm1;
+  *cpdep = hres * 0.5 * T * dlnamdT + num * num / (D * D - q->q * (2.0 * x + CU) * xm1 * xm1) - 1.0;
+  *hdep = hres + (q->Z - 1.0) - q->Cstar;
+}
+
+/* eos.mako:117-131 */
+static void gamma_c(const pgo_case *c, const cubicpt *q, double p, double T, double rho, double cp, double invZt,
+                    double *gamma, double *cs) {
+  const double invp = 1.0 / p, Tinv = c->Ru * q->invRuT;
+  const double dZtdT = q->dZdT + q->Cstar * Tinv;
+  const double dZdp = dZ(c, q, q->Astar * invp, q->Bstar * invp);
+  const double dZtdp = dZdp - q->Cstar * invp;
+  const double drhodp = rho * invp * (1.0 - p * dZtdp * invZt);
+  const double r = -Tinv * (1.0 + T * dZtdT * invZt);
+  const double csq = 1.0 / (drhodp - (T / cp) * r * r);
+  *gamma = drhodp * csq;
+  *cs = sqrt(fabs(csq));
+}
+
+/* cpg eos/cpg/kernels/eos.mako:5-15; tpg eos/tpg/kernels/eos.mako:8-20 and newton.mako:3-11;
+ * real gas eos/realgas/kernels/eos.mako:157-189 with newton.mako:3-15 */
+static void fromrhoE(const pgo_case *c, const mixmom *m, const double *Y, double rho, double e, double Tguess,
+                     eosstate *s) {
   if (c->eos == 0) {
     const double cv = m->cp - m->R;
     s->rho = rho;
@@ -199,10 +340,48 @@ static void fromrhoE(const pgo_case *c, const mixmom *m, double rho, double e, d
     s->cp = m->cp;
     s->gamma = m->cp / cv;
     s->c = sqrt(s->gamma * m->R * s->T);
+    s->Z = 1.0;
+    return;
+  }
+  const double tol = 1e5 * DBL_EPSILON;
+  if (c->eos == 2) {
+    cubicpt q;
+    cubicmix(c, m, &q);
+    q.v = q.MWmix / rho + q.cm;
+    q.I = pgo_elog((q.v + CSIG * q.bm) / (q.v + CEPS * q.bm)) / ((CSIG - CEPS) * q.bm);
+    const double invMWmix = 1.0 / q.MWmix;
+    double T = (Tguess < 1.0) ? 300.0 : Tguess, cpi = 0.0, hi = 0.0;
+    for (int k = 0; k < 12; k++) {
+      attraction(c, &q, Y, T);
+      ideal_cph(c, m, T, &cpi, &hi);
+      const double app = 0.5 * (q.S2 * q.S2 - q.dam) / T;
+      const double f = hi - m->R * T - (q.am - T * q.dam) * q.I * invMWmix - e;
+      const double cv = cpi - m->R + T * app * q.I * invMWmix;
+      const double dT = f / cv;
+      T -= dT;
+      if (k >= 2 && fabs(dT) <= tol * T) break;
+    }
+    attraction(c, &q, Y, T);
+    ideal_cph(c, m, T, &cpi, &hi);
+    const double vs = q.v + CSIG * q.bm, ve = q.v + CEPS * q.bm;
+    const double p = c->Ru * T / (q.v - q.bm) - q.am / (vs * ve);
+    dimless(c, &q, p, T);
+    q.Z = p * q.invRuT * q.v;
+    rootslope(&q);
+    double hdep, cpdep;
+    departures(c, &q, T, &hdep, &cpdep);
+    s->rho = rho;
+    s->rhoinv = 1.0 / rho;
+    s->e = e;
+    s->p = p;
+    s->T = T;
+    s->Z = q.Z;
+    s->cp = cpi + m->R * cpdep;
+    const double invZt = 1.0 / (q.Z - q.Cstar);
+    gamma_c(c, &q, p, T, rho, s->cp, invZt, &s->gamma, &s->c);
     return;
   }
   double T = (Tguess < 1.0) ? 300.0 : Tguess, cp = 0.0, h = 0.0;
-  const double tol = 1e5 * DBL_EPSILON;
   for (int k = 0; k < 12; k++) {
     ideal_cph(c, m, T, &cp, &h);
     const double dT = (e - (h - m->R * T)) / (cp - m->R);
@@ -217,10 +396,31 @@ static void fromrhoE(const pgo_case *c, const mixmom *m, double rho, double e, d
   s->cp = cp;
   s->gamma = cp / (cp - m->R);
   s->c = sqrt(s->gamma * m->R * T);
+  s->Z = 1.0;
 }
 
-/* cpg eos.mako:17-27; tpg eos.mako:22-33 */
-static void frompT(const pgo_case *c, const mixmom *m, double p, double T, eosstate *s) {
+/* cpg eos.mako:17-27; tpg eos.mako:22-33; real gas eos/realgas/kernels/eos.mako:137-155 */
+static void frompT(const pgo_case *c, const mixmom *m, const double *Y, double p, double T, eosstate *s) {
+  if (c->eos == 2) {
+    cubicpt q;
+    cubicmix(c, m, &q);
+    attraction(c, &q, Y, T);
+    double cpi, hi;
+    ideal_cph(c, m, T, &cpi, &hi);
+    zsolve(c, &q, p, T);
+    double hdep, cpdep;
+    departures(c, &q, T, &hdep, &cpdep);
+    s->p = p;
+    s->T = T;
+    s->Z = q.Z;
+    s->cp = cpi + m->R * cpdep;
+    const double Zt = q.Z - q.Cstar, invZt = 1.0 / Zt, porho = Zt * m->R * T;
+    s->rho = p * q.invRuT * q.MWmix * invZt;
+    s->rhoinv = 1.0 / s->rho;
+    s->e = hi + m->R * T * hdep - porho;
+    gamma_c(c, &q, p, T, s->rho, s->cp, invZt, &s->gamma, &s->c);
+    return;
+  }
   s->p = p;
   s->T = T;
   s->rho = p / (m->R * T);
@@ -238,6 +438,7 @@ static void frompT(const pgo_case *c, const mixmom *m, double p, double T, eosst
     s->gamma = cp / (cp - m->R);
   }
   s->c = sqrt(s->gamma * m->R * T);
+  s->Z = 1.0;
 }
 
 static void store(pgo_arr st, int i, int j, int k, const eosstate *s) {
@@ -249,6 +450,11 @@ static void store(pgo_arr st, int i, int j, int k, const eosstate *s) {
   *at(st, i, j, k, 5) = s->cp;
   *at(st, i, j, k, 6) = s->gamma;
   *at(st, i, j, k, 7) = s->c;
+}
+
+/* the real gas's ninth state field */
+static void storeZ(const pgo_case *c, pgo_arr st, int i, int j, int k, const eosstate *s) {
+  if (c->eos == 2) *at(st, i, j, k, 8) = s->Z;
 }
 
 static void load(pgo_arr st, int i, int j, int k, eosstate *s) {
@@ -279,8 +485,9 @@ void pgo_fromQ(const pgo_case *c, pgo_arr Q, pgo_arr state, pgo_box r) {
     mixmom m;
     mix(c, Y, &m);
     eosstate s;
-    fromrhoE(c, &m, rho, e, *at(state, i, j, k, 1), &s);
+    fromrhoE(c, &m, Y, rho, e, *at(state, i, j, k, 1), &s);
     store(state, i, j, k, &s);
+    storeZ(c, state, i, j, k, &s);
   }
 }
 
@@ -293,7 +500,7 @@ void pgo_fromprims(const pgo_case *c, pgo_arr prims, pgo_arr Q, pgo_arr state, p
     mixmom m;
     mix(c, Y, &m);
     eosstate s;
-    frompT(c, &m, *at(prims, i, j, k, 0), *at(prims, i, j, k, 4), &s);
+    frompT(c, &m, Y, *at(prims, i, j, k, 0), *at(prims, i, j, k, 4), &s);
     *at(Q, i, j, k, 0) = s.rho;
     *at(Q, i, j, k, 1) = s.rho * u[0];
     *at(Q, i, j, k, 2) = s.rho * u[1];
@@ -301,6 +508,7 @@ void pgo_fromprims(const pgo_case *c, pgo_arr prims, pgo_arr Q, pgo_arr state, p
     *at(Q, i, j, k, 4) = s.rho * (s.e + 0.5 * ((u[0]) * (u[0]) + (u[1]) * (u[1]) + (u[2]) * (u[2])));
     for (int n = 0; n < c->ns - 1; n++) *at(Q, i, j, k, 5 + n) = s.rho * Y[n];
     store(state, i, j, k, &s);
+    storeZ(c, state, i, j, k, &s);
   }
 }
 
@@ -371,12 +579,12 @@ static void sides(const pgo_case *c, pgo_arr Q, pgo_arr st, const int *cl, const
   }
   mixmom m;
   mix(c, L->Y, &m);
-  frompT(c, &m, pL, TL, &L->s);
+  frompT(c, &m, L->Y, pL, TL, &L->s);
   L->u = lft(lim, QQ(0, 1) * ri[0], QQ(1, 1) * ri[1], QQ(2, 1) * ri[2]);
   L->v = lft(lim, QQ(0, 2) * ri[0], QQ(1, 2) * ri[1], QQ(2, 2) * ri[2]);
   L->w = lft(lim, QQ(0, 3) * ri[0], QQ(1, 3) * ri[1], QQ(2, 3) * ri[2]);
   mix(c, R->Y, &m);
-  frompT(c, &m, pR, TR, &R->s);
+  frompT(c, &m, R->Y, pR, TR, &R->s);
   R->u = rgt(lim, QQ(1, 1) * ri[1], QQ(2, 1) * ri[2], QQ(3, 1) * ri[3]);
   R->v = rgt(lim, QQ(1, 2) * ri[1], QQ(2, 2) * ri[2], QQ(3, 2) * ri[3]);
   R->w = rgt(lim, QQ(1, 3) * ri[1], QQ(2, 3) * ri[2], QQ(3, 3) * ri[3]);
@@ -720,6 +928,100 @@ static double kappamix(const pgo_case *c, const double *X, const double *kappasp
   return 0.5 * (series + 1.0 / parallel);
 }
 
+/* transport/kernels/chung.mako:6-67: Chung et al.'s dense-gas factors on the dilute values,
+ * one-fluid mixing, no dipole or association terms; rho in kg/m3, Chung's units inside */
+static void chung(const pgo_case *c, const double *X, double T, double lnT, double rho, double *mu, double *kappa) {
+  const int ns = c->ns;
+  double sigma3 = 0.0, epssigma3 = 0.0, masssum = 0.0, omegasigma3 = 0.0, cvR = 0.0;
+  for (int i = 0; i < ns; i++) {
+    double cpR = 0.0;
+    for (int m = c->nt - 1; m >= 0; m--) cpR = cpR * lnT + c->cppoly[i * c->nt + m];
+    cvR += X[i] * (cpR - 1.0);
+    for (int j = 0; j < ns; j++) {
+      const double xx = X[i] * X[j];
+      const double sigma2ij = c->chungsigma[i] * c->chungsigma[j];
+      const double sigma3ij = sigma2ij * sqrt(sigma2ij);
+      const double epsij = sqrt(c->chungepsk[i] * c->chungepsk[j]);
+      const double Mij = 2.0 * c->MW[i] * c->MW[j] / (c->MW[i] + c->MW[j]);
+      sigma3 += xx * sigma3ij;
+      epssigma3 += xx * epsij * sigma3ij;
+      masssum += xx * epsij * sigma2ij * sqrt(Mij);
+      omegasigma3 += xx * 0.5 * (c->acentric[i] + c->acentric[j]) * sigma3ij;
+    }
+  }
+  const double epsk = epssigma3 / sigma3;
+  const double cbrtsigma3 = cbrt(sigma3);
+  const double sigma2 = cbrtsigma3 * cbrtsigma3;
+  const double sqrtMm = masssum / (epsk * sigma2);
+  const double Mm = sqrtMm * sqrtMm;
+  const double omega = omegasigma3 / sigma3;
+  const double Tc = 1.2593 * epsk;
+  const double Vc = sigma3 / (0.809 * 0.809 * 0.809);
+  const double Fc = 1.0 - 0.2756 * omega;
+  const double cbrtVc = cbrt(Vc);
+  const double Vc23 = cbrtVc * cbrtVc;
+  const double Mprime = Mm * 1.0e-3;
+  double E[10], B[7];
+  for (int i = 0; i < 10; i++) E[i] = c->chungvisc[4 * i] + c->chungvisc[4 * i + 1] * omega;
+  for (int i = 0; i < 7; i++) B[i] = c->chungcond[4 * i] + c->chungcond[4 * i + 1] * omega;
+  const double Tr = T / Tc;
+  const double Tstar = T / epsk, invTstar = epsk / T;
+  const double Omegav = 1.16145 * pow(Tstar, -0.14874) + 0.52487 * exp(-0.77320 * Tstar) + 2.16178 * exp(-2.43787 * Tstar);
+  const double Yy = rho / Mm * 1.0e-3 * Vc / 6.0;
+  const double omYy = 1.0 - Yy;
+  const double G1 = (1.0 - 0.5 * Yy) / (omYy * omYy * omYy);
+  const double G2 = (E[0] * (1.0 - exp(-E[3] * Yy)) / Yy + E[1] * G1 * exp(E[4] * Yy) + E[2] * G1) / (E[0] * E[3] + E[1] + E[2]);
+  const double etastarstar = E[6] * Yy * Yy * G2 * exp(E[7] + E[8] * invTstar + E[9] * invTstar * invTstar);
+  const double muratio = (1.0 / G2 + E[5] * Yy) + etastarstar * Omegav / (sqrt(Tstar) * Fc);
+  *mu = *mu * muratio;
+  const double eta0 = 4.0785e-6 * sqrt(Mm * T) / (Vc23 * Omegav) * Fc;
+  const double alpha = cvR - 1.5;
+  const double beta = 0.7862 - 0.7109 * omega + 1.3168 * omega * omega;
+  const double Zc = 2.0 + 10.5 * Tr * Tr;
+  const double Psi = 1.0 + alpha * ((0.215 + 0.28288 * alpha - 1.061 * beta + 0.26665 * Zc) / (0.6366 + beta * Zc + 1.061 * alpha * beta));
+  const double lambda0 = 31.2 * eta0 * Psi / Mprime;
+  const double H2 = (B[0] * (1.0 - exp(-B[3] * Yy)) / Yy + B[1] * G1 * exp(B[4] * Yy) + B[2] * G1) / (B[0] * B[3] + B[1] + B[2]);
+  const double kapparatio = (1.0 / H2 + B[5] * Yy) + 3.586e-3 * sqrt(Tc / Mprime) / Vc23 * B[6] * Yy * Yy * H2 * sqrt(Tr) / lambda0;
+  *kappa = *kappa * kapparatio;
+}
+
+/* eos/realgas/kernels/eos.mako:192-273 hspecies: the partial molar enthalpies of the cubic,
+ * -Ru T^2 d ln phi_n / dT plus the ideal part and the volume translation's -p c_n, per mass,
+ * at the state's own root Z */
+static void hpartial(const pgo_case *c, const double *Yall, double p, double T, double Z, double *h) {
+  const int ns = c->ns;
+  mixmom m;
+  mix(c, Yall, &m);
+  cubicpt q;
+  cubicmix(c, &m, &q);
+  attraction(c, &q, Yall, T);
+  dimless(c, &q, p, T);
+  q.Z = Z;
+  rootslope(&q);
+  double hdep, cpdep;
+  departures(c, &q, T, &hdep, &cpdep);
+  const double lnT = pgo_elog(T), Tinv = c->Ru * q.invRuT, sqrtT = sqrt(T);
+  const double ZT = q.dZdT, Bs = q.Bstar, BT = -Bs * Tinv;
+  const double invZsB = 1.0 / (Z + CSIG * Bs), invZeB = 1.0 / (Z + CEPS * Bs);
+  const double Iz = -q.logratio / (CSIG - CEPS);
+  const double IzT = ((ZT + CSIG * BT) * invZsB - (ZT + CEPS * BT) * invZeB) / (CSIG - CEPS);
+  const double dlnZmB = (ZT - BT) / (Z - Bs);
+  const double invS1 = 1.0 / q.S1, invBm = 1.0 / q.bm, invamsqrtT = invS1 * invS1 / sqrtT;
+  for (int n = 0; n < ns; n++) {
+    double hRT = 0.0;
+    for (int mm = c->nt - 1; mm >= 0; mm--) hRT = fma(hRT, lnT, c->hpoly[n * c->nt + mm]);
+    hRT += c->h298[n] * Tinv;
+    const double qT = (q.qd - q.q) * Tinv;
+    const double sqrtan = c->cubicsqrtac[n] * modulus(c, n, sqrtT);
+    const double r = 2.0 * sqrtan * invS1;
+    const double rT = (sqrtan * q.S2 - c->cubicda[n] * q.S1) * invamsqrtT;
+    const double bb = c->cubicb[n] * invBm;
+    const double qbar = q.q * (r - bb), qbarT = qT * (r - bb) + q.q * rT;
+    const double dlnphidT = bb * q.dZdT - dlnZmB - qbarT * Iz - qbar * IzT;
+    h[n] = (hRT * T * c->Ru - c->Ru * T * T * dlnphidT - c->vshift[n] * p) * c->MWinv[n];
+  }
+}
+
 /* navstokes/kernels/transport.mako:5-35 with transport/kernels/{constant,kinetic,dilute}.mako,
  * diffusion/kernels/lewis.mako:3-7 and the ideal hspecies (eos/kernels/species.mako:29-36).
  * ln T is taken by the EOS's log (pgo_elog), as everywhere in this path. */
@@ -758,12 +1060,19 @@ void pgo_transport(const pgo_case *c, pgo_arr Q, pgo_arr state, pgo_arr trstate,
         kappasp[n] = fit * sqrtT;
       }
     }
-    const double mu = viscmix(c, X, sqrtmu);
-    const double kappa = kappamix(c, X, kappasp);
+    double mu = viscmix(c, X, sqrtmu);
+    double kappa = kappamix(c, X, kappasp);
+    if (c->trans == 2) chung(c, X, T, lnT, rho, &mu, &kappa);
     *at(trstate, i, j, k, 0) = mu;
     *at(trstate, i, j, k, 1) = kappa;
     const double kappaoverrhocp = kappa * rhoinv / *at(state, i, j, k, 5);
     for (int n = 0; n < ns; n++) *at(trstate, i, j, k, 2 + n) = kappaoverrhocp / c->lewis[n];
+    if (c->eos == 2) {
+      double h[MAXNS];
+      hpartial(c, Y, *at(state, i, j, k, 0), T, *at(state, i, j, k, 8), h);
+      for (int n = 0; n < ns; n++) *at(hsp, i, j, k, n) = h[n];
+      continue;
+    }
     const double Tinv = 1.0 / T;
     for (int n = 0; n < ns; n++) {
       double hRT = 0.0;

@@ -42,10 +42,17 @@ typedef struct {
   const double *logA, *b, *EaR, *logprA, *prb, *prEaR, *nusum;
   const double *fwdorder, *reactnu, *prodnu, *netnu, *netnuMW, *lfcpoly, *tbdefeff, *tbeffdev;
   double logprefRu, prefRu, Rupref, logcfloor, expclamp, troeuc, troeuhinv;
-  /* transport: -1 none (Euler), 0 constant, 1 kinetic theory; Wilke mixing, Lewis diffusion */
+  /* transport: -1 none (Euler), 0 constant, 1 kinetic theory, 2 Chung (dense, on kinetic
+   * theory); Wilke mixing, Lewis diffusion */
   int trans, nmu, nkappa;
   const double *hpoly, *mu0, *kappa0, *lewis, *MWqinv, *wilkepair, *mupoly, *kappapoly;
   double Ru;
+  /* the cubic (eos 2): u, w, sigma, eps of v^2 + u b v + w b^2, and per species constants */
+  double cubic[4];
+  const double *cubicsqrtac, *cubicb, *cubicda, *cubicmTc, *cubiconepm, *cubicsqrtacMW, *cubicbMW, *vshift,
+      *vshiftMW, *cubicdaMW, *acentric;
+  /* Chung: cp/R fits, sigma (angstrom), epsilon/k (K), the E (10 x 4) and B (7 x 4) tables */
+  const double *cppoly, *chungsigma, *chungepsk, *chungvisc, *chungcond;
 } pgo_case;
 
 double pgo_tlog(double x);

@@ -49,7 +49,13 @@ extern "C" int32_t pga_const_offset(int32_t ns, const char *name, int32_t *count
       {"h298R", o.h298R, ns},     {"cp0", o.cp0, ns},         {"gpoly", o.gpoly, ns * NG}, {"h298", o.h298, ns},
       {"mu0", o.mu0, ns},         {"kappa0", o.kappa0, ns},   {"lewis", o.lewis, ns},
       {"hpoly", o.hpoly, ns * NT}, {"MWqinv", o.MWqinv, ns},  {"wilkepair", o.wilkepair, ns * ns},
-      {"mupoly", o.mupoly, ns * NT}, {"kappapoly", o.kappapoly, ns * NT},
+      {"mupoly", o.mupoly, ns * NT}, {"kappapoly", o.kappapoly, ns * NT}, {"cppoly", o.cppoly, ns * NT},
+      {"cubic", o.cubic, 4},       {"cubicsqrtac", o.cubicsqrtac, ns}, {"cubicb", o.cubicb, ns},
+      {"cubicda", o.cubicda, ns},  {"cubicmTc", o.cubicmTc, ns}, {"cubiconepm", o.cubiconepm, ns},
+      {"cubicsqrtacMW", o.cubicsqrtacMW, ns}, {"cubicbMW", o.cubicbMW, ns}, {"vshift", o.vshift, ns},
+      {"vshiftMW", o.vshiftMW, ns}, {"cubicdaMW", o.cubicdaMW, ns}, {"acentric", o.acentric, ns},
+      {"chungsigma", o.chungsigma, ns}, {"chungepsk", o.chungepsk, ns}, {"chungvisc", o.chungvisc, 40},
+      {"chungcond", o.chungcond, 28},
   };
   for (auto &r : rows)
     if (!strcmp(r.n, name)) {
@@ -86,12 +92,17 @@ extern "C" int32_t pga_case_create(const pga_case_desc *desc, const double *dev_
     set_error("pga_case_create: %d species; 1 to %d", desc->ns, MAXNS);
     return PGA_EINVAL;
   }
-  if (desc->eos < 0 || desc->eos > 1) {
-    set_error("pga_case_create: no eos %d (0 cpg, 1 tpg)", desc->eos);
+  if (desc->eos < 0 || desc->eos > 2) {
+    set_error("pga_case_create: no eos %d (0 cpg, 1 tpg, 2 cubic real gas)", desc->eos);
     return PGA_EUNSUP;
   }
-  if (desc->trans < -1 || desc->trans > 1) {
-    set_error("pga_case_create: no transport model %d (-1 none, 0 constant, 1 kinetic theory)", desc->trans);
+  if (desc->eos == 2 && !realgas_compiled(desc->ns)) {
+    set_error("pga_case_create: the real gas is compiled for 1 or 2 species, not %d", desc->ns);
+    return PGA_EUNSUP;
+  }
+  if (desc->trans < -1 || desc->trans > 2 || (desc->trans == 2 && desc->eos != 2)) {
+    set_error("pga_case_create: no transport model %d (-1 none, 0 constant, 1 kinetic theory, 2 Chung on a real gas)",
+              desc->trans);
     return PGA_EUNSUP;
   }
   if (desc->mechanism >= 0) {

@@ -33,7 +33,7 @@ __global__ __launch_bounds__(256) void k_fromprims(const double *__restrict__ K,
   MixMom<NS, EOS> m;
   mixmom<NS, EOS>(g, [&](int n) { return Y[n]; }, m);
   EosState s;
-  frompT<NS, EOS>(m, P.at(o, 0), P.at(o, 4), s);
+  frompT<NS, EOS>(g, m, [&](int n) { return Y[n]; }, P.at(o, 0), P.at(o, 4), s);
   Q.at(o, 0) = s.rho;
   Q.at(o, 1) = s.rho * u0;
   Q.at(o, 2) = s.rho * u1;
@@ -41,7 +41,7 @@ __global__ __launch_bounds__(256) void k_fromprims(const double *__restrict__ K,
   Q.at(o, 4) = s.rho * (s.e + 0.5 * (u0 * u0 + u1 * u1 + u2 * u2));
 #pragma unroll
   for (int n = 0; n < NS - 1; n++) Q.at(o, 5 + n) = s.rho * Y[n];
-  store_state(st, o, s);
+  store_state<EOS>(st, o, s);
 }
 
 // advec/kernels/applyflux.mako:2-11, the sum in the reference's order
@@ -153,24 +153,15 @@ extern "C" int32_t pga_fromQ(const pga_case *c, pga_view Q, pga_view state, cons
       return PGA_EINVAL;
     }
     if (empty(boxes[n])) continue;
-    PGA_REQUIRE(covers(Q, boxes[n], c->desc.ns + 4) && covers(state, boxes[n], 8),
+    PGA_REQUIRE(covers(Q, boxes[n], c->desc.ns + 4) && covers(state, boxes[n], nstate(c->desc)),
                 "pga_fromQ: box %d reaches outside Q or state, or they hold too few components", n);
     const Box3 b = box3(boxes[n]);
     const dim3 grid = grid_of(b), block(TI, TJ);
-    const int ns = c->desc.ns, eos = c->desc.eos;
-    bool done = false;
-#define X(N)                                                                                              \
-  if (ns == N) {                                                                                          \
-    if (eos == EOS_CPG)                                                                                   \
-      hipLaunchKernelGGL((k_fromQ<N, EOS_CPG>), grid, block, 0, stream_of(stream), c->K, view(Q), view(state), b); \
-    else                                                                                                  \
-      hipLaunchKernelGGL((k_fromQ<N, EOS_TPG>), grid, block, 0, stream_of(stream), c->K, view(Q), view(state), b); \
-    done = true;                                                                                          \
-  }
-    PGA_NS_CASES(X)
-#undef X
+    const bool done = dispatch_eos(c->desc, [&]<int N, int E>() {
+      hipLaunchKernelGGL((k_fromQ<N, E>), grid, block, 0, stream_of(stream), c->K, view(Q), view(state), b);
+    });
     if (!done) {
-      set_error("pga_fromQ: no kernel for %d species", ns);
+      set_error("pga_fromQ: no kernel for %d species with eos %d", c->desc.ns, c->desc.eos);
       return PGA_EUNSUP;
     }
     const int32_t rc = launched("pga_fromQ");
@@ -186,26 +177,16 @@ extern "C" int32_t pga_fromprims(const pga_case *c, pga_view prims, pga_view Q, 
     return PGA_EINVAL;
   }
   if (empty(box)) return PGA_OK;
-  PGA_REQUIRE(covers(prims, box, c->desc.ns + 4) && covers(Q, box, c->desc.ns + 4) && covers(state, box, 8),
+  PGA_REQUIRE(covers(prims, box, c->desc.ns + 4) && covers(Q, box, c->desc.ns + 4) && covers(state, box, nstate(c->desc)),
               "pga_fromprims: the box reaches outside prims, Q or state");
   const Box3 b = box3(box);
   const dim3 grid = grid_of(b), block(TI, TJ);
-  const int ns = c->desc.ns, eos = c->desc.eos;
-  bool done = false;
-#define X(N)                                                                                                       \
-  if (ns == N) {                                                                                                   \
-    if (eos == EOS_CPG)                                                                                            \
-      hipLaunchKernelGGL((k_fromprims<N, EOS_CPG>), grid, block, 0, stream_of(stream), c->K, view(prims), view(Q), \
-                         view(state), b);                                                                          \
-    else                                                                                                           \
-      hipLaunchKernelGGL((k_fromprims<N, EOS_TPG>), grid, block, 0, stream_of(stream), c->K, view(prims), view(Q), \
-                         view(state), b);                                                                          \
-    done = true;                                                                                                   \
-  }
-  PGA_NS_CASES(X)
-#undef X
+  const bool done = dispatch_eos(c->desc, [&]<int N, int E>() {
+    hipLaunchKernelGGL((k_fromprims<N, E>), grid, block, 0, stream_of(stream), c->K, view(prims), view(Q), view(state),
+                       b);
+  });
   if (!done) {
-    set_error("pga_fromprims: no kernel for %d species", ns);
+    set_error("pga_fromprims: no kernel for %d species with eos %d", c->desc.ns, c->desc.eos);
     return PGA_EUNSUP;
   }
   return launched("pga_fromprims");
@@ -286,7 +267,7 @@ extern "C" int32_t pga_source(const pga_case *c, pga_view Q, pga_view state, pga
     return PGA_EUNSUP;
   }
   if (empty(box)) return PGA_OK;
-  PGA_REQUIRE(covers(Q, box, c->desc.ns + 4) && covers(state, box, 8) && covers(dQ, box, c->desc.ns + 4),
+  PGA_REQUIRE(covers(Q, box, c->desc.ns + 4) && covers(state, box, nstate(c->desc)) && covers(dQ, box, c->desc.ns + 4),
               "pga_source: the box reaches outside Q, state or dQ");
   const Box3 b = box3(box);
   bool done = false;

@@ -24,7 +24,8 @@ constexpr double Ru = 8314.46261815324;
 // ---------------------------------------------------------------------------
 struct Offsets {
   int MW, MWinv, Rsp, cppolyR, h298R, cp0, gpoly, h298, mu0, kappa0, lewis, hpoly, MWqinv, wilkepair, mupoly,
-      kappapoly, size;
+      kappapoly, cppoly, cubic, cubicsqrtac, cubicb, cubicda, cubicmTc, cubiconepm, cubicsqrtacMW, cubicbMW, vshift,
+      vshiftMW, cubicdaMW, acentric, chungsigma, chungepsk, chungvisc, chungcond, size;
 };
 
 __host__ __device__ constexpr Offsets offsets(int ns) {
@@ -45,7 +46,26 @@ __host__ __device__ constexpr Offsets offsets(int ns) {
   o.wilkepair = o.MWqinv + ns;
   o.mupoly = o.wilkepair + ns * ns;
   o.kappapoly = o.mupoly + ns * NT;
-  o.size = o.kappapoly + ns * NT;
+  o.cppoly = o.kappapoly + ns * NT;
+  // the cubic real gas: u, w, sigma, eps, then per species
+  o.cubic = o.cppoly + ns * NT;
+  o.cubicsqrtac = o.cubic + 4;
+  o.cubicb = o.cubicsqrtac + ns;
+  o.cubicda = o.cubicb + ns;
+  o.cubicmTc = o.cubicda + ns;
+  o.cubiconepm = o.cubicmTc + ns;
+  o.cubicsqrtacMW = o.cubiconepm + ns;
+  o.cubicbMW = o.cubicsqrtacMW + ns;
+  o.vshift = o.cubicbMW + ns;
+  o.vshiftMW = o.vshift + ns;
+  o.cubicdaMW = o.vshiftMW + ns;
+  o.acentric = o.cubicdaMW + ns;
+  // Chung: sigma, epsilon/k per species, the E (10 x 4) and B (7 x 4) tables
+  o.chungsigma = o.acentric + ns;
+  o.chungepsk = o.chungsigma + ns;
+  o.chungvisc = o.chungepsk + ns;
+  o.chungcond = o.chungvisc + 40;
+  o.size = o.chungcond + 28;
   return o;
 }
 
@@ -73,6 +93,24 @@ struct Gas {
   __device__ __forceinline__ double wilkepair(int n, int m) const { return K[O.wilkepair + n * NS + m]; }
   __device__ __forceinline__ double mupoly(int n, int m) const { return K[O.mupoly + n * NT + m]; }
   __device__ __forceinline__ double kappapoly(int n, int m) const { return K[O.kappapoly + n * NT + m]; }
+  __device__ __forceinline__ double cppoly(int n, int m) const { return K[O.cppoly + n * NT + m]; }
+  // the cubic: cubic(0..3) = u, w, sigma, eps
+  __device__ __forceinline__ double cubic(int i) const { return K[O.cubic + i]; }
+  __device__ __forceinline__ double cubicsqrtac(int n) const { return K[O.cubicsqrtac + n]; }
+  __device__ __forceinline__ double cubicb(int n) const { return K[O.cubicb + n]; }
+  __device__ __forceinline__ double cubicda(int n) const { return K[O.cubicda + n]; }
+  __device__ __forceinline__ double cubicmTc(int n) const { return K[O.cubicmTc + n]; }
+  __device__ __forceinline__ double cubiconepm(int n) const { return K[O.cubiconepm + n]; }
+  __device__ __forceinline__ double cubicsqrtacMW(int n) const { return K[O.cubicsqrtacMW + n]; }
+  __device__ __forceinline__ double cubicbMW(int n) const { return K[O.cubicbMW + n]; }
+  __device__ __forceinline__ double vshift(int n) const { return K[O.vshift + n]; }
+  __device__ __forceinline__ double vshiftMW(int n) const { return K[O.vshiftMW + n]; }
+  __device__ __forceinline__ double cubicdaMW(int n) const { return K[O.cubicdaMW + n]; }
+  __device__ __forceinline__ double acentric(int n) const { return K[O.acentric + n]; }
+  __device__ __forceinline__ double chungsigma(int n) const { return K[O.chungsigma + n]; }
+  __device__ __forceinline__ double chungepsk(int n) const { return K[O.chungepsk + n]; }
+  __device__ __forceinline__ double chungvisc(int i, int j) const { return K[O.chungvisc + 4 * i + j]; }
+  __device__ __forceinline__ double chungcond(int i, int j) const { return K[O.chungcond + 4 * i + j]; }
 };
 
 // ---------------------------------------------------------------------------

@@ -100,7 +100,7 @@ __device__ __forceinline__ void face_sides(const Gas<NS> &g, const View &Q, cons
     {
       MixMom<NS, EOS> m;
       mixmom<NS, EOS>(g, [&](int n) { return L.Y[n]; }, m);
-      frompT<NS, EOS>(m, pL, TL, L.s);
+      frompT<NS, EOS>(g, m, [&](int n) { return L.Y[n]; }, pL, TL, L.s);
     }
     double vel[3][4];
 #pragma unroll
@@ -116,7 +116,7 @@ __device__ __forceinline__ void face_sides(const Gas<NS> &g, const View &Q, cons
     {
       MixMom<NS, EOS> m;
       mixmom<NS, EOS>(g, [&](int n) { return R.Y[n]; }, m);
-      frompT<NS, EOS>(m, pR, TR, R.s);
+      frompT<NS, EOS>(g, m, [&](int n) { return R.Y[n]; }, pR, TR, R.s);
     }
     R.u = right_of<LIM>(vel[0][1], vel[0][2], vel[0][3]);
     R.v = right_of<LIM>(vel[1][1], vel[1][2], vel[1][3]);

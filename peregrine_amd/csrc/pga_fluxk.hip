@@ -67,24 +67,14 @@ extern "C" int32_t pga_flux(const pga_case *c, int32_t axis, pga_view Q, pga_vie
     int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
     lo[axis] = d.recon == 1 ? 2 : 1;
     hi[axis] = d.recon == 1 ? 1 : 0;
-    PGA_REQUIRE(covers(Q, box, lo, hi, d.ns + 4) && covers(state, box, lo, hi, 8),
+    PGA_REQUIRE(covers(Q, box, lo, hi, d.ns + 4) && covers(state, box, lo, hi, nstate(d)),
                 "pga_flux: the faces' stencil reaches outside Q or state");
     PGA_REQUIRE(covers(S, box, 3) && covers(F, box, d.ns + 4), "pga_flux: the faces reach outside S or F");
   }
   const Box3 b = box3(box);
-  bool done = false;
-#define X(N)                                                                          \
-  if (d.ns == N) {                                                                    \
-    if (d.eos == EOS_CPG)                                                             \
-      launch_flux<N, EOS_CPG>(c, axis, Q, state, S, F, b, stream_of(stream));         \
-    else                                                                              \
-      launch_flux<N, EOS_TPG>(c, axis, Q, state, S, F, b, stream_of(stream));         \
-    done = true;                                                                      \
-  }
-  PGA_NS_CASES(X)
-#undef X
+  const bool done = dispatch_eos(d, [&]<int N, int E>() { launch_flux<N, E>(c, axis, Q, state, S, F, b, stream_of(stream)); });
   if (!done) {
-    set_error("pga_flux: no kernel for %d species", d.ns);
+    set_error("pga_flux: no kernel for %d species with eos %d", d.ns, d.eos);
     return PGA_EUNSUP;
   }
   return launched("pga_flux");

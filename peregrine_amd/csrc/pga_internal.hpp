@@ -65,4 +65,36 @@ inline bool covers(const pga_view &v, const pga_box &b, int ncomp) {
 
 #define PGA_NS_CASES(X) X(1) X(2) X(3) X(9)
 
+// the species counts the cubic real gas is compiled for (its kernels are the heaviest to build)
+constexpr bool realgas_compiled(int ns) { return ns == 1 || ns == 2; }
+
+// the state fields a case stores: the real gas adds its compressibility Z
+inline int nstate(const pga_case_desc &d) { return d.eos == 2 ? 9 : 8; }
+
+// f.template operator()<NS, EOS>() for the case's species count and eos (0 cpg, 1 tpg, 2 the
+// cubic real gas where compiled); false when no variant is compiled for them
+template <class F>
+bool dispatch_eos(const pga_case_desc &d, F &&f) {
+  bool done = false;
+  const auto one = [&]<int N>() {
+    if (d.ns != N) return;
+    if (d.eos == 0) {
+      f.template operator()<N, 0>();
+      done = true;
+    } else if (d.eos == 1) {
+      f.template operator()<N, 1>();
+      done = true;
+    } else if constexpr (realgas_compiled(N)) {
+      if (d.eos == 2) {
+        f.template operator()<N, 2>();
+        done = true;
+      }
+    }
+  };
+#define X(N) one.template operator()<N>();
+  PGA_NS_CASES(X)
+#undef X
+  return done;
+}
+
 }  // namespace pga

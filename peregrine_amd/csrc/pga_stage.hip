@@ -56,7 +56,7 @@ __device__ __forceinline__ void fromQ_regs(const Gas<NS> &g, const double (&Q)[N
   for (int n = 0; n < NS - 1; n++) Y[n] = Q[5 + n] * rhoinv;
   MixMom<NS, EOS> m;
   mixmom<NS, EOS>(g, [&](int n) { return Y[n]; }, m);
-  fromrhoE<NS, EOS>(m, rho, e, Tguess, s);
+  fromrhoE<NS, EOS>(g, m, [&](int n) { return Y[n]; }, rho, e, Tguess, s);
 }
 
 template <int NS, int EOS, bool CHEM>
@@ -124,7 +124,7 @@ __global__ __launch_bounds__(WT) void k_stage(StageArgs a) {
     const int64_t oo = Qout.off(i, j, k);
 #pragma unroll
     for (int l = 0; l < NE; l++) Qout.at(oo, l) = Qn[l];
-    store_state(stout, stout.off(i, j, k), s);
+    store_state<EOS>(stout, stout.off(i, j, k), s);
   }
 }
 
@@ -182,7 +182,7 @@ __global__ __launch_bounds__(256) void k_halo(HaloArgs a) {
   const int64_t ost = a.stdst.off(t.d0 + ii, t.d1 + jj, t.d2 + kk);
   EosState s;
   fromQ_regs<NS, EOS>(g, Q, a.stguess.at(ost, 1), s);
-  store_state(a.stdst, ost, s);
+  store_state<EOS>(a.stdst, ost, s);
 }
 
 }  // namespace pga
@@ -191,11 +191,12 @@ using namespace pga;
 
 extern "C" int32_t pga_stage_supported(const pga_case *c) {
   if (!c) return 0;
+  int ok = 0;
 #define X(N) \
-  if (c->desc.ns == N) return 1;
+  if (c->desc.ns == N) ok = c->desc.eos != 2 || realgas_compiled(N);
   PGA_NS_CASES(X)
 #undef X
-  return 0;
+  return ok;
 }
 
 extern "C" int32_t pga_stage(const pga_case *c, pga_view Qin, pga_view stin, pga_view Q0, pga_view Qout,
@@ -211,8 +212,8 @@ extern "C" int32_t pga_stage(const pga_case *c, pga_view Qin, pga_view stin, pga
   const int ne = d.ns + 4;
   const bool chem = d.mechanism >= 0;
   {
-    PGA_REQUIRE(covers(Qin, box, ne) && covers(stin, box, 8) && covers(Q0, box, ne) && covers(Qout, box, ne) &&
-                    covers(stout, box, 8) && covers(detJinv, box, 1),
+    PGA_REQUIRE(covers(Qin, box, ne) && covers(stin, box, nstate(d)) && covers(Q0, box, ne) && covers(Qout, box, ne) &&
+                    covers(stout, box, nstate(d)) && covers(detJinv, box, 1),
                 "pga_stage: the box reaches outside Qin, Q0, Qout, a state or detJinv");
     const int z[3] = {0, 0, 0}, fh[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
     PGA_REQUIRE(covers(iF, box, z, fh[0], ne) && covers(jF, box, z, fh[1], ne) && covers(kF, box, z, fh[2], ne),
@@ -234,21 +235,17 @@ extern "C" int32_t pga_stage(const pga_case *c, pga_view Qin, pga_view stin, pga
   a.n0 = box.extent[0], a.n1 = box.extent[1], a.n2 = box.extent[2];
   a.kc = kchunk;
   const dim3 grid((a.n0 + WI - 1) / WI, (a.n1 + WJ - 1) / WJ, (a.n2 + kchunk - 1) / kchunk);
-  bool done = false;
-#define X(N)                                                                                                \
-  if (d.ns == N) {                                                                                          \
-    if (d.eos == EOS_CPG)                                                                                   \
-      hipLaunchKernelGGL((k_stage<N, EOS_CPG, false>), grid, dim3(WT), 0, stream_of(stream), a);            \
-    else if (chem)                                                                                          \
-      hipLaunchKernelGGL((k_stage<N, EOS_TPG, true>), grid, dim3(WT), 0, stream_of(stream), a);             \
-    else                                                                                                    \
-      hipLaunchKernelGGL((k_stage<N, EOS_TPG, false>), grid, dim3(WT), 0, stream_of(stream), a);            \
-    done = true;                                                                                            \
-  }
-  PGA_NS_CASES(X)
-#undef X
+  const bool done = dispatch_eos(d, [&]<int N, int E>() {
+    if constexpr (E == EOS_TPG) {
+      if (chem) {
+        hipLaunchKernelGGL((k_stage<N, E, true>), grid, dim3(WT), 0, stream_of(stream), a);
+        return;
+      }
+    }
+    hipLaunchKernelGGL((k_stage<N, E, false>), grid, dim3(WT), 0, stream_of(stream), a);
+  });
   if (!done) {
-    set_error("pga_stage: no kernel for %d species", d.ns);
+    set_error("pga_stage: no kernel for %d species with eos %d", d.ns, d.eos);
     return PGA_EUNSUP;
   }
   return launched("pga_stage");
@@ -265,7 +262,7 @@ extern "C" int32_t pga_rates(const pga_case *c, pga_view Q, pga_view state, pga_
     return PGA_EUNSUP;
   }
   if (empty(box)) return PGA_OK;
-  PGA_REQUIRE(covers(Q, box, c->desc.ns + 4) && covers(state, box, 8) && covers(rates, box, c->desc.ns - 1),
+  PGA_REQUIRE(covers(Q, box, c->desc.ns + 4) && covers(state, box, nstate(c->desc)) && covers(rates, box, c->desc.ns - 1),
               "pga_rates: the box reaches outside Q, state or rates");
   const Box3 b = box3(box);
   bool done = false;
@@ -301,7 +298,7 @@ extern "C" int32_t pga_halo(const pga_case *c, int32_t ntask, const pga_view *Qd
     const int *ds = dst_start + 3 * n, *ss = src_start + 3 * n, *e = extent + 3 * n;
     const pga_box bd{{ds[0], ds[1], ds[2]}, {e[0], e[1], e[2]}}, bs{{ss[0], ss[1], ss[2]}, {e[0], e[1], e[2]}};
     PGA_REQUIRE(box_ok(bd) && box_ok(bs) && covers(Qdst[n], bd, ne) && covers(Qsrc[n], bs, ne) &&
-                    covers(stdst, bd, 8) && covers(stguess, bd, 8),
+                    covers(stdst, bd, nstate(c->desc)) && covers(stguess, bd, nstate(c->desc)),
                 "pga_halo: slab %d reaches outside its arrays", n);
     PGA_REQUIRE(Qdst[n].sj == stdst.sj && Qdst[n].sk == stdst.sk, "pga_halo: Q and state must share their layout");
     HaloTask &t = a.t[n];
@@ -317,20 +314,11 @@ extern "C" int32_t pga_halo(const pga_case *c, int32_t ntask, const pga_view *Qd
   const int64_t nblk = (total + 255) / 256;
   PGA_REQUIRE(nblk <= 0x7fffffff, "pga_halo: too many halo cells");
   const dim3 grid((unsigned)nblk);
-  const pga_case_desc &d = c->desc;
-  bool done = false;
-#define X(N)                                                                                                    \
-  if (d.ns == N) {                                                                                              \
-    if (d.eos == EOS_CPG)                                                                                       \
-      hipLaunchKernelGGL((k_halo<N, EOS_CPG>), grid, dim3(256), 0, stream_of(stream), a);                    \
-    else                                                                                                        \
-      hipLaunchKernelGGL((k_halo<N, EOS_TPG>), grid, dim3(256), 0, stream_of(stream), a);                    \
-    done = true;                                                                                                \
-  }
-  PGA_NS_CASES(X)
-#undef X
+  const bool done = dispatch_eos(c->desc, [&]<int N, int E>() {
+    hipLaunchKernelGGL((k_halo<N, E>), grid, dim3(256), 0, stream_of(stream), a);
+  });
   if (!done) {
-    set_error("pga_halo: no kernel for %d species", d.ns);
+    set_error("pga_halo: no kernel for %d species with eos %d", c->desc.ns, c->desc.eos);
     return PGA_EUNSUP;
   }
   return launched("pga_halo");

@@ -13,14 +13,120 @@
 
 namespace pga {
 
-enum { TRANS_CONSTANT = 0, TRANS_KINETIC = 1 };
+enum { TRANS_CONSTANT = 0, TRANS_KINETIC = 1, TRANS_CHUNG = 2 };
+
+// transport/kernels/chung.mako:6-67: Chung et al.'s dense-gas factors on the dilute mu and kappa,
+// one-fluid mixing, without the dipole and association terms; Chung's units inside
+template <int NS>
+__device__ __forceinline__ void chung(const Gas<NS> &g, const double (&X)[NS], double T, double lnT, double rho,
+                                      double &mu, double &kappa) {
+#pragma clang fp contract(off)
+  double sigma3 = 0.0, epssigma3 = 0.0, masssum = 0.0, omegasigma3 = 0.0, cvR = 0.0;
+#pragma unroll
+  for (int i = 0; i < NS; i++) {
+    double cpR = 0.0;
+#pragma unroll
+    for (int m = NT - 1; m >= 0; m--) cpR = cpR * lnT + g.cppoly(i, m);
+    cvR += X[i] * (cpR - 1.0);
+#pragma unroll
+    for (int j = 0; j < NS; j++) {
+      const double xx = X[i] * X[j];
+      const double sigma2ij = g.chungsigma(i) * g.chungsigma(j);
+      const double sigma3ij = sigma2ij * sqrt(sigma2ij);
+      const double epsij = sqrt(g.chungepsk(i) * g.chungepsk(j));
+      const double Mij = 2.0 * g.MW(i) * g.MW(j) / (g.MW(i) + g.MW(j));
+      sigma3 += xx * sigma3ij;
+      epssigma3 += xx * epsij * sigma3ij;
+      masssum += xx * epsij * sigma2ij * sqrt(Mij);
+      omegasigma3 += xx * 0.5 * (g.acentric(i) + g.acentric(j)) * sigma3ij;
+    }
+  }
+  const double epsk = epssigma3 / sigma3;
+  const double cbrtsigma3 = cbrt(sigma3);
+  const double sigma2 = cbrtsigma3 * cbrtsigma3;
+  const double sqrtMm = masssum / (epsk * sigma2);
+  const double Mm = sqrtMm * sqrtMm;
+  const double omega = omegasigma3 / sigma3;
+  const double Tc = 1.2593 * epsk;
+  const double Vc = sigma3 / (0.809 * 0.809 * 0.809);
+  const double Fc = 1.0 - 0.2756 * omega;
+  const double cbrtVc = cbrt(Vc);
+  const double Vc23 = cbrtVc * cbrtVc;
+  const double Mprime = Mm * 1.0e-3;
+  double E[10], B[7];
+#pragma unroll
+  for (int i = 0; i < 10; i++) E[i] = g.chungvisc(i, 0) + g.chungvisc(i, 1) * omega;
+#pragma unroll
+  for (int i = 0; i < 7; i++) B[i] = g.chungcond(i, 0) + g.chungcond(i, 1) * omega;
+  const double Tr = T / Tc;
+  const double Tstar = T / epsk, invTstar = epsk / T;
+  const double Omegav = 1.16145 * pow(Tstar, -0.14874) + 0.52487 * exp(-0.77320 * Tstar) + 2.16178 * exp(-2.43787 * Tstar);
+  const double Yy = rho / Mm * 1.0e-3 * Vc / 6.0;
+  const double omYy = 1.0 - Yy;
+  const double G1 = (1.0 - 0.5 * Yy) / (omYy * omYy * omYy);
+  const double G2 = (E[0] * (1.0 - exp(-E[3] * Yy)) / Yy + E[1] * G1 * exp(E[4] * Yy) + E[2] * G1) / (E[0] * E[3] + E[1] + E[2]);
+  const double etastarstar = E[6] * Yy * Yy * G2 * exp(E[7] + E[8] * invTstar + E[9] * invTstar * invTstar);
+  const double muratio = (1.0 / G2 + E[5] * Yy) + etastarstar * Omegav / (sqrt(Tstar) * Fc);
+  mu = mu * muratio;
+  const double eta0 = 4.0785e-6 * sqrt(Mm * T) / (Vc23 * Omegav) * Fc;
+  const double alpha = cvR - 1.5;
+  const double beta = 0.7862 - 0.7109 * omega + 1.3168 * omega * omega;
+  const double Zc = 2.0 + 10.5 * Tr * Tr;
+  const double Psi = 1.0 + alpha * ((0.215 + 0.28288 * alpha - 1.061 * beta + 0.26665 * Zc) / (0.6366 + beta * Zc + 1.061 * alpha * beta));
+  const double lambda0 = 31.2 * eta0 * Psi / Mprime;
+  const double H2 = (B[0] * (1.0 - exp(-B[3] * Yy)) / Yy + B[1] * G1 * exp(B[4] * Yy) + B[2] * G1) / (B[0] * B[3] + B[1] + B[2]);
+  const double kapparatio = (1.0 / H2 + B[5] * Yy) + 3.586e-3 * sqrt(Tc / Mprime) / Vc23 * B[6] * Yy * Yy * H2 * sqrt(Tr) / lambda0;
+  kappa = kappa * kapparatio;
+}
+
+// eos/realgas/kernels/eos.mako:192-273: the cubic's partial enthalpies per mass at the state's own
+// root Z, -Ru T^2 d ln phi_n / dT plus the ideal part and the translation's -p c_n
+template <int NS>
+__device__ __forceinline__ void hpartial(const Gas<NS> &g, const double (&Y)[NS], double p, double T, double Z,
+                                         const View &hsp, int64_t oh) {
+#pragma clang fp contract(off)
+  const double sig = g.cubic(2), eps = g.cubic(3);
+  const auto YF = [&](int n) { return Y[n]; };
+  MixMom<NS, EOS_REAL> m;
+  mixmom<NS, EOS_REAL>(g, YF, m);
+  Cubic c;
+  cubicmix(m, c);
+  attraction(g, c, YF, T);
+  dimless(g, c, p, T);
+  c.Z = Z;
+  rootslope(c);
+  double hdep, cpdep;
+  departures(g, c, T, hdep, cpdep);
+  const double lnT = elog(T), Tinv = Ru * c.invRuT, sqrtT = sqrt(T);
+  const double ZT = c.dZdT, Bs = c.Bstar, BT = -Bs * Tinv;
+  const double invZsB = 1.0 / (Z + sig * Bs), invZeB = 1.0 / (Z + eps * Bs);
+  const double Iz = -c.logratio / (sig - eps);
+  const double IzT = ((ZT + sig * BT) * invZsB - (ZT + eps * BT) * invZeB) / (sig - eps);
+  const double dlnZmB = (ZT - BT) / (Z - Bs);
+  const double invS1 = 1.0 / c.S1, invBm = 1.0 / c.bm, invamsqrtT = invS1 * invS1 / sqrtT;
+#pragma unroll
+  for (int n = 0; n < NS; n++) {
+    double hRT = 0.0;
+#pragma unroll
+    for (int mm = NT - 1; mm >= 0; mm--) hRT = fma(hRT, lnT, g.hpoly(n, mm));
+    hRT += g.h298(n) * Tinv;
+    const double qT = (c.qd - c.q) * Tinv;
+    const double sqrtan = g.cubicsqrtac(n) * modulus(g, n, sqrtT);
+    const double r = 2.0 * sqrtan * invS1;
+    const double rT = (sqrtan * c.S2 - g.cubicda(n) * c.S1) * invamsqrtT;
+    const double bb = g.cubicb(n) * invBm;
+    const double qbar = c.q * (r - bb), qbarT = qT * (r - bb) + c.q * rT;
+    const double dlnphidT = bb * c.dZdT - dlnZmB - qbarT * Iz - qbar * IzT;
+    hsp.at(oh, n) = (hRT * T * Ru - Ru * T * T * dlnphidT - g.vshift(n) * p) * g.MWinv(n);
+  }
+}
 
 // navstokes/kernels/transport.mako:5-35: mu and kappa by Wilke's rule and the conductivity mean
 // (mixrule/kernels/wilke.mako:3-17, transport/kernels/mixing.mako:3-10) from constant species
 // values (transport/kernels/constant.mako:5-16) or the kinetic-theory fits
 // (transport/kernels/dilute.mako:5-26); D_n by the Lewis numbers (diffusion/kernels/lewis.mako:3-7);
 // h_n ideal (eos/kernels/species.mako:29-36)
-template <int NS, int TRANS>
+template <int NS, int TRANS, int EOS>
 __device__ __forceinline__ void transport_cell(const Gas<NS> &g, const View &Q, const View &st, int64_t o,
                                                const View &tr, const View &hsp, int64_t ot, int64_t oh) {
 #pragma clang fp contract(off)
@@ -84,29 +190,34 @@ __device__ __forceinline__ void transport_cell(const Gas<NS> &g, const View &Q, 
     series += X[n] * kappasp[n];
     parallel += X[n] / kappasp[n];
   }
-  const double kappa = 0.5 * (series + 1.0 / parallel);
+  double kappa = 0.5 * (series + 1.0 / parallel);
+  if constexpr (TRANS == TRANS_CHUNG) chung(g, X, T, lnT, rho, mu, kappa);
   tr.at(ot, 0) = mu;
   tr.at(ot, 1) = kappa;
   const double kappaoverrhocp = kappa * rhoinv / cp;
 #pragma unroll
   for (int n = 0; n < NS; n++) tr.at(ot, 2 + n) = kappaoverrhocp / g.lewis(n);
-  const double Tinv = 1.0 / T;
+  if constexpr (EOS == EOS_REAL) {
+    hpartial(g, Y, st.at(o, 0), T, st.at(o, 8), hsp, oh);
+  } else {
+    const double Tinv = 1.0 / T;
 #pragma unroll
-  for (int n = 0; n < NS; n++) {
-    double hRT = 0.0;
+    for (int n = 0; n < NS; n++) {
+      double hRT = 0.0;
 #pragma unroll
-    for (int m = NT - 1; m >= 0; m--) hRT = fma(hRT, lnT, g.hpoly(n, m));
-    hRT += g.h298(n) * Tinv;
-    hsp.at(oh, n) = hRT * T * Ru * g.MWinv(n);
+      for (int m = NT - 1; m >= 0; m--) hRT = fma(hRT, lnT, g.hpoly(n, m));
+      hRT += g.h298(n) * Tinv;
+      hsp.at(oh, n) = hRT * T * Ru * g.MWinv(n);
+    }
   }
 }
 
-template <int NS, int TRANS>
+template <int NS, int TRANS, int EOS>
 __global__ __launch_bounds__(256) void k_transport(const double *K, View Q, View st, View tr, View hsp, Box3 b) {
   int i, j, k;
   if (!cell_of(b, i, j, k)) return;
   const Gas<NS> g{K};
-  transport_cell<NS, TRANS>(g, Q, st, Q.off(i, j, k), tr, hsp, tr.off(i, j, k), hsp.off(i, j, k));
+  transport_cell<NS, TRANS, EOS>(g, Q, st, Q.off(i, j, k), tr, hsp, tr.off(i, j, k), hsp.off(i, j, k));
 }
 
 // navstokes/kernels/dqdxyz.mako:16-41
@@ -263,32 +374,28 @@ extern "C" int32_t pga_transport(const pga_case *c, pga_view Q, pga_view state, 
   const int ne = d.ns + 4;
   for (int n = 0; n < nboxes; n++) {
     PGA_REQUIRE(box_ok(boxes[n]), "pga_transport: box %d has a negative extent", n);
-    PGA_REQUIRE(covers(Q, boxes[n], ne) && covers(state, boxes[n], 8) && covers(trstate, boxes[n], 2 + d.ns) &&
+    PGA_REQUIRE(covers(Q, boxes[n], ne) && covers(state, boxes[n], nstate(d)) && covers(trstate, boxes[n], 2 + d.ns) &&
                     covers(hsp, boxes[n], d.ns),
                 "pga_transport: box %d reaches outside its arrays", n);
-  }
-  bool known = false;
-#define X(N) known = known || d.ns == N;
-  PGA_NS_CASES(X)
-#undef X
-  if (!known) {
-    set_error("pga_transport: no kernel for %d species", d.ns);
-    return PGA_EUNSUP;
   }
   for (int n = 0; n < nboxes; n++) {
     if (empty(boxes[n])) continue;
     const Box3 b = box3(boxes[n]);
-#define X(N)                                                                                                   \
-  if (d.ns == N) {                                                                                             \
-    if (d.trans == TRANS_CONSTANT)                                                                             \
-      hipLaunchKernelGGL((k_transport<N, TRANS_CONSTANT>), grid_of(b), dim3(TI, TJ), 0, stream_of(stream),     \
-                         c->K, view(Q), view(state), view(trstate), view(hsp), b);                             \
-    else                                                                                                       \
-      hipLaunchKernelGGL((k_transport<N, TRANS_KINETIC>), grid_of(b), dim3(TI, TJ), 0, stream_of(stream),      \
-                         c->K, view(Q), view(state), view(trstate), view(hsp), b);                             \
-  }
-    PGA_NS_CASES(X)
-#undef X
+    const bool done = dispatch_eos(d, [&]<int N, int E>() {
+      if (d.trans == TRANS_CONSTANT)
+        hipLaunchKernelGGL((k_transport<N, TRANS_CONSTANT, E>), grid_of(b), dim3(TI, TJ), 0, stream_of(stream), c->K,
+                           view(Q), view(state), view(trstate), view(hsp), b);
+      else if (d.trans == TRANS_KINETIC)
+        hipLaunchKernelGGL((k_transport<N, TRANS_KINETIC, E>), grid_of(b), dim3(TI, TJ), 0, stream_of(stream), c->K,
+                           view(Q), view(state), view(trstate), view(hsp), b);
+      else if constexpr (E == EOS_REAL)
+        hipLaunchKernelGGL((k_transport<N, TRANS_CHUNG, E>), grid_of(b), dim3(TI, TJ), 0, stream_of(stream), c->K,
+                           view(Q), view(state), view(trstate), view(hsp), b);
+    });
+    if (!done) {
+      set_error("pga_transport: no kernel for %d species with eos %d", d.ns, d.eos);
+      return PGA_EUNSUP;
+    }
   }
   return launched("pga_transport");
 }
@@ -303,7 +410,7 @@ extern "C" int32_t pga_grads(const pga_case *c, pga_view Q, pga_view state, pga_
   const pga_case_desc &d = c->desc;
   const int ne = d.ns + 4;
   const int one[3] = {1, 1, 1};
-  PGA_REQUIRE(covers(Q, box, one, one, ne) && covers(state, box, one, one, 8),
+  PGA_REQUIRE(covers(Q, box, one, one, ne) && covers(state, box, one, one, nstate(d)),
               "pga_grads: the stencil reaches outside Q or the state (one halo layer needed)");
   PGA_REQUIRE(covers(Jinv, box, 9) && covers(grads, box, 3 * (ne - 1)), "pga_grads: the box reaches outside Jinv or grads");
   PGA_REQUIRE(Q.sj == state.sj && Q.sk == state.sk, "pga_grads: Q and state must share their layout");
@@ -337,7 +444,7 @@ extern "C" int32_t pga_diffflux(const pga_case *c, int32_t axis, double alpha, p
   int lo[3] = {0, 0, 0};
   const int hi[3] = {0, 0, 0};
   lo[axis] = 1;
-  PGA_REQUIRE(covers(Q, box, lo, hi, ne) && covers(state, box, lo, hi, 8) && covers(trstate, box, lo, hi, 2 + d.ns) &&
+  PGA_REQUIRE(covers(Q, box, lo, hi, ne) && covers(state, box, lo, hi, nstate(d)) && covers(trstate, box, lo, hi, 2 + d.ns) &&
                   covers(hsp, box, lo, hi, d.ns) && covers(grads, box, lo, hi, 3 * (ne - 1)) &&
                   covers(xcc, box, lo, hi, 3),
               "pga_diffflux: the face's cells reach outside a cell array");

@@ -145,4 +145,86 @@ class Tpg(BaseEos):
         super().__init__(cfgsect, TempDepCp)
 
 
-EOS_MODELS = {m.name: m for m in (Cpg, Tpg)}
+class RealGas(BaseEos):
+    """A cubic equation of state on the thermally perfect caloric model
+    (``peregrine/simulator/mixture/eos/realgas/model.py:10-98``):
+    p = Ru T / (v - b) - a(T) / (v^2 + u b v + w b^2), van der Waals one-fluid mixing with
+    k_ij = 0 (a = (sum_n X_n sqrt(a_n))^2, b = sum_n X_n b_n), Soave's alpha(T) =
+    (1 + m (1 - sqrt(T / Tc)))^2 with m a quadratic in the acentric factor. The reference's
+    real gas is Soave-Redlich-Kwong (u = 1, w = 0); ``peng-robinson`` is the same kernels with
+    u = 2, w = -1 and its own constants. A species without a critical point gets one from its
+    Lennard-Jones parameters (Chung's relations backwards, Pitzer's Zc at w = 0). The state
+    carries the compressibility Z as a ninth field."""
+
+    name = "real-gas"
+    kind = 2
+    fields = BaseEos.fields + ("Z",)
+    critical = ("Tcrit", "pcrit", "Vcrit", "acentric")
+    cubic = dict(u=1.0, w=0.0, omega_a=0.4274802, omega_b=0.0866403, m=(0.480, 1.574, -0.176))
+
+    def __init__(self, cfgsect):
+        super().__init__(cfgsect, TempDepCp)
+        self.provides["critpoint"] = self.critical
+
+    def check(self, species):
+        super().check(species)
+        lacking = [name for name, sp in species.items()
+                   if sp.missing(self.critical) and sp.missing(("well", "diam"))]
+        if lacking:
+            raise ValueError(f"{self.name} needs Tcrit, pcrit, Vcrit, acentric (or well and diam) for {', '.join(lacking)}")
+
+    def critpoint(self, species):
+        from peregrine_amd.mixture.constants import kB
+
+        out = {key: [] for key in self.critical}
+        for sp in species.values():
+            if sp.missing(self.critical):
+                Tc = 1.2593 * sp["well"] / kB
+                # m3/kmol, from Chung's sigma = 0.809 Vc^1/3 (angstrom, cm3/mol)
+                Vcmol = (sp["diam"] * 1e10 / 0.809) ** 3 * 1e-3
+                got = dict(Tcrit=Tc, pcrit=0.291 * Ru * Tc / Vcmol, Vcrit=Vcmol / sp["MW"], acentric=0.0)
+            else:
+                got = {key: sp[key] for key in self.critical}
+            for key in self.critical:
+                out[key].append(got[key])
+        return tuple(out[key] for key in self.critical)
+
+    def constants(self, species):
+        c = self.cubic
+        Tc, pc, w = (self.collect(k, species) for k in ("Tcrit", "pcrit", "acentric"))
+        MWinv = 1.0 / self.collect("MW", species)
+        sq = np.sqrt(c["u"] ** 2 - 4.0 * c["w"])
+        sqrtac = np.sqrt(c["omega_a"] / pc) * Ru * Tc
+        b = c["omega_b"] * Ru * Tc / pc
+        m = c["m"][0] + c["m"][1] * w + c["m"][2] * w**2
+        invsqrtTc = Tc**-0.5
+        vshift = np.array([sp.get("vshift", 0.0) for sp in species.values()], float)
+        out = dict(
+            super().constants(species),
+            # the form's quadratic v^2 + u b v + w b^2 = (v + sigma b)(v + eps b)
+            cubic=np.array([c["u"], c["w"], 0.5 * (c["u"] + sq), 0.5 * (c["u"] - sq)]),
+            acentric=w,
+            cubicsqrtac=sqrtac,
+            cubicb=b,
+            # -(da_n/dT) / sqrt(a_n) times sqrt(T) ... sqrt(ac) m / sqrt(Tc)
+            cubicda=sqrtac * m * invsqrtTc,
+            cubicmTc=m * invsqrtTc,
+            cubiconepm=1.0 + m,
+            vshift=vshift,
+        )
+        out.update(cubicsqrtacMW=out["cubicsqrtac"] * MWinv, cubicbMW=b * MWinv, vshiftMW=vshift * MWinv,
+                   cubicdaMW=out["cubicda"] * MWinv)
+        return out
+
+
+class PengRobinson(RealGas):
+    """Peng-Robinson: u = 2, w = -1, Omega_a = 0.45724, Omega_b = 0.07780,
+    m = 0.37464 + 1.54226 w - 0.26992 w^2 (Peng and Robinson, Ind. Eng. Chem. Fundam. 15, 59,
+    1976)."""
+
+    name = "peng-robinson"
+    kind = 2
+    cubic = dict(u=2.0, w=-1.0, omega_a=0.457235529, omega_b=0.077796074, m=(0.37464, 1.54226, -0.26992))
+
+
+EOS_MODELS = {m.name: m for m in (Cpg, Tpg, RealGas, PengRobinson)}

@@ -36,6 +36,8 @@ class Mixture:
         self.chemistry = FiniteRate(cfgsect, self.reactions) if chem else None
         if self.chemistry and cfgsect["eos"] != "tpg":
             raise ValueError("finite-rate needs the thermally perfect caloric model (eos tpg)")
+        if cfgsect["trans"] == "chung" and self.eos.kind != 2:
+            raise ValueError("chung's dense-gas transport reads the critical point of a real-gas eos")
         self.species = Species.build(usersp, self.models)
         self.spnames = list(self.species)
         self.ns = len(self.spnames)

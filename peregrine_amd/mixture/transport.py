@@ -40,6 +40,125 @@ class Wilke(BaseModel):
         return dict(MWqinv=MW**-0.25, wilkepair=1.0 / np.sqrt(8.0 * (1.0 + MW[:, None] / MW[None, :])))
 
 
-TRANSPORT_MODELS = {m.name: m for m in (ConstantProps,)}
+class KineticTheory(BaseModel):
+    """Dilute-gas viscosity and conductivity from the Lennard-Jones (Stockmayer) parameters, fitted
+    in ln T over the case's T-range (``peregrine/simulator/mixture/transport/kinetic.py:12-141``):
+
+    - mu = 5/16 sqrt(pi m kB T) / (pi sigma^2 Omega(2,2)*), fitted as sqrt(mu) / T^1/4;
+    - kappa by the Warnatz form of the Eucken correction: self-diffusion
+      D = 3/16 sqrt(2 pi / m_r) (kB T)^3/2 / (pi sigma^2 Omega(1,1)*), Parker's rotational
+      relaxation Z_rot(T) = Z_rot(298) F(298) / F(T), and translational, rotational and internal
+      shares of cv; fitted as kappa / sqrt(T);
+
+    a polar/non-polar pair's well and diameter corrected for induction, and delta* from the
+    dipole. The fits take the lowest degree within refit-tol (at most refit-max-deg), as the
+    caloric refits do."""
+
+    name = "kinetic-theory"
+    kind = 1
+
+    def __init__(self, cfgsect):
+        super().__init__(cfgsect)
+        self.needs += ("well", "diam", "dipole", "polarize", "zrot", "geometry")
+        # the conductivity reads cppoly, which the eos stores first; mupoly before kappapoly
+        self.provides.update(mupoly="mupoly", kappapoly="kappapoly")
+
+    def constants(self, species):
+        from peregrine_amd.mixture.eos import NT
+
+        return dict(mupoly=self.collect_poly("mupoly", species, NT), kappapoly=self.collect_poly("kappapoly", species, NT))
+
+    def _params(self, species):
+        from peregrine_amd.mixture.constants import avogadro, epsilon0
+        from peregrine_amd.mixture.integrals import CollisionIntegrals
+        from peregrine_amd.mixture.species import Species
+
+        MW = self.collect("MW", species)
+        well, diam = self.collect("well", species), self.collect("diam", species)
+        dipole = self.collect("dipole", species)
+        rotdof = np.array([Species.rotdof[sp["geometry"]] for sp in species.values()])
+        mass = MW / avogadro
+        # the reduced dipole of each species with itself (the pair terms a mixture rule would read
+        # are the diagonal here: a species' own viscosity and conductivity)
+        delta = 0.5 * dipole**2 / (4.0 * np.pi * epsilon0 * well * diam**3)
+        return dict(Ts=np.linspace(*self.trange, 50), ci=CollisionIntegrals(), MW=MW, well=well, diam=diam,
+                    mass=mass, delta=delta, zrot=self.collect("zrot", species), rotdof=rotdof)
+
+    def _fit(self, lnT, values):
+        from peregrine_amd.mixture.fits import fit
+
+        tol, deg = self.cfgsect["refit-tol"], self.cfgsect["refit-max-deg"]
+        return [fit(lnT, v, tol, deg)[0] for v in values.T]
+
+    def mupoly(self, species):
+        from peregrine_amd.mixture.constants import kB
+
+        p = self._params(species)
+        Ts = p["Ts"]
+        Tstar = np.outer(Ts, kB / p["well"])
+        omega22 = p["ci"].omega22(Tstar, np.broadcast_to(p["delta"], Tstar.shape))
+        mu = (5.0 / 16.0) * np.sqrt(np.pi * p["mass"] * kB * Ts[:, None]) / (np.pi * p["diam"] ** 2 * omega22)
+        return self._fit(np.log(Ts), np.sqrt(mu / np.sqrt(Ts)[:, None]))
+
+    def kappapoly(self, species):
+        from peregrine_amd.mixture.constants import Ru, Tref, kB
+        from peregrine_amd.mixture.fits import evaluate
+
+        p = self._params(species)
+        Ts = p["Ts"]
+        lnT = np.log(Ts)
+        T = Ts[:, None]
+        sps = list(species.values())
+        mu = np.array([evaluate(sp["mupoly"], lnT) for sp in sps]).T ** 2 * np.sqrt(T)
+        cpR = np.array([evaluate(sp["cppoly"], lnT) for sp in sps]).T
+        Tstar = np.outer(Ts, kB / p["well"])
+        delta = np.broadcast_to(p["delta"], Tstar.shape)
+        omega11 = p["ci"].omega22(Tstar, delta) / p["ci"].astar(Tstar, delta)
+        # a species with itself: the reduced mass is half the molecular mass
+        mr = 0.5 * p["mass"]
+        D = (3.0 / 16.0) * np.sqrt(2.0 * np.pi / mr) * (kB * T) ** 1.5 / (np.pi * p["diam"] ** 2 * omega11)
+
+        def parker(ts):
+            return 1.0 + np.pi**1.5 / np.sqrt(ts) * (0.5 + 1.0 / ts) + (0.25 * np.pi**2 + 2.0) / ts
+
+        MW, rotdof = p["MW"], p["rotdof"]
+        fint = MW / (Ru * T) * D / mu
+        A = 2.5 - fint
+        zrot = p["zrot"] * parker(kB * Tref / p["well"]) / parker(Tstar)
+        B = zrot + 2.0 / np.pi * (5.0 / 3.0 * rotdof + fint)
+        c1 = 2.0 / np.pi * A / B
+        cvint = cpR - 2.5 - rotdof
+        frot = fint * (1.0 + c1)
+        ftrans = 2.5 * (1.0 - c1 * rotdof / 1.5)
+        kappa = mu / MW * Ru * (ftrans * 1.5 + frot * rotdof + fint * cvint)
+        return self._fit(lnT, kappa / np.sqrt(T))
+
+
+class ChungDenseGas(KineticTheory):
+    """Chung's dense-gas corrections on the kinetic-theory dilute values
+    (``peregrine/simulator/mixture/transport/chung.py:9-33``, ``kernels/chung.mako``): Chung,
+    Ajlan, Lee and Starling, Ind. Eng. Chem. Res. 27, 671 (1988) as Poling et al. (5th ed.,
+    9-5, 9-6, 10-5) state it, without the dipole and association terms, one-fluid mixing of
+    sigma, epsilon, M and the acentric factor. Its sigma (angstrom) and epsilon/k (K) come from
+    the real gas's critical point."""
+
+    name = "chung"
+
+    def constants(self, species):
+        from peregrine_amd.mixture.species import database
+
+        chung = database("chung")
+        MW = self.collect("MW", species)
+        Vc = self.collect("Vcrit", species) * MW * 1e3
+        return dict(
+            super().constants(species),
+            chungsigma=0.809 * Vc ** (1.0 / 3.0),
+            chungepsk=self.collect("Tcrit", species) / 1.2593,
+            chungvisc=np.array(chung["viscosity"], dtype=np.float64),
+            chungcond=np.array(chung["conductivity"], dtype=np.float64),
+        )
+
+
+TRANSPORT_MODELS = {m.name: m for m in (ConstantProps, KineticTheory, ChungDenseGas)}
 DIFFUSION_MODELS = {m.name: m for m in (Lewis,)}
 MIXRULE_MODELS = {m.name: m for m in (Wilke,)}

@@ -120,7 +120,9 @@ def const_layout(ns):
     """{section: (offset, count)} of the species constant table, and its size."""
     L = lib()
     names = ("MW", "MWinv", "Rsp", "cppolyR", "h298R", "cp0", "gpoly", "h298", "mu0", "kappa0", "lewis", "hpoly",
-             "MWqinv", "wilkepair", "mupoly", "kappapoly")
+             "MWqinv", "wilkepair", "mupoly", "kappapoly", "cppoly", "cubic", "cubicsqrtac", "cubicb", "cubicda",
+             "cubicmTc", "cubiconepm", "cubicsqrtacMW", "cubicbMW", "vshift", "vshiftMW", "cubicdaMW", "acentric",
+             "chungsigma", "chungepsk", "chungvisc", "chungcond")
     out = {}
     for name in names:
         count = _i32(0)

@@ -14,11 +14,11 @@ from peregrine_amd.config import ConfigError
 from peregrine_amd.misc import find_subclass
 from peregrine_amd.mixture import get_mixture
 
-EOS = {"cpg": 0, "tpg": 1}
+EOS = {"cpg": 0, "tpg": 1, "real-gas": 2, "peng-robinson": 2}
 RECON = {"constant": 0, "muscl": 1}
 LIMITER = {None: 0, "koren": 1, "minmod": 2, "mc": 3, "superbee": 4, "vanleer": 5}
 PRIMARY = {"kepaec": 0, "hllc": 1, "rusanov": 2}
-TRANS = {None: -1, "constant": 0, "kinetic-theory": 1}
+TRANS = {None: -1, "constant": 0, "kinetic-theory": 1, "chung": 2}
 
 
 class IoVars:
@@ -85,6 +85,11 @@ class EulerSimulator(BaseSimulator):
     @property
     def ne(self):
         return 4 + self.mixture.ns
+
+    @property
+    def nstate(self):
+        """The state fields per cell: p, T, 1/rho, rho, e, cp, gamma, c, and Z for a real gas."""
+        return len(self.mixture.eos.fields)
 
     @property
     def ng(self):

@@ -19,7 +19,6 @@ from peregrine_amd.integrators import get_integrator
 from peregrine_amd.mesher import Interface
 from peregrine_amd.simulator import NativeCase, get_simulator
 
-NSTATE = 8
 
 
 class Block:
@@ -91,7 +90,7 @@ class Block:
         ng, cs = self.ng, self.cellshape
         ne = sim.ne
         self.regs = [backend.alloc(cs + (ne,), name=f"Q{r}") for r in range(nregs)]
-        self.state = backend.alloc(cs + (NSTATE,), name="state")
+        self.state = backend.alloc(cs + (sim.nstate,), name="state")
         self.iF, self.jF, self.kF = (backend.alloc(self.faceshape(a) + (ne,), name=f"{'ijk'[a]}F") for a in range(3))
         self.iS, self.jS, self.kS = (backend.alloc(self.faceshape(a) + (3,), name=f"{'ijk'[a]}S") for a in range(3))
         self.detJinv = backend.alloc(cs + (1,), name="detJinv")

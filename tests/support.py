@@ -60,6 +60,10 @@ def config(gas="air-cpg", recon="constant", primary="kepaec", limiter=None, chem
         m["species"], m["eos"], m["T-range"] = AIR_NASA7, "tpg", (300.0, 3500.0)
     elif gas == "binary":
         m["species"], m["eos"] = BINARY_CPG, "cpg"
+    elif gas in ("o2n2-srk", "o2n2-pr"):
+        # the transcritical pair: library O2 and N2 (JANAF thermo, critical constants), a cubic real gas
+        m["species"], m["eos"], m["T-range"] = ["O2", "N2"], "real-gas" if gas == "o2n2-srk" else "peng-robinson", (
+            100.0, 1000.0)
     elif gas == "h2o2":
         m["species"], m["eos"], m["T-range"] = "h2o2.yaml", "tpg", (300.0, 3500.0)
         if chem:
@@ -87,7 +91,7 @@ def to_host(a):
 class HostBlock:
     """One periodic cube block's arrays on the host, in the device layout."""
 
-    def __init__(self, ncell, ng, ne, nregs, lengths=(1.0, 1.0, 1.0), jostle=0.0, seed=0):
+    def __init__(self, ncell, ng, ne, nregs, lengths=(1.0, 1.0, 1.0), jostle=0.0, seed=0, nstate=8):
         self.ncell, self.ng, self.ne = tuple(ncell), ng, ne
         nodes = [np.linspace(0.0, L, n + 1) for L, n in zip(lengths, ncell)]
         inner = np.stack(np.meshgrid(*nodes, indexing="ij"), axis=-1)
@@ -103,7 +107,7 @@ class HostBlock:
         self.nodes = grid.halo_nodes(inner, ng, (True, True, True), lengths)
         self.cshape = tuple(n + 2 * ng for n in ncell)
         self.regs = [np.zeros((ne,) + self.cshape[::-1]) for _ in range(nregs)]
-        self.state = np.zeros((8,) + self.cshape[::-1])
+        self.state = np.zeros((nstate,) + self.cshape[::-1])
         self.S = [to_dev(grid.areavecs(self.nodes, a)) for a in range(3)]
         self.F = [np.zeros((ne,) + s.shape[1:]) for s in self.S]
         self.detJinv = to_dev(grid.detJinv(self.nodes, ng)[..., None])

@@ -253,7 +253,7 @@ def test_halo_copies(dev):
 
 
 def _run_pair(gas, recon, primary, limiter, chem, integrator, ncell, steps, dt, prims, lengths, fused=True,
-              trans=None, species=None):
+              trans=None, species=None, nstate=8):
     """The same case stepped by the solver on the device and by the oracle on the host."""
     import peregrine_amd as pg
 
@@ -268,7 +268,7 @@ def _run_pair(gas, recon, primary, limiter, chem, integrator, ncell, steps, dt, 
     sim = mb.sim
     oc = O.OracleCase(sim.mixture, recon, limiter, primary)
     nregs = 4 if integrator == "rk4" else 3
-    hb = sp.HostBlock(ncell, sim.ng, sim.ne, nregs, lengths=lengths)
+    hb = sp.HostBlock(ncell, sim.ng, sim.ne, nregs, lengths=lengths, nstate=nstate)
     q = prims(hb.xcc)
     mb.set_prims([q])
     full = O.box((0, 0, 0), hb.cshape)
@@ -437,16 +437,18 @@ def test_blocks_of_a_lattice_step_as_one_block(dev, fused):
 # --- the viscous terms -----------------------------------------------------------------------
 
 
-@pytest.mark.parametrize("gas", ["air-tpg", "binary"])
-def test_transport_gradients_and_viscous_flux(dev, gas):
+@pytest.mark.parametrize("gas,trans", [("air-tpg", "constant"), ("binary", "constant"), ("h2o2", "kinetic-theory")])
+def test_transport_gradients_and_viscous_flux(dev, gas, trans):
     """pga_transport, pga_grads and pga_diffflux against the oracle on a jostled block: the same
     operations in the same order, so the same bits."""
-    cfg = sp.config(gas, trans="constant")
+    cfg = sp.config(gas, trans=trans)
     sim = get_simulator(cfg)
     nc = NativeCase(sim, Mi355xBackend(cfg))
     oc = O.OracleCase(sim.mixture)
     hb = sp.HostBlock((67, 9, 6), 1, sim.ne, 2, jostle=0.3, seed=5)
     q = sp.smooth_prims(hb.xcc, sim.ne, ns=sim.ns)
+    if gas == "h2o2":
+        q[..., 5:] *= 0.1
     _prepared(sim, oc, hb, q)
     L = native.lib()
     full = O.box((0, 0, 0), hb.cshape)
@@ -531,4 +533,100 @@ def test_viscous_binary_muscl_steps_match(dev, fused):
     got = mb.blocks[0].Q.get()[g:-g, g:-g, g:-g].transpose(3, 2, 1, 0)
     err = _rowerr(got, hb.regs[0][hb.interior()])
     print(f"viscous binary 10 steps ({'fused' if fused else 'per-kernel'}): {err:.3e}")
+    assert err < 1e-10
+
+
+# --- the cubic real gas ----------------------------------------------------------------------
+
+REAL = ["o2n2-srk", "o2n2-pr"]
+
+
+def _transcritical(xcc, ne, seed=7):
+    """60 bar, O2 at 120 K below N2 at 300 K across a tanh layer in y, with a shear and a small
+    seeded perturbation: both sides of the pseudo-critical point in one block."""
+    rng = np.random.default_rng(seed)
+    y = xcc[..., 1]
+    Ly = xcc[..., 1].max() + xcc[..., 1].min()
+    s = np.tanh((y - 0.5 * Ly) / (0.1 * Ly))
+    q = np.zeros(xcc.shape[:3] + (ne,))
+    q[..., 0] = 6e6
+    q[..., 1] = 10.0 * s + 0.5 * np.sin(2 * np.pi * xcc[..., 0] / Ly + rng.uniform(0, 6.3))
+    q[..., 2] = 0.5 * np.sin(2 * np.pi * xcc[..., 2] / Ly + rng.uniform(0, 6.3))
+    q[..., 4] = 210.0 - 90.0 * s
+    q[..., 5] = 0.5 * (1.0 - s) * 0.999 + 0.0005
+    return q
+
+
+@pytest.mark.parametrize("gas", REAL)
+def test_real_gas_fromprims_and_fromQ(dev, gas):
+    cfg, sim, nc, oc = _case(gas)
+    hb = sp.HostBlock((70, 9, 5), 1, sim.ne, 2, jostle=0.2, nstate=9, lengths=(0.7e-3, 0.09e-3, 0.05e-3))
+    q = _transcritical(hb.xcc, sim.ne)
+    P = sp.to_dev(q)
+    full = O.box((0, 0, 0), hb.cshape)
+    O.lib.pgo_fromprims(oc.ref, O.arr(P), O.arr(hb.regs[0]), O.arr(hb.state), full)
+    dP, dQ, dS = dev.put(P), dev.put(np.zeros_like(hb.regs[0])), dev.put(np.zeros_like(hb.state))
+    L = native.lib()
+    native.check(L.pga_fromprims(nc.handle, dev.view(dP), dev.view(dQ), dev.view(dS), native.box((0, 0, 0), hb.cshape), None),
+                 "pga_fromprims")
+    # the cubic's root takes cbrt or acos/cos: the device and host libraries round those apart
+    err = max(_rowerr(dev.get(dQ), hb.regs[0]), _rowerr(dev.get(dS), hb.state))
+    print(f"fromprims {gas}: {err:.3e}")
+    assert err < 1e-12
+    st = hb.state.copy()
+    st[1] *= 1.02
+    dS = dev.put(st)
+    O.lib.pgo_fromQ(oc.ref, O.arr(hb.regs[0]), O.arr(st), full)
+    arr = (native.Box * 1)(native.box((0, 0, 0), hb.cshape))
+    native.check(L.pga_fromQ(nc.handle, dev.view(dev.put(hb.regs[0])), dev.view(dS), arr, 1, None), "pga_fromQ")
+    err = _rowerr(dev.get(dS), st)
+    print(f"fromQ {gas}: {err:.3e}")
+    assert err < 1e-12
+
+
+@pytest.mark.parametrize("gas", REAL)
+def test_real_gas_muscl_faces_and_chung_transport(dev, gas):
+    cfg = sp.config(gas, recon="muscl", primary="rusanov", limiter="koren", trans="chung")
+    sim = get_simulator(cfg)
+    nc = NativeCase(sim, Mi355xBackend(cfg))
+    oc = O.OracleCase(sim.mixture, "muscl", "koren", "rusanov")
+    hb = sp.HostBlock((67, 9, 6), sim.ng, sim.ne, 3, jostle=0.3, seed=5, nstate=9, lengths=(0.67e-3, 0.09e-3, 0.06e-3))
+    q = _transcritical(hb.xcc, sim.ne)
+    full = O.box((0, 0, 0), hb.cshape)
+    O.lib.pgo_fromprims(oc.ref, O.arr(sp.to_dev(q)), O.arr(hb.regs[0]), O.arr(hb.state), full)
+    ng, L = hb.ng, native.lib()
+    dQv, dSt = dev.put(hb.regs[0]), dev.put(hb.state)
+    for a in range(3):
+        ext = list(hb.ncell)
+        ext[a] += 1
+        O.lib.pgo_flux(oc.ref, a, O.arr(hb.regs[0]), O.arr(hb.state), O.arr(hb.S[a]), O.arr(hb.F[a]), O.box((ng,) * 3, ext))
+        F = dev.put(np.zeros_like(hb.F[a]))
+        native.check(L.pga_flux(nc.handle, a, dev.view(dQv), dev.view(dSt), dev.view(dev.put(hb.S[a])), dev.view(F),
+                                native.box((ng,) * 3, ext), None), "pga_flux")
+        err = _rowerr(dev.get(F), hb.F[a])
+        print(f"real-gas flux {gas} axis {a}: {err:.3e}")
+        assert err < 1e-12
+    O.lib.pgo_transport(oc.ref, O.arr(hb.regs[0]), O.arr(hb.state), O.arr(hb.trstate), O.arr(hb.hsp), full)
+    dTr, dH = dev.put(np.zeros_like(hb.trstate)), dev.put(np.zeros_like(hb.hsp))
+    boxes = (native.Box * 1)(native.box((0, 0, 0), hb.cshape))
+    native.check(L.pga_transport(nc.handle, dev.view(dQv), dev.view(dSt), dev.view(dTr), dev.view(dH), boxes, 1, None),
+                 "pga_transport")
+    err = max(_rowerr(dev.get(dTr), hb.trstate), _rowerr(dev.get(dH), hb.hsp))
+    print(f"chung transport {gas}: {err:.3e}")
+    assert err < 1e-12
+
+
+@pytest.mark.parametrize("fused", [True, False], ids=["fused", "per-kernel"])
+def test_transcritical_mixing_layer_steps_match(dev, fused):
+    """O2/N2 at 60 bar, Peng-Robinson with Chung transport, MUSCL-Koren + Rusanov, Navier-Stokes,
+    tvd-rk3, 10 steps on a periodic 48 x 24 x 8 block (10 um cells); 1e-10 per conserved row."""
+    ncell = (48, 24, 8)
+    L = tuple(1e-5 * n for n in ncell)
+    mb, hb = _run_pair("o2n2-pr", "muscl", "rusanov", "koren", False, "tvd-rk3", ncell, 10, 2e-9,
+                       lambda x: _transcritical(x, 6), L, fused=fused, trans="chung", nstate=9)
+    assert mb.fused == fused
+    g = hb.ng
+    got = mb.blocks[0].Q.get()[g:-g, g:-g, g:-g].transpose(3, 2, 1, 0)
+    err = _rowerr(got, hb.regs[0][hb.interior()])
+    print(f"transcritical 10 steps ({'fused' if fused else 'per-kernel'}): {err:.3e}")
     assert err < 1e-10
