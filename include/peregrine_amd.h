@@ -196,6 +196,23 @@ int32_t pga_diffflux(const pga_case *c, int32_t axis, double alpha, pga_view Q, 
                      pga_view hsp, pga_view grads, pga_view xcc, pga_view xf, pga_view S, pga_view F, pga_box box,
                      void *stream);
 
+/* --- boundaries -------------------------------------------------------------------------- */
+
+/* The adiabatic slip wall's halo on face `side` (0 low, 1 high) of `axis` of a block of ncell
+ * interior cells with ng halo layers: each halo cell h planes out takes the interior cell h planes
+ * in, its momentum's normal part (the wall face's unit normal from S) taken off twice, its state
+ * copied. Replaces the preadvflux hook of slp-adia-wall:
+ * peregrine/simulator/euler/kernels/bcs/slp-adia-wall.mako. */
+int32_t pga_wall_mirror(const pga_case *c, int32_t axis, int32_t side, const int32_t *ncell, int32_t ng, pga_view Q,
+                        pga_view state, pga_view S, void *stream);
+
+/* The flux through the same wall's faces, written over F: the pressure p_w = max(p + rho c u_n, 0)
+ * of the cell next to the wall, and for a viscous case (trstate non-null) the normal stress of the
+ * two-point normal velocity gradient, no heat flux. Replaces the postflux hook of slp-adia-wall:
+ * peregrine/simulator/euler/kernels/bcs/wall.mako, navstokes/kernels/bcs/{slp-adia-wall,wall}.mako. */
+int32_t pga_wall_flux(const pga_case *c, int32_t axis, int32_t side, const int32_t *ncell, int32_t ng, pga_view Q,
+                      pga_view state, pga_view trstate, pga_view xcc, pga_view xf, pga_view S, pga_view F, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

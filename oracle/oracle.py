@@ -63,6 +63,7 @@ class Block(C.Structure):
         ("detJinv", Arr), ("integrator", C.c_int), ("chemistry", C.c_int),
         ("viscous", C.c_int), ("alpha", C.c_double),
         ("trstate", Arr), ("hsp", Arr), ("grads", Arr), ("Jinv", Arr), ("xcc", Arr), ("xf", Arr * 3),
+        ("bc", C.c_int * 6), ("nstate", C.c_int),
     ]
 
 
@@ -76,6 +77,8 @@ def _load():
     lib.pgo_texp.argtypes = [C.c_double]
     lib.pgo_elog.restype = C.c_double
     lib.pgo_elog.argtypes = [C.c_double]
+    lib.pgo_dcbrt.restype = C.c_double
+    lib.pgo_dcbrt.argtypes = [C.c_double]
     lib.pgo_threads.restype = C.c_int
     P = C.POINTER
     lib.pgo_fromQ.argtypes = [P(Case), Arr, Arr, Box]
@@ -87,11 +90,15 @@ def _load():
     lib.pgo_combine.argtypes = [C.c_int, Arr, C.c_int, C.c_double, C.c_int, P(Arr), _dp, C.c_double, C.c_int, Box]
     lib.pgo_fill_periodic.argtypes = [Arr, C.c_int, P(C.c_int), C.c_int, C.c_int, C.c_int]
     lib.pgo_transport.argtypes = [P(Case), Arr, Arr, Arr, Arr, Box]
+    lib.pgo_wall_mirror.argtypes = [P(Case), C.c_int, C.c_int, P(C.c_int), C.c_int, C.c_int, Arr, Arr, Arr]
+    lib.pgo_wall_flux.argtypes = [P(Case), C.c_int, C.c_int, P(C.c_int), C.c_int, C.c_int, Arr, Arr, Arr, Arr, Arr,
+                                  Arr, Arr]
     lib.pgo_grads.argtypes = [P(Case), Arr, Arr, Arr, Arr, Box]
     lib.pgo_diffflux.argtypes = [P(Case), C.c_int, C.c_double, Arr, Arr, Arr, Arr, Arr, Arr, Arr, Arr, Arr, Box]
     lib.pgo_step.argtypes = [P(Case), P(Block), C.c_double]
     lib.pgo_stage.argtypes = [P(Case), P(Block), C.c_int, C.c_double]
     lib.pgo_advance.argtypes = [P(Case), P(Block), C.c_int, C.c_double]
+    lib.pgo_rhs.argtypes = [P(Case), P(Block), C.c_int, C.c_int]
     return lib
 
 
@@ -99,6 +106,7 @@ lib = _load()
 tlog = lib.pgo_tlog
 texp = lib.pgo_texp
 elog = lib.pgo_elog
+dcbrt = lib.pgo_dcbrt
 
 EOS = {"cpg": 0, "tpg": 1, "real-gas": 2, "peng-robinson": 2}
 TRANS = {None: -1, "constant": 0, "kinetic-theory": 1, "chung": 2}

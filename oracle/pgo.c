@@ -246,7 +246,24 @@ static void dimless(const pgo_case *c, cubicpt *q, double p, double T) {
 /* eos.mako:86-89 */
 static void rootslope(cubicpt *q) { q->invslope = 1.0 / (3.0 * q->Z * q->Z + 2.0 * q->Z * q->z2 + q->z1); }
 
-/* eos.mako:46-83: Cardano; of the roots above B*, the least Gibbs energy departure */
+/* the cube root by an exponent split and Newton steps in plain IEEE arithmetic: the device's
+ * dcbrt, the same sequence (x = a 2^(3q), a in [0.5, 4), six steps from a linear start) */
+double pgo_dcbrt(double x) {
+  if (x == 0.0 || !(fabs(x) <= DBL_MAX)) return x;
+  int e;
+  const double m = frexp(fabs(x), &e);
+  int q = e / 3, r = e - 3 * q;
+  if (r < 0) r += 3, q -= 1;
+  const double a = ldexp(m, r);
+  double y = 0.75 + 0.25 * a;
+  for (int k = 0; k < 6; k++) y = y - (y * y * y - a) / (3.0 * (y * y));
+  return copysign(ldexp(y, q), x);
+}
+
+/* eos.mako:46-83: the cubic's root; of the roots above B*, the least Gibbs energy departure's.
+ * One real root: Cardano (with pgo_dcbrt). Three: the largest by Newton from its upper bound
+ * 2 sqrt(Q) - z2/3, the other two from the deflated quadratic -- the roots the reference takes by
+ * the trigonometric form, here in IEEE operations only so the device gives the same bits. */
 static void zsolve(const pgo_case *c, cubicpt *q, double p, double T) {
   dimless(c, q, p, T);
   const double B = q->Bstar, B2 = B * B, z1 = q->z1, z2 = q->z2;
@@ -256,14 +273,28 @@ static void zsolve(const pgo_case *c, cubicpt *q, double p, double T) {
   const double M = R * R - Q * Q * Q;
   const double z2o3 = z2 * (1.0 / 3.0);
   if (M > 0.0) {
-    const double S = -copysign(cbrt(fabs(R) + sqrt(M)), R);
+    const double S = -copysign(pgo_dcbrt(fabs(R) + sqrt(M)), R);
     q->Z = S + Q / S - z2o3;
   } else {
-    const double sqQ = sqrt(Q), twopio3 = 2.0943951023931954923;
-    const double theta3 = acos(R / (Q * sqQ)) * (1.0 / 3.0);
-    const double roots[3] = {-2.0 * sqQ * cos(theta3) - z2o3, -2.0 * sqQ * cos(theta3 + twopio3) - z2o3,
-                             -2.0 * sqQ * cos(theta3 - twopio3) - z2o3};
-    double Z = fmax(roots[0], fmax(roots[1], roots[2]));
+    double r0 = 2.0 * sqrt(Q) - z2o3;
+    for (int k = 0; k < 100; k++) {
+      const double f = ((r0 + z2) * r0 + z1) * r0 + z0;
+      const double df = (3.0 * r0 + 2.0 * z2) * r0 + z1;
+      if (!(df > 0.0)) break;
+      const double dr = f / df;
+      r0 -= dr;
+      if (!(fabs(dr) > 1e-15 * fabs(r0))) break;
+    }
+    const double bq = z2 + r0, cq = z1 + bq * r0, disc = bq * bq - 4.0 * cq;
+    double r1 = r0, r2 = r0;
+    if (disc >= 0.0) {
+      const double qq = -0.5 * (bq + copysign(sqrt(disc), bq));
+      const double ra = qq, rb = qq != 0.0 ? cq / qq : 0.0;
+      r1 = fmax(ra, rb);
+      r2 = fmin(ra, rb);
+    }
+    const double roots[3] = {r0, r1, r2};
+    double Z = r0;
     int above = 0;
     for (int k = 0; k < 3; k++) above += roots[k] > B;
     if (above > 1) {
@@ -950,7 +981,7 @@ static void chung(const pgo_case *c, const double *X, double T, double lnT, doub
     }
   }
   const double epsk = epssigma3 / sigma3;
-  const double cbrtsigma3 = cbrt(sigma3);
+  const double cbrtsigma3 = pgo_dcbrt(sigma3);
   const double sigma2 = cbrtsigma3 * cbrtsigma3;
   const double sqrtMm = masssum / (epsk * sigma2);
   const double Mm = sqrtMm * sqrtMm;
@@ -958,7 +989,7 @@ static void chung(const pgo_case *c, const double *X, double T, double lnT, doub
   const double Tc = 1.2593 * epsk;
   const double Vc = sigma3 / (0.809 * 0.809 * 0.809);
   const double Fc = 1.0 - 0.2756 * omega;
-  const double cbrtVc = cbrt(Vc);
+  const double cbrtVc = pgo_dcbrt(Vc);
   const double Vc23 = cbrtVc * cbrtVc;
   const double Mprime = Mm * 1.0e-3;
   double E[10], B[7];
@@ -966,12 +997,15 @@ static void chung(const pgo_case *c, const double *X, double T, double lnT, doub
   for (int i = 0; i < 7; i++) B[i] = c->chungcond[4 * i] + c->chungcond[4 * i + 1] * omega;
   const double Tr = T / Tc;
   const double Tstar = T / epsk, invTstar = epsk / T;
-  const double Omegav = 1.16145 * pow(Tstar, -0.14874) + 0.52487 * exp(-0.77320 * Tstar) + 2.16178 * exp(-2.43787 * Tstar);
+  /* pow and exp by the table methods (the device evaluates the same) */
+  const double Omegav = 1.16145 * pgo_texp(-0.14874 * pgo_elog(Tstar)) + 0.52487 * pgo_texp(-0.77320 * Tstar) +
+                        2.16178 * pgo_texp(-2.43787 * Tstar);
   const double Yy = rho / Mm * 1.0e-3 * Vc / 6.0;
   const double omYy = 1.0 - Yy;
   const double G1 = (1.0 - 0.5 * Yy) / (omYy * omYy * omYy);
-  const double G2 = (E[0] * (1.0 - exp(-E[3] * Yy)) / Yy + E[1] * G1 * exp(E[4] * Yy) + E[2] * G1) / (E[0] * E[3] + E[1] + E[2]);
-  const double etastarstar = E[6] * Yy * Yy * G2 * exp(E[7] + E[8] * invTstar + E[9] * invTstar * invTstar);
+  const double G2 = (E[0] * (1.0 - pgo_texp(-E[3] * Yy)) / Yy + E[1] * G1 * pgo_texp(E[4] * Yy) + E[2] * G1) /
+                    (E[0] * E[3] + E[1] + E[2]);
+  const double etastarstar = E[6] * Yy * Yy * G2 * pgo_texp(E[7] + E[8] * invTstar + E[9] * invTstar * invTstar);
   const double muratio = (1.0 / G2 + E[5] * Yy) + etastarstar * Omegav / (sqrt(Tstar) * Fc);
   *mu = *mu * muratio;
   const double eta0 = 4.0785e-6 * sqrt(Mm * T) / (Vc23 * Omegav) * Fc;
@@ -980,7 +1014,8 @@ static void chung(const pgo_case *c, const double *X, double T, double lnT, doub
   const double Zc = 2.0 + 10.5 * Tr * Tr;
   const double Psi = 1.0 + alpha * ((0.215 + 0.28288 * alpha - 1.061 * beta + 0.26665 * Zc) / (0.6366 + beta * Zc + 1.061 * alpha * beta));
   const double lambda0 = 31.2 * eta0 * Psi / Mprime;
-  const double H2 = (B[0] * (1.0 - exp(-B[3] * Yy)) / Yy + B[1] * G1 * exp(B[4] * Yy) + B[2] * G1) / (B[0] * B[3] + B[1] + B[2]);
+  const double H2 = (B[0] * (1.0 - pgo_texp(-B[3] * Yy)) / Yy + B[1] * G1 * pgo_texp(B[4] * Yy) + B[2] * G1) /
+                    (B[0] * B[3] + B[1] + B[2]);
   const double kapparatio = (1.0 / H2 + B[5] * Yy) + 3.586e-3 * sqrt(Tc / Mprime) / Vc23 * B[6] * Yy * Yy * H2 * sqrt(Tr) / lambda0;
   *kappa = *kappa * kapparatio;
 }
@@ -1217,6 +1252,96 @@ void pgo_diffflux(const pgo_case *c, int axis, double alpha, pgo_arr Q, pgo_arr 
   }
 }
 
+/* ------------------------------------------------------------------------
+ * the adiabatic slip wall: euler/kernels/bcs/slp-adia-wall.mako, bcs/wall.mako;
+ * navstokes/kernels/bcs/slp-adia-wall.mako, bcs/wall.mako
+ * ---------------------------------------------------------------------- */
+
+/* the halo cell h planes out from the wall takes the interior cell h planes in, its momentum's
+ * normal part (along the wall face's normal at that row) taken off twice, the state copied */
+void pgo_wall_mirror(const pgo_case *c, int axis, int side, const int *ncell, int ng, int nstate, pgo_arr Q,
+                     pgo_arr state, pgo_arr S) {
+  const int n = ncell[axis];
+  const int t1 = (axis + 1) % 3, t2 = (axis + 2) % 3;
+  for (int h = 0; h < ng; h++) {
+    const int halo = side ? ng + n + h : ng - 1 - h, mirror = side ? ng + n - 1 - h : ng + h, face = side ? ng + n : ng;
+#pragma omp parallel for collapse(2)
+    for (int b2 = ng; b2 < ng + ncell[t2]; b2++)
+      for (int b1 = ng; b1 < ng + ncell[t1]; b1++) {
+        int ih[3], im[3], iface[3];
+        ih[axis] = halo, im[axis] = mirror, iface[axis] = face;
+        ih[t1] = im[t1] = iface[t1] = b1;
+        ih[t2] = im[t2] = iface[t2] = b2;
+        const double Sx = *at(S, iface[0], iface[1], iface[2], 0), Sy = *at(S, iface[0], iface[1], iface[2], 1),
+                     Sz = *at(S, iface[0], iface[1], iface[2], 2);
+        const double Sinv = 1.0 / sqrt(Sx * Sx + Sy * Sy + Sz * Sz);
+        const double nx = Sx * Sinv, ny = Sy * Sinv, nz = Sz * Sinv;
+        const double q1 = *at(Q, im[0], im[1], im[2], 1), q2 = *at(Q, im[0], im[1], im[2], 2),
+                     q3 = *at(Q, im[0], im[1], im[2], 3);
+        const double mn = 2.0 * (q1 * nx + q2 * ny + q3 * nz);
+        *at(Q, ih[0], ih[1], ih[2], 0) = *at(Q, im[0], im[1], im[2], 0);
+        *at(Q, ih[0], ih[1], ih[2], 1) = q1 - mn * nx;
+        *at(Q, ih[0], ih[1], ih[2], 2) = q2 - mn * ny;
+        *at(Q, ih[0], ih[1], ih[2], 3) = q3 - mn * nz;
+        for (int l = 4; l < c->ne; l++) *at(Q, ih[0], ih[1], ih[2], l) = *at(Q, im[0], im[1], im[2], l);
+        for (int f = 0; f < nstate; f++) *at(state, ih[0], ih[1], ih[2], f) = *at(state, im[0], im[1], im[2], f);
+      }
+  }
+}
+
+/* the flux through the wall's faces from the interior cell: the pressure p_w = max(p + rho c u_n,
+ * 0) (wall.mako wallpressure: the acoustic star pressure to first order, u_n toward the wall);
+ * a viscous case adds the normal stress of the two-point normal velocity gradient and no heat
+ * flux (navstokes wall.mako wallfaceflux, slip: the normal component alone) */
+void pgo_wall_flux(const pgo_case *c, int axis, int side, const int *ncell, int ng, int viscous, pgo_arr Q,
+                   pgo_arr state, pgo_arr trstate, pgo_arr xcc, pgo_arr xf, pgo_arr S, pgo_arr F) {
+  const int n = ncell[axis];
+  const int t1 = (axis + 1) % 3, t2 = (axis + 2) % 3;
+  const int face = side ? ng + n : ng, cell = side ? ng + n - 1 : ng;
+  const double outward = side ? 1.0 : -1.0;
+#pragma omp parallel for collapse(2)
+  for (int b2 = ng; b2 < ng + ncell[t2]; b2++)
+    for (int b1 = ng; b1 < ng + ncell[t1]; b1++) {
+      int f[3], ic[3];
+      f[axis] = face, ic[axis] = cell;
+      f[t1] = ic[t1] = b1;
+      f[t2] = ic[t2] = b2;
+      const double S3[3] = {*at(S, f[0], f[1], f[2], 0), *at(S, f[0], f[1], f[2], 1), *at(S, f[0], f[1], f[2], 2)};
+      const double q[4] = {*at(Q, ic[0], ic[1], ic[2], 0), *at(Q, ic[0], ic[1], ic[2], 1), *at(Q, ic[0], ic[1], ic[2], 2),
+                           *at(Q, ic[0], ic[1], ic[2], 3)};
+      const double area = sqrt(S3[0] * S3[0] + S3[1] * S3[1] + S3[2] * S3[2]);
+      const double un = outward * (q[1] * S3[0] + q[2] * S3[1] + q[3] * S3[2]) / (q[0] * area);
+      const double pw = fmax(*at(state, ic[0], ic[1], ic[2], 0) + q[0] * *at(state, ic[0], ic[1], ic[2], 7) * un, 0.0);
+      double Fv[MAXNS + 4];
+      Fv[0] = 0.0;
+      Fv[1] = pw * S3[0];
+      Fv[2] = pw * S3[1];
+      Fv[3] = pw * S3[2];
+      for (int l = 4; l < c->ne; l++) Fv[l] = 0.0;
+      if (viscous) {
+        const double mu = *at(trstate, ic[0], ic[1], ic[2], 0);
+        const double Sabs = sqrt(S3[0] * S3[0] + S3[1] * S3[1] + S3[2] * S3[2]), Sinv = 1.0 / Sabs;
+        const double nv[3] = {S3[0] * Sinv, S3[1] * Sinv, S3[2] * Sinv};
+        const double dn = fabs((*at(xcc, ic[0], ic[1], ic[2], 0) - *at(xf, f[0], f[1], f[2], 0)) * nv[0] +
+                               (*at(xcc, ic[0], ic[1], ic[2], 1) - *at(xf, f[0], f[1], f[2], 1)) * nv[1] +
+                               (*at(xcc, ic[0], ic[1], ic[2], 2) - *at(xf, f[0], f[1], f[2], 2)) * nv[2]);
+        const double rhoinv = 1.0 / q[0];
+        double dun[3];
+        for (int d = 0; d < 3; d++) dun[d] = outward * (0.0 - q[1 + d] * rhoinv) / dn;
+        const double div = dun[0] * nv[0] + dun[1] * nv[1] + dun[2] * nv[2];
+        for (int d = 0; d < 3; d++) dun[d] = div * nv[d] + 0.0 * (dun[d] - div * nv[d]);
+        double work = 0.0;
+        for (int d = 0; d < 3; d++) {
+          const double tS = -mu * Sabs * (dun[d] + div * nv[d] / 3.0);
+          Fv[1 + d] += tS;
+          work += 0.0 * tS;
+        }
+        Fv[4] += work - *at(trstate, ic[0], ic[1], ic[2], 1) * outward * (0.0) / dn * Sabs;
+      }
+      for (int l = 0; l < c->ne; l++) *at(F, f[0], f[1], f[2], l) = Fv[l];
+    }
+}
+
 static pgo_box interior(const pgo_block *b) {
   pgo_box r;
   for (int a = 0; a < 3; a++) {
@@ -1240,28 +1365,38 @@ static pgo_box haloslab(const pgo_block *b, int a, int side, int depth) {
   return r;
 }
 
-/* simulator/advec/simulator.py:57-75: the exchange, the halo EOS, the source; for a viscous
- * case the transport over the interior and the first halo layer ("shallow",
- * launch/ranges.py:52-53; advecdiff/simulator.py postbc) */
+/* simulator/advec/simulator.py:57-75: the exchange and the halo EOS on periodic faces, the
+ * walls' mirror halos (preadvflux), the source; for a viscous case the transport over the
+ * interior and the first halo layer of each interface ("shallow", launch/ranges.py:52-53;
+ * advecdiff/simulator.py postbc) */
 static void prerhs(const pgo_case *c, pgo_block *b, pgo_arr Q, pgo_arr dQ, int withdQ) {
-  for (int a = 0; a < 3; a++) pgo_fill_periodic(Q, c->ne, b->ncell, b->ng, b->ng, a);
+  pgo_arr S[3] = {b->iS, b->jS, b->kS};
   for (int a = 0; a < 3; a++)
-    for (int side = 0; side < 2; side++) pgo_fromQ(c, Q, b->state, haloslab(b, a, side, b->ng));
+    if (!b->bc[2 * a]) pgo_fill_periodic(Q, c->ne, b->ncell, b->ng, b->ng, a);
+  for (int a = 0; a < 3; a++)
+    for (int side = 0; side < 2; side++)
+      if (!b->bc[2 * a + side]) pgo_fromQ(c, Q, b->state, haloslab(b, a, side, b->ng));
+  for (int a = 0; a < 3; a++)
+    for (int side = 0; side < 2; side++)
+      if (b->bc[2 * a + side]) pgo_wall_mirror(c, a, side, b->ncell, b->ng, b->nstate, Q, b->state, S[a]);
   if (withdQ && b->chemistry) pgo_source(c, Q, b->state, dQ, interior(b));
   if (b->viscous) {
     pgo_transport(c, Q, b->state, b->trstate, b->hsp, interior(b));
     for (int a = 0; a < 3; a++)
-      for (int side = 0; side < 2; side++) pgo_transport(c, Q, b->state, b->trstate, b->hsp, haloslab(b, a, side, 1));
+      for (int side = 0; side < 2; side++)
+        if (!b->bc[2 * a + side]) pgo_transport(c, Q, b->state, b->trstate, b->hsp, haloslab(b, a, side, 1));
   }
 }
 
 /* advec/simulator.py:76-80; advecdiff/simulator.py:20-35: gradients over the interior and their
- * one-plane exchange, the advective flux, the viscous flux added to it, the divergence */
+ * one-plane exchange (a wall's halo gradients are read by nothing), the advective flux, the
+ * viscous flux added to it, the walls' own flux through their faces (postflux), the divergence */
 static void rhs(const pgo_case *c, pgo_block *b, pgo_arr Q, pgo_arr dQ) {
   pgo_arr F[3] = {b->iF, b->jF, b->kF}, S[3] = {b->iS, b->jS, b->kS};
   if (b->viscous) {
     pgo_grads(c, Q, b->state, b->Jinv, b->grads, interior(b));
-    for (int a = 0; a < 3; a++) pgo_fill_periodic(b->grads, 3 * (c->ne - 1), b->ncell, b->ng, 1, a);
+    for (int a = 0; a < 3; a++)
+      if (!b->bc[2 * a]) pgo_fill_periodic(b->grads, 3 * (c->ne - 1), b->ncell, b->ng, 1, a);
   }
   for (int a = 0; a < 3; a++) {
     pgo_flux(c, a, Q, b->state, S[a], F[a], facebox(b, a));
@@ -1269,6 +1404,10 @@ static void rhs(const pgo_case *c, pgo_block *b, pgo_arr Q, pgo_arr dQ) {
       pgo_diffflux(c, a, b->alpha, Q, b->state, b->trstate, b->hsp, b->grads, b->xcc, b->xf[a], S[a], F[a],
                    facebox(b, a));
   }
+  for (int a = 0; a < 3; a++)
+    for (int side = 0; side < 2; side++)
+      if (b->bc[2 * a + side])
+        pgo_wall_flux(c, a, side, b->ncell, b->ng, b->viscous, Q, b->state, b->trstate, b->xcc, b->xf[a], S[a], F[a]);
   pgo_applyflux(c->ne, dQ, b->iF, b->jF, b->kF, b->detJinv, b->chemistry, interior(b));
 }
 
@@ -1337,6 +1476,11 @@ void pgo_stage(const pgo_case *c, pgo_block *b, int n, double dt) {
   rhs(c, b, Q, dQ);
   pgo_advance(c, b, n, dt);
   pgo_fromQ(c, Qout, b->state, interior(b));
+}
+
+void pgo_rhs(const pgo_case *c, pgo_block *b, int q, int dq) {
+  prerhs(c, b, b->regs[q], b->regs[dq], 1);
+  rhs(c, b, b->regs[q], b->regs[dq]);
 }
 
 void pgo_step(const pgo_case *c, pgo_block *b, double dt) {

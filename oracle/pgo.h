@@ -58,6 +58,7 @@ typedef struct {
 double pgo_tlog(double x);
 double pgo_elog(double x);
 double pgo_texp(double x);
+double pgo_dcbrt(double x);
 
 void pgo_fromQ(const pgo_case *c, pgo_arr Q, pgo_arr state, pgo_box r);
 void pgo_fromprims(const pgo_case *c, pgo_arr prims, pgo_arr Q, pgo_arr state, pgo_box r);
@@ -88,9 +89,23 @@ typedef struct {
   int viscous;
   double alpha;
   pgo_arr trstate, hsp, grads, Jinv, xcc, xf[3];
+  /* per face (i low, i high, j low, ...): 0 periodic, 1 adiabatic slip wall; nstate the state
+   * fields (9 for the real gas) */
+  int bc[6];
+  int nstate;
 } pgo_block;
 
+/* the adiabatic slip wall (simulator/euler/bcs.py:20-27): the halo as the interior's mirror image
+ * across the wall face (side 0 the low face of axis, 1 the high), and the wall's own flux through
+ * its faces (the pressure's, plus for a viscous case the normal stress and no heat) */
+void pgo_wall_mirror(const pgo_case *c, int axis, int side, const int *ncell, int ng, int nstate, pgo_arr Q,
+                     pgo_arr state, pgo_arr S);
+void pgo_wall_flux(const pgo_case *c, int axis, int side, const int *ncell, int ng, int viscous, pgo_arr Q,
+                   pgo_arr state, pgo_arr trstate, pgo_arr xcc, pgo_arr xf, pgo_arr S, pgo_arr F);
+
 void pgo_step(const pgo_case *c, pgo_block *b, double dt);
+/* one right-hand side of register q into register dq (prerhs then rhs), as a stage begins it */
+void pgo_rhs(const pgo_case *c, pgo_block *b, int q, int dq);
 void pgo_stage(const pgo_case *c, pgo_block *b, int n, double dt);
 /* the RK combine(s) of stage n alone, over the interior */
 void pgo_advance(const pgo_case *c, pgo_block *b, int n, double dt);

@@ -227,4 +227,23 @@ __device__ __forceinline__ double elog(double x) {
   return hi + (r + (p + lo));
 }
 
+// ---------------------------------------------------------------------------
+// dcbrt: the cube root by an exponent split and Newton steps in plain IEEE arithmetic, so the oracle
+// and the device give the same bits (the libraries' cbrt round apart by an ulp, and the cubic EOS's
+// root amplifies that across a pseudo-critical layer). x = a 2^(3q) with a in [0.5, 4); six Newton
+// steps from a linear start settle within an ulp of cbrt(a); the 2^q is exact.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double dcbrt(double x) {
+#pragma clang fp contract(off)
+  if (x == 0.0 || !(fabs(x) <= 1.7976931348623157e308)) return x;
+  int e;
+  const double m = frexp(fabs(x), &e);
+  int q = e / 3, r = e - 3 * q;
+  if (r < 0) r += 3, q -= 1;
+  const double a = ldexp(m, r);
+  double y = 0.75 + 0.25 * a;
+  for (int k = 0; k < 6; k++) y = y - (y * y * y - a) / (3.0 * (y * y));
+  return copysign(ldexp(y, q), x);
+}
+
 }  // namespace pga

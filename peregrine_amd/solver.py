@@ -163,14 +163,28 @@ class Solver:
     def nblocks(self):
         return len(self.blocks)
 
+    # the boundary kinds this build has (simulator/euler/bcs.py, simulator/navstokes/bcs.py)
+    BC_KINDS = ("slp-adia-wall",)
+
     def _check_bcs(self):
+        """Every boundary named and its name in bcvals with a kind this build has
+        (multiblock/solver.py:158-173)."""
         for blk in self.blocks:
+            blk.walls = []
             for f in blk.faces.values():
-                if f.nbr is None:
-                    raise ValueError(
-                        f"block {blk.nblki} interface {f.nface} is a boundary ({f.bcname!r}); this build runs "
-                        "periodic and block-to-block interfaces only"
-                    )
+                if f.nbr is not None:
+                    continue
+                where = f"block {blk.nblki} interface {f.nface}"
+                if f.bcname is None:
+                    raise ValueError(f"{where} is a boundary the grid did not name; name it (a mesher's bcnames or "
+                                     "defaultbc) and say what it is in bcvals")
+                if f.bcname not in self.cfg["bcvals"]:
+                    raise ValueError(f"{where} is named {f.bcname!r}, which bcvals does not have; it has "
+                                     f"{sorted(self.cfg['bcvals'])}")
+                kind = dict(self.cfg["bcvals"][f.bcname]).get("kind")
+                if kind not in self.BC_KINDS:
+                    raise ValueError(f"{where}: bcvals.{f.bcname} is of kind {kind!r}; this build has {self.BC_KINDS}")
+                blk.walls.append((f.axis, 0 if f.is_low else 1))
             if min(blk.ncell) < self.ng:
                 raise ValueError(f"block {blk.nblki}: {blk.ncell} cells, thinner than the {self.ng}-deep halo")
 
@@ -250,6 +264,8 @@ class Solver:
         ng = self.ng
         for blk in self.blocks:
             for f in blk.faces.values():
+                if f.nbr is None:
+                    continue
                 nbr = self.blocks[f.nbr]
                 a = f.axis
                 ext = list(blk.ncell)
@@ -285,13 +301,28 @@ class Solver:
         self._exchange(lambda b: b.grads, 3 * self.sim.ngrad, 1)
 
     def fluxes(self, blk, Qv, st):
-        """The advective flux of each axis, with the viscous flux added for a viscous case."""
+        """The advective flux of each axis, with the viscous flux added for a viscous case, then the
+        walls' own flux through their faces (the postflux hook)."""
         for a, (S, F) in enumerate(((blk.iS, blk.iF), (blk.jS, blk.jF), (blk.kS, blk.kF))):
             self._call("pga_flux", self.case.handle, a, Qv, st, S.view(), F.view(), blk.face_box(a))
             if self.sim.viscous:
                 self._call("pga_diffflux", self.case.handle, a, self.sim.alpha, Qv, st, blk.trstate.view(),
                            blk.hsp.view(), blk.grads.view(), blk.xcc.view(), blk.xf[a].view(), S.view(), F.view(),
                            blk.face_box(a))
+        for axis, side in blk.walls:
+            S, F = (blk.iS, blk.jS, blk.kS)[axis], (blk.iF, blk.jF, blk.kF)[axis]
+            visc = self.sim.viscous
+            self._call("pga_wall_flux", self.case.handle, axis, side, native.ints(blk.ncell), self.ng, Qv, st,
+                       blk.trstate.view() if visc else native.View(), blk.xcc.view(),
+                       blk.xf[axis].view() if visc else native.View(), S.view(), F.view())
+
+    def wall_halos(self, Q):
+        """Each wall's halo as the interior's mirror image (the preadvflux hook)."""
+        for blk in self.blocks:
+            for axis, side in blk.walls:
+                S = (blk.iS, blk.jS, blk.kS)[axis]
+                self._call("pga_wall_mirror", self.case.handle, axis, side, native.ints(blk.ncell), self.ng,
+                           blk.regs[Q].view(), blk.state.view(), S.view())
 
     def prerhs(self, Q, dQ=None):
         self.exchange(Q)
@@ -299,6 +330,7 @@ class Solver:
             boxes = blk.halo_boxes()
             arr = (native.Box * len(boxes))(*boxes)
             self._call("pga_fromQ", self.case.handle, blk.regs[Q].view(), blk.state.view(), arr, len(boxes))
+        self.wall_halos(Q)
         if dQ is not None:
             self.begin_dQ(Q, dQ)
         if self.sim.viscous:
@@ -370,6 +402,8 @@ class Solver:
         for blk in self.blocks:
             dsts, srcs, ds, ss, es = [], [], [], [], []
             for f in blk.faces.values():
+                if f.nbr is None:
+                    continue
                 nbr = self.blocks[f.nbr]
                 a = f.axis
                 ext = list(blk.ncell)
@@ -408,6 +442,7 @@ class Solver:
         for n, regs in enumerate(self.integ.stageregs(self.regs)):
             if regs.Q not in self.readied:
                 self._halo_pass(regs.Q)
+                self.wall_halos(regs.Q)
                 if self.sim.viscous:
                     self.transport(regs.Q)
             self.readied.clear()

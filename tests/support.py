@@ -126,7 +126,8 @@ class HostBlock:
         return np.s_[:, g:-g, g:-g, g:-g]
 
 
-def oracle_block(hb, integrator, chemistry, viscous=False, alpha=1.0):
+def oracle_block(hb, integrator, chemistry, viscous=False, alpha=1.0, walls=()):
+    """walls: the faces (0 i-low, 1 i-high, 2 j-low, ...) that are adiabatic slip walls."""
     from oracle import oracle as O
 
     b = O.Block()
@@ -146,6 +147,9 @@ def oracle_block(hb, integrator, chemistry, viscous=False, alpha=1.0):
     b.Jinv, b.xcc = O.arr(hb.Jinv), O.arr(hb.xccd)
     for a in range(3):
         b.xf[a] = O.arr(hb.xf[a])
+    for f in range(6):
+        b.bc[f] = int(f in walls)
+    b.nstate = hb.state.shape[0]
     return b
 
 

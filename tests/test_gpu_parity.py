@@ -253,17 +253,22 @@ def test_halo_copies(dev):
 
 
 def _run_pair(gas, recon, primary, limiter, chem, integrator, ncell, steps, dt, prims, lengths, fused=True,
-              trans=None, species=None, nstate=8):
-    """The same case stepped by the solver on the device and by the oracle on the host."""
+              trans=None, species=None, nstate=8, walls=()):
+    """The same case stepped by the solver on the device and by the oracle on the host; walls: the
+    faces (0 i-low .. 5 k-high) that are adiabatic slip walls, the rest periodic."""
     import peregrine_amd as pg
 
     cfg = sp.config(gas, recon=recon, primary=primary, limiter=limiter, chem=chem, integrator=integrator, dt=dt,
                     trans=trans)
     if species is not None:
         cfg["mixture"]["species"] = species
+    if walls:
+        cfg["bcvals"]["wall"] = {"kind": "slp-adia-wall"}
     cfg["backend-mi355x"]["fused-stage"] = fused
     backend = pg.get_backend("mi355x", cfg)
-    mesh = pg.mesher.CubeMesher(blkdims=[n + 1 for n in ncell], lengths=lengths, periodic=[True] * 3)
+    periodic = [not (2 * a in walls or 2 * a + 1 in walls) for a in range(3)]
+    mesh = pg.mesher.CubeMesher(blkdims=[n + 1 for n in ncell], lengths=lengths, periodic=periodic,
+                                bcnames={f + 1: "wall" for f in walls})
     mb = pg.get_solver(cfg, mesh, backend)
     sim = mb.sim
     oc = O.OracleCase(sim.mixture, recon, limiter, primary)
@@ -277,7 +282,7 @@ def _run_pair(gas, recon, primary, limiter, chem, integrator, ncell, steps, dt, 
     # the EOS over the interior, then the halo exchange and the EOS over the halos, which its first
     # stage then takes as ready; the oracle's first stage exchanges and takes the halo EOS itself
     O.lib.pgo_fromQ(oc.ref, O.arr(hb.regs[0]), O.arr(hb.state), O.box((hb.ng,) * 3, hb.ncell))
-    b = sp.oracle_block(hb, integrator, chem, viscous=sim.viscous, alpha=getattr(sim, "alpha", 1.0))
+    b = sp.oracle_block(hb, integrator, chem, viscous=sim.viscous, alpha=getattr(sim, "alpha", 1.0), walls=walls)
     for _ in range(steps):
         mb.step(dt)
         O.lib.pgo_step(oc.ref, ctypes.byref(b), dt)
@@ -569,7 +574,6 @@ def test_real_gas_fromprims_and_fromQ(dev, gas):
     L = native.lib()
     native.check(L.pga_fromprims(nc.handle, dev.view(dP), dev.view(dQ), dev.view(dS), native.box((0, 0, 0), hb.cshape), None),
                  "pga_fromprims")
-    # the cubic's root takes cbrt or acos/cos: the device and host libraries round those apart
     err = max(_rowerr(dev.get(dQ), hb.regs[0]), _rowerr(dev.get(dS), hb.state))
     print(f"fromprims {gas}: {err:.3e}")
     assert err < 1e-12
@@ -619,14 +623,64 @@ def test_real_gas_muscl_faces_and_chung_transport(dev, gas):
 @pytest.mark.parametrize("fused", [True, False], ids=["fused", "per-kernel"])
 def test_transcritical_mixing_layer_steps_match(dev, fused):
     """O2/N2 at 60 bar, Peng-Robinson with Chung transport, MUSCL-Koren + Rusanov, Navier-Stokes,
-    tvd-rk3, 10 steps on a periodic 48 x 24 x 8 block (10 um cells); 1e-10 per conserved row."""
+    tvd-rk3, 10 steps on a 48 x 24 x 8 block (10 um cells), periodic in x and z with adiabatic
+    slip walls at both y ends, as the mixing layer is stated; 1e-10 per conserved row."""
     ncell = (48, 24, 8)
     L = tuple(1e-5 * n for n in ncell)
     mb, hb = _run_pair("o2n2-pr", "muscl", "rusanov", "koren", False, "tvd-rk3", ncell, 10, 2e-9,
-                       lambda x: _transcritical(x, 6), L, fused=fused, trans="chung", nstate=9)
+                       lambda x: _transcritical(x, 6), L, fused=fused, trans="chung", nstate=9, walls=(2, 3))
     assert mb.fused == fused
     g = hb.ng
     got = mb.blocks[0].Q.get()[g:-g, g:-g, g:-g].transpose(3, 2, 1, 0)
     err = _rowerr(got, hb.regs[0][hb.interior()])
     print(f"transcritical 10 steps ({'fused' if fused else 'per-kernel'}): {err:.3e}")
+    # the cubic's root and Chung's factors take only IEEE operations, dcbrt, elog and texp: the
+    # same bits on both sides (with the libraries' cbrt, acos, cos, pow and exp, which round apart
+    # by an ulp, this case measured 3.9e-10 after 10 steps: the pseudo-critical layer amplifies)
+    assert err < 1e-10
+
+
+def test_transcritical_rhs_matches(dev):
+    """One right-hand side of the mixing layer (the halo EOS, the walls' mirror, transport,
+    gradients, advective, viscous and wall fluxes, divergence) against the oracle's: 1e-12 per row."""
+    import peregrine_amd as pg
+
+    ncell = (48, 24, 8)
+    L = tuple(1e-5 * n for n in ncell)
+    cfg = sp.config("o2n2-pr", recon="muscl", primary="rusanov", limiter="koren", trans="chung", dt=2e-9)
+    cfg["bcvals"]["wall"] = {"kind": "slp-adia-wall"}
+    mesh = pg.mesher.CubeMesher(blkdims=[n + 1 for n in ncell], lengths=L, periodic=[True, False, True],
+                                bcnames={3: "wall", 4: "wall"})
+    mb = pg.get_solver(cfg, mesh, pg.get_backend("mi355x", cfg))
+    sim = mb.sim
+    oc = O.OracleCase(sim.mixture, "muscl", "koren", "rusanov")
+    hb = sp.HostBlock(ncell, sim.ng, sim.ne, 3, lengths=L, nstate=9)
+    q = _transcritical(hb.xcc, sim.ne)
+    mb.set_prims([q])
+    O.lib.pgo_fromprims(oc.ref, O.arr(sp.to_dev(q)), O.arr(hb.regs[0]), O.arr(hb.state), O.box((0, 0, 0), hb.cshape))
+    O.lib.pgo_fromQ(oc.ref, O.arr(hb.regs[0]), O.arr(hb.state), O.box((hb.ng,) * 3, hb.ncell))
+    b = sp.oracle_block(hb, "tvd-rk3", False, viscous=True, alpha=sim.alpha, walls=(2, 3))
+    O.lib.pgo_rhs(oc.ref, ctypes.byref(b), 0, 2)
+    regs = mb.integ.stageregs(mb.regs)[0]
+    mb.rhs(Q=regs.Q, dQ=regs.dQ)
+    g = hb.ng
+    got = mb.blocks[0].regs[regs.dQ].get()[g:-g, g:-g, g:-g].transpose(3, 2, 1, 0)
+    err = _rowerr(got, hb.regs[2][hb.interior()])
+    print(f"transcritical rhs: {err:.3e}")
+    assert err < 1e-12
+
+
+@pytest.mark.parametrize("fused", [True, False], ids=["fused", "per-kernel"])
+def test_slip_walls_steps_match(dev, fused):
+    """Two species, MUSCL-Koren + HLLC, Euler, tvd-rk3 between adiabatic slip walls at both j ends
+    and both k ends, periodic in i: 10 steps, by the fused stage and the per-kernel sequence."""
+    ncell = (40, 20, 12)
+    L = (1.0, 0.5, 0.3)
+    mb, hb = _run_pair("binary", "muscl", "hllc", "koren", False, "tvd-rk3", ncell, 10, 2e-6,
+                       lambda x: sp.smooth_prims(x, 6, ns=2), L, fused=fused, walls=(2, 3, 4, 5))
+    assert mb.fused == fused
+    g = hb.ng
+    got = mb.blocks[0].Q.get()[g:-g, g:-g, g:-g].transpose(3, 2, 1, 0)
+    err = _rowerr(got, hb.regs[0][hb.interior()])
+    print(f"slip walls 10 steps ({'fused' if fused else 'per-kernel'}): {err:.3e}")
     assert err < 1e-10

@@ -152,3 +152,12 @@ def test_chung_tends_to_the_dilute_values():
     tr, hs = np.zeros((4, 1, 1, 1)), np.zeros((2, 1, 1, 1))
     O.lib.pgo_transport(oc.ref, O.arr(hb.regs[0]), O.arr(hb.state), O.arr(tr), O.arr(hs), O.box((0, 0, 0), (1, 1, 1)))
     assert st[0, 3] > 300.0 and tr[0, 0, 0, 0] > 3e-5
+
+
+def test_the_deterministic_cube_root_is_within_an_ulp():
+    rng = np.random.default_rng(9)
+    x = np.concatenate([np.exp(rng.uniform(-600, 600, 20000)), rng.uniform(1e-3, 10.0, 20000), -rng.uniform(0.1, 5, 2000)])
+    got = np.array([O.dcbrt(v) for v in x])
+    want = np.cbrt(x)
+    assert (np.abs(got - want) <= np.spacing(np.abs(want))).all()
+    assert O.dcbrt(0.0) == 0.0 and O.dcbrt(8.0) == 2.0 and O.dcbrt(-27.0) == -3.0
