@@ -10,7 +10,7 @@ Alongside: the dominant kernel's achieved rate against its roofline (device even
 launch stream over the timed steps), and the CPU oracle timed on the host's cores on a bounded
 sample of the same case.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 128] [--cpu-n 32] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 128] [--cpu-n 64] [--cpu-seconds 10] [--no-cpu]
 """
 
 import argparse
@@ -68,8 +68,9 @@ def algorithmic_bytes(name, ne, ng):
     return 0
 
 
-def cpu_baseline(support, n, stages):
-    """The oracle (C, OpenMP) on an n^3 block of the same case, all host cores."""
+def cpu_baseline(support, n, seconds):
+    """The oracle (C, OpenMP over the host's cores) on an n^3 block of the same case: stages in
+    turn (RK3's three, repeated) until `seconds` of them have run, the median stage reported."""
     from oracle import oracle as O
     from peregrine_amd.simulator import get_simulator
 
@@ -82,13 +83,28 @@ def cpu_baseline(support, n, stages):
     b = support.oracle_block(hb, "tvd-rk3", True)
     O.lib.pgo_stage(oc.ref, ctypes.byref(b), 0, DT)  # warm-up
     times = []
-    for s in range(stages):
+    start = time.perf_counter()
+    s = 1
+    while time.perf_counter() - start < seconds or len(times) < 6:
         t0 = time.perf_counter()
         O.lib.pgo_stage(oc.ref, ctypes.byref(b), s % 3, DT)
         times.append(time.perf_counter() - t0)
+        s += 1
     med = float(np.median(times))
     return dict(value=n**3 / med, unit="cell-updates/s", cores=O.threads(), kind="port",
-                sample=f"{n}^3 block of the same case, median of {stages} stages, C oracle with OpenMP")
+                sample=f"{n}^3 block of the same case, median of {len(times)} RK3 stages ({sum(times):.1f} s), "
+                       "C oracle with OpenMP")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of a kernel, from the rocprofv3 FETCH_SIZE and WRITE_SIZE passes over this
+    benchmark, each scaled by the same counter's reading of a plain copy of known size; None when no
+    profile has been recorded."""
+    path = ROOT / "profiles" / "traffic.json"
+    if not path.is_file():
+        return None
+    got = json.loads(path.read_text()).get(kernel)
+    return None if got is None else got["bytes_per_launch"]
 
 
 def main():
@@ -97,11 +113,13 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=128)
-    ap.add_argument("--cpu-n", type=int, default=32)
-    ap.add_argument("--cpu-stages", type=int, default=5)
+    ap.add_argument("--cpu-n", type=int, default=64)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--per-kernel", action="store_true", help="the reference's launch sequence instead of the fused stage")
     ap.add_argument("--k-chunk", type=int, default=None)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only a plain device copy of 512 MiB, five times: the reference for HBM byte counters")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU: this build has no multi-GPU path (the block sharding is design only)",
@@ -109,6 +127,16 @@ def main():
         sys.exit(2)
 
     import torch
+
+    if args.calibrate_copy:
+        n = 512 * 2**20 // 8
+        src = torch.rand(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": 8 * n, "copy_bytes_written": 8 * n, "launches": 5}), flush=True)
+        return
 
     import peregrine_amd as pg
     import support
@@ -144,12 +172,13 @@ def main():
     items = {"pga_flux": (n + 1) * n * n}.get(dom, n**3)
     per = algorithmic_bytes(dom, ne, mb.ng)
     achieved = per * items / (ms * 1e-3) / 1e9
+    kname = {"pga_flux": "k_flux", "pga_stage": "k_stage"}.get(dom, dom)
     roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK, unit="GB/s", frac=round(achieved / HBM_PEAK, 4),
-                traffic=None, kernel=dom, kernel_ms=round(ms, 4), launches=count,
+                traffic=measured_traffic(kname), kernel=dom, kernel_ms=round(ms, 4), launches=count,
                 bytes_per_item=per, items_per_launch=items)
     breakdown = {k: {"launches": c, "mean_ms": round(t, 4), "share": round(c * t / sum(total.values()), 3)}
                  for k, (c, t) in sorted(times.items(), key=lambda kv: -kv[1][0] * kv[1][1])}
-    cpu = None if args.no_cpu else cpu_baseline(support, args.cpu_n, args.cpu_stages)
+    cpu = None if args.no_cpu else cpu_baseline(support, args.cpu_n, args.cpu_seconds)
     line = {
         "metric": "cell-updates/s per RK stage (FP64, H2-air 3D)",
         "value": value,

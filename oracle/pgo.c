@@ -110,8 +110,12 @@ double pgo_texp(double x) {
  * parts. Tables: pga_elog_tables.h, written once by the build.
  * ---------------------------------------------------------------------- */
 double pgo_elog(double x) {
-  if (!(x >= DBL_MIN) || x > DBL_MAX) return log(x);
-  if (fabs(x - 1.0) < 0.03125) {
+  /* zero, negative, NaN and infinity: the limits; a subnormal is scaled into the normal range */
+  if (!(x > 0.0)) return x == 0.0 ? -HUGE_VAL : (x - x) / 0.0;
+  if (x > DBL_MAX) return x;
+  int escale = 0;
+  if (x < DBL_MIN) x *= 18014398509481984.0, escale = -54; /* 2^54 */
+  if (escale == 0 && fabs(x - 1.0) < 0.03125) {
     /* near 1 the table's terms would cancel: ln(1 + r) by its series, r = x - 1 exact */
     const double r = x - 1.0;
     double q = -1.0 / 14.0;
@@ -120,7 +124,7 @@ double pgo_elog(double x) {
   }
   int64_t bits;
   memcpy(&bits, &x, sizeof bits);
-  const int e = (int)(bits >> 52) - 1023;
+  const int e = (int)(bits >> 52) - 1023 + escale;
   const int k = (int)((bits >> 45) & 127);
   int64_t mb = (bits & 0x000fffffffffffffLL) | 0x3ff0000000000000LL;
   double m;

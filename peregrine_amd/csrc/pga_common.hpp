@@ -199,11 +199,13 @@ __device__ __forceinline__ double texp(double x) {
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ double elog(double x) {
 #pragma clang fp contract(off)
-  if (!(x >= 2.2250738585072014e-308) || x > 1.7976931348623157e308) {
-    // zero, negative, NaN, subnormal or infinite: the library's answer (never reached by a gas state)
-    return log(x);
-  }
-  if (fabs(x - 1.0) < 0.03125) {
+  // zero, negative, NaN and infinity: the limits; a subnormal is scaled into the normal range. No
+  // library call: a call's save and restore around the EOS costs registers on every path
+  if (!(x > 0.0)) return x == 0.0 ? -__builtin_huge_val() : (x - x) / 0.0;
+  if (x > 1.7976931348623157e308) return x;
+  int escale = 0;
+  if (x < 2.2250738585072014e-308) x *= 18014398509481984.0, escale = -54;
+  if (escale == 0 && fabs(x - 1.0) < 0.03125) {
     // near 1 the table's terms would cancel: ln(1 + r) by its series, r = x - 1 exact
     const double r = x - 1.0;
     double q = -1.0 / 14.0;
@@ -211,7 +213,7 @@ __device__ __forceinline__ double elog(double x) {
     return r + (r * r) * q;
   }
   const long long bits = __double_as_longlong(x);
-  const int e = (int)(bits >> 52) - 1023;
+  const int e = (int)(bits >> 52) - 1023 + escale;
   const int k = (int)((bits >> 45) & 127);
   const double m = __longlong_as_double((bits & 0x000fffffffffffffLL) | 0x3ff0000000000000LL);
   const double r = fma(m, ELOG_RCP[k], -1.0);

@@ -78,6 +78,65 @@ __device__ __forceinline__ void mixmom(const Gas<NS> &g, const YF &Y, MixMom<NS,
   }
 }
 
+// mixmom for two compositions at once (a face's two sides): each species' constants are loaded
+// once and folded into both sums while they are live, rather than held (~80 values) across two
+// separate passes; each side's sums are the single mixmom's to the bit
+template <int NS, int EOS, class YA, class YB>
+__device__ __forceinline__ void mixmom2(const Gas<NS> &g, const YA &Ya, const YB &Yb, MixMom<NS, EOS> &ma,
+                                        MixMom<NS, EOS> &mb) {
+#pragma clang fp contract(off)
+  ma.R = mb.R = 0.0;
+  ma.h298 = mb.h298 = 0.0;
+  if constexpr (EOS != EOS_CPG) {
+#pragma unroll
+    for (int t = 0; t < NT; t++) ma.cpcoeff[t] = mb.cpcoeff[t] = 0.0;
+  } else {
+    ma.cp = mb.cp = 0.0;
+  }
+  if constexpr (EOS == EOS_REAL) ma.b = ma.c = ma.da = mb.b = mb.c = mb.da = 0.0;
+  double la = 1.0, lb = 1.0;
+#pragma unroll
+  for (int n = 0; n < NS; n++) {
+    double ya, yb;
+    if (n < NS - 1) {
+      ya = Ya(n);
+      yb = Yb(n);
+      la -= ya;
+      lb -= yb;
+    } else {
+      ya = la;
+      yb = lb;
+    }
+    const double Rn = g.Rsp(n);
+    ma.R = fma(ya, Rn, ma.R);
+    mb.R = fma(yb, Rn, mb.R);
+    if constexpr (EOS != EOS_CPG) {
+#pragma unroll
+      for (int t = 0; t < NT; t++) {
+        const double cn = g.cppolyR(n, t);
+        ma.cpcoeff[t] = fma(ya, cn, ma.cpcoeff[t]);
+        mb.cpcoeff[t] = fma(yb, cn, mb.cpcoeff[t]);
+      }
+    } else {
+      const double cn = g.cp0(n);
+      ma.cp = fma(ya, cn, ma.cp);
+      mb.cp = fma(yb, cn, mb.cp);
+    }
+    const double hn = g.h298R(n);
+    ma.h298 = fma(ya, hn, ma.h298);
+    mb.h298 = fma(yb, hn, mb.h298);
+    if constexpr (EOS == EOS_REAL) {
+      const double bn = g.cubicbMW(n), vn = g.vshiftMW(n), dn = g.cubicdaMW(n);
+      ma.b = fma(ya, bn, ma.b);
+      mb.b = fma(yb, bn, mb.b);
+      ma.c = fma(ya, vn, ma.c);
+      mb.c = fma(yb, vn, mb.c);
+      ma.da = fma(ya, dn, ma.da);
+      mb.da = fma(yb, dn, mb.da);
+    }
+  }
+}
+
 // caloric/tdepcp/kernels/caloric.mako:4-15: cp and h per mass at T, h's coefficients
 // built in the same descent as hcoeff[m] = cpcoeff[m] - (m + 1) hcoeff[m + 1]
 template <class M>

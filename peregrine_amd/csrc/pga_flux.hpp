@@ -97,11 +97,9 @@ __device__ __forceinline__ void face_sides(const Gas<NS> &g, const View &Q, cons
       L.Y[n] = left_of<LIM>(y0, y1, y2);
       R.Y[n] = right_of<LIM>(y1, y2, y3);
     }
-    {
-      MixMom<NS, EOS> m;
-      mixmom<NS, EOS>(g, [&](int n) { return L.Y[n]; }, m);
-      frompT<NS, EOS>(g, m, [&](int n) { return L.Y[n]; }, pL, TL, L.s);
-    }
+    MixMom<NS, EOS> mL, mR;
+    mixmom2<NS, EOS>(g, [&](int n) { return L.Y[n]; }, [&](int n) { return R.Y[n]; }, mL, mR);
+    frompT<NS, EOS>(g, mL, [&](int n) { return L.Y[n]; }, pL, TL, L.s);
     double vel[3][4];
 #pragma unroll
     for (int c = 0; c < 3; c++) {
@@ -113,11 +111,7 @@ __device__ __forceinline__ void face_sides(const Gas<NS> &g, const View &Q, cons
     L.u = left_of<LIM>(vel[0][0], vel[0][1], vel[0][2]);
     L.v = left_of<LIM>(vel[1][0], vel[1][1], vel[1][2]);
     L.w = left_of<LIM>(vel[2][0], vel[2][1], vel[2][2]);
-    {
-      MixMom<NS, EOS> m;
-      mixmom<NS, EOS>(g, [&](int n) { return R.Y[n]; }, m);
-      frompT<NS, EOS>(g, m, [&](int n) { return R.Y[n]; }, pR, TR, R.s);
-    }
+    frompT<NS, EOS>(g, mR, [&](int n) { return R.Y[n]; }, pR, TR, R.s);
     R.u = right_of<LIM>(vel[0][1], vel[0][2], vel[0][3]);
     R.v = right_of<LIM>(vel[1][1], vel[1][2], vel[1][3]);
     R.w = right_of<LIM>(vel[2][1], vel[2][2], vel[2][3]);

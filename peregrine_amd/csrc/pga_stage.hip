@@ -59,10 +59,10 @@ __device__ __forceinline__ void fromQ_regs(const Gas<NS> &g, const double (&Q)[N
   fromrhoE<NS, EOS>(g, m, [&](int n) { return Y[n]; }, rho, e, Tguess, s);
 }
 
-template <int NS, int EOS, bool CHEM>
+// the divergence, the source and the RK combine of a column of cells: memory-bound
+template <int NE, bool CHEM>
 __global__ __launch_bounds__(WT) void k_stage(StageArgs a) {
-  constexpr int NE = NS + 4;
-  const View Qin = a.Qin, stin = a.stin, Q0 = a.Q0, Qout = a.Qout, stout = a.stout, det = a.det;
+  const View Qin = a.Qin, Q0 = a.Q0, Qout = a.Qout, det = a.det;
   const View iF = a.iF, jF = a.jF, kF = a.kF;
   const CombineSpec comb = a.comb;
   const View rates = a.rates;
@@ -72,60 +72,55 @@ __global__ __launch_bounds__(WT) void k_stage(StageArgs a) {
   const int kbeg = a.s2 + blockIdx.z * a.kc;
   const int kend = min(kbeg + a.kc, a.s2 + a.n2);
   const double dt = *a.dt;
-  const Gas<NS> g{a.K};
 #pragma unroll 1
   for (int k = kbeg; k < kend; k++) {
-    const int64_t o = Qin.off(i, j, k);
-    double dQ[NE];
-    {
+    const int64_t o = Qin.off(i, j, k), o0 = Q0.off(i, j, k), oo = Qout.off(i, j, k);
+    const int64_t fil = iF.off(i, j, k), fjl = jF.off(i, j, k), fkl = kF.off(i, j, k);
+    const int64_t fih = fil + 1, fjh = fjl + jF.sj, fkh = fkl + kF.sk;
+    const int64_t orr = CHEM ? rates.off(i, j, k) : 0;
+    const double J = det.at(det.off(i, j, k), 0);
+    // per conserved row: its six faces in the reference's order of sums, times detJinv, the
+    // source first where there is one (chemint/kernels/source.mako:3-10), then the RK combine in
+    // the reference's order of terms (integrators/kernels/combine.mako:1-13); a row at a time in a
+    // rolled loop keeps ~30 registers, so eight waves per SIMD hide the loads' latency
+#pragma unroll 1
+    for (int l = 0; l < NE; l++) {
 #pragma clang fp contract(off)
-      const double J = det.at(det.off(i, j, k), 0);
-      const int64_t fil = iF.off(i, j, k), fjl = jF.off(i, j, k), fkl = kF.off(i, j, k);
-      const int64_t fih = fil + 1, fjh = fjl + jF.sj, fkh = fkl + kF.sk;
-#pragma unroll
-      for (int l = 0; l < NE; l++)
-        dQ[l] = ((((iF.at(fil, l) + jF.at(fjl, l)) + kF.at(fkl, l)) - iF.at(fih, l)) - jF.at(fjh, l)) - kF.at(fkh, l);
-#pragma unroll
-      for (int l = 0; l < NE; l++) dQ[l] = dQ[l] * J;
-    }
-    if constexpr (CHEM) {
-      // chemint/kernels/source.mako:3-10: the source (pga_rates) begins dQ, the divergence is added
-      const int64_t orr = rates.off(i, j, k);
-#pragma unroll
-      for (int n = 0; n < NS - 1; n++) {
-#pragma clang fp contract(off)
-        dQ[5 + n] = rates.at(orr, n) + dQ[5 + n];
+      double d = ((((iF.at(fil, l) + jF.at(fjl, l)) + kF.at(fkl, l)) - iF.at(fih, l)) - jF.at(fjh, l)) - kF.at(fkh, l);
+      d = d * J;
+      if constexpr (CHEM) {
+        if (l >= 5) d = rates.at(orr, l - 5) + d;
       }
-    }
-    double Qn[NE];
-    {
-#pragma clang fp contract(off)
-      // integrators/kernels/combine.mako:1-13, terms in the reference's order
-      // the terms' values are selected as values: a select of pointers would put dQ in memory
-      const int64_t o0 = Q0.off(i, j, k);
+      // the terms' values are selected as values: a select of pointers would put them in memory
+      const double qin = Qin.at(o, l), q0 = Q0.at(o0, l);
+      double acc = 0.0;
 #pragma unroll
-      for (int l = 0; l < NE; l++) {
-        const double qin = Qin.at(o, l), q0 = Q0.at(o0, l);
-        double acc = 0.0;
-#pragma unroll
-        for (int t = 0; t < 4; t++) {
-          if (t < comb.n) {
-            const int src = comb.src[t];
-            const double x = src == SRC_QIN ? qin : src == SRC_Q0 ? q0 : dQ[l];
-            const double term = comb.withdt[t] ? comb.w[t] * dt * x : comb.w[t] * x;
-            acc = t == 0 ? term : acc + term;
-          }
+      for (int t = 0; t < 4; t++) {
+        if (t < comb.n) {
+          const int src = comb.src[t];
+          const double x = src == SRC_QIN ? qin : src == SRC_Q0 ? q0 : d;
+          const double term = comb.withdt[t] ? comb.w[t] * dt * x : comb.w[t] * x;
+          acc = t == 0 ? term : acc + term;
         }
-        Qn[l] = acc;
       }
+      Qout.at(oo, l) = acc;
     }
-    EosState s;
-    fromQ_regs<NS, EOS>(g, Qn, stin.at(o, 1), s);
-    const int64_t oo = Qout.off(i, j, k);
-#pragma unroll
-    for (int l = 0; l < NE; l++) Qout.at(oo, l) = Qn[l];
-    store_state<EOS>(stout, stout.off(i, j, k), s);
   }
+}
+
+// the EOS of the stage's result, T started from the input state (stin may be stout)
+template <int NS, int EOS>
+__global__ __launch_bounds__(256) void k_stage_eos(const double *__restrict__ K, View Q, View stin, View stout, Box3 b) {
+  int i, j, k;
+  if (!cell_of(b, i, j, k)) return;
+  const Gas<NS> g{K};
+  const int64_t o = Q.off(i, j, k);
+  double Qn[NS + 4];
+#pragma unroll
+  for (int l = 0; l < NS + 4; l++) Qn[l] = Q.at(o, l);
+  EosState s;
+  fromQ_regs<NS, EOS>(g, Qn, stin.at(stin.off(i, j, k), 1), s);
+  store_state<EOS>(stout, stout.off(i, j, k), s);
 }
 
 // the carried species' production rates of each cell (chemint/kernels/cell.mako:5-20), the
@@ -235,14 +230,14 @@ extern "C" int32_t pga_stage(const pga_case *c, pga_view Qin, pga_view stin, pga
   a.n0 = box.extent[0], a.n1 = box.extent[1], a.n2 = box.extent[2];
   a.kc = kchunk;
   const dim3 grid((a.n0 + WI - 1) / WI, (a.n1 + WJ - 1) / WJ, (a.n2 + kchunk - 1) / kchunk);
+  const Box3 b = box3(box);
   const bool done = dispatch_eos(d, [&]<int N, int E>() {
-    if constexpr (E == EOS_TPG) {
-      if (chem) {
-        hipLaunchKernelGGL((k_stage<N, E, true>), grid, dim3(WT), 0, stream_of(stream), a);
-        return;
-      }
-    }
-    hipLaunchKernelGGL((k_stage<N, E, false>), grid, dim3(WT), 0, stream_of(stream), a);
+    if (chem)
+      hipLaunchKernelGGL((k_stage<N + 4, true>), grid, dim3(WT), 0, stream_of(stream), a);
+    else
+      hipLaunchKernelGGL((k_stage<N + 4, false>), grid, dim3(WT), 0, stream_of(stream), a);
+    hipLaunchKernelGGL((k_stage_eos<N, E>), grid_of(b), dim3(TI, TJ), 0, stream_of(stream), c->K, a.Qout, a.stin,
+                       a.stout, b);
   });
   if (!done) {
     set_error("pga_stage: no kernel for %d species with eos %d", d.ns, d.eos);
