@@ -213,6 +213,17 @@ int32_t pga_wall_mirror(const pga_case *c, int32_t axis, int32_t side, const int
 int32_t pga_wall_flux(const pga_case *c, int32_t axis, int32_t side, const int32_t *ncell, int32_t ng, pga_view Q,
                       pga_view state, pga_view trstate, pga_view xcc, pga_view xf, pga_view S, pga_view F, void *stream);
 
+/* The CFL reduction over the cells of `box`: out[0..2] (device memory) receive the largest
+ * acoustic c sqrt(sum 1/d^2), convective sum |u . n| / d and combined (their sum) inverse time
+ * scales, each folded by fmax from -inf; d the cell's lengths (dijk, infinite along an axis one
+ * cell thick), n the mean of its two faces' unit normals per axis. `work` is device memory of at
+ * least pga_cflmax_worksize(box) doubles for the per-workgroup partials. Replaces the cflmax
+ * kernel and its reduction: peregrine/simulator/euler/kernels/cflmax.mako:3-33,
+ * peregrine/multiblock/solver.py:292-296, peregrine/backend/device/backend.py:24-41. */
+int64_t pga_cflmax_worksize(pga_box box);
+int32_t pga_cflmax(const pga_case *c, pga_view Q, pga_view state, pga_view dijk, pga_view iS, pga_view jS,
+                   pga_view kS, pga_box box, double *work, int64_t nwork, double *out, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

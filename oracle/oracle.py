@@ -95,6 +95,7 @@ def _load():
                                   Arr, Arr]
     lib.pgo_grads.argtypes = [P(Case), Arr, Arr, Arr, Arr, Box]
     lib.pgo_diffflux.argtypes = [P(Case), C.c_int, C.c_double, Arr, Arr, Arr, Arr, Arr, Arr, Arr, Arr, Arr, Box]
+    lib.pgo_cflmax.argtypes = [Arr, Arr, Arr, Arr, Arr, Arr, Box, _dp]
     lib.pgo_step.argtypes = [P(Case), P(Block), C.c_double]
     lib.pgo_stage.argtypes = [P(Case), P(Block), C.c_int, C.c_double]
     lib.pgo_advance.argtypes = [P(Case), P(Block), C.c_int, C.c_double]

@@ -1491,3 +1491,38 @@ void pgo_step(const pgo_case *c, pgo_block *b, double dt) {
   const int nst = b->integrator == 3 ? 4 : b->integrator + 1;
   for (int n = 0; n < nst; n++) pgo_stage(c, b, n, dt);
 }
+
+/* euler/kernels/cflmax.mako:3-33: per interior cell the acoustic speed c sqrt(sum 1/d^2) over
+ * the acoustic length, the convective sum of |u . n| / d per axis (n the mean of the cell's two
+ * faces' unit normals) and their sum; folded by fmax from -inf (codegen/grammar.py:34-42) */
+void pgo_cflmax(pgo_arr Q, pgo_arr state, pgo_arr dijk, pgo_arr iS, pgo_arr jS, pgo_arr kS, pgo_box r, double out[3]) {
+  const pgo_arr S[3] = {iS, jS, kS};
+  double best[3] = {-HUGE_VAL, -HUGE_VAL, -HUGE_VAL};
+  for (int k = r.start[2]; k < r.start[2] + r.extent[2]; k++)
+    for (int j = r.start[1]; j < r.start[1] + r.extent[1]; j++)
+      for (int i = r.start[0]; i < r.start[0] + r.extent[0]; i++) {
+        const double di = *at(dijk, i, j, k, 0), dj = *at(dijk, i, j, k, 1), dk = *at(dijk, i, j, k, 2);
+        const double rhoinv = 1.0 / *at(Q, i, j, k, 0);
+        const double u = *at(Q, i, j, k, 1) * rhoinv, v = *at(Q, i, j, k, 2) * rhoinv, w = *at(Q, i, j, k, 3) * rhoinv;
+        double un[3];
+        for (int a = 0; a < 3; a++) {
+          const int hi[3] = {i + (a == 0), j + (a == 1), k + (a == 2)};
+          const double L0 = *at(S[a], i, j, k, 0), L1 = *at(S[a], i, j, k, 1), L2 = *at(S[a], i, j, k, 2);
+          const double H0 = *at(S[a], hi[0], hi[1], hi[2], 0), H1 = *at(S[a], hi[0], hi[1], hi[2], 1),
+                       H2 = *at(S[a], hi[0], hi[1], hi[2], 2);
+          const double S0inv = 1.0 / sqrt(L0 * L0 + L1 * L1 + L2 * L2);
+          const double S1inv = 1.0 / sqrt(H0 * H0 + H1 * H1 + H2 * H2);
+          const double nx = 0.5 * (L0 * S0inv + H0 * S1inv);
+          const double ny = 0.5 * (L1 * S0inv + H1 * S1inv);
+          const double nz = 0.5 * (L2 * S0inv + H2 * S1inv);
+          un[a] = fabs(nx * u + ny * v + nz * w);
+        }
+        const double c = *at(state, i, j, k, 7);
+        const double acoustic = c * sqrt(1.0 / (di * di) + 1.0 / (dj * dj) + 1.0 / (dk * dk));
+        const double convective = un[0] / di + un[1] / dj + un[2] / dk;
+        best[0] = fmax(best[0], acoustic);
+        best[1] = fmax(best[1], convective);
+        best[2] = fmax(best[2], acoustic + convective);
+      }
+  for (int n = 0; n < 3; n++) out[n] = best[n];
+}

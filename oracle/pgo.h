@@ -103,6 +103,10 @@ void pgo_wall_mirror(const pgo_case *c, int axis, int side, const int *ncell, in
 void pgo_wall_flux(const pgo_case *c, int axis, int side, const int *ncell, int ng, int viscous, pgo_arr Q,
                    pgo_arr state, pgo_arr trstate, pgo_arr xcc, pgo_arr xf, pgo_arr S, pgo_arr F);
 
+/* the largest acoustic, convective and combined inverse time scales over the box's cells (the
+ * CFL controller's step is cfl over the combined one) */
+void pgo_cflmax(pgo_arr Q, pgo_arr state, pgo_arr dijk, pgo_arr iS, pgo_arr jS, pgo_arr kS, pgo_box r, double out[3]);
+
 void pgo_step(const pgo_case *c, pgo_block *b, double dt);
 /* one right-hand side of register q into register dq (prerhs then rhs), as a stage begins it */
 void pgo_rhs(const pgo_case *c, pgo_block *b, int q, int dq);

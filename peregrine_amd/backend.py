@@ -6,15 +6,22 @@ is chosen by name) and the allocation interface (``peregrine/backend/allocation.
 tensors on the device) and lent to the HIP library as raw pointers; the
 library never allocates or frees it.
 
-Layout: an array of host-visible shape (ni, nj, nk, ncomp) is held as one
-contiguous block in Fortran order, i fastest and each component a separate
-plane (``peregrine/backend/device/backend.py:56-57``).
+Layout: an array of host-visible shape (ni, nj, nk, ncomp) is held in Fortran
+order, i fastest and each component a separate plane
+(``peregrine/backend/device/backend.py:56-57``). Each i-row is padded to a
+multiple of 16 doubles (128 B, one L2 cache line) and the array is placed so
+that its first interior index i = lead starts a line: a wave's 64 lanes along
+i then read four whole lines rather than touching five. The padding lies
+between rows, never inside one; the library takes the strides with the
+pointer.
 """
 
 import numpy as np
 
 from peregrine_amd import native
 from peregrine_amd.misc import find_subclass
+
+ROW_ALIGN = 16
 
 
 class BaseBackend:
@@ -37,13 +44,18 @@ def _torch():
 class DeviceArray:
     """A block array on the device: host shape (ni, nj, nk, ncomp), device planes per component."""
 
-    def __init__(self, backend, shape, name=None):
+    def __init__(self, backend, shape, name=None, lead=0):
         torch = _torch()
         self.backend = backend
         self.shape = tuple(int(n) for n in shape)
         self.name = name
         ni, nj, nk, nc = self.shape
-        self.t = torch.zeros((nc, nk, nj, ni), dtype=torch.float64, device=backend.device)
+        sj = -(-ni // ROW_ALIGN) * ROW_ALIGN
+        self.strides = (sj, sj * nj, sj * nj * nk)
+        off = (-int(lead)) % ROW_ALIGN
+        # the allocator hands out blocks aligned to at least 128 B
+        buf = torch.zeros(off + self.strides[2] * nc, dtype=torch.float64, device=backend.device)
+        self.t = buf.as_strided((nc, nk, nj, ni), self.strides[::-1] + (1,), off)
 
     def __repr__(self):
         return f"<DeviceArray {self.name} {self.shape}>"
@@ -53,7 +65,7 @@ class DeviceArray:
         return self.t.data_ptr()
 
     def view(self):
-        return native.view(self.ptr, *self.shape)
+        return native.view(self.ptr, *self.shape, strides=self.strides)
 
     def get(self, component=None):
         self.backend.synchronize()
@@ -126,8 +138,8 @@ class Mi355xBackend(BaseBackend):
         """The HIP stream every launch of this backend goes on: torch's current one."""
         return self.torch.cuda.current_stream(self.device).cuda_stream
 
-    def alloc(self, shape, name=None):
-        return DeviceArray(self, shape, name)
+    def alloc(self, shape, name=None, lead=0):
+        return DeviceArray(self, shape, name, lead)
 
     def scalar(self, name, value=0.0):
         return DeviceScalar(self, name, value)

@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "../../include/peregrine_amd.h"
 #include "generated/pga_tables.hpp"
 #define PGA_ELOG_TABLE static __constant__ const
@@ -141,6 +143,48 @@ __host__ inline dim3 grid_of(const Box3 &b) {
   return dim3((b.e0 + TI - 1) / TI, (b.e1 + TJ - 1) / TJ, b.e2);
 }
 
+// Flat launches: 256-thread blocks over the box's cells in i-fastest order, so a row
+// of 129 faces costs 129 lanes rather than three 64-lane waves.  The hardware deals
+// consecutive blocks round-robin to the 8 XCDs, each with its own L2; the block
+// number is remapped so that every XCD walks one contiguous run of the box and the
+// rows a stencil shares with its neighbours stay in one L2.
+constexpr int FLAT = 256;
+constexpr int NXCD = 8;
+
+// a flat launch indexes its cells in 32 bits: boxes beyond FLAT_MAX cells go in k-slabs
+constexpr int64_t FLAT_MAX = int64_t(1) << 31;
+
+__host__ inline dim3 flat_grid(const Box3 &b) {
+  return dim3((unsigned)(((int64_t)b.e0 * b.e1 * b.e2 + FLAT - 1) / FLAT));
+}
+
+// launch(sub-box) over slabs of whole k-planes, each under FLAT_MAX cells
+template <class F>
+__host__ inline void flat_slabs(const Box3 &b, F &&launch) {
+  const int64_t plane = (int64_t)b.e0 * b.e1;
+  const int per = (int)std::max<int64_t>(1, (FLAT_MAX - FLAT) / std::max<int64_t>(plane, 1));
+  for (int k0 = 0; k0 < b.e2; k0 += per) {
+    Box3 s = b;
+    s.s2 = b.s2 + k0;
+    s.e2 = std::min(per, b.e2 - k0);
+    launch(s);
+  }
+}
+
+__device__ __forceinline__ unsigned xcd_block() {
+  const unsigned nb = gridDim.x, bid = blockIdx.x, x = bid % NXCD, q = nb / NXCD, r = nb % NXCD;
+  return x * q + (x < r ? x : r) + bid / NXCD;
+}
+
+__device__ __forceinline__ bool flat_cell_of(const Box3 &b, int &i, int &j, int &k) {
+  const unsigned t = xcd_block() * FLAT + threadIdx.x;
+  const unsigned row = t / (unsigned)b.e0;
+  i = b.s0 + (int)(t - row * (unsigned)b.e0);
+  j = b.s1 + (int)(row % (unsigned)b.e1);
+  k = b.s2 + (int)(row / (unsigned)b.e1);
+  return row < (unsigned)b.e1 * (unsigned)b.e2;
+}
+
 // the cell a thread takes, false when the block overhangs the box
 __device__ __forceinline__ bool cell_of(const Box3 &b, int &i, int &j, int &k) {
   const int ii = blockIdx.x * TI + threadIdx.x, jj = blockIdx.y * TJ + threadIdx.y;
@@ -199,21 +243,15 @@ __device__ __forceinline__ double texp(double x) {
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ double elog(double x) {
 #pragma clang fp contract(off)
-  // zero, negative, NaN and infinity: the limits; a subnormal is scaled into the normal range. No
-  // library call: a call's save and restore around the EOS costs registers on every path
-  if (!(x > 0.0)) return x == 0.0 ? -__builtin_huge_val() : (x - x) / 0.0;
-  if (x > 1.7976931348623157e308) return x;
-  int escale = 0;
-  if (x < 2.2250738585072014e-308) x *= 18014398509481984.0, escale = -54;
-  if (escale == 0 && fabs(x - 1.0) < 0.03125) {
-    // near 1 the table's terms would cancel: ln(1 + r) by its series, r = x - 1 exact
-    const double r = x - 1.0;
-    double q = -1.0 / 14.0;
-    for (int n = 13; n >= 2; n--) q = fma(q, r, ((n & 1) ? 1.0 : -1.0) / n);
-    return r + (r * r) * q;
-  }
-  const long long bits = __double_as_longlong(x);
-  const int e = (int)(bits >> 52) - 1023 + escale;
+  // Every lane evaluates the table path, the series near 1 and the limits, and selects: the EOS
+  // calls this on both sides of a face, and without branches the two calls and the code around them
+  // stay one block the scheduler can interleave. A subnormal is scaled into the normal range;
+  // zero, negative, NaN and infinity take their limits. No library call: a call's save and restore
+  // around the EOS costs registers on every path
+  const bool sub = x < 2.2250738585072014e-308;
+  const double xs = sub ? x * 18014398509481984.0 : x;
+  const long long bits = __double_as_longlong(xs);
+  const int e = (int)((bits >> 52) & 0x7ff) - 1023 - (sub ? 54 : 0);
   const int k = (int)((bits >> 45) & 127);
   const double m = __longlong_as_double((bits & 0x000fffffffffffffLL) | 0x3ff0000000000000LL);
   const double r = fma(m, ELOG_RCP[k], -1.0);
@@ -226,7 +264,16 @@ __device__ __forceinline__ double elog(double x) {
   const double p = (r * r) * q;
   const double hi = e * PGA_ELOG_LN2HI + ELOG_LHI[k];
   const double lo = e * PGA_ELOG_LN2LO + ELOG_LLO[k];
-  return hi + (r + (p + lo));
+  const double table = hi + (r + (p + lo));
+  // near 1 the table's terms would cancel: ln(1 + r) by its series, r = x - 1 exact
+  const double r1 = x - 1.0;
+  double q1 = -1.0 / 14.0;
+#pragma unroll
+  for (int n = 13; n >= 2; n--) q1 = fma(q1, r1, ((n & 1) ? 1.0 : -1.0) / n);
+  const double series = r1 + (r1 * r1) * q1;
+  const double finite = fabs(r1) < 0.03125 ? series : table;
+  const double limit = x == 0.0 ? -__builtin_huge_val() : x > 0.0 ? x : __builtin_nan("");
+  return x > 0.0 && x <= 1.7976931348623157e308 ? finite : limit;
 }
 
 // ---------------------------------------------------------------------------

@@ -34,16 +34,15 @@ __device__ __forceinline__ double limit(double a, double b) {
   return (a + fabs(a)) * b / (b + fabs(a));
 }
 
-// recon/muscl.mako:5-13: the limited slope across the face carried to dp's sign and back
+// recon/muscl.mako:5-13: the limited slope across the face carried to dp's sign and back; zero
+// where dp is. Evaluated on every lane and selected, so the 26 slopes of a face stay one basic
+// block the scheduler can interleave (a branch per slope would fence each one off)
 template <int LIM>
 __device__ __forceinline__ double phi(double dm, double dp) {
 #pragma clang fp contract(off)
-  double p = 0.0;
-  if (dp != 0.0) {
-    const double s = copysign(1.0, dp);
-    p = s * limit<LIM>(s * dm, fabs(dp));
-  }
-  return p;
+  const double s = copysign(1.0, dp);
+  const double p = s * limit<LIM>(s * dm, fabs(dp));
+  return dp != 0.0 ? p : 0.0;
 }
 
 template <int LIM>

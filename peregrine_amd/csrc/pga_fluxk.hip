@@ -10,7 +10,7 @@ template <int NS, int EOS, int RECON, int PRIMARY, int LIM>
 __global__ __launch_bounds__(256) void k_flux(const double *__restrict__ K, int axis, View Q, View st, View S,
                                               View F, Box3 b) {
   int i, j, k;
-  if (!cell_of(b, i, j, k)) return;
+  if (!flat_cell_of(b, i, j, k)) return;
   const Gas<NS> g{K};
   const int64_t o = Q.off(i, j, k);
   const int64_t d = axis == 0 ? 1 : axis == 1 ? Q.sj : Q.sk;
@@ -28,12 +28,13 @@ __global__ __launch_bounds__(256) void k_flux(const double *__restrict__ K, int 
 template <int NS, int EOS>
 static void launch_flux(const pga_case *c, int axis, pga_view Q, pga_view st, pga_view S, pga_view F, Box3 b,
                         hipStream_t s) {
-  const dim3 grid = grid_of(b), block(TI, TJ);
   const int lim = c->desc.limiter, recon = c->desc.recon, prim = c->desc.primary;
 #define L(RECON, PRIM, LIM)                                                                                    \
   if (recon == RECON && prim == PRIM && (RECON == RECON_CONSTANT || lim == LIM))                              \
-    hipLaunchKernelGGL((k_flux<NS, EOS, RECON, PRIM, LIM>), grid, block, 0, s, c->K, axis, view(Q), view(st),   \
-                       view(S), view(F), b);
+    flat_slabs(b, [&](const Box3 &sb) {                                                                        \
+      hipLaunchKernelGGL((k_flux<NS, EOS, RECON, PRIM, LIM>), flat_grid(sb), dim3(FLAT), 0, s, c->K, axis,      \
+                         view(Q), view(st), view(S), view(F), sb);                                              \
+    });
   L(RECON_CONSTANT, FLUX_KEPAEC, LIM_NONE)
   L(RECON_CONSTANT, FLUX_HLLC, LIM_NONE)
   L(RECON_CONSTANT, FLUX_RUSANOV, LIM_NONE)
@@ -71,6 +72,7 @@ extern "C" int32_t pga_flux(const pga_case *c, int32_t axis, pga_view Q, pga_vie
                 "pga_flux: the faces' stencil reaches outside Q or state");
     PGA_REQUIRE(covers(S, box, 3) && covers(F, box, d.ns + 4), "pga_flux: the faces reach outside S or F");
   }
+  if (!flat_fits(box)) return PGA_ERANGE;
   const Box3 b = box3(box);
   const bool done = dispatch_eos(d, [&]<int N, int E>() { launch_flux<N, E>(c, axis, Q, state, S, F, b, stream_of(stream)); });
   if (!done) {

@@ -36,6 +36,13 @@ inline bool box_ok(const pga_box &b) {
   return b.extent[2] <= 65535;
 }
 
+// a flat launch takes a box whose every k-plane indexes in 32 bits
+inline bool flat_fits(const pga_box &b) {
+  if ((int64_t)b.extent[0] * b.extent[1] < FLAT_MAX - FLAT) return true;
+  set_error("a plane of %d x %d cells exceeds the flat launch's 2^31 cells", b.extent[0], b.extent[1]);
+  return false;
+}
+
 inline bool empty(const pga_box &b) { return b.extent[0] == 0 || b.extent[1] == 0 || b.extent[2] == 0; }
 
 // whether the cells [start - lo, start + extent + hi) along each axis lie inside the view,

@@ -272,7 +272,7 @@ template <int NS, int AXIS>
 __global__ __launch_bounds__(256) void k_diffflux(DiffArgs a) {
 #pragma clang fp contract(off)
   int i, j, k;
-  if (!cell_of(a.b, i, j, k)) return;
+  if (!flat_cell_of(a.b, i, j, k)) return;
   const View Q = a.Q, st = a.st, tr = a.tr, hsp = a.hsp, gr = a.gr, xcc = a.xcc, xf = a.xf, S = a.S, F = a.F;
   const int64_t oR = Q.off(i, j, k), oL = oR - (AXIS == 0 ? 1 : AXIS == 1 ? Q.sj : Q.sk);
   const int64_t ofa = S.off(i, j, k), off_ = F.off(i, j, k), oxf = xf.off(i, j, k);
@@ -454,17 +454,21 @@ extern "C" int32_t pga_diffflux(const pga_case *c, int32_t axis, double alpha, p
   PGA_REQUIRE(Q.sj == state.sj && Q.sk == state.sk && Q.sj == trstate.sj && Q.sk == trstate.sk && Q.sj == hsp.sj &&
                   Q.sk == hsp.sk && Q.sj == grads.sj && Q.sk == grads.sk && Q.sj == xcc.sj && Q.sk == xcc.sk,
               "pga_diffflux: the cell arrays must share their layout");
+  if (!flat_fits(box)) return PGA_ERANGE;
   DiffArgs a{view(Q), view(state), view(trstate), view(hsp), view(grads), view(xcc), view(xf), view(S), view(F),
              alpha, box3(box)};
   bool done = false;
 #define X(N)                                                                                                     \
   if (d.ns == N) {                                                                                               \
-    if (axis == 0)                                                                                               \
-      hipLaunchKernelGGL((k_diffflux<N, 0>), grid_of(a.b), dim3(TI, TJ), 0, stream_of(stream), a);              \
-    else if (axis == 1)                                                                                          \
-      hipLaunchKernelGGL((k_diffflux<N, 1>), grid_of(a.b), dim3(TI, TJ), 0, stream_of(stream), a);              \
-    else                                                                                                         \
-      hipLaunchKernelGGL((k_diffflux<N, 2>), grid_of(a.b), dim3(TI, TJ), 0, stream_of(stream), a);              \
+    flat_slabs(box3(box), [&](const Box3 &sb) {                                                                  \
+      a.b = sb;                                                                                                  \
+      if (axis == 0)                                                                                             \
+        hipLaunchKernelGGL((k_diffflux<N, 0>), flat_grid(sb), dim3(FLAT), 0, stream_of(stream), a);             \
+      else if (axis == 1)                                                                                        \
+        hipLaunchKernelGGL((k_diffflux<N, 1>), flat_grid(sb), dim3(FLAT), 0, stream_of(stream), a);             \
+      else                                                                                                       \
+        hipLaunchKernelGGL((k_diffflux<N, 2>), flat_grid(sb), dim3(FLAT), 0, stream_of(stream), a);             \
+    });                                                                                                          \
     done = true;                                                                                                 \
   }
   PGA_NS_CASES(X)

@@ -56,6 +56,21 @@ def facecenters(nodes, a):
     return 0.25 * (c[0] + c[1] + c[2] + c[3])
 
 
+def dijk(nodes, ncell):
+    """Per cell and axis, the distance between the centres of its two faces along that axis; an
+    axis one cell thick is not marched in, and its length is infinite so that no step limit binds
+    on it (``multiblock/block/grid.py:164-176``)."""
+    lengths = []
+    for a in range(3):
+        centers = facecenters(nodes, a)
+        far, near = [slice(None)] * 3, [slice(None)] * 3
+        far[a], near[a] = np.s_[1:], np.s_[:-1]
+        span = centers[tuple(far)] - centers[tuple(near)]
+        length = np.sqrt((span**2).sum(axis=-1))
+        lengths.append(np.full_like(length, np.inf) if ncell[a] == 1 else length)
+    return np.stack(lengths, axis=-1)
+
+
 def areavecs(nodes, a):
     diagonal = [(a + 1) % 3, (a + 2) % 3]
     c00, c01, c10, c11 = (_corner(nodes, diagonal, *h) for h in ((0, 0), (0, 1), (1, 0), (1, 1)))

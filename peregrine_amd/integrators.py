@@ -26,6 +26,8 @@ def combine(out, *pairs, step=True):
 
 
 class FixedController:
+    """``integrators/controllers.py:13-17``: the configured dt."""
+
     name = "fixed"
 
     def __init__(self, cfg):
@@ -33,6 +35,26 @@ class FixedController:
 
     def stepsize(self, solver):
         return self.dt
+
+
+class CFLController:
+    """``integrators/controllers.py:19-29``: cfl over the largest combined inverse time scale, no
+    more than dt-max, shortened to land on the next time a plugin acts at."""
+
+    name = "cfl"
+
+    def __init__(self, cfg):
+        self.cfl = float(cfg["simulation"]["cfl"])
+        self.dtmax = float(cfg["simulation"]["dt-max"])
+
+    def stepsize(self, solver):
+        dt = min(self.cfl / solver.cflmax()["combined"], self.dtmax)
+        times = [p.tnext(solver) for p in solver.plugins]
+        now = solver.tme.get()
+        return min([dt, *(t - now for t in times if t is not None)])
+
+
+CONTROLLERS = {c.name: c for c in (FixedController, CFLController)}
 
 
 class BaseRungeKutta:
@@ -105,6 +127,7 @@ class RK4(BaseRungeKutta):
 def get_integrator(cfg):
     name = cfg["time-integration"]["integrator"]
     cls = find_subclass(BaseRungeKutta, name=name)
-    if cfg["simulation"]["controller"] != "fixed":
-        raise ValueError("this build steps at a fixed dt (controller: fixed)")
-    return cls(FixedController(cfg))
+    name = cfg["simulation"]["controller"]
+    if name not in CONTROLLERS:
+        raise ValueError(f"controller {name!r}: this build has {sorted(CONTROLLERS)}")
+    return cls(CONTROLLERS[name](cfg))

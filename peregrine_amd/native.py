@@ -20,9 +20,11 @@ class View(C.Structure):
     _fields_ = [("data", C.c_void_p), ("sj", C.c_int64), ("sk", C.c_int64), ("sc", C.c_int64), ("n", _i32 * 4)]
 
 
-def view(ptr, ni, nj, nk, nc):
-    """The ABI view of a contiguous array with extents (ni, nj, nk, nc), i fastest."""
-    return View(ptr, ni, ni * nj, ni * nj * nk, (_i32 * 4)(ni, nj, nk, nc))
+def view(ptr, ni, nj, nk, nc, strides=None):
+    """The ABI view of an array with extents (ni, nj, nk, nc), i fastest; strides (sj, sk, sc) in
+    elements, contiguous when not given."""
+    sj, sk, sc = strides if strides is not None else (ni, ni * nj, ni * nj * nk)
+    return View(ptr, sj, sk, sc, (_i32 * 4)(ni, nj, nk, nc))
 
 
 class Box(C.Structure):
@@ -72,6 +74,9 @@ SIGNATURES = {
                             C.c_void_p]),
     "pga_wall_mirror": (_i32, [C.c_void_p, _i32, _i32, _ip, _i32, View, View, View, C.c_void_p]),
     "pga_wall_flux": (_i32, [C.c_void_p, _i32, _i32, _ip, _i32, View, View, View, View, View, View, View, C.c_void_p]),
+    "pga_cflmax_worksize": (C.c_int64, [Box]),
+    "pga_cflmax": (_i32, [C.c_void_p, View, View, View, View, View, View, Box, C.c_void_p, C.c_int64, C.c_void_p,
+                          C.c_void_p]),
 }
 
 

@@ -1,4 +1,5 @@
-"""Solver plugins: called after steps, with the reference's start/due/__call__/finalize protocol.
+"""Solver plugins: called after steps, with the reference's start/due/tnext/__call__/finalize
+protocol and its cadence (niter-out or dt-out).
 
 Mirrors ``peregrine/plugins/solver/base.py:5-26``; ``report`` prints the banner and the
 seconds per iteration per cell (``plugins/solver/report.py:9-60``), ``nancheck`` stops a run
@@ -7,19 +8,51 @@ whose interior holds a non-finite value (``plugins/solver/nancheck.py:10-44``).
 
 import time
 
+import numpy as np
+
+
+class Cadence:
+    """When a plugin acts: every niter-out steps, or each time the clock passes a multiple of
+    dt-out; every step when neither is given (``plugins/solver/cadence.py:4-25``)."""
+
+    def __init__(self, cfgsect):
+        self.niterout = cfgsect.get("niter-out")
+        self.dtout = cfgsect.get("dt-out")
+        if self.niterout is None and self.dtout is None:
+            self.niterout = 1
+
+    def nreached(self, tme):
+        # a step that lands on a multiple of dt-out can fall a few ulps short of it: counted reached
+        tol = 8.0 * np.spacing(max(abs(tme), self.dtout))
+        return int((tme + tol) / self.dtout)
+
+    def next(self, solver):
+        if self.dtout is None:
+            return None
+        return (self.nreached(solver.tme.get()) + 1) * self.dtout
+
+    def due(self, solver):
+        if self.dtout is None:
+            return solver.nrt % self.niterout == 0
+        tme, dt = solver.tme.get(), solver.dt.get()
+        return self.nreached(tme) != self.nreached(tme - dt)
+
 
 class BasePlugin:
     name = None
-    defaults = {"every": 1}
 
     def __init__(self, cfgsect, solver):
-        self.cfg = {**self.defaults, **(cfgsect or {})}
+        self.cfg = dict(cfgsect or {})
+        self.cadence = Cadence(self.cfg)
 
     def start(self, solver):
         pass
 
     def due(self, solver):
-        return solver.nrt % int(self.cfg["every"]) == 0
+        return self.cadence.due(solver)
+
+    def tnext(self, solver):
+        return self.cadence.next(solver)
 
     def __call__(self, solver):
         pass

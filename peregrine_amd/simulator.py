@@ -54,6 +54,8 @@ class BaseSimulator:
 class EulerSimulator(BaseSimulator):
     name = "euler"
     viscous = False
+    # the CFL reduction's three speeds, in its order (simulator/euler/simulator.py:23)
+    cflspeeds = ("acoustic", "convective", "combined")
 
     def __init__(self, cfg):
         super().__init__(cfg)

@@ -27,6 +27,7 @@ class Block:
         self.mb = mb
         self.faces = {n: Interface(n) for n in range(1, 7)}
         self.interior_nodes = None
+        self.dijk = None
 
     def intface(self, nface):
         return self.faces[nface]
@@ -89,23 +90,27 @@ class Block:
     def alloc(self, backend, sim, nregs):
         ng, cs = self.ng, self.cellshape
         ne = sim.ne
-        self.regs = [backend.alloc(cs + (ne,), name=f"Q{r}") for r in range(nregs)]
-        self.state = backend.alloc(cs + (sim.nstate,), name="state")
-        self.iF, self.jF, self.kF = (backend.alloc(self.faceshape(a) + (ne,), name=f"{'ijk'[a]}F") for a in range(3))
-        self.iS, self.jS, self.kS = (backend.alloc(self.faceshape(a) + (3,), name=f"{'ijk'[a]}S") for a in range(3))
-        self.detJinv = backend.alloc(cs + (1,), name="detJinv")
-        self.xcc = backend.alloc(cs + (3,), name="xcc")
-        self.q = backend.alloc(cs + (ne,), name="q")
+        # every array's rows start a cache line at the first interior index, i = ng
+        def alloc(shape, name):
+            return backend.alloc(shape, name=name, lead=ng)
+
+        self.regs = [alloc(cs + (ne,), f"Q{r}") for r in range(nregs)]
+        self.state = alloc(cs + (sim.nstate,), "state")
+        self.iF, self.jF, self.kF = (alloc(self.faceshape(a) + (ne,), f"{'ijk'[a]}F") for a in range(3))
+        self.iS, self.jS, self.kS = (alloc(self.faceshape(a) + (3,), f"{'ijk'[a]}S") for a in range(3))
+        self.detJinv = alloc(cs + (1,), "detJinv")
+        self.xcc = alloc(cs + (3,), "xcc")
+        self.q = alloc(cs + (ne,), "q")
         self.viscous = sim.viscous
         if sim.viscous:
             ns = sim.ns
             # navstokes/simulator.py:30-36, advecdiff/simulator.py:12-15: the transport state, the
             # species enthalpies, the gradients; the metrics the gradients and viscous flux read
-            self.trstate = backend.alloc(cs + (2 + ns,), name="trstate")
-            self.hsp = backend.alloc(cs + (ns,), name="hsp")
-            self.grads = backend.alloc(cs + (3 * sim.ngrad,), name="grads")
-            self.Jinv = backend.alloc(cs + (9,), name="Jinv")
-            self.xf = [backend.alloc(self.faceshape(a) + (3,), name=f"{'ijk'[a]}xc") for a in range(3)]
+            self.trstate = alloc(cs + (2 + ns,), "trstate")
+            self.hsp = alloc(cs + (ns,), "hsp")
+            self.grads = alloc(cs + (3 * sim.ngrad,), "grads")
+            self.Jinv = alloc(cs + (9,), "Jinv")
+            self.xf = [alloc(self.faceshape(a) + (3,), f"{'ijk'[a]}xc") for a in range(3)]
 
     @property
     def Q(self):
@@ -133,6 +138,7 @@ class Solver:
         self.nrt = 0
         # register -> the derivative register its prerhs began (None: halos ready, no derivative)
         self.readied = {}
+        self._cflwork = None
         self.tme = HostScalar(0.0)
         self.dt = backend.scalar("dt", cfg["simulation"]["dt"])
         self.ng = sim.ng
@@ -147,7 +153,8 @@ class Solver:
         self.fused = self._fused_supported()
         if self.fused:
             for blk in self.blocks:
-                blk.rates = backend.alloc(blk.cellshape + (max(sim.ns - 1, 1),), name="rates") if sim.chemistry else None
+                rshape = blk.cellshape + (max(sim.ns - 1, 1),)
+                blk.rates = backend.alloc(rshape, name="rates", lead=self.ng) if sim.chemistry else None
         self.plugins = self._plugins()
         self._set_state(state)
         for p in self.plugins:
@@ -486,6 +493,26 @@ class Solver:
         finally:
             for p in self.plugins:
                 p.finalize(self)
+
+    def cflmax(self):
+        """{acoustic, convective, combined}: the largest inverse time scales over every block's
+        interior, reduced on the device per block and across blocks here
+        (``multiblock/solver.py:292-296``)."""
+        best = np.full(3, -np.inf)
+        torch = self.backend.torch
+        for blk in self.blocks:
+            if blk.dijk is None:
+                blk.dijk = self.backend.alloc(blk.cellshape + (3,), name="dijk", lead=self.ng)
+                blk.dijk.set(grid.dijk(blk.nodes, blk.ncell))
+            box = blk.interior_box()
+            n = int(native.lib().pga_cflmax_worksize(box))
+            if self._cflwork is None or self._cflwork.numel() < n + 3:
+                self._cflwork = torch.empty(n + 3, dtype=torch.float64, device=self.backend.device)
+            work = self._cflwork
+            self._call("pga_cflmax", self.case.handle, blk.Q.view(), blk.state.view(), blk.dijk.view(), blk.iS.view(),
+                       blk.jS.view(), blk.kS.view(), box, work.data_ptr() + 24, n, work.data_ptr())
+            best = np.fmax(best, work[:3].cpu().numpy())
+        return dict(zip(self.sim.cflspeeds, best.tolist()))
 
     def synchronize(self):
         self.backend.synchronize()

@@ -684,3 +684,76 @@ def test_slip_walls_steps_match(dev, fused):
     err = _rowerr(got, hb.regs[0][hb.interior()])
     print(f"slip walls 10 steps ({'fused' if fused else 'per-kernel'}): {err:.3e}")
     assert err < 1e-10
+
+
+def test_cflmax_matches_to_the_bit(dev):
+    """The CFL reduction on a jostled grid with a random state, through the C ABI, and its work
+    size for a box that needs more than one workgroup."""
+    from peregrine_amd import grid
+
+    cfg, sim, nc, oc = _case("binary", recon="muscl", primary="hllc", limiter="koren")
+    ncell = (37, 9, 6)
+    hb = sp.HostBlock(ncell, sim.ng, sim.ne, 1, lengths=(0.37, 0.09, 0.06), jostle=0.3, seed=2)
+    rng = np.random.default_rng(21)
+    q = np.zeros(hb.cshape + (sim.ne,))
+    q[..., 0] = 101325.0 * rng.uniform(0.5, 2.0, hb.cshape)
+    q[..., 1:4] = rng.uniform(-80, 80, hb.cshape + (3,))
+    q[..., 4] = rng.uniform(250.0, 900.0, hb.cshape)
+    q[..., 5] = rng.uniform(0.0, 1.0, hb.cshape)
+    full = O.box((0, 0, 0), hb.cshape)
+    O.lib.pgo_fromprims(oc.ref, O.arr(sp.to_dev(q)), O.arr(hb.regs[0]), O.arr(hb.state), full)
+    d = sp.to_dev(grid.dijk(hb.nodes, ncell))
+    g = hb.ng
+    want = np.zeros(3)
+    O.lib.pgo_cflmax(O.arr(hb.regs[0]), O.arr(hb.state), O.arr(d), *(O.arr(s) for s in hb.S), O.box((g,) * 3, ncell),
+                     want.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    L = native.lib()
+    box = native.box((g,) * 3, ncell)
+    n = L.pga_cflmax_worksize(box)
+    assert n == 3 * ((37 * 9 * 6 + 255) // 256)
+    work, out = dev.put(np.full(n, np.nan)), dev.put(np.zeros(3))
+    native.check(L.pga_cflmax(nc.handle, dev.view(dev.put(hb.regs[0])), dev.view(dev.put(hb.state)), dev.view(dev.put(d)),
+                              *(dev.view(dev.put(s)) for s in hb.S), box, work.data_ptr(), n, out.data_ptr(), None),
+                 "pga_cflmax")
+    got = dev.get(out)
+    print("cflmax", got, want)
+    assert np.array_equal(got, want)
+    # too small a work array is refused before any launch
+    rc = L.pga_cflmax(nc.handle, dev.view(dev.put(hb.regs[0])), dev.view(dev.put(hb.state)), dev.view(dev.put(d)),
+                      *(dev.view(dev.put(s)) for s in hb.S), box, work.data_ptr(), n - 1, out.data_ptr(), None)
+    assert rc == -1
+
+
+def test_cfl_controlled_steps_match(dev):
+    """Three steps under the CFL controller: each dt from the device's reduction, the oracle
+    stepping with dt from its own; the solutions agree to the bit."""
+    import peregrine_amd as pg
+    from peregrine_amd import grid
+
+    ncell, lengths = (16, 12, 8), (1.0, 0.75, 0.5)
+    cfg = sp.config("air-tpg", recon="muscl", primary="hllc", limiter="koren")
+    cfg["simulation"]["controller"], cfg["simulation"]["cfl"], cfg["simulation"]["dt-max"] = "cfl", 0.4, 1.0
+    cfg["simulation"]["niter"] = 3
+    mb = pg.get_solver(cfg, pg.mesher.CubeMesher(blkdims=[n + 1 for n in ncell], lengths=lengths, periodic=[True] * 3),
+                       pg.get_backend("mi355x", cfg))
+    sim = mb.sim
+    oc = O.OracleCase(sim.mixture, "muscl", "koren", "hllc")
+    hb = sp.HostBlock(ncell, sim.ng, sim.ne, 3, lengths=lengths)
+    q = sp.smooth_prims(hb.xcc, sim.ne)
+    mb.set_prims([q])
+    O.lib.pgo_fromprims(oc.ref, O.arr(sp.to_dev(q)), O.arr(hb.regs[0]), O.arr(hb.state), O.box((0, 0, 0), hb.cshape))
+    O.lib.pgo_fromQ(oc.ref, O.arr(hb.regs[0]), O.arr(hb.state), O.box((hb.ng,) * 3, hb.ncell))
+    b = sp.oracle_block(hb, "tvd-rk3", False)
+    d = sp.to_dev(grid.dijk(hb.nodes, ncell))
+    g = hb.ng
+    dts = []
+    for _ in range(3):
+        speeds = np.zeros(3)
+        O.lib.pgo_cflmax(O.arr(hb.regs[0]), O.arr(hb.state), O.arr(d), *(O.arr(s) for s in hb.S),
+                         O.box((g,) * 3, ncell), speeds.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        dt = min(0.4 / speeds[2], 1.0)
+        dts.append(dt)
+        O.lib.pgo_step(oc.ref, ctypes.byref(b), dt)
+    mb.run()
+    assert mb.nrt == 3 and mb.tme.get() == sum(dts)
+    assert np.array_equal(sp.to_dev(mb.blocks[0].Q.get())[hb.interior()], hb.regs[0][hb.interior()])
