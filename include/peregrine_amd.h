@@ -224,6 +224,32 @@ int64_t pga_cflmax_worksize(pga_box box);
 int32_t pga_cflmax(const pga_case *c, pga_view Q, pga_view state, pga_view dijk, pga_view iS, pga_view jS,
                    pga_view kS, pga_box box, double *work, int64_t nwork, double *out, void *stream);
 
+/* Sub-stepped chemistry's options: chem-integration.max-substeps, .bulk-substeps, .stiff-bins and
+ * .equilibrium-tolerance (peregrine/simulator/chemint/substepped.py:14-33) */
+typedef struct {
+  int32_t max_substeps;  /* the cap on a cell's substeps in a step, >= 1 */
+  int32_t bulk_substeps; /* the bulk pass's limit, 1..max_substeps; below the cap the stiff cells are listed */
+  int32_t stiff_bins;    /* 1..16 bins of the stiff list, by the substep count of the previous call */
+  int32_t reserved;
+  double equilibrium_tolerance; /* 0 <= tol < 1; 0 for no equilibrium exit */
+} pga_substep_opts;
+
+/* Int32 entries of the stiff list a compacted call needs for `box`: a header of 1 + bins counts and
+ * a segment per bin as long as the box; -1 for bad options or box. */
+int64_t pga_substep_listsize(const pga_substep_opts *o, pga_box box);
+
+/* Sub-stepped chemistry over the cells of `box` for a step of *dt (device): each cell integrates its
+ * finite-rate chemistry in explicit substeps, each as long as its limiting species allows, with the
+ * equilibrium exit at the first substep, and writes the mean rate over the step: into dQ (rows 0..4
+ * zero, 5.. the rates) when begin_dQ, else into the ns - 1 rows of `out` as pga_rates does. The
+ * substep counts are kept per cell in `substeps` (int32, device, box-shaped, i fastest) and key the
+ * next call's bins; `list` (device int32, pga_substep_listsize entries) is needed when
+ * bulk_substeps < max_substeps. Replaces chemint/substepped.py:52-68 (begin_dQ_graph) and the
+ * kernels peregrine/simulator/chemint/kernels/{substep,bulk,stiff,reset}.mako. */
+int32_t pga_substep(const pga_case *c, const pga_substep_opts *o, pga_view Q, pga_view state, pga_view out,
+                    int32_t begin_dQ, int32_t *substeps, int64_t nsubsteps, int32_t *list, int64_t nlist,
+                    const double *dt, pga_box box, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

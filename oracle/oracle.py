@@ -44,7 +44,7 @@ class Case(C.Structure):
         ("fwdorder", _dp), ("reactnu", _dp), ("prodnu", _dp), ("netnu", _dp), ("netnuMW", _dp),
         ("lfcpoly", _dp), ("tbdefeff", _dp), ("tbeffdev", _dp),
         ("logprefRu", C.c_double), ("prefRu", C.c_double), ("Rupref", C.c_double), ("logcfloor", C.c_double),
-        ("expclamp", C.c_double), ("troeuc", C.c_double), ("troeuhinv", C.c_double),
+        ("expclamp", C.c_double), ("troeuc", C.c_double), ("troeuhinv", C.c_double), ("cfloor", C.c_double),
         ("trans", C.c_int), ("nmu", C.c_int), ("nkappa", C.c_int),
         ("hpoly", _dp), ("mu0", _dp), ("kappa0", _dp), ("lewis", _dp), ("MWqinv", _dp), ("wilkepair", _dp),
         ("mupoly", _dp), ("kappapoly", _dp), ("Ru", C.c_double),
@@ -61,6 +61,7 @@ class Block(C.Structure):
         ("ncell", C.c_int * 3), ("ng", C.c_int), ("regs", Arr * 4), ("nregs", C.c_int),
         ("state", Arr), ("iF", Arr), ("jF", Arr), ("kF", Arr), ("iS", Arr), ("jS", Arr), ("kS", Arr),
         ("detJinv", Arr), ("integrator", C.c_int), ("chemistry", C.c_int),
+        ("subcap", C.c_int), ("subeqtol", C.c_double), ("substeps", C.POINTER(C.c_int32)), ("dt", C.c_double),
         ("viscous", C.c_int), ("alpha", C.c_double),
         ("trstate", Arr), ("hsp", Arr), ("grads", Arr), ("Jinv", Arr), ("xcc", Arr), ("xf", Arr * 3),
         ("bc", C.c_int * 6), ("nstate", C.c_int),
@@ -96,6 +97,10 @@ def _load():
     lib.pgo_grads.argtypes = [P(Case), Arr, Arr, Arr, Arr, Box]
     lib.pgo_diffflux.argtypes = [P(Case), C.c_int, C.c_double, Arr, Arr, Arr, Arr, Arr, Arr, Arr, Arr, Arr, Box]
     lib.pgo_cflmax.argtypes = [Arr, Arr, Arr, Arr, Arr, Arr, Box, _dp]
+    lib.pgo_sources.argtypes = [P(Case), C.c_double, _dp, C.c_double, _dp, _dp]
+    lib.pgo_dlog1p.restype = C.c_double
+    lib.pgo_dlog1p.argtypes = [C.c_double]
+    lib.pgo_substep.argtypes = [P(Case), Arr, Arr, Arr, C.c_int, P(C.c_int32), C.c_int, C.c_double, C.c_double, Box]
     lib.pgo_step.argtypes = [P(Case), P(Block), C.c_double]
     lib.pgo_stage.argtypes = [P(Case), P(Block), C.c_int, C.c_double]
     lib.pgo_advance.argtypes = [P(Case), P(Block), C.c_int, C.c_double]
@@ -108,6 +113,7 @@ tlog = lib.pgo_tlog
 texp = lib.pgo_texp
 elog = lib.pgo_elog
 dcbrt = lib.pgo_dcbrt
+dlog1p = lib.pgo_dlog1p
 
 EOS = {"cpg": 0, "tpg": 1, "real-gas": 2, "peng-robinson": 2}
 TRANS = {None: -1, "constant": 0, "kinetic-theory": 1, "chung": 2}
@@ -188,7 +194,7 @@ class OracleCase:
             for name in ("logA", "b", "EaR", "logprA", "prb", "prEaR", "nusum", "fwdorder", "reactnu", "prodnu",
                          "netnu", "netnuMW", "lfcpoly", "tbdefeff", "tbeffdev"):
                 setattr(c, name, d(cols[name]))
-            for name in ("logprefRu", "prefRu", "Rupref", "logcfloor", "expclamp", "troeuc", "troeuhinv"):
+            for name in ("logprefRu", "prefRu", "Rupref", "logcfloor", "expclamp", "troeuc", "troeuhinv", "cfloor"):
                 setattr(c, name, sc[name])
         self.c = c
         self.ns, self.ne = ns, 4 + ns

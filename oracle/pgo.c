@@ -758,7 +758,16 @@ static double troeblend(const pgo_case *c, int k, double logT, double logpr) {
   return lfc * B2 / (B2 + A * A) * ln10;
 }
 
-void pgo_production(const pgo_case *c, double rho, const double *Y, double T, double *omega) {
+/* ln(1 + x) through elog: u = 1 + x rounded, then the first-order correction for that rounding,
+ * ((u - 1) - x) / u. One specified method on both sides, in place of the libraries' log1p, whose
+ * last bits differ between libm and the device */
+double pgo_dlog1p(double x) {
+  const double u = 1.0 + x;
+  return pgo_elog(u) - ((u - 1.0) - x) / u;
+}
+
+/* omega, and the destruction rates D unless D is null: one fold for both (finiterate.mako:185-236) */
+void pgo_sources(const pgo_case *c, double rho, const double *Y, double T, double *omega, double *D) {
   const int ns = c->ns;
   /* ratestate, finiterate.mako:136-182 */
   const double logT = pgo_tlog(T), Tinv = 1.0 / T;
@@ -789,10 +798,12 @@ void pgo_production(const pgo_case *c, double rho, const double *Y, double T, do
   /* fold, finiterate.mako:185-226 */
   const double expclamp = c->expclamp;
   for (int n = 0; n < ns; n++) omega[n] = 0.0;
+  if (D)
+    for (int n = 0; n < ns; n++) D[n] = 0.0;
   for (int k = 0; k < c->nr; k++) {
     const int kind = c->kind[k], thirdbody = kind >= 1, falloff = kind >= 2, troe = kind == 3;
     const int tb = c->tb[k];
-    double rp;
+    double rp, fwd, rev;
     double lk = c->logA[k];
     if (c->b[k] != 0.0) lk += c->b[k] * logT;
     if (c->EaR[k] != 0.0) lk -= c->EaR[k] * Tinv;
@@ -812,10 +823,10 @@ void pgo_production(const pgo_case *c, double rho, const double *Y, double T, do
       double K = pgo_texp(fmin(xk, expclamp));
       if (thirdbody && !falloff) K *= ctb[tb];
       if (falloff) K *= pfac;
-      double fwd = K;
+      fwd = K;
       for (int j = 0; j < c->nreact[k]; j++)
         for (int t = 0; t < (int)c->reactnu[4 * k + j]; t++) fwd *= cG[c->reactsp[4 * k + j]];
-      double rev = 0.0;
+      rev = 0.0;
       if (c->reversible[k]) {
         rev = K;
         const double nus = c->nusum[k];
@@ -833,7 +844,7 @@ void pgo_production(const pgo_case *c, double rho, const double *Y, double T, do
         if (c->prb[k] != 0.0) logpr += c->prb[k] * logT;
         if (c->prEaR[k] != 0.0) logpr -= c->prEaR[k] * Tinv;
         const double prexp = pgo_texp(-fabs(logpr));
-        lk += fmin(logpr, 0.0) - log1p(prexp);
+        lk += fmin(logpr, 0.0) - pgo_dlog1p(prexp);
       }
       if (troe) lk += troeblend(c, k, logT, logpr);
       double lrp = lk;
@@ -844,14 +855,26 @@ void pgo_production(const pgo_case *c, double rho, const double *Y, double T, do
           const int sp = c->netsp[4 * k + j];
           diff -= c->netnu[4 * k + j] * (gRT[sp] + logc[sp]);
         }
-        const double fwd = pgo_texp(fmin(lrp, expclamp)), rev = pgo_texp(fmin(lrp - diff, expclamp));
+        fwd = pgo_texp(fmin(lrp, expclamp));
+        rev = pgo_texp(fmin(lrp - diff, expclamp));
         rp = fwd - rev;
       } else {
-        rp = pgo_texp(fmin(lrp, expclamp));
+        fwd = pgo_texp(fmin(lrp, expclamp));
+        rev = 0.0;
+        rp = fwd;
       }
     }
     for (int j = 0; j < c->nnet[k]; j++) omega[c->netsp[4 * k + j]] += c->netnuMW[4 * k + j] * rp;
+    if (D)
+      for (int j = 0; j < c->nnet[k]; j++) {
+        const double nuMW = c->netnuMW[4 * k + j];
+        D[c->netsp[4 * k + j]] += (nuMW < 0.0 ? -nuMW : 0.0) * fwd + (nuMW > 0.0 ? nuMW : 0.0) * rev;
+      }
   }
+}
+
+void pgo_production(const pgo_case *c, double rho, const double *Y, double T, double *omega) {
+  pgo_sources(c, rho, Y, T, omega, NULL);
 }
 
 /* chemint/kernels/source.mako:3-10, cell.mako:5-20, massfrac solution.mako:25-35 */
@@ -1383,7 +1406,9 @@ static void prerhs(const pgo_case *c, pgo_block *b, pgo_arr Q, pgo_arr dQ, int w
   for (int a = 0; a < 3; a++)
     for (int side = 0; side < 2; side++)
       if (b->bc[2 * a + side]) pgo_wall_mirror(c, a, side, b->ncell, b->ng, b->nstate, Q, b->state, S[a]);
-  if (withdQ && b->chemistry) pgo_source(c, Q, b->state, dQ, interior(b));
+  if (withdQ && b->chemistry == 1) pgo_source(c, Q, b->state, dQ, interior(b));
+  if (withdQ && b->chemistry == 2)
+    pgo_substep(c, Q, b->state, dQ, 1, b->substeps, b->subcap, b->subeqtol, b->dt, interior(b));
   if (b->viscous) {
     pgo_transport(c, Q, b->state, b->trstate, b->hsp, interior(b));
     for (int a = 0; a < 3; a++)
@@ -1476,6 +1501,7 @@ void pgo_stage(const pgo_case *c, pgo_block *b, int n, double dt) {
   const int nst = b->integrator == 3 ? 4 : b->integrator + 1, last = nst - 1;
   pgo_arr Q = n == 0 ? b->regs[0] : b->regs[1], dQ = b->regs[b->nregs - 1];
   pgo_arr Qout = n == last ? b->regs[0] : b->regs[1];
+  b->dt = dt;
   prerhs(c, b, Q, dQ, 1);
   rhs(c, b, Q, dQ);
   pgo_advance(c, b, n, dt);
@@ -1525,4 +1551,102 @@ void pgo_cflmax(pgo_arr Q, pgo_arr state, pgo_arr dijk, pgo_arr iS, pgo_arr jS, 
         best[2] = fmax(best[2], acoustic + convective);
       }
   for (int n = 0; n < 3; n++) out[n] = best[n];
+}
+
+/* chemint/kernels/substep.mako:5-93 with bulk.mako's one-pass form (bulk-substeps = max-substeps):
+ * each cell of the box integrates its chemistry over dt in substeps as long as its limiting species
+ * allows, the equilibrium exit judged at the first substep when eqtol > 0, and writes the mean rate
+ * over the step (into dQ rows 5.. with rows 0..4 zero when begin_dQ, else into rows 0..ns-2 of out);
+ * substeps[box index, i fastest] receives each cell's count */
+void pgo_substep(const pgo_case *c, pgo_arr Q, pgo_arr state, pgo_arr out, int begin_dQ, int32_t *substeps, int cap,
+                 double eqtol, double dt, pgo_box r) {
+  const int ns = c->ns, nt = c->nt;
+  const double eps = DBL_EPSILON, tiny = DBL_MIN, stepfrac = 1e-12;
+  FOR_BOX(r) {
+    const double rho = *at(Q, i, j, k, 0), rhoinv = 1.0 / rho, step = dt, stepinv = 1.0 / step;
+    double Y[MAXNS], src[MAXNS], rate[MAXNS], D[MAXNS];
+    /* massfrac (solution.mako:25-35) */
+    double Ylast = 1.0, Ysum = 0.0;
+    for (int n = 0; n < ns - 1; n++) {
+      const double Yn = fmax(fmin(*at(Q, i, j, k, 5 + n) * rhoinv, 1.0), 0.0);
+      Y[n] = Yn;
+      Ylast -= Yn;
+      Ysum += Yn;
+    }
+    const double Yscale = 1.0 / fmax(Ysum, 1.0);
+    for (int n = 0; n < ns - 1; n++) Y[n] *= Yscale;
+    Y[ns - 1] = fmax(Ylast, 0.0);
+    for (int n = 0; n < ns; n++) src[n] = 0.0;
+    double T = *at(state, i, j, k, 1), remaining = 1.0;
+    const int exitq = eqtol > 0.0;
+    int sub = 0, early = 0;
+    for (; sub < cap && remaining > stepfrac; sub++) {
+      pgo_sources(c, rho, Y, T, rate, exitq && sub == 0 ? D : NULL);
+      double abest = 0.0, bbest = 0.0;
+      for (int n = 0; n < ns; n++) {
+        const double sign = copysign(1.0, rate[n]);
+        const double headroom = 0.5 * ((1.0 - sign) * Y[n] + (1.0 + sign) * (1.0 - Y[n]));
+        const double a = rho * fmax(headroom, eps), b = fmax(fabs(rate[n]), tiny);
+        if (n == 0 || a * bbest < abest * b) {
+          abest = a;
+          bbest = b;
+        }
+      }
+      const double share = fmin(remaining, abest / bbest * stepinv);
+      /* the substep reads ln T from the rate state's table log (finiterate.mako:143) */
+      const double u = pgo_tlog(T), Tinv = 1.0 / T;
+      double cv = 0.0, dTdt = 0.0, e[MAXNS];
+      for (int n = 0; n < ns; n++) {
+        double cpR = 0.0, hRT = 0.0;
+        for (int m = nt - 1; m >= 0; m--) cpR = cpR * u + c->cppoly[n * nt + m];
+        for (int m = nt - 1; m >= 0; m--) hRT = hRT * u + c->hpoly[n * nt + m];
+        hRT += c->h298[n] * Tinv;
+        e[n] = (hRT - 1.0) * T * c->Rsp[n];
+        cv += (cpR - 1.0) * c->Rsp[n] * Y[n];
+        dTdt -= e[n] * rate[n];
+      }
+      const double cvrhoinv = 1.0 / (cv * rho);
+      dTdt *= cvrhoinv;
+      if (exitq) {
+        const double tleft = remaining * step, tleftinv = 1.0 / tleft;
+        int settled = sub == 0 && tleft > 0.0 && fabs(dTdt) * tleft < 0.1;
+        for (int n = 0; n < ns && settled; n++) settled = fabs(rate[n] * rhoinv) * tleft < eqtol;
+        if (settled) {
+          double dT = 0.0;
+          for (int n = 0; n < ns; n++) {
+            const double rhoden = rho * fmax(Y[n], c->cfloor * c->MW[n] * rhoinv);
+            const double move = fmax((rate[n] * rhoinv) * tleft * rhoden / (rhoden + D[n] * tleft), -Y[n]);
+            const double rr = move * rho * tleftinv;
+            dT -= e[n] * rr;
+            Y[n] += move;
+            src[n] += rr * remaining;
+          }
+          T += dT * cvrhoinv * tleft;
+          remaining = 0.0;
+          early = 1;
+          break;
+        }
+      }
+      const double tsub = share * step;
+      for (int n = 0; n < ns; n++) {
+        Y[n] += (rate[n] * rhoinv) * tsub;
+        src[n] += rate[n] * share;
+      }
+      T += dTdt * tsub;
+      remaining -= share;
+      if (share <= stepfrac) {
+        remaining = 0.0;
+        early = 1;
+        break;
+      }
+    }
+    const int64_t code = (i - r.start[0]) + (int64_t)r.extent[0] * ((j - r.start[1]) + (int64_t)r.extent[1] * (k - r.start[2]));
+    substeps[code] = early ? sub + 1 : sub;
+    if (begin_dQ) {
+      for (int l = 0; l < 5; l++) *at(out, i, j, k, l) = 0.0;
+      for (int n = 0; n < ns - 1; n++) *at(out, i, j, k, 5 + n) = src[n];
+    } else {
+      for (int n = 0; n < ns - 1; n++) *at(out, i, j, k, n) = src[n];
+    }
+  }
 }

@@ -41,7 +41,7 @@ typedef struct {
   const int32_t *nfwd, *fwdsp, *nreact, *reactsp, *nprod, *prodsp, *nnet, *netsp, *ntbeff, *tbeffsp;
   const double *logA, *b, *EaR, *logprA, *prb, *prEaR, *nusum;
   const double *fwdorder, *reactnu, *prodnu, *netnu, *netnuMW, *lfcpoly, *tbdefeff, *tbeffdev;
-  double logprefRu, prefRu, Rupref, logcfloor, expclamp, troeuc, troeuhinv;
+  double logprefRu, prefRu, Rupref, logcfloor, expclamp, troeuc, troeuhinv, cfloor;
   /* transport: -1 none (Euler), 0 constant, 1 kinetic theory, 2 Chung (dense, on kinetic
    * theory); Wilke mixing, Lewis diffusion */
   int trans, nmu, nkappa;
@@ -65,6 +65,13 @@ void pgo_fromprims(const pgo_case *c, pgo_arr prims, pgo_arr Q, pgo_arr state, p
 void pgo_flux(const pgo_case *c, int axis, pgo_arr Q, pgo_arr state, pgo_arr S, pgo_arr F, pgo_box r);
 void pgo_applyflux(int ne, pgo_arr dQ, pgo_arr iF, pgo_arr jF, pgo_arr kF, pgo_arr detJinv, int append, pgo_box r);
 void pgo_production(const pgo_case *c, double rho, const double *Y, double T, double *omega);
+/* omega and the destruction rates D (kg/m^3/s) in one fold; D may be null */
+void pgo_sources(const pgo_case *c, double rho, const double *Y, double T, double *omega, double *D);
+double pgo_dlog1p(double x);
+/* sub-stepped chemistry, one pass per cell (chemint/kernels/substep.mako, bulk.mako): the mean
+ * rate over dt into dQ (begin_dQ) or the ns - 1 rows of out, each cell's count into substeps */
+void pgo_substep(const pgo_case *c, pgo_arr Q, pgo_arr state, pgo_arr out, int begin_dQ, int32_t *substeps, int cap,
+                 double eqtol, double dt, pgo_box r);
 void pgo_source(const pgo_case *c, pgo_arr Q, pgo_arr state, pgo_arr dQ, pgo_box r);
 void pgo_combine(int ne, pgo_arr out, int own, double wout, int nterms, const pgo_arr *X, const double *w,
                  double dt, int step, pgo_box r);
@@ -84,7 +91,13 @@ typedef struct {
   int nregs;
   pgo_arr state, iF, jF, kF, iS, jS, kS, detJinv;
   int integrator; /* 0 rk1, 1 rk2, 2 tvd-rk3, 3 rk4 */
-  int chemistry;  /* 1: the coupled source begins dQ */
+  int chemistry;  /* 1: the coupled source begins dQ; 2: sub-stepped chemistry does */
+  /* sub-stepped chemistry: the cap, the equilibrium tolerance, the counts per interior cell, and the
+   * step's dt (set by pgo_stage; pgo_rhs reads the last one) */
+  int subcap;
+  double subeqtol;
+  int32_t *substeps;
+  double dt;
   /* Navier-Stokes: the viscous arrays and the damping coefficient */
   int viscous;
   double alpha;

@@ -73,7 +73,7 @@ _DEFAULTS = {
         "tile-i": 64,
         "tile-j": 4,
         "fused-stage": True,
-        "k-chunk": 8,
+        "k-chunk": 1,
         "graphs": True,
     },
     "halo-exchange": {"kind": "hoststaged"},

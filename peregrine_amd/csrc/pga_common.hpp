@@ -122,7 +122,12 @@ struct View {
   double *__restrict__ d;
   int64_t sj, sk, sc;
   __device__ __forceinline__ int64_t off(int i, int j, int k) const { return i + j * sj + k * sk; }
-  __device__ __forceinline__ double &at(int64_t o, int c) const { return d[o + c * sc]; }
+  // the component's plane from the uniform base, then a 32-bit byte offset into it: a load takes
+  // the scalar base and one vector offset, with no 64-bit address arithmetic per lane (every entry
+  // point checks that a plane spans less than 4 GiB)
+  __device__ __forceinline__ double &at(int64_t o, int c) const {
+    return *reinterpret_cast<double *>(reinterpret_cast<char *>(d + c * sc) + (uint32_t)((uint64_t)o << 3));
+  }
 };
 
 __host__ inline View view(const pga_view &v) { return View{v.data, v.sj, v.sk, v.sc}; }
@@ -274,6 +279,15 @@ __device__ __forceinline__ double elog(double x) {
   const double finite = fabs(r1) < 0.03125 ? series : table;
   const double limit = x == 0.0 ? -__builtin_huge_val() : x > 0.0 ? x : __builtin_nan("");
   return x > 0.0 && x <= 1.7976931348623157e308 ? finite : limit;
+}
+
+// ln(1 + x) through elog: u = 1 + x rounded, then the first-order correction for that rounding,
+// ((u - 1) - x) / u; one specified method, so the device and the oracle give the same bits where the
+// libraries' log1p differ in the last
+__device__ __forceinline__ double dlog1p(double x) {
+#pragma clang fp contract(off)
+  const double u = 1.0 + x;
+  return elog(u) - ((u - 1.0) - x) / u;
 }
 
 // ---------------------------------------------------------------------------

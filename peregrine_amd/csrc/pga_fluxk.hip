@@ -1,16 +1,38 @@
 // The advective face flux of the per-kernel path: one launch per axis over the
 // faces of a box, one face per thread, lanes along the contiguous i index so the
 // stencil reads of every axis are coalesced.
+//
+// The i and j faces take a flat launch (rows of faces back to back, XCD-ordered
+// blocks): the rows a j-face stencil spans are shared with the blocks dispatched
+// beside it on the same XCD, so L2 serves the reuse. A k-face stencil spans four
+// planes, further apart than an XCD's blocks in flight reach: there a workgroup
+// takes a 64 (i) x 4 (k) tile of faces, its four waves four consecutive k-faces,
+// and the seven planes of cells they span are read once into L1 and L2 for all
+// four rather than four times.
 #include "pga_flux.hpp"
 #include "pga_internal.hpp"
 
 namespace pga {
 
+constexpr int KTILE = 4;
+
+__host__ inline dim3 ktile_grid(const Box3 &b) {
+  return dim3((unsigned)(((b.e0 + 63) / 64) * b.e1 * ((b.e2 + KTILE - 1) / KTILE)));
+}
+
 template <int NS, int EOS, int RECON, int PRIMARY, int LIM>
 __global__ __launch_bounds__(256) void k_flux(const double *__restrict__ K, int axis, View Q, View st, View S,
                                               View F, Box3 b) {
   int i, j, k;
-  if (!flat_cell_of(b, i, j, k)) return;
+  if (axis == 2) {
+    const unsigned t = xcd_block(), nbi = (b.e0 + 63) / 64, rest = t / nbi;
+    i = b.s0 + (int)(t % nbi) * 64 + (threadIdx.x & 63);
+    j = b.s1 + (int)(rest % b.e1);
+    k = b.s2 + (int)(rest / b.e1) * KTILE + (threadIdx.x >> 6);
+    if (i >= b.s0 + b.e0 || k >= b.s2 + b.e2) return;
+  } else if (!flat_cell_of(b, i, j, k)) {
+    return;
+  }
   const Gas<NS> g{K};
   const int64_t o = Q.off(i, j, k);
   const int64_t d = axis == 0 ? 1 : axis == 1 ? Q.sj : Q.sk;
@@ -32,8 +54,8 @@ static void launch_flux(const pga_case *c, int axis, pga_view Q, pga_view st, pg
 #define L(RECON, PRIM, LIM)                                                                                    \
   if (recon == RECON && prim == PRIM && (RECON == RECON_CONSTANT || lim == LIM))                              \
     flat_slabs(b, [&](const Box3 &sb) {                                                                        \
-      hipLaunchKernelGGL((k_flux<NS, EOS, RECON, PRIM, LIM>), flat_grid(sb), dim3(FLAT), 0, s, c->K, axis,      \
-                         view(Q), view(st), view(S), view(F), sb);                                              \
+      hipLaunchKernelGGL((k_flux<NS, EOS, RECON, PRIM, LIM>), axis == 2 ? ktile_grid(sb) : flat_grid(sb),       \
+                         dim3(FLAT), 0, s, c->K, axis, view(Q), view(st), view(S), view(F), sb);                \
     });
   L(RECON_CONSTANT, FLUX_KEPAEC, LIM_NONE)
   L(RECON_CONSTANT, FLUX_HLLC, LIM_NONE)

@@ -30,6 +30,9 @@ inline int32_t launched(const char *what) {
   return PGA_OK;
 }
 
+// the elements of one component plane a view may hold: 4 GiB of doubles
+constexpr int64_t PLANE_MAX = int64_t(1) << 29;
+
 inline bool box_ok(const pga_box &b) {
   for (int a = 0; a < 3; a++)
     if (b.start[a] < 0 || b.extent[a] < 0) return false;
@@ -53,8 +56,9 @@ inline bool covers(const pga_view &v, const pga_box &b, const int lo[3], const i
     if (b.extent[a] == 0) continue;
     if (b.start[a] - lo[a] < 0 || b.start[a] + b.extent[a] + hi[a] > v.n[a]) return false;
   }
-  // the strides must not let one plane overlap the next
-  return v.sj >= v.n[0] && v.sk >= v.sj * v.n[1] && v.sc >= v.sk * v.n[2];
+  // the strides must not let one plane overlap the next, and a plane is addressed by 32-bit byte
+  // offsets from its base (PLANE_MAX elements)
+  return v.sj >= v.n[0] && v.sk >= v.sj * v.n[1] && v.sc >= v.sk * v.n[2] && v.sk * v.n[2] <= PLANE_MAX;
 }
 
 inline bool covers(const pga_view &v, const pga_box &b, int ncomp) {

@@ -72,12 +72,29 @@ class EulerSimulator(BaseSimulator):
         if adv["recon"] == "muscl" and adv["primary"] == "kepaec":
             raise ConfigError("adv-flux", "kepaec is the scheme of the two cells as they are; it takes recon constant")
         self.recon, self.limiter, self.primary = adv["recon"], adv["limiter"], adv["primary"]
-        kind = cfg["chem-integration"]["kind"]
-        if kind not in (None, "coupled"):
-            raise ConfigError("chem-integration.kind", "this build has coupled chemistry")
+        ci = cfg["chem-integration"]
+        kind = ci["kind"]
+        if kind not in (None, "coupled", "sub-stepped"):
+            raise ConfigError("chem-integration.kind", "this build has coupled and sub-stepped chemistry")
         if kind and self.mixture.chemistry is None:
-            raise ConfigError("chem-integration.kind", "coupled chemistry needs mixture.chemistry finite-rate")
+            raise ConfigError("chem-integration.kind", f"{kind} needs mixture.chemistry")
         self.chemistry = bool(kind)
+        self.chemkind = kind
+        self.substep = None
+        if kind == "sub-stepped":
+            # chemint/substepped.py:22-33
+            cap, bulk, bins, tol = (ci[k] for k in ("max-substeps", "bulk-substeps", "stiff-bins",
+                                                    "equilibrium-tolerance"))
+            if not isinstance(cap, int) or isinstance(cap, bool) or cap < 1:
+                raise ConfigError("chem-integration.max-substeps", f"{cap!r} is not a count of one or more")
+            if not isinstance(bulk, int) or isinstance(bulk, bool) or not 1 <= bulk <= cap:
+                raise ConfigError("chem-integration.bulk-substeps", f"{bulk!r} is not a count from one to max-substeps")
+            if not isinstance(bins, int) or isinstance(bins, bool) or not 1 <= bins <= 16:
+                raise ConfigError("chem-integration.stiff-bins", f"{bins!r} is not a count from one to sixteen")
+            if not isinstance(tol, (int, float)) or isinstance(tol, bool) or not 0.0 <= tol < 1.0:
+                raise ConfigError("chem-integration.equilibrium-tolerance",
+                                  f"{tol!r} is not a mass fraction below one, zero for no exit")
+            self.substep = dict(cap=cap, bulk=bulk, bins=bins, eqtol=float(tol))
         self.iovars = IoVars(self)
 
     @property

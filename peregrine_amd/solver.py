@@ -1,7 +1,8 @@
 """The multiblock solver: blocks, their arrays, and the slots of an RK stage.
 
 Mirrors ``peregrine/multiblock/solver.py:23-391``. A stage is
-  prerhs   halo exchange of Q, the EOS over the halos, the chemistry source into dQ
+  prerhs   halo exchange of Q, the EOS over the halos, the chemistry's source into dQ (coupled or
+           sub-stepped)
   rhs      the advective flux through every face, its divergence into dQ
   advance  the RK combine into the stage's output register
   postrhs  the EOS over the interior of the output register
@@ -28,6 +29,7 @@ class Block:
         self.faces = {n: Interface(n) for n in range(1, 7)}
         self.interior_nodes = None
         self.dijk = None
+        self.substeps = None
 
     def intface(self, nface):
         return self.faces[nface]
@@ -140,7 +142,9 @@ class Solver:
         self.readied = {}
         self._cflwork = None
         self.tme = HostScalar(0.0)
-        self.dt = backend.scalar("dt", cfg["simulation"]["dt"])
+        # the step scalar is nothing until a step sets it (multiblock/solver.py:36, allocation.py:80-85):
+        # a right-hand side before any step judges no equilibrium exit over no time
+        self.dt = backend.scalar("dt", 0.0)
         self.ng = sim.ng
         self.blocks = []
         mesh.fill(self)
@@ -345,12 +349,35 @@ class Solver:
         self.readied[Q] = dQ
 
     def begin_dQ(self, Q, dQ):
-        """The derivative's first term, the coupled source, before the fluxes append to it."""
+        """The derivative's first term, the chemistry's source, before the fluxes append to it."""
         if self.sim.chemistry:
             for blk in self.blocks:
-                self._call("pga_source", self.case.handle, blk.regs[Q].view(), blk.state.view(), blk.regs[dQ].view(),
-                           blk.interior_box())
+                self.chemistry_source(blk, blk.regs[Q].view(), blk.regs[dQ].view(), True)
         self.readied[Q] = dQ
+
+    def chemistry_source(self, blk, Qv, out, begin_dQ):
+        """The chemistry's term of the derivative over the interior: the coupled source (the
+        production rates at the stage's state) or the sub-stepped mean rate over the step
+        (``chemint/coupled.py:10-13``, ``chemint/substepped.py:52-68``); into dQ, rows 0..4 zero,
+        when begin_dQ, else into the rates array the fused stage adds."""
+        box = blk.interior_box()
+        if self.sim.chemkind == "coupled":
+            if begin_dQ:
+                self._call("pga_source", self.case.handle, Qv, blk.state.view(), out, box)
+            else:
+                self._call("pga_rates", self.case.handle, Qv, blk.state.view(), out, box)
+            return
+        opts, torch = self.sim.substep, self.backend.torch
+        if blk.substeps is None:
+            o = native.SubstepOpts(opts["cap"], opts["bulk"], opts["bins"], 0, opts["eqtol"])
+            blk.subopts = o
+            blk.substeps = torch.zeros(blk.ncells, dtype=torch.int32, device=self.backend.device)
+            n = int(native.lib().pga_substep_listsize(ctypes.byref(o), box)) if opts["bulk"] < opts["cap"] else 0
+            blk.stifflist = torch.zeros(max(n, 1), dtype=torch.int32, device=self.backend.device)
+            blk.nlist = n
+        self._call("pga_substep", self.case.handle, ctypes.byref(blk.subopts), Qv, blk.state.view(), out,
+                   int(begin_dQ), blk.substeps.data_ptr(), blk.ncells, blk.stifflist.data_ptr(), blk.nlist,
+                   self.dt.ptr, box)
 
     def _ready(self, Q, dQ):
         """Halos of Q exchanged and their EOS taken once per solution: a register whose halos are
@@ -463,7 +490,7 @@ class Solver:
                 box = blk.interior_box()
                 Qin, st = blk.regs[regs.Q].view(), blk.state.view()
                 if self.sim.chemistry:
-                    self._call("pga_rates", self.case.handle, Qin, st, blk.rates.view(), box)
+                    self.chemistry_source(blk, Qin, blk.rates.view(), False)
                 rates = blk.rates.view() if self.sim.chemistry else native.View()
                 self.fluxes(blk, Qin, st)
                 self._call("pga_stage", self.case.handle, Qin, st, blk.regs[0].view(), blk.regs[regs.Qout].view(),

@@ -253,13 +253,19 @@ def test_halo_copies(dev):
 
 
 def _run_pair(gas, recon, primary, limiter, chem, integrator, ncell, steps, dt, prims, lengths, fused=True,
-              trans=None, species=None, nstate=8, walls=()):
+              trans=None, species=None, nstate=8, walls=(), substep=None):
     """The same case stepped by the solver on the device and by the oracle on the host; walls: the
-    faces (0 i-low .. 5 k-high) that are adiabatic slip walls, the rest periodic."""
+    faces (0 i-low .. 5 k-high) that are adiabatic slip walls, the rest periodic; substep: the
+    sub-stepped chemistry's (max-substeps, bulk-substeps, stiff-bins, equilibrium-tolerance), the
+    oracle taking the one-pass form."""
     import peregrine_amd as pg
 
     cfg = sp.config(gas, recon=recon, primary=primary, limiter=limiter, chem=chem, integrator=integrator, dt=dt,
                     trans=trans)
+    if substep is not None:
+        ci = cfg["chem-integration"]
+        ci["kind"] = "sub-stepped"
+        ci["max-substeps"], ci["bulk-substeps"], ci["stiff-bins"], ci["equilibrium-tolerance"] = substep
     if species is not None:
         cfg["mixture"]["species"] = species
     if walls:
@@ -283,6 +289,10 @@ def _run_pair(gas, recon, primary, limiter, chem, integrator, ncell, steps, dt, 
     # stage then takes as ready; the oracle's first stage exchanges and takes the halo EOS itself
     O.lib.pgo_fromQ(oc.ref, O.arr(hb.regs[0]), O.arr(hb.state), O.box((hb.ng,) * 3, hb.ncell))
     b = sp.oracle_block(hb, integrator, chem, viscous=sim.viscous, alpha=getattr(sim, "alpha", 1.0), walls=walls)
+    if substep is not None:
+        b.chemistry, b.subcap, b.subeqtol = 2, substep[0], substep[3]
+        hb.counts = np.zeros(int(np.prod(ncell)), dtype=np.int32)
+        b.substeps = hb.counts.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
     for _ in range(steps):
         mb.step(dt)
         O.lib.pgo_step(oc.ref, ctypes.byref(b), dt)
@@ -718,10 +728,10 @@ def test_cflmax_matches_to_the_bit(dev):
     got = dev.get(out)
     print("cflmax", got, want)
     assert np.array_equal(got, want)
-    # too small a work array is refused before any launch
+    # too small a work array is refused before any launch, as a size out of range
     rc = L.pga_cflmax(nc.handle, dev.view(dev.put(hb.regs[0])), dev.view(dev.put(hb.state)), dev.view(dev.put(d)),
                       *(dev.view(dev.put(s)) for s in hb.S), box, work.data_ptr(), n - 1, out.data_ptr(), None)
-    assert rc == -1
+    assert rc == -4 and b"work holds" in L.pga_last_error()
 
 
 def test_cfl_controlled_steps_match(dev):
@@ -757,3 +767,87 @@ def test_cfl_controlled_steps_match(dev):
     mb.run()
     assert mb.nrt == 3 and mb.tme.get() == sum(dts)
     assert np.array_equal(sp.to_dev(mb.blocks[0].Q.get())[hb.interior()], hb.regs[0][hb.interior()])
+
+
+def _hot_and_cold(sim, ncell, radicals=2e-3):
+    """Stoichiometric H2-air at 10 bar: the low-i half at 400 K, the high-i half at 2500 K with
+    radicals, so one step is stiff for half the cells and settled for the other."""
+    names = sim.mixture.spnames
+    q = np.zeros(ncell + (sim.ne,))
+    q[..., 0] = 10e5
+    q[..., 4] = 400.0
+    q[ncell[0] // 2:, ..., 4] = 2500.0
+    Y = np.zeros(sim.ns)
+    for name, v in sp.H2_AIR_Y.items():
+        Y[names.index(name)] = v
+    hot = Y.copy()
+    for name in ("H", "O", "OH"):
+        hot[names.index(name)] = radicals
+    q[..., 5:] = Y[:-1]
+    q[ncell[0] // 2:, ..., 5:] = hot[:-1]
+    return q
+
+
+@pytest.mark.parametrize("bulk,bins", [(200, 1), (1, 1), (2, 8)], ids=["one-pass", "bulk1-bin1", "bulk2-bins8"])
+@pytest.mark.parametrize("begin_dQ", [True, False], ids=["dQ", "rates"])
+def test_substep_passes_match_the_one_pass_oracle(dev, bulk, bins, begin_dQ):
+    """pga_substep through the C ABI over a stiff and a settled half, twice in a row (the second
+    call's bins keyed by the first's counts): the mean rates and the counts are the oracle's
+    one-pass form's to the bit, however the cells were listed."""
+    cfg, sim, nc, oc = _case("h2o2", chem=True)
+    ncell = (16, 6, 5)
+    q = _hot_and_cold(sim, ncell)
+    hb = sp.HostBlock(ncell, 0, sim.ne, 1)
+    full = O.box((0, 0, 0), ncell)
+    O.lib.pgo_fromprims(oc.ref, O.arr(sp.to_dev(q)), O.arr(hb.regs[0]), O.arr(hb.state), full)
+    cap, eqtol, dt = 200, 1e-7, 1e-5
+    rows = sim.ne if begin_dQ else sim.ns - 1
+    want = np.zeros((rows,) + hb.regs[0].shape[1:])
+    wcounts = np.zeros(int(np.prod(ncell)), dtype=np.int32)
+    O.lib.pgo_substep(oc.ref, O.arr(hb.regs[0]), O.arr(hb.state), O.arr(want), int(begin_dQ),
+                      wcounts.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), cap, eqtol, dt, full)
+    import torch
+
+    L = native.lib()
+    opts = native.SubstepOpts(cap, bulk, bins, 0, eqtol)
+    box = native.box((0, 0, 0), ncell)
+    nlist = L.pga_substep_listsize(ctypes.byref(opts), box)
+    assert nlist == 1 + bins + bins * int(np.prod(ncell))
+    dQv, dS = dev.put(hb.regs[0]), dev.put(hb.state)
+    out = dev.put(np.full_like(want, np.nan))
+    counts = torch.zeros(int(np.prod(ncell)), dtype=torch.int32, device="cuda")
+    lst = torch.zeros(nlist, dtype=torch.int32, device="cuda")
+    ddt = dev.put(np.array([dt]))
+    for call in range(2):
+        native.check(L.pga_substep(nc.handle, ctypes.byref(opts), dev.view(dQv), dev.view(dS), dev.view(out),
+                                   int(begin_dQ), counts.data_ptr(), counts.numel(), lst.data_ptr(), nlist,
+                                   ddt.data_ptr(), box, None), "pga_substep")
+        got, gcounts = dev.get(out), counts.cpu().numpy()
+        print(f"call {call}: counts {np.bincount(gcounts)[:3]}... max {gcounts.max()}, listed {int(lst[0])}")
+        assert np.array_equal(gcounts, wcounts)
+        assert np.array_equal(got, want)
+    # the hot half ran to many substeps, the cold half one; a compacted call listed the hot half
+    per = wcounts.reshape(ncell[::-1])
+    assert (per[..., : ncell[0] // 2] == 1).all() and (per[..., ncell[0] // 2:] > 2).all()
+    if bulk < cap:
+        assert int(lst[0]) == int(np.prod(ncell)) // 2
+
+
+@pytest.mark.parametrize("fused", [True, False], ids=["fused", "per-kernel"])
+def test_substepped_reacting_steps_match(dev, fused):
+    """The reacting case with sub-stepped chemistry (compacted: bulk 2 of 50 substeps, 8 bins, the
+    equilibrium exit on), 4 steps of tvd-rk3 by the solver against the oracle's one-pass form."""
+    n = 12
+    L = (0.01,) * 3
+    mb, hb = _run_pair("h2o2", "muscl", "hllc", "koren", True, "tvd-rk3", (20, n, n), 4, 2e-7,
+                       lambda x: sp.reacting_flow(x, n, L), (0.01 * 20 / 12, 0.01, 0.01), fused=fused,
+                       substep=(50, 2, 8, 1e-7))
+    assert mb.fused == fused
+    g = hb.ng
+    got = mb.blocks[0].Q.get()[g:-g, g:-g, g:-g].transpose(3, 2, 1, 0)
+    want = hb.regs[0][hb.interior()]
+    err = _rowerr(got, want)
+    counts = mb.blocks[0].substeps.cpu().numpy()
+    print(f"sub-stepped 4 steps ({'fused' if fused else 'per-kernel'}): {err:.3e}, counts up to {counts.max()}")
+    assert np.array_equal(counts, hb.counts)
+    assert err < 1e-10
