@@ -144,8 +144,9 @@ int32_t pga_copy_box(pga_view dst, const int32_t *dst_start, pga_view src, const
 int32_t pga_stage_supported(const pga_case *c);
 
 /* One RK stage of a block's interior in one pass after its three flux sweeps: the divergence of
- * the six face fluxes (iF, jF, kF from pga_flux on Qin), the coupled source (a reacting case's
- * rates from pga_rates, ns - 1 components per cell), the RK combine and the EOS of the result,
+ * the six face fluxes (iF, jF, kF from pga_flux on Qin), the chemistry's source (a reacting
+ * case's rates, ns - 1 components per cell, from pga_rates or pga_substep with begin_dQ = 0),
+ * the RK combine and the EOS of the result,
  * its Newton started from stin's T. Q0 is the step's start. The combine is
  * out = sum_t w[t] x_t over nterms terms in order, x_t = Qin (src 0), Q0 (1) or the stage's
  * dQ (2), a term with withdt[t] taking w[t] * (*dt) * x_t; dQ is never stored. Each cell reads

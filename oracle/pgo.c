@@ -503,8 +503,11 @@ static void load(pgo_arr st, int i, int j, int k, eosstate *s) {
   s->c = *at(st, i, j, k, 7);
 }
 
+/* threads only where there is work for them: a region over a few cells costs more in the fork
+ * and its barrier than it saves, and under a loaded host the barrier's spin is slower still */
+#define PGO_PAR_CELLS 16384
 #define FOR_BOX(r)                                                     \
-  _Pragma("omp parallel for collapse(2) schedule(static)")             \
+  _Pragma("omp parallel for collapse(2) schedule(static) if ((int64_t)(r).extent[0] * (r).extent[1] * (r).extent[2] > PGO_PAR_CELLS)") \
   for (int k = (r).start[2]; k < (r).start[2] + (r).extent[2]; k++)    \
     for (int j = (r).start[1]; j < (r).start[1] + (r).extent[1]; j++)  \
       for (int i = (r).start[0]; i < (r).start[0] + (r).extent[0]; i++)
@@ -931,7 +934,7 @@ void pgo_fill_periodic(pgo_arr A, int ncomp, const int *ncell, int ng, int depth
   }
   lo[axis] = 0;
   hi[axis] = depth;
-#pragma omp parallel for collapse(2) schedule(static)
+#pragma omp parallel for collapse(2) schedule(static) if ((int64_t)ncomp * (hi[0] - lo[0]) * (hi[1] - lo[1]) * (hi[2] - lo[2]) > PGO_PAR_CELLS)
   for (int l = 0; l < ncomp; l++)
     for (int k = lo[2]; k < hi[2]; k++)
       for (int j = lo[1]; j < hi[1]; j++)
@@ -1292,7 +1295,7 @@ void pgo_wall_mirror(const pgo_case *c, int axis, int side, const int *ncell, in
   const int t1 = (axis + 1) % 3, t2 = (axis + 2) % 3;
   for (int h = 0; h < ng; h++) {
     const int halo = side ? ng + n + h : ng - 1 - h, mirror = side ? ng + n - 1 - h : ng + h, face = side ? ng + n : ng;
-#pragma omp parallel for collapse(2)
+#pragma omp parallel for collapse(2) if ((int64_t)ncell[t1] * ncell[t2] > PGO_PAR_CELLS)
     for (int b2 = ng; b2 < ng + ncell[t2]; b2++)
       for (int b1 = ng; b1 < ng + ncell[t1]; b1++) {
         int ih[3], im[3], iface[3];
@@ -1326,7 +1329,7 @@ void pgo_wall_flux(const pgo_case *c, int axis, int side, const int *ncell, int 
   const int t1 = (axis + 1) % 3, t2 = (axis + 2) % 3;
   const int face = side ? ng + n : ng, cell = side ? ng + n - 1 : ng;
   const double outward = side ? 1.0 : -1.0;
-#pragma omp parallel for collapse(2)
+#pragma omp parallel for collapse(2) if ((int64_t)ncell[t1] * ncell[t2] > PGO_PAR_CELLS)
   for (int b2 = ng; b2 < ng + ncell[t2]; b2++)
     for (int b1 = ng; b1 < ng + ncell[t1]; b1++) {
       int f[3], ic[3];

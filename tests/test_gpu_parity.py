@@ -839,9 +839,19 @@ def test_substepped_reacting_steps_match(dev, fused):
     equilibrium exit on), 4 steps of tvd-rk3 by the solver against the oracle's one-pass form."""
     n = 12
     L = (0.01,) * 3
-    mb, hb = _run_pair("h2o2", "muscl", "hllc", "koren", True, "tvd-rk3", (20, n, n), 4, 2e-7,
-                       lambda x: sp.reacting_flow(x, n, L), (0.01 * 20 / 12, 0.01, 0.01), fused=fused,
-                       substep=(50, 2, 8, 1e-7))
+
+    def stiff(x):
+        # the reacting flow with its high-x half 1000 K hotter and seeded with radicals, so those
+        # cells run past the bulk pass's two substeps and are listed
+        q = sp.reacting_flow(x, n, L)
+        hot = x[..., 0] > 0.5 * x[..., 0].max()
+        q[hot, 4] += 1000.0
+        for sp_idx in (1, 2, 4):  # H, O, OH
+            q[hot, 5 + sp_idx] = 2e-3
+        return q
+
+    mb, hb = _run_pair("h2o2", "muscl", "hllc", "koren", True, "tvd-rk3", (20, n, n), 4, 2e-7, stiff,
+                       (0.01 * 20 / 12, 0.01, 0.01), fused=fused, substep=(50, 2, 8, 1e-7))
     assert mb.fused == fused
     g = hb.ng
     got = mb.blocks[0].Q.get()[g:-g, g:-g, g:-g].transpose(3, 2, 1, 0)
@@ -850,4 +860,5 @@ def test_substepped_reacting_steps_match(dev, fused):
     counts = mb.blocks[0].substeps.cpu().numpy()
     print(f"sub-stepped 4 steps ({'fused' if fused else 'per-kernel'}): {err:.3e}, counts up to {counts.max()}")
     assert np.array_equal(counts, hb.counts)
+    assert counts.max() > 2
     assert err < 1e-10
