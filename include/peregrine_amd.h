@@ -60,8 +60,9 @@ typedef struct {
   int32_t limiter;   /* 0 none, 1 koren, 2 minmod, 3 mc, 4 superbee, 5 vanleer */
   int32_t primary;   /* 0 kepaec, 1 hllc, 2 rusanov */
   int32_t mechanism; /* -1 none, else the index of a compiled mechanism (pga_mechanism_find) */
-  int32_t trans;     /* -1 none (Euler), 0 constant, 1 kinetic theory; Wilke mixing, Lewis diffusion */
-  int32_t reserved[9];
+  int32_t trans;     /* -1 none (Euler), 0 constant, 1 kinetic theory, 2 Chung; Wilke mixing */
+  int32_t diffusion; /* 0 Lewis numbers, 1 binary (mixture-averaged pair coefficients) */
+  int32_t reserved[8];
 } pga_case_desc;
 
 typedef struct pga_case pga_case;

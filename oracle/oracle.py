@@ -53,6 +53,8 @@ class Case(C.Structure):
         ("cubicsqrtacMW", _dp), ("cubicbMW", _dp), ("vshift", _dp), ("vshiftMW", _dp), ("cubicdaMW", _dp),
         ("acentric", _dp),
         ("cppoly", _dp), ("chungsigma", _dp), ("chungepsk", _dp), ("chungvisc", _dp), ("chungcond", _dp),
+        ("diffusion", C.c_int), ("npair", C.c_int), ("diffwidth", C.c_int), ("diffpartial", C.c_int),
+        ("pairfit", _dp), ("selffit", _dp), ("bathof", _dp),
     ]
 
 
@@ -180,7 +182,13 @@ class OracleCase:
             for name in ("chungsigma", "chungepsk", "chungvisc", "chungcond"):
                 setattr(c, name, d(k[name]))
         if mixture.trans is not None:
-            c.lewis, c.MWqinv, c.wilkepair = d(k["lewis"]), d(k["MWqinv"]), d(k["wilkepair"])
+            c.MWqinv, c.wilkepair = d(k["MWqinv"]), d(k["wilkepair"])
+            if mixture.diffusion.name == "binary":
+                c.diffusion, c.npair = 1, k["pairfit"].shape[-1]
+                c.diffwidth, c.diffpartial = int(k["diffmeta"][0]), int(k["diffmeta"][1])
+                c.pairfit, c.selffit, c.bathof = d(k["pairfit"]), d(k["selffit"]), d(k["bathof"])
+            else:
+                c.lewis = d(k["lewis"])
             c.mu0, c.kappa0 = d(k.get("mu0", np.zeros(ns))), d(k.get("kappa0", np.zeros(ns)))
             mup, kap = k.get("mupoly", np.zeros((ns, 1))), k.get("kappapoly", np.zeros((ns, 1)))
             c.nmu, c.nkappa = mup.shape[1], kap.shape[1]

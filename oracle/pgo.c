@@ -1087,8 +1087,54 @@ static void hpartial(const pgo_case *c, const double *Yall, double p, double T, 
   }
 }
 
+/* diffusion/kernels/binary.mako:11-62: each species' mixture-averaged coefficient over the bath,
+ * D_n = rest / (p (rest sum1 + X_n sum2)) with rest = sum X_m MW_m, sum1 = sum X_m / D_nm and
+ * sum2 = sum X_m MW_m / D_nm over the partners (a partial bath scales the first sum from the
+ * bath's fraction to the whole); a species alone takes its self-diffusion coefficient */
+static void binary_diffusion(const pgo_case *c, const double *X, double p, double T, double lnT, double *D) {
+  const int ns = c->ns, np = c->npair, width = c->diffwidth, partial = c->diffpartial;
+  const double invT32 = 1.0 / (T * sqrt(T)), tiny = 64.0 * DBL_EPSILON;
+  for (int n = 0; n < ns; n++) {
+    double sum1 = 0.0, sum2 = 0.0, rest = 0.0, bathx = 0.0;
+    for (int k = 0; k < width; k++) {
+      int m;
+      double Xm;
+      if (partial) {
+        m = (int)c->bathof[n * ns + k];
+        Xm = m != n ? fmax(X[m], 0.0) : 0.0;
+      } else {
+        m = k + (k >= n);
+        Xm = fmax(X[m], 0.0);
+      }
+      double Dinv = 0.0;
+      for (int t = np - 1; t >= 0; t--) Dinv = Dinv * lnT + c->pairfit[(n * ns + k) * np + t];
+      Dinv *= invT32;
+      sum1 += Xm * Dinv;
+      sum2 += Xm * c->MW[m] * Dinv;
+      rest += Xm * c->MW[m];
+      bathx += Xm;
+    }
+    double Dinvself = 0.0;
+    for (int t = np - 1; t >= 0; t--) Dinvself = Dinvself * lnT + c->selffit[n * np + t];
+    Dinvself *= invT32;
+    const double Xn = fmax(X[n], 0.0);
+    int alone;
+    double num, den;
+    if (partial) {
+      alone = 1.0 - Xn <= tiny || bathx <= tiny;
+      num = alone ? 1.0 : rest * bathx;
+      den = alone ? p * Dinvself : p * ((1.0 - Xn) * rest * sum1 + Xn * bathx * sum2);
+    } else {
+      alone = 1.0 - Xn <= tiny;
+      num = alone ? 1.0 : rest;
+      den = alone ? p * Dinvself : p * (rest * sum1 + Xn * sum2);
+    }
+    D[n] = (alone || (rest > 0.0 && den > 0.0)) ? num / den : 0.0;
+  }
+}
+
 /* navstokes/kernels/transport.mako:5-35 with transport/kernels/{constant,kinetic,dilute}.mako,
- * diffusion/kernels/lewis.mako:3-7 and the ideal hspecies (eos/kernels/species.mako:29-36).
+ * diffusion/kernels/{lewis,binary}.mako and the ideal hspecies (eos/kernels/species.mako:29-36).
  * ln T is taken by the EOS's log (pgo_elog), as everywhere in this path. */
 void pgo_transport(const pgo_case *c, pgo_arr Q, pgo_arr state, pgo_arr trstate, pgo_arr hsp, pgo_box r) {
   const int ns = c->ns;
@@ -1130,8 +1176,14 @@ void pgo_transport(const pgo_case *c, pgo_arr Q, pgo_arr state, pgo_arr trstate,
     if (c->trans == 2) chung(c, X, T, lnT, rho, &mu, &kappa);
     *at(trstate, i, j, k, 0) = mu;
     *at(trstate, i, j, k, 1) = kappa;
-    const double kappaoverrhocp = kappa * rhoinv / *at(state, i, j, k, 5);
-    for (int n = 0; n < ns; n++) *at(trstate, i, j, k, 2 + n) = kappaoverrhocp / c->lewis[n];
+    if (c->diffusion == 1) {
+      double D[MAXNS];
+      binary_diffusion(c, X, *at(state, i, j, k, 0), T, lnT, D);
+      for (int n = 0; n < ns; n++) *at(trstate, i, j, k, 2 + n) = D[n];
+    } else {
+      const double kappaoverrhocp = kappa * rhoinv / *at(state, i, j, k, 5);
+      for (int n = 0; n < ns; n++) *at(trstate, i, j, k, 2 + n) = kappaoverrhocp / c->lewis[n];
+    }
     if (c->eos == 2) {
       double h[MAXNS];
       hpartial(c, Y, *at(state, i, j, k, 0), T, *at(state, i, j, k, 8), h);

@@ -53,6 +53,11 @@ typedef struct {
       *vshiftMW, *cubicdaMW, *acentric;
   /* Chung: cp/R fits, sigma (angstrom), epsilon/k (K), the E (10 x 4) and B (7 x 4) tables */
   const double *cppoly, *chungsigma, *chungepsk, *chungvisc, *chungcond;
+  /* binary diffusion (diffusion 1; 0 is Lewis numbers): per species n and partner slot k < width,
+   * pairfit[(n ns + k) npair + t] (T^3/2 / D at unit pressure in ln T) and bathof[n ns + k] the
+   * partner, selffit[n npair + t] its own pair; partial: a bath short of every species */
+  int diffusion, npair, diffwidth, diffpartial;
+  const double *pairfit, *selffit, *bathof;
 } pgo_case;
 
 double pgo_tlog(double x);

@@ -55,7 +55,8 @@ extern "C" int32_t pga_const_offset(int32_t ns, const char *name, int32_t *count
       {"cubicsqrtacMW", o.cubicsqrtacMW, ns}, {"cubicbMW", o.cubicbMW, ns}, {"vshift", o.vshift, ns},
       {"vshiftMW", o.vshiftMW, ns}, {"cubicdaMW", o.cubicdaMW, ns}, {"acentric", o.acentric, ns},
       {"chungsigma", o.chungsigma, ns}, {"chungepsk", o.chungepsk, ns}, {"chungvisc", o.chungvisc, 40},
-      {"chungcond", o.chungcond, 28},
+      {"chungcond", o.chungcond, 28}, {"pairfit", o.pairfit, ns * ns * NT}, {"selffit", o.selffit, ns * NT},
+      {"bathof", o.bathof, ns * ns}, {"diffmeta", o.diffmeta, 2},
   };
   for (auto &r : rows)
     if (!strcmp(r.n, name)) {
@@ -103,6 +104,10 @@ extern "C" int32_t pga_case_create(const pga_case_desc *desc, const double *dev_
   if (desc->trans < -1 || desc->trans > 2 || (desc->trans == 2 && desc->eos != 2)) {
     set_error("pga_case_create: no transport model %d (-1 none, 0 constant, 1 kinetic theory, 2 Chung on a real gas)",
               desc->trans);
+    return PGA_EUNSUP;
+  }
+  if (desc->diffusion < 0 || desc->diffusion > 1) {
+    set_error("pga_case_create: no diffusion model %d (0 Lewis numbers, 1 binary)", desc->diffusion);
     return PGA_EUNSUP;
   }
   if (desc->mechanism >= 0) {

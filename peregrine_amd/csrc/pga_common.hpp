@@ -27,7 +27,8 @@ constexpr double Ru = 8314.46261815324;
 struct Offsets {
   int MW, MWinv, Rsp, cppolyR, h298R, cp0, gpoly, h298, mu0, kappa0, lewis, hpoly, MWqinv, wilkepair, mupoly,
       kappapoly, cppoly, cubic, cubicsqrtac, cubicb, cubicda, cubicmTc, cubiconepm, cubicsqrtacMW, cubicbMW, vshift,
-      vshiftMW, cubicdaMW, acentric, chungsigma, chungepsk, chungvisc, chungcond, size;
+      vshiftMW, cubicdaMW, acentric, chungsigma, chungepsk, chungvisc, chungcond, pairfit, selffit, bathof, diffmeta,
+      size;
 };
 
 __host__ __device__ constexpr Offsets offsets(int ns) {
@@ -67,7 +68,13 @@ __host__ __device__ constexpr Offsets offsets(int ns) {
   o.chungepsk = o.chungsigma + ns;
   o.chungvisc = o.chungepsk + ns;
   o.chungcond = o.chungvisc + 40;
-  o.size = o.chungcond + 28;
+  // binary diffusion: each species' partner fits (ns x ns rows of NT, the first `width` used), its
+  // own pair's fit, the partners' indices, and (width, partial bath)
+  o.pairfit = o.chungcond + 28;
+  o.selffit = o.pairfit + ns * ns * NT;
+  o.bathof = o.selffit + ns * NT;
+  o.diffmeta = o.bathof + ns * ns;
+  o.size = o.diffmeta + 2;
   return o;
 }
 
@@ -113,6 +120,10 @@ struct Gas {
   __device__ __forceinline__ double chungepsk(int n) const { return K[O.chungepsk + n]; }
   __device__ __forceinline__ double chungvisc(int i, int j) const { return K[O.chungvisc + 4 * i + j]; }
   __device__ __forceinline__ double chungcond(int i, int j) const { return K[O.chungcond + 4 * i + j]; }
+  __device__ __forceinline__ double pairfit(int n, int k, int t) const { return K[O.pairfit + (n * NS + k) * NT + t]; }
+  __device__ __forceinline__ double selffit(int n, int t) const { return K[O.selffit + n * NT + t]; }
+  __device__ __forceinline__ double bathof(int n, int k) const { return K[O.bathof + n * NS + k]; }
+  __device__ __forceinline__ double diffmeta(int i) const { return K[O.diffmeta + i]; }
 };
 
 // ---------------------------------------------------------------------------

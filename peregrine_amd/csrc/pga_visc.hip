@@ -123,14 +123,86 @@ __device__ __forceinline__ void hpartial(const Gas<NS> &g, const double (&Y)[NS]
   }
 }
 
+// X[m] for a partner index m the table holds (uniform across lanes): a select over the species,
+// which keeps the mole fractions in registers where a dynamic index would move them to memory
+template <int NS>
+__device__ __forceinline__ double pick(const double (&X)[NS], int m) {
+  double r = X[0];
+#pragma unroll
+  for (int i = 1; i < NS; i++) r = m == i ? X[i] : r;
+  return r;
+}
+
+// diffusion/kernels/binary.mako:11-62: each species' mixture-averaged coefficient from its binary
+// pairs with the bath, fits of T^3/2 / D_nm at unit pressure divided by p; a species alone (its
+// mole fraction within 64 eps of one, or none of a partial bath's others present) takes its
+// self-diffusion coefficient
+template <int NS>
+__device__ __forceinline__ void binary_diffusion(const Gas<NS> &g, const double (&X)[NS], double p, double T,
+                                                 double lnT, double (&D)[NS]) {
+#pragma clang fp contract(off)
+  const double invT32 = 1.0 / (T * sqrt(T));
+  const int width = (int)g.diffmeta(0);
+  const bool partial = g.diffmeta(1) != 0.0;
+  const double tiny = 64.0 * 2.220446049250313e-16;
+#pragma unroll
+  for (int n = 0; n < NS; n++) {
+    double sum1 = 0.0, sum2 = 0.0, rest = 0.0, bathx = 0.0;
+#pragma unroll
+    for (int k = 0; k < NS; k++) {
+      if (k < width) {
+        // the fraction clipped at zero: a negative trace would let rest cancel while sum1 stays
+        // tiny and positive
+        int m;
+        double Xm;
+        if (partial) {
+          m = (int)g.bathof(n, k);
+          Xm = m != n ? fmax(pick(X, m), 0.0) : 0.0;
+        } else {
+          m = k + (k >= n);
+          Xm = fmax(X[m < NS ? m : NS - 1], 0.0);
+        }
+        double Dinv = 0.0;
+#pragma unroll
+        for (int t = NT - 1; t >= 0; t--) Dinv = Dinv * lnT + g.pairfit(n, k, t);
+        Dinv *= invT32;
+        const double MWm = g.MW(m);
+        sum1 += Xm * Dinv;
+        sum2 += Xm * MWm * Dinv;
+        rest += Xm * MWm;
+        bathx += Xm;
+      }
+    }
+    double Dinvself = 0.0;
+#pragma unroll
+    for (int t = NT - 1; t >= 0; t--) Dinvself = Dinvself * lnT + g.selffit(n, t);
+    Dinvself *= invT32;
+    const double Xn = fmax(X[n], 0.0);
+    bool alone;
+    double num, den;
+    if (partial) {
+      alone = 1.0 - Xn <= tiny || bathx <= tiny;
+      num = alone ? 1.0 : rest * bathx;
+      den = alone ? p * Dinvself : p * ((1.0 - Xn) * rest * sum1 + Xn * bathx * sum2);
+    } else {
+      alone = 1.0 - Xn <= tiny;
+      num = alone ? 1.0 : rest;
+      den = alone ? p * Dinvself : p * (rest * sum1 + Xn * sum2);
+    }
+    D[n] = (alone || (rest > 0.0 && den > 0.0)) ? num / den : 0.0;
+  }
+}
+
 // navstokes/kernels/transport.mako:5-35: mu and kappa by Wilke's rule and the conductivity mean
 // (mixrule/kernels/wilke.mako:3-17, transport/kernels/mixing.mako:3-10) from constant species
 // values (transport/kernels/constant.mako:5-16) or the kinetic-theory fits
-// (transport/kernels/dilute.mako:5-26); D_n by the Lewis numbers (diffusion/kernels/lewis.mako:3-7);
+// (transport/kernels/dilute.mako:5-26); D_n by the Lewis numbers (diffusion/kernels/lewis.mako:3-7)
+// or the binary pairs;
 // h_n ideal (eos/kernels/species.mako:29-36)
 template <int NS, int TRANS, int EOS>
 __device__ __forceinline__ void transport_cell(const Gas<NS> &g, const View &Q, const View &st, int64_t o,
-                                               const View &tr, const View &hsp, int64_t ot, int64_t oh) {
+                                               const View &tr, const View &hsp, int64_t ot, int64_t oh,
+                                               bool binary) {
 #pragma clang fp contract(off)
   const double rho = Q.at(o, 0), rhoinv = 1.0 / rho;
   double Y[NS];
@@ -196,9 +268,16 @@ __device__ __forceinline__ void transport_cell(const Gas<NS> &g, const View &Q, 
   if constexpr (TRANS == TRANS_CHUNG) chung(g, X, T, lnT, rho, mu, kappa);
   tr.at(ot, 0) = mu;
   tr.at(ot, 1) = kappa;
-  const double kappaoverrhocp = kappa * rhoinv / cp;
+  if (binary) {
+    double D[NS];
+    binary_diffusion(g, X, st.at(o, 0), T, lnT, D);
 #pragma unroll
-  for (int n = 0; n < NS; n++) tr.at(ot, 2 + n) = kappaoverrhocp / g.lewis(n);
+    for (int n = 0; n < NS; n++) tr.at(ot, 2 + n) = D[n];
+  } else {
+    const double kappaoverrhocp = kappa * rhoinv / cp;
+#pragma unroll
+    for (int n = 0; n < NS; n++) tr.at(ot, 2 + n) = kappaoverrhocp / g.lewis(n);
+  }
   if constexpr (EOS == EOS_REAL) {
     hpartial(g, Y, st.at(o, 0), T, st.at(o, 8), hsp, oh);
   } else {
@@ -215,11 +294,12 @@ __device__ __forceinline__ void transport_cell(const Gas<NS> &g, const View &Q, 
 }
 
 template <int NS, int TRANS, int EOS>
-__global__ __launch_bounds__(256) void k_transport(const double *K, View Q, View st, View tr, View hsp, Box3 b) {
+__global__ __launch_bounds__(256) void k_transport(const double *K, View Q, View st, View tr, View hsp, Box3 b,
+                                                   int binary) {
   int i, j, k;
   if (!cell_of(b, i, j, k)) return;
   const Gas<NS> g{K};
-  transport_cell<NS, TRANS, EOS>(g, Q, st, Q.off(i, j, k), tr, hsp, tr.off(i, j, k), hsp.off(i, j, k));
+  transport_cell<NS, TRANS, EOS>(g, Q, st, Q.off(i, j, k), tr, hsp, tr.off(i, j, k), hsp.off(i, j, k), binary != 0);
 }
 
 // navstokes/kernels/dqdxyz.mako:16-41
@@ -386,13 +466,13 @@ extern "C" int32_t pga_transport(const pga_case *c, pga_view Q, pga_view state, 
     const bool done = dispatch_eos(d, [&]<int N, int E>() {
       if (d.trans == TRANS_CONSTANT)
         hipLaunchKernelGGL((k_transport<N, TRANS_CONSTANT, E>), grid_of(b), dim3(TI, TJ), 0, stream_of(stream), c->K,
-                           view(Q), view(state), view(trstate), view(hsp), b);
+                           view(Q), view(state), view(trstate), view(hsp), b, d.diffusion);
       else if (d.trans == TRANS_KINETIC)
         hipLaunchKernelGGL((k_transport<N, TRANS_KINETIC, E>), grid_of(b), dim3(TI, TJ), 0, stream_of(stream), c->K,
-                           view(Q), view(state), view(trstate), view(hsp), b);
+                           view(Q), view(state), view(trstate), view(hsp), b, d.diffusion);
       else if constexpr (E == EOS_REAL)
         hipLaunchKernelGGL((k_transport<N, TRANS_CHUNG, E>), grid_of(b), dim3(TI, TJ), 0, stream_of(stream), c->K,
-                           view(Q), view(state), view(trstate), view(hsp), b);
+                           view(Q), view(state), view(trstate), view(hsp), b, d.diffusion);
     });
     if (!done) {
       set_error("pga_transport: no kernel for %d species with eos %d", d.ns, d.eos);

@@ -26,6 +26,10 @@ class Mixture:
         self.eos = EOS_MODELS[cfgsect["eos"]](cfgsect)
         self.trans = self.diffusion = self.mixrule = None
         if cfgsect["trans"]:
+            for key, models in (("trans", TRANSPORT_MODELS), ("diffusion", DIFFUSION_MODELS),
+                                ("mixing-rule", MIXRULE_MODELS)):
+                if cfgsect[key] not in models:
+                    raise ValueError(f"mixture.{key} {cfgsect[key]!r}: this build has {sorted(models)}")
             self.trans = TRANSPORT_MODELS[cfgsect["trans"]](cfgsect)
             self.diffusion = DIFFUSION_MODELS[cfgsect["diffusion"]](cfgsect)
             self.mixrule = MIXRULE_MODELS[cfgsect["mixing-rule"]](cfgsect)

@@ -159,6 +159,105 @@ class ChungDenseGas(KineticTheory):
         )
 
 
+def pair_parameters(species):
+    """Every species pair's Lennard-Jones parameters by the usual combining rules: the reduced mass,
+    the mean diameter, the geometric-mean well, the reduced dipole, and for a polar/non-polar pair
+    the induction correction xi that deepens the well by xi^2 and shrinks the diameter by
+    xi^-1/6 (Hirschfelder, Curtiss and Bird; as ``transport/kinetic.py:30-64`` combines them)."""
+    from peregrine_amd.mixture.constants import avogadro, epsilon0
+
+    def col(key):
+        return np.array([sp[key] for sp in species.values()], float)
+
+    MW, well, diam, dipole, polarize = col("MW"), col("well"), col("diam"), col("dipole"), col("polarize")
+    mass = MW / avogadro
+    rmass = np.outer(mass, mass) / np.add.outer(mass, mass)
+    rdiam = 0.5 * np.add.outer(diam, diam)
+    rwell = np.sqrt(np.outer(well, well))
+    rdipole = np.sqrt(np.outer(dipole, dipole))
+    rdelta = 0.5 * rdipole**2 / (4.0 * np.pi * epsilon0 * rwell * rdiam**3)
+    polar = dipole > 0.0
+    mixed = polar[:, None] != polar[None, :]
+    n = np.arange(len(MW))
+    ip = np.where(polar[:, None], n[:, None], n[None, :])
+    inp = np.where(polar[:, None], n[None, :], n[:, None])
+    alphastar = polarize[inp] / diam[inp] ** 3
+    mustar = dipole[ip] / np.sqrt(4.0 * np.pi * epsilon0 * diam[ip] ** 3 * well[ip])
+    xi = 1.0 + 0.25 * alphastar * mustar**2 * np.sqrt(well[ip] / well[inp])
+    rwell = np.where(mixed, rwell * xi**2, rwell)
+    rdiam = np.where(mixed, rdiam * xi ** (-1.0 / 6.0), rdiam)
+    return dict(rmass=rmass, rdiam=rdiam, rwell=rwell, rdelta=rdelta)
+
+
+class BinaryDiffusion(BaseModel):
+    """Mixture-averaged diffusion from binary pair coefficients (``diffusion/binary.py:9-64``,
+    ``diffusion/kernels/binary.mako:11-62``): each pair's Chapman-Enskog coefficient at unit
+    pressure, D_nm = 3/16 sqrt(2 pi / m_nm) (kB T)^3/2 / (pi sigma_nm^2 Omega(1,1)*), fitted as
+    T^3/2 / D_nm in ln T over the case's T-range; the kernel sums over the bath (the case's
+    `bath` species, or all of them) and divides by the pressure.
+
+    Table layout: pairfit[n][k] the fit of species n with its k-th partner, bathof[n][k] that
+    partner's index (every other species in order for the whole bath; the bath's species, its
+    own slot included, for a partial one), selffit[n] its own pair, diffmeta (width, partial)."""
+
+    name = "binary"
+    kind = 1
+
+    def __init__(self, cfgsect):
+        super().__init__(cfgsect)
+        self.needs += ("well", "diam", "dipole", "polarize")
+
+    def bath(self, species):
+        names = list(species)
+        chosen = self.cfgsect["bath"]
+        unknown = sorted(set(chosen or ()) - set(names))
+        if unknown:
+            raise ValueError(f"mixture.bath names {unknown}, not species of the case")
+        return np.array([names.index(n) for n in (names if chosen is None else chosen)], dtype=np.int32)
+
+    def dijinv(self, species):
+        """The fit of T^3/2 / D_nm (at unit pressure) per unordered pair, in triu order."""
+        from peregrine_amd.mixture.constants import kB
+        from peregrine_amd.mixture.fits import fit
+        from peregrine_amd.mixture.integrals import CollisionIntegrals
+
+        pp = pair_parameters(species)
+        Ts = np.linspace(*self.trange, 50)
+        ns = len(species)
+        k, j = np.triu_indices(ns)
+        Tstar = np.outer(Ts, kB / pp["rwell"][k, j])
+        delta = np.broadcast_to(pp["rdelta"][k, j], Tstar.shape)
+        ci = CollisionIntegrals()
+        omega11 = ci.omega22(Tstar, delta) / ci.astar(Tstar, delta)
+        kT = kB * Ts[:, None]
+        D = 3.0 / 16.0 * np.sqrt(2.0 * np.pi / pp["rmass"][k, j]) * kT**1.5 / (np.pi * pp["rdiam"][k, j] ** 2 * omega11)
+        tol, deg = self.cfgsect["refit-tol"], self.cfgsect["refit-max-deg"]
+        return [fit(np.log(Ts), Dinv, tol, deg)[0] for Dinv in (Ts[:, None] ** 1.5 / D).T]
+
+    def constants(self, species):
+        from peregrine_amd.mixture.eos import NT
+        from peregrine_amd.mixture.fits import padded
+
+        ns = len(species)
+        dijinv = padded(self.dijinv(species), width=NT)
+        bath = self.bath(species)
+        slot = np.zeros((ns, ns), dtype=int)
+        slot[np.triu_indices(ns)] = np.arange(ns * (ns + 1) // 2)
+        slot = np.maximum(slot, slot.T)
+        whole = len(bath) == ns
+        rows = [[m for m in range(ns) if m != n] if whole else list(bath) for n in range(ns)]
+        width = len(rows[0])
+        pairfit = np.zeros((ns, ns, NT))
+        bathof = np.zeros((ns, ns))
+        for n, r in enumerate(rows):
+            for k, m in enumerate(r):
+                bathof[n, k] = m
+                if m != n:
+                    pairfit[n, k] = dijinv[slot[n, m]]
+        return dict(pairfit=pairfit, selffit=dijinv[np.diag(slot)], bathof=bathof, bath=bath,
+                    diffmeta=np.array([width, 0.0 if whole else 1.0]), dijinv=dijinv)
+
+
 TRANSPORT_MODELS = {m.name: m for m in (ConstantProps, KineticTheory, ChungDenseGas)}
-DIFFUSION_MODELS = {m.name: m for m in (Lewis,)}
+DIFFUSION_MODELS = {m.name: m for m in (Lewis, BinaryDiffusion)}
 MIXRULE_MODELS = {m.name: m for m in (Wilke,)}

@@ -39,7 +39,7 @@ class SubstepOpts(C.Structure):
 class CaseDesc(C.Structure):
     _fields_ = [
         ("ns", _i32), ("eos", _i32), ("recon", _i32), ("limiter", _i32), ("primary", _i32), ("mechanism", _i32),
-        ("trans", _i32), ("reserved", _i32 * 9),
+        ("trans", _i32), ("diffusion", _i32), ("reserved", _i32 * 8),
     ]
 
 
@@ -137,7 +137,7 @@ def const_layout(ns):
     names = ("MW", "MWinv", "Rsp", "cppolyR", "h298R", "cp0", "gpoly", "h298", "mu0", "kappa0", "lewis", "hpoly",
              "MWqinv", "wilkepair", "mupoly", "kappapoly", "cppoly", "cubic", "cubicsqrtac", "cubicb", "cubicda",
              "cubicmTc", "cubiconepm", "cubicsqrtacMW", "cubicbMW", "vshift", "vshiftMW", "cubicdaMW", "acentric",
-             "chungsigma", "chungepsk", "chungvisc", "chungcond")
+             "chungsigma", "chungepsk", "chungvisc", "chungcond", "pairfit", "selffit", "bathof", "diffmeta")
     out = {}
     for name in names:
         count = _i32(0)

@@ -203,6 +203,7 @@ class NativeCase:
         desc.recon, desc.limiter, desc.primary = RECON[sim.recon], LIMITER[sim.limiter], PRIMARY[sim.primary]
         desc.mechanism = mechanism
         desc.trans = TRANS[mix.trans.name if mix.trans is not None else None]
+        desc.diffusion = mix.diffusion.kind if mix.diffusion is not None else 0
         handle = C.c_void_p()
         chem_ptr = self.chem.data_ptr() if self.chem is not None else None
         native.check(native.lib().pga_case_create(C.byref(desc), self.consts.data_ptr(), chem_ptr, C.byref(handle)),

@@ -253,7 +253,7 @@ def test_halo_copies(dev):
 
 
 def _run_pair(gas, recon, primary, limiter, chem, integrator, ncell, steps, dt, prims, lengths, fused=True,
-              trans=None, species=None, nstate=8, walls=(), substep=None):
+              trans=None, species=None, nstate=8, walls=(), substep=None, mixture=None):
     """The same case stepped by the solver on the device and by the oracle on the host; walls: the
     faces (0 i-low .. 5 k-high) that are adiabatic slip walls, the rest periodic; substep: the
     sub-stepped chemistry's (max-substeps, bulk-substeps, stiff-bins, equilibrium-tolerance), the
@@ -268,6 +268,8 @@ def _run_pair(gas, recon, primary, limiter, chem, integrator, ncell, steps, dt, 
         ci["max-substeps"], ci["bulk-substeps"], ci["stiff-bins"], ci["equilibrium-tolerance"] = substep
     if species is not None:
         cfg["mixture"]["species"] = species
+    for key, value in (mixture or {}).items():
+        cfg["mixture"][key] = value
     if walls:
         cfg["bcvals"]["wall"] = {"kind": "slp-adia-wall"}
     cfg["backend-mi355x"]["fused-stage"] = fused
@@ -452,11 +454,16 @@ def test_blocks_of_a_lattice_step_as_one_block(dev, fused):
 # --- the viscous terms -----------------------------------------------------------------------
 
 
-@pytest.mark.parametrize("gas,trans", [("air-tpg", "constant"), ("binary", "constant"), ("h2o2", "kinetic-theory")])
-def test_transport_gradients_and_viscous_flux(dev, gas, trans):
+@pytest.mark.parametrize("gas,trans,diffusion,bath", [
+    ("air-tpg", "constant", "lewis", None), ("binary", "constant", "lewis", None),
+    ("h2o2", "kinetic-theory", "lewis", None), ("h2o2", "kinetic-theory", "binary", None),
+    ("h2o2", "kinetic-theory", "binary", ["H2", "O2", "H2O", "N2"])])
+def test_transport_gradients_and_viscous_flux(dev, gas, trans, diffusion, bath):
     """pga_transport, pga_grads and pga_diffflux against the oracle on a jostled block: the same
-    operations in the same order, so the same bits."""
+    operations in the same order, so the same bits; Lewis numbers and binary diffusion, over every
+    species and over a partial bath."""
     cfg = sp.config(gas, trans=trans)
+    cfg["mixture"]["diffusion"], cfg["mixture"]["bath"] = diffusion, bath
     sim = get_simulator(cfg)
     nc = NativeCase(sim, Mi355xBackend(cfg))
     oc = O.OracleCase(sim.mixture)
@@ -548,6 +555,23 @@ def test_viscous_binary_muscl_steps_match(dev, fused):
     got = mb.blocks[0].Q.get()[g:-g, g:-g, g:-g].transpose(3, 2, 1, 0)
     err = _rowerr(got, hb.regs[0][hb.interior()])
     print(f"viscous binary 10 steps ({'fused' if fused else 'per-kernel'}): {err:.3e}")
+    assert err < 1e-10
+
+
+@pytest.mark.parametrize("fused", [True, False], ids=["fused", "per-kernel"])
+def test_reacting_navier_stokes_with_binary_diffusion_matches(dev, fused):
+    """H2/O2-N2 with kinetic-theory transport, binary (mixture-averaged) diffusion and coupled
+    chemistry, MUSCL + HLLC, tvd-rk3: 5 steps on 20 x 12 x 12 by the solver against the oracle."""
+    n = 12
+    L = (0.01,) * 3
+    mb, hb = _run_pair("h2o2", "muscl", "hllc", "koren", True, "tvd-rk3", (20, n, n), 5, 1e-8,
+                       lambda x: sp.reacting_flow(x, n, L), (0.01 * 20 / 12, 0.01, 0.01), fused=fused,
+                       trans="kinetic-theory", mixture={"diffusion": "binary"})
+    assert mb.fused == fused and mb.sim.mixture.diffusion.name == "binary"
+    g = hb.ng
+    got = mb.blocks[0].Q.get()[g:-g, g:-g, g:-g].transpose(3, 2, 1, 0)
+    err = _rowerr(got, hb.regs[0][hb.interior()])
+    print(f"reacting navier-stokes, binary diffusion, 5 steps ({'fused' if fused else 'per-kernel'}): {err:.3e}")
     assert err < 1e-10
 
 
