@@ -200,20 +200,36 @@ int32_t pga_diffflux(const pga_case *c, int32_t axis, double alpha, pga_view Q, 
 
 /* --- boundaries -------------------------------------------------------------------------- */
 
-/* The adiabatic slip wall's halo on face `side` (0 low, 1 high) of `axis` of a block of ncell
- * interior cells with ng halo layers: each halo cell h planes out takes the interior cell h planes
- * in, its momentum's normal part (the wall face's unit normal from S) taken off twice, its state
- * copied. Replaces the preadvflux hook of slp-adia-wall:
- * peregrine/simulator/euler/kernels/bcs/slp-adia-wall.mako. */
-int32_t pga_wall_mirror(const pga_case *c, int32_t axis, int32_t side, const int32_t *ncell, int32_t ng, pga_view Q,
-                        pga_view state, pga_view S, void *stream);
+/* A wall: its kind and values (peregrine/simulator/euler/bcs.py:15-27, navstokes/bcs.py:27-53).
+ * kind: 0 slp-adia-wall, 1 slp-isot-wall (Euler and Navier-Stokes); 2 no-slp-adia-wall,
+ * 3 no-slp-isot-wall, 4 mov-adia-wall, 5 mov-isot-wall (Navier-Stokes). u: a moving wall's velocity;
+ * T: an isothermal wall's temperature. A null pga_wall pointer is an adiabatic slip wall. */
+typedef struct {
+  int32_t kind;
+  int32_t reserved;
+  double u[3];
+  double T;
+} pga_wall;
+
+/* A wall's halo on face `side` (0 low, 1 high) of `axis` of a block of ncell interior cells with ng
+ * halo layers: each halo cell h planes out from the interior cell h planes in. A slip wall mirrors
+ * the velocity's normal part (the wall face's unit normal from S); a no-slip or moving wall reflects
+ * the velocity about the wall's, the energy carrying the change in its kinetic part; an adiabatic
+ * wall copies the state, an isothermal one takes the image's p and Y at the wall's T by the EOS.
+ * Replaces the preadvflux hooks: peregrine/simulator/euler/kernels/bcs/slp-{adia,isot}-wall.mako,
+ * navstokes/kernels/bcs/{no-slp,mov}-{adia,isot}-wall.mako and wall.mako (stuckhalo, stuckhaloisot). */
+int32_t pga_wall_mirror(const pga_case *c, int32_t axis, int32_t side, const pga_wall *w, const int32_t *ncell,
+                        int32_t ng, pga_view Q, pga_view state, pga_view S, void *stream);
 
 /* The flux through the same wall's faces, written over F: the pressure p_w = max(p + rho c u_n, 0)
- * of the cell next to the wall, and for a viscous case (trstate non-null) the normal stress of the
- * two-point normal velocity gradient, no heat flux. Replaces the postflux hook of slp-adia-wall:
- * peregrine/simulator/euler/kernels/bcs/wall.mako, navstokes/kernels/bcs/{slp-adia-wall,wall}.mako. */
-int32_t pga_wall_flux(const pga_case *c, int32_t axis, int32_t side, const int32_t *ncell, int32_t ng, pga_view Q,
-                      pga_view state, pga_view trstate, pga_view xcc, pga_view xf, pga_view S, pga_view F, void *stream);
+ * of the cell next to the wall, and for a viscous case (trstate non-null) the stress of the two-point
+ * normal velocity gradient relative to the wall (its normal part alone on a slip wall), the wall's
+ * work and, on an isothermal wall, the heat flux from T_w with mu and kappa at the wall. Replaces
+ * the postflux hooks: peregrine/simulator/euler/kernels/bcs/wall.mako,
+ * navstokes/kernels/bcs/wall.mako (wallfaceflux, adiaflux, isotflux, wallcoeffs). */
+int32_t pga_wall_flux(const pga_case *c, int32_t axis, int32_t side, const pga_wall *w, const int32_t *ncell,
+                      int32_t ng, pga_view Q, pga_view state, pga_view trstate, pga_view xcc, pga_view xf, pga_view S,
+                      pga_view F, void *stream);
 
 /* The CFL reduction over the cells of `box`: out[0..2] (device memory) receive the largest
  * acoustic c sqrt(sum 1/d^2), convective sum |u . n| / d and combined (their sum) inverse time

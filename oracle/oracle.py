@@ -66,7 +66,7 @@ class Block(C.Structure):
         ("subcap", C.c_int), ("subeqtol", C.c_double), ("substeps", C.POINTER(C.c_int32)), ("dt", C.c_double),
         ("viscous", C.c_int), ("alpha", C.c_double),
         ("trstate", Arr), ("hsp", Arr), ("grads", Arr), ("Jinv", Arr), ("xcc", Arr), ("xf", Arr * 3),
-        ("bc", C.c_int * 6), ("nstate", C.c_int),
+        ("bc", C.c_int * 6), ("nstate", C.c_int), ("bcwall", (C.c_double * 3) * 6), ("bcT", C.c_double * 6),
     ]
 
 
@@ -93,9 +93,10 @@ def _load():
     lib.pgo_combine.argtypes = [C.c_int, Arr, C.c_int, C.c_double, C.c_int, P(Arr), _dp, C.c_double, C.c_int, Box]
     lib.pgo_fill_periodic.argtypes = [Arr, C.c_int, P(C.c_int), C.c_int, C.c_int, C.c_int]
     lib.pgo_transport.argtypes = [P(Case), Arr, Arr, Arr, Arr, Box]
-    lib.pgo_wall_mirror.argtypes = [P(Case), C.c_int, C.c_int, P(C.c_int), C.c_int, C.c_int, Arr, Arr, Arr]
-    lib.pgo_wall_flux.argtypes = [P(Case), C.c_int, C.c_int, P(C.c_int), C.c_int, C.c_int, Arr, Arr, Arr, Arr, Arr,
-                                  Arr, Arr]
+    lib.pgo_wall_mirror.argtypes = [P(Case), C.c_int, C.c_int, C.c_int, _dp, C.c_double, P(C.c_int), C.c_int,
+                                    C.c_int, Arr, Arr, Arr]
+    lib.pgo_wall_flux.argtypes = [P(Case), C.c_int, C.c_int, C.c_int, _dp, C.c_double, P(C.c_int), C.c_int, C.c_int,
+                                  Arr, Arr, Arr, Arr, Arr, Arr, Arr]
     lib.pgo_grads.argtypes = [P(Case), Arr, Arr, Arr, Arr, Box]
     lib.pgo_diffflux.argtypes = [P(Case), C.c_int, C.c_double, Arr, Arr, Arr, Arr, Arr, Arr, Arr, Arr, Arr, Box]
     lib.pgo_cflmax.argtypes = [Arr, Arr, Arr, Arr, Arr, Arr, Box, _dp]
@@ -138,6 +139,10 @@ def arr(a):
 
 def box(start, extent):
     return Box((C.c_int * 3)(*start), (C.c_int * 3)(*extent))
+
+
+def dbls(values):
+    return (C.c_double * len(values))(*values)
 
 
 def ints(values):

@@ -1087,6 +1087,35 @@ static void hpartial(const pgo_case *c, const double *Yall, double p, double T, 
   }
 }
 
+/* the mixture's mu (returned) and kappa at T (lnT its log) and rho: constant species values or the
+ * kinetic-theory fits (transport/kernels/{constant,dilute}.mako), Wilke's rule and the
+ * conductivity mean (mixrule/kernels/wilke.mako:3-17, transport/kernels/mixing.mako:3-10), Chung's
+ * dense factors (transport/kernels/chung.mako) */
+static double mu_kappa(const pgo_case *c, const double *X, double T, double lnT, double rho, double *kappa) {
+  const int ns = c->ns;
+  double sqrtmu[MAXNS], kappasp[MAXNS];
+  if (c->trans == 0) {
+    for (int n = 0; n < ns; n++) sqrtmu[n] = sqrt(c->mu0[n]), kappasp[n] = c->kappa0[n];
+  } else {
+    /* dilute.mako:5-26: sqrt(mu) = fit(ln T) T^(1/4), kappa = fit(ln T) sqrt(T) */
+    const double sqrtT = sqrt(T), sqrtsqrtT = sqrt(sqrtT);
+    for (int n = 0; n < ns; n++) {
+      double fit = 0.0;
+      for (int m = c->nmu - 1; m >= 0; m--) fit = fit * lnT + c->mupoly[n * c->nmu + m];
+      sqrtmu[n] = fit * sqrtsqrtT;
+    }
+    for (int n = 0; n < ns; n++) {
+      double fit = 0.0;
+      for (int m = c->nkappa - 1; m >= 0; m--) fit = fit * lnT + c->kappapoly[n * c->nkappa + m];
+      kappasp[n] = fit * sqrtT;
+    }
+  }
+  double mu = viscmix(c, X, sqrtmu);
+  *kappa = kappamix(c, X, kappasp);
+  if (c->trans == 2) chung(c, X, T, lnT, rho, &mu, kappa);
+  return mu;
+}
+
 /* diffusion/kernels/binary.mako:11-62: each species' mixture-averaged coefficient over the bath,
  * D_n = rest / (p (rest sum1 + X_n sum2)) with rest = sum X_m MW_m, sum1 = sum X_m / D_nm and
  * sum2 = sum X_m MW_m / D_nm over the partners (a partial bath scales the first sum from the
@@ -1154,26 +1183,8 @@ void pgo_transport(const pgo_case *c, pgo_arr Q, pgo_arr state, pgo_arr trstate,
     double X[MAXNS];
     for (int n = 0; n < ns; n++) X[n] = Y[n] * c->MWinv[n] * MWmix;
     const double T = *at(state, i, j, k, 1), lnT = pgo_elog(T);
-    double sqrtmu[MAXNS], kappasp[MAXNS];
-    if (c->trans == 0) {
-      for (int n = 0; n < ns; n++) sqrtmu[n] = sqrt(c->mu0[n]), kappasp[n] = c->kappa0[n];
-    } else {
-      /* dilute.mako:5-26: sqrt(mu) = fit(ln T) T^(1/4), kappa = fit(ln T) sqrt(T) */
-      const double sqrtT = sqrt(T), sqrtsqrtT = sqrt(sqrtT);
-      for (int n = 0; n < ns; n++) {
-        double fit = 0.0;
-        for (int m = c->nmu - 1; m >= 0; m--) fit = fit * lnT + c->mupoly[n * c->nmu + m];
-        sqrtmu[n] = fit * sqrtsqrtT;
-      }
-      for (int n = 0; n < ns; n++) {
-        double fit = 0.0;
-        for (int m = c->nkappa - 1; m >= 0; m--) fit = fit * lnT + c->kappapoly[n * c->nkappa + m];
-        kappasp[n] = fit * sqrtT;
-      }
-    }
-    double mu = viscmix(c, X, sqrtmu);
-    double kappa = kappamix(c, X, kappasp);
-    if (c->trans == 2) chung(c, X, T, lnT, rho, &mu, &kappa);
+    double kappa;
+    const double mu = mu_kappa(c, X, T, lnT, rho, &kappa);
     *at(trstate, i, j, k, 0) = mu;
     *at(trstate, i, j, k, 1) = kappa;
     if (c->diffusion == 1) {
@@ -1339,10 +1350,31 @@ void pgo_diffflux(const pgo_case *c, int axis, double alpha, pgo_arr Q, pgo_arr 
  * navstokes/kernels/bcs/slp-adia-wall.mako, bcs/wall.mako
  * ---------------------------------------------------------------------- */
 
-/* the halo cell h planes out from the wall takes the interior cell h planes in, its momentum's
- * normal part (along the wall face's normal at that row) taken off twice, the state copied */
-void pgo_wall_mirror(const pgo_case *c, int axis, int side, const int *ncell, int ng, int nstate, pgo_arr Q,
-                     pgo_arr state, pgo_arr S) {
+/* solution.mako:14-23 fromprims for one cell */
+static void fromprims_cell(const pgo_case *c, double p, double T, const double u[3], const double *Y, pgo_arr Q,
+                           pgo_arr state, int i, int j, int k) {
+  mixmom m;
+  mix(c, Y, &m);
+  eosstate s;
+  frompT(c, &m, Y, p, T, &s);
+  *at(Q, i, j, k, 0) = s.rho;
+  *at(Q, i, j, k, 1) = s.rho * u[0];
+  *at(Q, i, j, k, 2) = s.rho * u[1];
+  *at(Q, i, j, k, 3) = s.rho * u[2];
+  *at(Q, i, j, k, 4) = s.rho * (s.e + 0.5 * ((u[0]) * (u[0]) + (u[1]) * (u[1]) + (u[2]) * (u[2])));
+  for (int n = 0; n < c->ns - 1; n++) *at(Q, i, j, k, 5 + n) = s.rho * Y[n];
+  store(state, i, j, k, &s);
+  storeZ(c, state, i, j, k, &s);
+}
+
+/* the walls (simulator/euler/bcs.py:15-27, navstokes/bcs.py:27-53): each halo cell h planes out
+ * from the interior cell h planes in. kind 0 slp-adia (euler/kernels/bcs/slp-adia-wall.mako: the
+ * momentum's normal part taken off twice, the state copied), 1 slp-isot (slp-isot-wall.mako: the
+ * velocity's normal part taken off twice, the image's p and Y at T_w by the EOS), 2 no-slp-adia and
+ * 4 mov-adia (navstokes wall.mako stuckhalo: the velocity reflected about the wall's, the energy
+ * carrying the kinetic change, the state copied), 3 no-slp-isot and 5 mov-isot (stuckhaloisot) */
+void pgo_wall_mirror(const pgo_case *c, int axis, int side, int kind, const double *wall, double Tw, const int *ncell,
+                     int ng, int nstate, pgo_arr Q, pgo_arr state, pgo_arr S) {
   const int n = ncell[axis];
   const int t1 = (axis + 1) % 3, t2 = (axis + 2) % 3;
   for (int h = 0; h < ng; h++) {
@@ -1350,37 +1382,71 @@ void pgo_wall_mirror(const pgo_case *c, int axis, int side, const int *ncell, in
 #pragma omp parallel for collapse(2) if ((int64_t)ncell[t1] * ncell[t2] > PGO_PAR_CELLS)
     for (int b2 = ng; b2 < ng + ncell[t2]; b2++)
       for (int b1 = ng; b1 < ng + ncell[t1]; b1++) {
-        int ih[3], im[3], iface[3];
-        ih[axis] = halo, im[axis] = mirror, iface[axis] = face;
-        ih[t1] = im[t1] = iface[t1] = b1;
-        ih[t2] = im[t2] = iface[t2] = b2;
-        const double Sx = *at(S, iface[0], iface[1], iface[2], 0), Sy = *at(S, iface[0], iface[1], iface[2], 1),
-                     Sz = *at(S, iface[0], iface[1], iface[2], 2);
+        int ih[3], im[3], f[3];
+        ih[axis] = halo, im[axis] = mirror, f[axis] = face;
+        ih[t1] = im[t1] = f[t1] = b1;
+        ih[t2] = im[t2] = f[t2] = b2;
+        const double Sx = *at(S, f[0], f[1], f[2], 0), Sy = *at(S, f[0], f[1], f[2], 1), Sz = *at(S, f[0], f[1], f[2], 2);
         const double Sinv = 1.0 / sqrt(Sx * Sx + Sy * Sy + Sz * Sz);
         const double nx = Sx * Sinv, ny = Sy * Sinv, nz = Sz * Sinv;
-        const double q1 = *at(Q, im[0], im[1], im[2], 1), q2 = *at(Q, im[0], im[1], im[2], 2),
-                     q3 = *at(Q, im[0], im[1], im[2], 3);
-        const double mn = 2.0 * (q1 * nx + q2 * ny + q3 * nz);
-        *at(Q, ih[0], ih[1], ih[2], 0) = *at(Q, im[0], im[1], im[2], 0);
-        *at(Q, ih[0], ih[1], ih[2], 1) = q1 - mn * nx;
-        *at(Q, ih[0], ih[1], ih[2], 2) = q2 - mn * ny;
-        *at(Q, ih[0], ih[1], ih[2], 3) = q3 - mn * nz;
-        for (int l = 4; l < c->ne; l++) *at(Q, ih[0], ih[1], ih[2], l) = *at(Q, im[0], im[1], im[2], l);
-        for (int f = 0; f < nstate; f++) *at(state, ih[0], ih[1], ih[2], f) = *at(state, im[0], im[1], im[2], f);
+        if (kind == 0) {
+          const double q1 = *at(Q, im[0], im[1], im[2], 1), q2 = *at(Q, im[0], im[1], im[2], 2),
+                       q3 = *at(Q, im[0], im[1], im[2], 3);
+          const double mn = 2.0 * (q1 * nx + q2 * ny + q3 * nz);
+          *at(Q, ih[0], ih[1], ih[2], 0) = *at(Q, im[0], im[1], im[2], 0);
+          *at(Q, ih[0], ih[1], ih[2], 1) = q1 - mn * nx;
+          *at(Q, ih[0], ih[1], ih[2], 2) = q2 - mn * ny;
+          *at(Q, ih[0], ih[1], ih[2], 3) = q3 - mn * nz;
+          for (int l = 4; l < c->ne; l++) *at(Q, ih[0], ih[1], ih[2], l) = *at(Q, im[0], im[1], im[2], l);
+          for (int f8 = 0; f8 < nstate; f8++) *at(state, ih[0], ih[1], ih[2], f8) = *at(state, im[0], im[1], im[2], f8);
+        } else if (kind == 2 || kind == 4) {
+          const double rho = *at(Q, im[0], im[1], im[2], 0), rhoinv = 1.0 / rho;
+          double ke = 0.0, kehalo = 0.0;
+          for (int d = 0; d < 3; d++) {
+            const double u = *at(Q, im[0], im[1], im[2], 1 + d) * rhoinv, uh = 2.0 * wall[d] - u;
+            *at(Q, ih[0], ih[1], ih[2], 1 + d) = rho * uh;
+            ke += u * u;
+            kehalo += uh * uh;
+          }
+          *at(Q, ih[0], ih[1], ih[2], 0) = rho;
+          *at(Q, ih[0], ih[1], ih[2], 4) = *at(Q, im[0], im[1], im[2], 4) + 0.5 * rho * (kehalo - ke);
+          for (int l = 5; l < c->ne; l++) *at(Q, ih[0], ih[1], ih[2], l) = *at(Q, im[0], im[1], im[2], l);
+          for (int f8 = 0; f8 < nstate; f8++) *at(state, ih[0], ih[1], ih[2], f8) = *at(state, im[0], im[1], im[2], f8);
+        } else {
+          const double rhoinv = 1.0 / *at(Q, im[0], im[1], im[2], 0);
+          double u[3], Y[MAXNS];
+          if (kind == 1) {
+            const double uin = *at(Q, im[0], im[1], im[2], 1) * rhoinv, vin = *at(Q, im[0], im[1], im[2], 2) * rhoinv,
+                         win = *at(Q, im[0], im[1], im[2], 3) * rhoinv;
+            const double un = 2.0 * (uin * nx + vin * ny + win * nz);
+            u[0] = uin - un * nx;
+            u[1] = vin - un * ny;
+            u[2] = win - un * nz;
+          } else {
+            for (int d = 0; d < 3; d++) u[d] = 2.0 * wall[d] - *at(Q, im[0], im[1], im[2], 1 + d) * rhoinv;
+          }
+          for (int l = 0; l < c->ns - 1; l++) Y[l] = *at(Q, im[0], im[1], im[2], 5 + l) * rhoinv;
+          fromprims_cell(c, *at(state, im[0], im[1], im[2], 0), Tw, u, Y, Q, state, ih[0], ih[1], ih[2]);
+        }
       }
   }
 }
 
-/* the flux through the wall's faces from the interior cell: the pressure p_w = max(p + rho c u_n,
- * 0) (wall.mako wallpressure: the acoustic star pressure to first order, u_n toward the wall);
- * a viscous case adds the normal stress of the two-point normal velocity gradient and no heat
- * flux (navstokes wall.mako wallfaceflux, slip: the normal component alone) */
-void pgo_wall_flux(const pgo_case *c, int axis, int side, const int *ncell, int ng, int viscous, pgo_arr Q,
-                   pgo_arr state, pgo_arr trstate, pgo_arr xcc, pgo_arr xf, pgo_arr S, pgo_arr F) {
+/* the wall's flux from the cell next to it, written over F: p_w = max(p + rho c u_n, 0)
+ * (euler/kernels/bcs/wall.mako wallpressure: the acoustic star pressure to first order, u_n toward
+ * the wall); a viscous wall adds the stress of the two-point normal velocity gradient relative to
+ * the wall, every component where the flow is held (stuck) and the normal one where it slips, the
+ * wall's work, and for an isothermal wall the heat flux from T_w with mu and kappa at the wall
+ * (navstokes/kernels/bcs/wall.mako wallfaceflux, adiaflux, isotflux, wallcoeffs) */
+void pgo_wall_flux(const pgo_case *c, int axis, int side, int kind, const double *wall, double Tw, const int *ncell,
+                   int ng, int viscous, pgo_arr Q, pgo_arr state, pgo_arr trstate, pgo_arr xcc, pgo_arr xf, pgo_arr S,
+                   pgo_arr F) {
   const int n = ncell[axis];
   const int t1 = (axis + 1) % 3, t2 = (axis + 2) % 3;
   const int face = side ? ng + n : ng, cell = side ? ng + n - 1 : ng;
   const double outward = side ? 1.0 : -1.0;
+  const int isot = kind == 1 || kind == 3 || kind == 5;
+  const double stuck = kind >= 2 ? 1.0 : 0.0;
 #pragma omp parallel for collapse(2) if ((int64_t)ncell[t1] * ncell[t2] > PGO_PAR_CELLS)
   for (int b2 = ng; b2 < ng + ncell[t2]; b2++)
     for (int b1 = ng; b1 < ng + ncell[t1]; b1++) {
@@ -1401,7 +1467,30 @@ void pgo_wall_flux(const pgo_case *c, int axis, int side, const int *ncell, int 
       Fv[3] = pw * S3[2];
       for (int l = 4; l < c->ne; l++) Fv[l] = 0.0;
       if (viscous) {
-        const double mu = *at(trstate, ic[0], ic[1], ic[2], 0);
+        double mu, kappa, dT;
+        if (isot) {
+          const double rhoinv = 1.0 / q[0];
+          double Y[MAXNS], Ylast = 1.0;
+          for (int l = 0; l < c->ns - 1; l++) {
+            Y[l] = *at(Q, ic[0], ic[1], ic[2], 5 + l) * rhoinv;
+            Ylast -= *at(Q, ic[0], ic[1], ic[2], 5 + l) * rhoinv;
+          }
+          Y[c->ns - 1] = fmax(Ylast, 0.0);
+          mixmom m;
+          mix(c, Y, &m);
+          eosstate thermo;
+          frompT(c, &m, Y, *at(state, ic[0], ic[1], ic[2], 0), Tw, &thermo);
+          double denom = 0.0, X[MAXNS];
+          for (int l = 0; l < c->ns; l++) denom += Y[l] * c->MWinv[l];
+          const double MWmix = 1.0 / denom;
+          for (int l = 0; l < c->ns; l++) X[l] = Y[l] * c->MWinv[l] * MWmix;
+          mu = mu_kappa(c, X, Tw, pgo_elog(Tw), thermo.rho, &kappa);
+          dT = Tw - *at(state, ic[0], ic[1], ic[2], 1);
+        } else {
+          mu = *at(trstate, ic[0], ic[1], ic[2], 0);
+          kappa = *at(trstate, ic[0], ic[1], ic[2], 1);
+          dT = 0.0;
+        }
         const double Sabs = sqrt(S3[0] * S3[0] + S3[1] * S3[1] + S3[2] * S3[2]), Sinv = 1.0 / Sabs;
         const double nv[3] = {S3[0] * Sinv, S3[1] * Sinv, S3[2] * Sinv};
         const double dn = fabs((*at(xcc, ic[0], ic[1], ic[2], 0) - *at(xf, f[0], f[1], f[2], 0)) * nv[0] +
@@ -1409,16 +1498,16 @@ void pgo_wall_flux(const pgo_case *c, int axis, int side, const int *ncell, int 
                                (*at(xcc, ic[0], ic[1], ic[2], 2) - *at(xf, f[0], f[1], f[2], 2)) * nv[2]);
         const double rhoinv = 1.0 / q[0];
         double dun[3];
-        for (int d = 0; d < 3; d++) dun[d] = outward * (0.0 - q[1 + d] * rhoinv) / dn;
+        for (int d = 0; d < 3; d++) dun[d] = outward * (wall[d] - q[1 + d] * rhoinv) / dn;
         const double div = dun[0] * nv[0] + dun[1] * nv[1] + dun[2] * nv[2];
-        for (int d = 0; d < 3; d++) dun[d] = div * nv[d] + 0.0 * (dun[d] - div * nv[d]);
+        for (int d = 0; d < 3; d++) dun[d] = div * nv[d] + stuck * (dun[d] - div * nv[d]);
         double work = 0.0;
         for (int d = 0; d < 3; d++) {
           const double tS = -mu * Sabs * (dun[d] + div * nv[d] / 3.0);
           Fv[1 + d] += tS;
-          work += 0.0 * tS;
+          work += wall[d] * tS;
         }
-        Fv[4] += work - *at(trstate, ic[0], ic[1], ic[2], 1) * outward * (0.0) / dn * Sabs;
+        Fv[4] += work - kappa * outward * (dT) / dn * Sabs;
       }
       for (int l = 0; l < c->ne; l++) *at(F, f[0], f[1], f[2], l) = Fv[l];
     }
@@ -1460,7 +1549,9 @@ static void prerhs(const pgo_case *c, pgo_block *b, pgo_arr Q, pgo_arr dQ, int w
       if (!b->bc[2 * a + side]) pgo_fromQ(c, Q, b->state, haloslab(b, a, side, b->ng));
   for (int a = 0; a < 3; a++)
     for (int side = 0; side < 2; side++)
-      if (b->bc[2 * a + side]) pgo_wall_mirror(c, a, side, b->ncell, b->ng, b->nstate, Q, b->state, S[a]);
+      if (b->bc[2 * a + side])
+        pgo_wall_mirror(c, a, side, b->bc[2 * a + side] - 1, b->bcwall[2 * a + side], b->bcT[2 * a + side], b->ncell,
+                        b->ng, b->nstate, Q, b->state, S[a]);
   if (withdQ && b->chemistry == 1) pgo_source(c, Q, b->state, dQ, interior(b));
   if (withdQ && b->chemistry == 2)
     pgo_substep(c, Q, b->state, dQ, 1, b->substeps, b->subcap, b->subeqtol, b->dt, interior(b));
@@ -1491,7 +1582,8 @@ static void rhs(const pgo_case *c, pgo_block *b, pgo_arr Q, pgo_arr dQ) {
   for (int a = 0; a < 3; a++)
     for (int side = 0; side < 2; side++)
       if (b->bc[2 * a + side])
-        pgo_wall_flux(c, a, side, b->ncell, b->ng, b->viscous, Q, b->state, b->trstate, b->xcc, b->xf[a], S[a], F[a]);
+        pgo_wall_flux(c, a, side, b->bc[2 * a + side] - 1, b->bcwall[2 * a + side], b->bcT[2 * a + side], b->ncell,
+                      b->ng, b->viscous, Q, b->state, b->trstate, b->xcc, b->xf[a], S[a], F[a]);
   pgo_applyflux(c->ne, dQ, b->iF, b->jF, b->kF, b->detJinv, b->chemistry, interior(b));
 }
 

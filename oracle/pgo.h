@@ -107,19 +107,23 @@ typedef struct {
   int viscous;
   double alpha;
   pgo_arr trstate, hsp, grads, Jinv, xcc, xf[3];
-  /* per face (i low, i high, j low, ...): 0 periodic, 1 adiabatic slip wall; nstate the state
-   * fields (9 for the real gas) */
+  /* per face (i low, i high, j low, ...): 0 periodic, else 1 + the wall's kind (0 slp-adia,
+   * 1 slp-isot, 2 no-slp-adia, 3 no-slp-isot, 4 mov-adia, 5 mov-isot) with its velocity and T;
+   * nstate the state fields (9 for the real gas) */
   int bc[6];
   int nstate;
+  double bcwall[6][3], bcT[6];
 } pgo_block;
 
-/* the adiabatic slip wall (simulator/euler/bcs.py:20-27): the halo as the interior's mirror image
- * across the wall face (side 0 the low face of axis, 1 the high), and the wall's own flux through
- * its faces (the pressure's, plus for a viscous case the normal stress and no heat) */
-void pgo_wall_mirror(const pgo_case *c, int axis, int side, const int *ncell, int ng, int nstate, pgo_arr Q,
-                     pgo_arr state, pgo_arr S);
-void pgo_wall_flux(const pgo_case *c, int axis, int side, const int *ncell, int ng, int viscous, pgo_arr Q,
-                   pgo_arr state, pgo_arr trstate, pgo_arr xcc, pgo_arr xf, pgo_arr S, pgo_arr F);
+/* the walls (simulator/euler/bcs.py:15-27, navstokes/bcs.py:27-53): the halo as the interior's
+ * image across the wall face (side 0 the low face of axis, 1 the high), and the wall's own flux
+ * through its faces; kind 0 slp-adia, 1 slp-isot, 2 no-slp-adia, 3 no-slp-isot, 4 mov-adia,
+ * 5 mov-isot, wall the wall's velocity, Tw its temperature */
+void pgo_wall_mirror(const pgo_case *c, int axis, int side, int kind, const double *wall, double Tw, const int *ncell,
+                     int ng, int nstate, pgo_arr Q, pgo_arr state, pgo_arr S);
+void pgo_wall_flux(const pgo_case *c, int axis, int side, int kind, const double *wall, double Tw, const int *ncell,
+                   int ng, int viscous, pgo_arr Q, pgo_arr state, pgo_arr trstate, pgo_arr xcc, pgo_arr xf, pgo_arr S,
+                   pgo_arr F);
 
 /* the largest acoustic, convective and combined inverse time scales over the box's cells (the
  * CFL controller's step is cfl over the combined one) */

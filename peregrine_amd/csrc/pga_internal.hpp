@@ -7,6 +7,7 @@
 #include <cstdio>
 
 #include "../../include/peregrine_amd.h"
+#include "pga_common.hpp"
 
 struct pga_case {
   pga_case_desc desc;

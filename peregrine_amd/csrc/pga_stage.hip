@@ -78,21 +78,23 @@ __global__ __launch_bounds__(WT) void k_stage(StageArgs a) {
     const int64_t fil = iF.off(i, j, k), fjl = jF.off(i, j, k), fkl = kF.off(i, j, k);
     const int64_t fih = fil + 1, fjh = fjl + jF.sj, fkh = fkl + kF.sk;
     const int64_t orr = CHEM ? rates.off(i, j, k) : 0;
-    const double J = det.at(det.off(i, j, k), 0);
+    const double J = __builtin_nontemporal_load(&det.at(det.off(i, j, k), 0));
     // per conserved row: its six faces in the reference's order of sums, times detJinv, the
     // source first where there is one (chemint/kernels/source.mako:3-10), then the RK combine in
     // the reference's order of terms (integrators/kernels/combine.mako:1-13); a row at a time in a
-    // rolled loop keeps ~30 registers, so eight waves per SIMD hide the loads' latency
+    // rolled loop keeps ~30 registers, so eight waves per SIMD hide the loads' latency. What only
+    // this cell reads (Q, Q0, the rates, 1/J) and what it writes stream past the caches
+    // (nontemporal); each face value is read by the two cells it lies between and is cached
 #pragma unroll 1
     for (int l = 0; l < NE; l++) {
 #pragma clang fp contract(off)
       double d = ((((iF.at(fil, l) + jF.at(fjl, l)) + kF.at(fkl, l)) - iF.at(fih, l)) - jF.at(fjh, l)) - kF.at(fkh, l);
       d = d * J;
       if constexpr (CHEM) {
-        if (l >= 5) d = rates.at(orr, l - 5) + d;
+        if (l >= 5) d = __builtin_nontemporal_load(&rates.at(orr, l - 5)) + d;
       }
       // the terms' values are selected as values: a select of pointers would put them in memory
-      const double qin = Qin.at(o, l), q0 = Q0.at(o0, l);
+      const double qin = __builtin_nontemporal_load(&Qin.at(o, l)), q0 = __builtin_nontemporal_load(&Q0.at(o0, l));
       double acc = 0.0;
 #pragma unroll
       for (int t = 0; t < 4; t++) {
@@ -103,7 +105,7 @@ __global__ __launch_bounds__(WT) void k_stage(StageArgs a) {
           acc = t == 0 ? term : acc + term;
         }
       }
-      Qout.at(oo, l) = acc;
+      __builtin_nontemporal_store(acc, &Qout.at(oo, l));
     }
   }
 }

@@ -31,6 +31,14 @@ class Box(C.Structure):
     _fields_ = [("start", _i32 * 3), ("extent", _i32 * 3)]
 
 
+class Wall(C.Structure):
+    _fields_ = [("kind", _i32), ("reserved", _i32), ("u", C.c_double * 3), ("T", C.c_double)]
+
+
+WALL_KINDS = ("slp-adia-wall", "slp-isot-wall", "no-slp-adia-wall", "no-slp-isot-wall", "mov-adia-wall",
+              "mov-isot-wall")
+
+
 class SubstepOpts(C.Structure):
     _fields_ = [("max_substeps", _i32), ("bulk_substeps", _i32), ("stiff_bins", _i32), ("reserved", _i32),
                 ("equilibrium_tolerance", C.c_double)]
@@ -77,8 +85,9 @@ SIGNATURES = {
     "pga_grads": (_i32, [C.c_void_p, View, View, View, View, Box, C.c_void_p]),
     "pga_diffflux": (_i32, [C.c_void_p, _i32, C.c_double, View, View, View, View, View, View, View, View, View, Box,
                             C.c_void_p]),
-    "pga_wall_mirror": (_i32, [C.c_void_p, _i32, _i32, _ip, _i32, View, View, View, C.c_void_p]),
-    "pga_wall_flux": (_i32, [C.c_void_p, _i32, _i32, _ip, _i32, View, View, View, View, View, View, View, C.c_void_p]),
+    "pga_wall_mirror": (_i32, [C.c_void_p, _i32, _i32, C.c_void_p, _ip, _i32, View, View, View, C.c_void_p]),
+    "pga_wall_flux": (_i32, [C.c_void_p, _i32, _i32, C.c_void_p, _ip, _i32, View, View, View, View, View, View, View,
+                             C.c_void_p]),
     "pga_cflmax_worksize": (C.c_int64, [Box]),
     "pga_substep_listsize": (C.c_int64, [C.POINTER(SubstepOpts), Box]),
     "pga_substep": (_i32, [C.c_void_p, C.POINTER(SubstepOpts), View, View, View, _i32, C.c_void_p, C.c_int64,

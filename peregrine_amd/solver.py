@@ -175,7 +175,27 @@ class Solver:
         return len(self.blocks)
 
     # the boundary kinds this build has (simulator/euler/bcs.py, simulator/navstokes/bcs.py)
-    BC_KINDS = ("slp-adia-wall",)
+    # the wall kinds each simulator has (simulator/euler/bcs.py:15-27, navstokes/bcs.py:27-53) and
+    # the values each reads from its bcvals section
+    BC_KINDS = {"euler": ("slp-adia-wall", "slp-isot-wall"), "navier-stokes": native.WALL_KINDS}
+    BC_VALUES = {"slp-isot-wall": ("T",), "no-slp-isot-wall": ("T",), "mov-adia-wall": ("u", "v", "w"),
+                 "mov-isot-wall": ("u", "v", "w", "T")}
+
+    def _wall(self, where, bcname, sect):
+        kind = sect.get("kind")
+        kinds = self.BC_KINDS[self.sim.name]
+        if kind not in kinds:
+            raise ValueError(f"{where}: bcvals.{bcname} is of kind {kind!r}; this build's {self.sim.name} has {kinds}")
+        need = self.BC_VALUES.get(kind, ())
+        missing = [v for v in need if v not in sect]
+        if missing:
+            raise ValueError(f"{where}: bcvals.{bcname} ({kind}) needs {missing}")
+        w = native.Wall()
+        w.kind = native.WALL_KINDS.index(kind)
+        for d, name in enumerate("uvw"):
+            w.u[d] = float(sect.get(name, 0.0)) if name in need else 0.0
+        w.T = float(sect["T"]) if "T" in need else 0.0
+        return w
 
     def _check_bcs(self):
         """Every boundary named and its name in bcvals with a kind this build has
@@ -192,10 +212,8 @@ class Solver:
                 if f.bcname not in self.cfg["bcvals"]:
                     raise ValueError(f"{where} is named {f.bcname!r}, which bcvals does not have; it has "
                                      f"{sorted(self.cfg['bcvals'])}")
-                kind = dict(self.cfg["bcvals"][f.bcname]).get("kind")
-                if kind not in self.BC_KINDS:
-                    raise ValueError(f"{where}: bcvals.{f.bcname} is of kind {kind!r}; this build has {self.BC_KINDS}")
-                blk.walls.append((f.axis, 0 if f.is_low else 1))
+                wall = self._wall(where, f.bcname, dict(self.cfg["bcvals"][f.bcname]))
+                blk.walls.append((f.axis, 0 if f.is_low else 1, wall))
             if min(blk.ncell) < self.ng:
                 raise ValueError(f"block {blk.nblki}: {blk.ncell} cells, thinner than the {self.ng}-deep halo")
 
@@ -320,20 +338,20 @@ class Solver:
                 self._call("pga_diffflux", self.case.handle, a, self.sim.alpha, Qv, st, blk.trstate.view(),
                            blk.hsp.view(), blk.grads.view(), blk.xcc.view(), blk.xf[a].view(), S.view(), F.view(),
                            blk.face_box(a))
-        for axis, side in blk.walls:
+        for axis, side, wall in blk.walls:
             S, F = (blk.iS, blk.jS, blk.kS)[axis], (blk.iF, blk.jF, blk.kF)[axis]
             visc = self.sim.viscous
-            self._call("pga_wall_flux", self.case.handle, axis, side, native.ints(blk.ncell), self.ng, Qv, st,
-                       blk.trstate.view() if visc else native.View(), blk.xcc.view(),
+            self._call("pga_wall_flux", self.case.handle, axis, side, ctypes.byref(wall), native.ints(blk.ncell),
+                       self.ng, Qv, st, blk.trstate.view() if visc else native.View(), blk.xcc.view(),
                        blk.xf[axis].view() if visc else native.View(), S.view(), F.view())
 
     def wall_halos(self, Q):
         """Each wall's halo as the interior's mirror image (the preadvflux hook)."""
         for blk in self.blocks:
-            for axis, side in blk.walls:
+            for axis, side, wall in blk.walls:
                 S = (blk.iS, blk.jS, blk.kS)[axis]
-                self._call("pga_wall_mirror", self.case.handle, axis, side, native.ints(blk.ncell), self.ng,
-                           blk.regs[Q].view(), blk.state.view(), S.view())
+                self._call("pga_wall_mirror", self.case.handle, axis, side, ctypes.byref(wall), native.ints(blk.ncell),
+                           self.ng, blk.regs[Q].view(), blk.state.view(), S.view())
 
     def prerhs(self, Q, dQ=None):
         self.exchange(Q)

@@ -127,7 +127,8 @@ class HostBlock:
 
 
 def oracle_block(hb, integrator, chemistry, viscous=False, alpha=1.0, walls=()):
-    """walls: the faces (0 i-low, 1 i-high, 2 j-low, ...) that are adiabatic slip walls."""
+    """walls: the faces (0 i-low, 1 i-high, 2 j-low, ...) that are adiabatic slip walls, or a dict
+    {face: (kind, (u, v, w), T)} with kind 0 slp-adia .. 5 mov-isot."""
     from oracle import oracle as O
 
     b = O.Block()
@@ -147,8 +148,13 @@ def oracle_block(hb, integrator, chemistry, viscous=False, alpha=1.0, walls=()):
     b.Jinv, b.xcc = O.arr(hb.Jinv), O.arr(hb.xccd)
     for a in range(3):
         b.xf[a] = O.arr(hb.xf[a])
+    spec = walls if isinstance(walls, dict) else {f: (0, (0.0, 0.0, 0.0), 0.0) for f in walls}
     for f in range(6):
-        b.bc[f] = int(f in walls)
+        kind, wall, T = spec.get(f, (-1, (0.0, 0.0, 0.0), 0.0))
+        b.bc[f] = kind + 1
+        for d in range(3):
+            b.bcwall[f][d] = wall[d]
+        b.bcT[f] = T
     b.nstate = hb.state.shape[0]
     return b
 

@@ -270,13 +270,17 @@ def _run_pair(gas, recon, primary, limiter, chem, integrator, ncell, steps, dt, 
         cfg["mixture"]["species"] = species
     for key, value in (mixture or {}).items():
         cfg["mixture"][key] = value
-    if walls:
-        cfg["bcvals"]["wall"] = {"kind": "slp-adia-wall"}
+    # walls: faces that are adiabatic slip walls, or {face: (kind, values)} of any wall kind
+    spec = walls if isinstance(walls, dict) else {f: ("slp-adia-wall", {}) for f in walls}
+    for f, (kind, values) in spec.items():
+        cfg["bcvals"][f"wall{f}"] = {"kind": kind, **values}
+    oracle_walls = {f: (native.WALL_KINDS.index(kind), tuple(values.get(v, 0.0) for v in "uvw"), values.get("T", 0.0))
+                    for f, (kind, values) in spec.items()}
     cfg["backend-mi355x"]["fused-stage"] = fused
     backend = pg.get_backend("mi355x", cfg)
     periodic = [not (2 * a in walls or 2 * a + 1 in walls) for a in range(3)]
     mesh = pg.mesher.CubeMesher(blkdims=[n + 1 for n in ncell], lengths=lengths, periodic=periodic,
-                                bcnames={f + 1: "wall" for f in walls})
+                                bcnames={f + 1: f"wall{f}" for f in spec})
     mb = pg.get_solver(cfg, mesh, backend)
     sim = mb.sim
     oc = O.OracleCase(sim.mixture, recon, limiter, primary)
@@ -290,7 +294,8 @@ def _run_pair(gas, recon, primary, limiter, chem, integrator, ncell, steps, dt, 
     # the EOS over the interior, then the halo exchange and the EOS over the halos, which its first
     # stage then takes as ready; the oracle's first stage exchanges and takes the halo EOS itself
     O.lib.pgo_fromQ(oc.ref, O.arr(hb.regs[0]), O.arr(hb.state), O.box((hb.ng,) * 3, hb.ncell))
-    b = sp.oracle_block(hb, integrator, chem, viscous=sim.viscous, alpha=getattr(sim, "alpha", 1.0), walls=walls)
+    b = sp.oracle_block(hb, integrator, chem, viscous=sim.viscous, alpha=getattr(sim, "alpha", 1.0),
+                        walls=oracle_walls)
     if substep is not None:
         b.chemistry, b.subcap, b.subeqtol = 2, substep[0], substep[3]
         hb.counts = np.zeros(int(np.prod(ncell)), dtype=np.int32)
@@ -885,4 +890,74 @@ def test_substepped_reacting_steps_match(dev, fused):
     print(f"sub-stepped 4 steps ({'fused' if fused else 'per-kernel'}): {err:.3e}, counts up to {counts.max()}")
     assert np.array_equal(counts, hb.counts)
     assert counts.max() > 2
+    assert err < 1e-10
+
+
+WALL_CASES = [
+    ("slp-adia-wall", {}), ("slp-isot-wall", {"T": 350.0}), ("no-slp-adia-wall", {}),
+    ("no-slp-isot-wall", {"T": 350.0}), ("mov-adia-wall", {"u": 20.0, "v": 0.0, "w": -4.0}),
+    ("mov-isot-wall", {"u": 20.0, "v": 0.0, "w": -4.0, "T": 350.0}),
+]
+
+
+@pytest.mark.parametrize("kind,values", WALL_CASES, ids=[k for k, _ in WALL_CASES])
+def test_wall_halos_and_fluxes_match(dev, kind, values):
+    """Each wall kind's mirror halo and wall flux through the C ABI against the oracle, on both sides
+    of every axis of a jostled viscous block (kinetic-theory transport): to the bit."""
+    cfg = sp.config("h2o2", trans="kinetic-theory")
+    sim = get_simulator(cfg)
+    nc = NativeCase(sim, Mi355xBackend(cfg))
+    oc = O.OracleCase(sim.mixture)
+    hb = sp.HostBlock((13, 9, 6), 1, sim.ne, 1, jostle=0.3, seed=4)
+    q = sp.smooth_prims(hb.xcc, sim.ne, ns=sim.ns)
+    q[..., 5:] *= 0.1
+    _prepared(sim, oc, hb, q)
+    O.lib.pgo_transport(oc.ref, O.arr(hb.regs[0]), O.arr(hb.state), O.arr(hb.trstate), O.arr(hb.hsp),
+                        O.box((0, 0, 0), hb.cshape))
+    w = native.Wall()
+    w.kind = native.WALL_KINDS.index(kind)
+    for d, v in enumerate("uvw"):
+        w.u[d] = values.get(v, 0.0)
+    w.T = values.get("T", 0.0)
+    wall = O.dbls([w.u[0], w.u[1], w.u[2]])
+    L = native.lib()
+    for axis in range(3):
+        for side in (0, 1):
+            Q, st = hb.regs[0].copy(), hb.state.copy()
+            dQv, dSt = dev.put(Q), dev.put(st)
+            O.lib.pgo_wall_mirror(oc.ref, axis, side, w.kind, wall, w.T, O.ints(hb.ncell), hb.ng, st.shape[0], O.arr(Q),
+                                  O.arr(st), O.arr(hb.S[axis]))
+            native.check(L.pga_wall_mirror(nc.handle, axis, side, ctypes.byref(w), native.ints(hb.ncell), hb.ng,
+                                           dev.view(dQv), dev.view(dSt), dev.view(dev.put(hb.S[axis])), None),
+                         "pga_wall_mirror")
+            assert np.array_equal(dev.get(dQv), Q), (kind, axis, side)
+            assert np.array_equal(dev.get(dSt), st), (kind, axis, side)
+            F0 = np.random.default_rng(axis).random(hb.F[axis].shape)
+            want = F0.copy()
+            O.lib.pgo_wall_flux(oc.ref, axis, side, w.kind, wall, w.T, O.ints(hb.ncell), hb.ng, 1, O.arr(Q),
+                                O.arr(st), O.arr(hb.trstate), O.arr(hb.xccd), O.arr(hb.xf[axis]), O.arr(hb.S[axis]),
+                                O.arr(want))
+            dF = dev.put(F0)
+            native.check(L.pga_wall_flux(nc.handle, axis, side, ctypes.byref(w), native.ints(hb.ncell), hb.ng,
+                                         dev.view(dQv), dev.view(dSt), dev.view(dev.put(hb.trstate)),
+                                         dev.view(dev.put(hb.xccd)), dev.view(dev.put(hb.xf[axis])),
+                                         dev.view(dev.put(hb.S[axis])), dev.view(dF), None), "pga_wall_flux")
+            assert np.array_equal(dev.get(dF), want), (kind, axis, side)
+
+
+@pytest.mark.parametrize("fused", [True, False], ids=["fused", "per-kernel"])
+def test_channel_between_held_walls_steps_match(dev, fused):
+    """A viscous binary mixture between an isothermal no-slip wall (j low, 350 K) and a moving
+    adiabatic wall (j high, 20 m/s along x), periodic in i and k, MUSCL + HLLC, tvd-rk3, 10 steps by
+    the solver against the oracle."""
+    ncell = (16, 12, 8)
+    L = (0.4, 0.3, 0.2)
+    walls = {2: ("no-slp-isot-wall", {"T": 350.0}), 3: ("mov-adia-wall", {"u": 20.0, "v": 0.0, "w": 0.0})}
+    mb, hb = _run_pair("binary", "muscl", "hllc", "koren", False, "tvd-rk3", ncell, 10, 2e-6,
+                       lambda x: sp.smooth_prims(x, 6, ns=2), L, fused=fused, trans="constant", walls=walls)
+    assert mb.fused == fused
+    g = hb.ng
+    got = mb.blocks[0].Q.get()[g:-g, g:-g, g:-g].transpose(3, 2, 1, 0)
+    err = _rowerr(got, hb.regs[0][hb.interior()])
+    print(f"channel 10 steps ({'fused' if fused else 'per-kernel'}): {err:.3e}")
     assert err < 1e-10
