@@ -1378,7 +1378,10 @@ void pgo_wall_mirror(const pgo_case *c, int axis, int side, int kind, const doub
   const int n = ncell[axis];
   const int t1 = (axis + 1) % 3, t2 = (axis + 2) % 3;
   for (int h = 0; h < ng; h++) {
-    const int halo = side ? ng + n + h : ng - 1 - h, mirror = side ? ng + n - 1 - h : ng + h, face = side ? ng + n : ng;
+    /* a block thinner than the halo reflects its last cell into every deeper layer
+     * (launch/policy.py:168-184) */
+    const int hm = h < n ? h : n - 1;
+    const int halo = side ? ng + n + h : ng - 1 - h, mirror = side ? ng + n - 1 - hm : ng + hm, face = side ? ng + n : ng;
 #pragma omp parallel for collapse(2) if ((int64_t)ncell[t1] * ncell[t2] > PGO_PAR_CELLS)
     for (int b2 = ng; b2 < ng + ncell[t2]; b2++)
       for (int b1 = ng; b1 < ng + ncell[t1]; b1++) {

@@ -70,9 +70,12 @@ __global__ __launch_bounds__(256) void k_wall_mirror(WallArgs a) {
   const double Sinv = 1.0 / sqrt(Sx * Sx + Sy * Sy + Sz * Sz);
   const double nx = Sx * Sinv, ny = Sy * Sinv, nz = Sz * Sinv;
   for (int h = 0; h < a.ng; h++) {
+    // a block thinner than the halo reflects its last cell into every deeper layer
+    // (launch/policy.py:168-184, MirrorOffset)
+    const int hm = h < a.n ? h : a.n - 1;
     int ih[3], im[3];
     wall_index(a, a.side ? a.ng + a.n + h : a.ng - 1 - h, b1, b2, ih);
-    wall_index(a, a.side ? a.ng + a.n - 1 - h : a.ng + h, b1, b2, im);
+    wall_index(a, a.side ? a.ng + a.n - 1 - hm : a.ng + hm, b1, b2, im);
     const int64_t oh = a.Q.off(ih[0], ih[1], ih[2]), om = a.Q.off(im[0], im[1], im[2]);
     const int64_t sh = a.st.off(ih[0], ih[1], ih[2]), sm = a.st.off(im[0], im[1], im[2]);
     if (a.kind == WALL_SLP_ADIA) {
@@ -257,7 +260,6 @@ extern "C" int32_t pga_wall_mirror(const pga_case *c, int32_t axis, int32_t side
   pga_box faces{{ng, ng, ng}, {ncell[0], ncell[1], ncell[2]}};
   faces.start[axis] = side ? ng + ncell[axis] : ng;
   faces.extent[axis] = 1;
-  PGA_REQUIRE(ncell[axis] >= ng, "pga_wall_mirror: a wall needs at least ng = %d interior cells to mirror", ng);
   PGA_REQUIRE(covers(Q, cells, a.ne) && covers(state, cells, a.nstate) && covers(S, faces, 3),
               "pga_wall_mirror: the halo, its mirror or the wall faces reach outside Q, state or S");
   PGA_REQUIRE(Q.sj == state.sj && Q.sk == state.sk, "pga_wall_mirror: Q and state must share their layout");

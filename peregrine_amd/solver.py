@@ -214,8 +214,11 @@ class Solver:
                                      f"{sorted(self.cfg['bcvals'])}")
                 wall = self._wall(where, f.bcname, dict(self.cfg["bcvals"][f.bcname]))
                 blk.walls.append((f.axis, 0 if f.is_low else 1, wall))
-            if min(blk.ncell) < self.ng:
-                raise ValueError(f"block {blk.nblki}: {blk.ncell} cells, thinner than the {self.ng}-deep halo")
+            # an exchanged interface copies ng interior planes; a wall reflects however thin the block
+            for f in blk.faces.values():
+                if f.nbr is not None and blk.ncell[f.axis] < self.ng:
+                    raise ValueError(f"block {blk.nblki}: {blk.ncell} cells, thinner along axis {f.axis} than the "
+                                     f"{self.ng}-deep halo its interface {f.nface} exchanges")
 
     def _plugins(self):
         from peregrine_amd.plugins import claim

@@ -961,3 +961,46 @@ def test_channel_between_held_walls_steps_match(dev, fused):
     err = _rowerr(got, hb.regs[0][hb.interior()])
     print(f"channel 10 steps ({'fused' if fused else 'per-kernel'}): {err:.3e}")
     assert err < 1e-10
+
+
+@pytest.mark.parametrize("fused", [True, False], ids=["fused", "per-kernel"])
+def test_sods_tube_steps_match_and_meet_the_exact_solution(dev, fused):
+    """Sod's tube (Toro test 1 as the reference numbers it 0), MUSCL-Koren + HLLC, on a walled line of
+    200 cells one cell thick, 2000 steps of tvd-rk3 by the solver: the oracle's line to the bit, and
+    the reference's measured mean density error over the fan (tests/simulator/test_flux.py:87)."""
+    import peregrine_amd as pg
+    import test_oracle_riemann as tr
+
+    case, scheme = 0, "muscl-koren-hllc"
+    x_o, rho_o, u_o, p_o, r, _ = tr.tube(case, scheme)
+    c = pg.Config()
+    c["simulation"]["simulator"] = "euler"
+    c["mixture"]["species"], c["mixture"]["eos"] = tr.AIR, "cpg"
+    a = c["adv-flux"]
+    a["primary"], a["recon"], a["limiter"] = "hllc", "muscl", "koren"
+    c["bcvals"]["walls"] = {"kind": "slp-adia-wall"}
+    c["backend-mi355x"]["fused-stage"] = fused
+    c.validate()
+    n, L = 200, 1.0
+    mesh = pg.mesher.CubeMesher(blkdims=[n + 1, 2, 2], lengths=[L, 0.1, 0.1], periodic=[False] * 3,
+                                bcnames=dict.fromkeys(range(1, 7), "walls"))
+    mb = pg.get_solver(c, mesh, pg.get_backend("mi355x", c))
+    blk = mb.blocks[0]
+    x = blk.xcc.get()[..., 0] + (r.x0 - 0.5 * L)
+    left = x <= r.x0
+    q = np.zeros(blk.cellshape + (mb.sim.ne,))
+    q[..., 0] = np.where(left, r.pL, r.pR)
+    q[..., 1] = np.where(left, r.uL, r.uR)
+    q[..., 4] = np.where(left, r.TL, r.TR)
+    mb.set_prims([q])
+    for _ in range(int(round(r.t / r.dt))):
+        mb.step(r.dt)
+    g = mb.ng
+    Q = blk.Q.get()[g:-g, g, g]
+    rho = Q[:, 0]
+    diff = np.abs(rho - rho_o)
+    print(f"sod device vs oracle: max |drho| {diff.max():.3e} at {int(diff.argmax())}, cells differing {int((diff > 0).sum())}")
+    assert np.array_equal(rho, rho_o) and np.array_equal(Q[:, 1] / Q[:, 0], u_o)
+    rho_err, _, _ = tr.errors(x_o, rho, u_o, p_o, r)
+    print(f"sod on the device ({'fused' if fused else 'per-kernel'}): rho error {rho_err:.4e}")
+    assert rho_err == pytest.approx(tr.TUBES[case]["rho"][scheme], rel=1e-3)
