@@ -160,3 +160,60 @@ def test_the_tube_against_the_exact_solution(case, scheme):
         return
     within = PLATEAU[case]["muscl" if "muscl" in scheme else "constant"]
     assert p_err < within and u_err < within
+
+
+# --- the density wave once around a periodic line (tests/simulator/test_flux.py:26-48,
+# tests/simulator/support.py:118-138): measured (max order, mean order, e80) for each scheme
+
+WAVE_SCHEMES = dict(SCHEMES, kepaec=dict(primary="kepaec", recon="constant", limiter=None))
+ORDERS = [
+    ("kepaec", 1.95, 1.95, 3.872e-3),
+    ("muscl-koren-rusanov", 1.6, 2.0, 1.298e-2),
+    ("muscl-koren-hllc", 1.5, 2.0, 6.922e-3),
+    ("rusanov", 0.3, 0.3, 4.069e-1),
+    ("hllc", 0.7, 0.7, 1.312e-1),
+]
+
+
+def wave_errors(scheme, nx, u=100.0):
+    """An entropy wave rho = 1.2 + 0.6 sin(2 pi x) at 1 atm advected once around a periodic line of
+    nx - 1 cells (one cell thick, slip walls across), by tvd-rk3 at a tenth of the acoustic CFL;
+    the max and mean |rho - rho_0| along the line."""
+    c = Config()
+    c["simulation"]["simulator"] = "euler"
+    c["mixture"]["species"], c["mixture"]["eos"] = AIR, "cpg"
+    a = c["adv-flux"]
+    for key in ("primary", "recon", "limiter"):
+        a[key] = WAVE_SCHEMES[scheme][key]
+    c.validate()
+    sim = get_simulator(c)
+    oc = O.OracleCase(sim.mixture, a["recon"], a["limiter"], a["primary"])
+    hb = sp.HostBlock((nx - 1, 1, 1), sim.ng, sim.ne, 3, lengths=(1.0, 0.1, 0.1))
+    xc = hb.xcc[..., 0]
+    rho = 1.2 + 0.6 * np.sin(2 * np.pi * xc)
+    q = np.zeros(hb.cshape + (sim.ne,))
+    q[..., 0], q[..., 1] = 101325.0, u
+    q[..., 4] = q[..., 0] / (rho * R_AIR)
+    O.lib.pgo_fromprims(oc.ref, O.arr(sp.to_dev(q)), O.arr(hb.regs[0]), O.arr(hb.state), O.box((0, 0, 0), hb.cshape))
+    O.lib.pgo_fromQ(oc.ref, O.arr(hb.regs[0]), O.arr(hb.state), O.box((hb.ng,) * 3, hb.ncell))
+    b = sp.oracle_block(hb, "tvd-rk3", False, walls=(2, 3, 4, 5))
+    cmax = np.sqrt(GAMMA * q[..., 0].max() / rho.min())
+    dx = 1.0 / (nx - 1)
+    n = int(round(1.0 / (abs(u) * 0.1 * dx / (abs(u) + cmax))))
+    dt = 1.0 / (abs(u) * n)
+    for _ in range(n):
+        O.lib.pgo_step(oc.ref, ctypes.byref(b), dt)
+    g = hb.ng
+    error = np.abs(hb.regs[0][0][g, g, g:-g] - rho[g:-g, g, g])
+    return error.max(), error.mean()
+
+
+@pytest.mark.parametrize("scheme,atleast,mean_atleast,e80", ORDERS, ids=[o[0] for o in ORDERS])
+def test_the_order_on_the_wave(scheme, atleast, mean_atleast, e80):
+    e40, e80got = wave_errors(scheme, 41), wave_errors(scheme, 81)
+    print(f"wave {scheme}: orders {np.log2(e40[0] / e80got[0]):.2f} {np.log2(e40[1] / e80got[1]):.2f}, "
+          f"e80 {e80got[0]:.4e} (reference {e80:.3e})")
+    assert np.log2(e40[0] / e80got[0]) >= atleast
+    assert np.log2(e40[1] / e80got[1]) >= mean_atleast
+    # the reference locks e80 a fifth above its measurement; the same schemes land on it
+    assert e80got[0] == pytest.approx(e80, rel=1e-3)
