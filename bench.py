@@ -118,6 +118,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--per-kernel", action="store_true", help="the reference's launch sequence instead of the fused stage")
     ap.add_argument("--k-chunk", type=int, default=None)
+    ap.add_argument("--stage-slabs", type=int, default=None, help="k-slabs of the fused stage")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only a plain device copy of 512 MiB, five times: the reference for HBM byte counters")
     args = ap.parse_args()
@@ -145,6 +146,8 @@ def main():
     cfg["backend-mi355x"]["fused-stage"] = not args.per_kernel
     if args.k_chunk:
         cfg["backend-mi355x"]["k-chunk"] = args.k_chunk
+    if args.stage_slabs is not None:
+        cfg["backend-mi355x"]["stage-slabs"] = args.stage_slabs
     backend = pg.get_backend("mi355x", cfg)
     n = args.n
     mesh = pg.mesher.CubeMesher(blkdims=[n + 1] * 3, lengths=L, periodic=[True] * 3)
@@ -169,7 +172,9 @@ def main():
     dom = max(total, key=total.get)
     count, ms = times[dom]
     ne = mb.sim.ne
-    items = {"pga_flux": (n + 1) * n * n}.get(dom, n**3)
+    # items per launch: a stage's faces (three axes) or cells over the launches a stage makes
+    per_stage = {"pga_flux": 3 * (n + 1) * n * n}.get(dom, n**3)
+    items = per_stage * nst * args.steps / count
     per = algorithmic_bytes(dom, ne, mb.ng)
     achieved = per * items / (ms * 1e-3) / 1e9
     kname = {"pga_flux": "k_flux", "pga_stage": "k_stage"}.get(dom, dom)
@@ -194,7 +199,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"h2o2-9sp-28rx {n}^3 periodic, muscl-koren + hllc, coupled chemistry, tvd-rk3",
                    "cells": ncells, "stages_per_step": nst, "path": "fused" if mb.fused else "per-kernel",
-                   "k_chunk": cfg["backend-mi355x"]["k-chunk"]},
+                   "k_chunk": cfg["backend-mi355x"]["k-chunk"],
+                   "stage_slabs": cfg["backend-mi355x"]["stage-slabs"]},
         "roofline": roof,
         "cpu_baseline": cpu,
         "kernels": breakdown,

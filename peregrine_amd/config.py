@@ -66,14 +66,17 @@ _DEFAULTS = {
         "launch-waves": 2,
         "constform": "literal",
     },
-    # this package's device: the block shape of the cell and face kernels and whether
-    # a stage runs as the fused pass or as the per-kernel sequence
+    # this package's device: the block shape of the cell and face kernels, whether a stage
+    # runs as the fused pass or as the per-kernel sequence, and the k-slabs a fused stage is
+    # run in (a slab's face arrays stay in the 256 MB Infinity Cache between the flux sweeps
+    # and the divergence that reads them)
     "backend-mi355x": {
         "device": 0,
         "tile-i": 64,
         "tile-j": 4,
         "fused-stage": True,
         "k-chunk": 1,
+        "stage-slabs": 1,
         "graphs": True,
     },
     "halo-exchange": {"kind": "hoststaged"},

@@ -134,6 +134,22 @@ class Block:
                 xf.set(grid.facecenters(nodes, a))
 
 
+def stage_slabs(nk, n):
+    """The k-slabs (k0, k1, f0, f1) a fused stage runs a block of nk cells along k in, n of them
+    (at most nk): the stage of cells k0 <= k < k1 follows the fluxes it reads and writes Q and the
+    state in place, so a slab also takes every k face whose stencil (cells k - 2 to k + 1) reaches
+    one of its cells before they change: f1 = k1 + 1 (nk at the last), and the next slab's k faces
+    start at f1 + 1. Every face is computed once."""
+    n = max(1, min(n, nk))
+    slabs, f0 = [], 0
+    for c in range(n):
+        k0, k1 = nk * c // n, nk * (c + 1) // n
+        f1 = min(k1 + 1, nk)
+        slabs.append((k0, k1, f0, f1))
+        f0 = f1 + 1
+    return slabs
+
+
 class Solver:
     def __init__(self, cfg, mesh, backend, sim, integ, state=None):
         self.cfg, self.mesh, self.backend, self.sim, self.integ = cfg, mesh, backend, sim, integ
@@ -332,15 +348,23 @@ class Solver:
                        blk.grads.view(), blk.interior_box())
         self._exchange(lambda b: b.grads, 3 * self.sim.ngrad, 1)
 
-    def fluxes(self, blk, Qv, st):
+    def fluxes(self, blk, Qv, st, slab=None):
         """The advective flux of each axis, with the viscous flux added for a viscous case, then the
-        walls' own flux through their faces (the postflux hook)."""
+        walls' own flux through their faces (the postflux hook). A slab (k0, k1, f0, f1) limits the
+        i and j faces to the cells k0 <= k < k1 and the k faces to f0 <= k <= f1."""
         for a, (S, F) in enumerate(((blk.iS, blk.iF), (blk.jS, blk.jF), (blk.kS, blk.kF))):
-            self._call("pga_flux", self.case.handle, a, Qv, st, S.view(), F.view(), blk.face_box(a))
+            box = blk.face_box(a)
+            if slab is not None:
+                k0, k1, f0, f1 = slab
+                lo, hi = (f0, f1 + 1) if a == 2 else (k0, k1)
+                start, ext = list(box.start), list(box.extent)
+                start[2], ext[2] = self.ng + lo, max(0, hi - lo)
+                box = native.box(start, ext)
+            self._call("pga_flux", self.case.handle, a, Qv, st, S.view(), F.view(), box)
             if self.sim.viscous:
                 self._call("pga_diffflux", self.case.handle, a, self.sim.alpha, Qv, st, blk.trstate.view(),
                            blk.hsp.view(), blk.grads.view(), blk.xcc.view(), blk.xf[a].view(), S.view(), F.view(),
-                           blk.face_box(a))
+                           box)
         for axis, side, wall in blk.walls:
             S, F = (blk.iS, blk.jS, blk.kS)[axis], (blk.iF, blk.jF, blk.kF)[axis]
             visc = self.sim.viscous
@@ -489,6 +513,9 @@ class Solver:
             terms.append((src(r), w, int(c.step and m == len(c.X) - 1)))
         return terms
 
+    def _slabs(self, blk):
+        return stage_slabs(blk.ncell[2], int(self.cfg["backend-mi355x"]["stage-slabs"]))
+
     def _fused_step(self):
         """A step by stages of five launches per block: the halo pass (exchange and halo EOS, unless
         from_prims left the halos ready), the chemistry rates, the three flux sweeps, and the fused
@@ -513,10 +540,13 @@ class Solver:
                 if self.sim.chemistry:
                     self.chemistry_source(blk, Qin, blk.rates.view(), False)
                 rates = blk.rates.view() if self.sim.chemistry else native.View()
-                self.fluxes(blk, Qin, st)
-                self._call("pga_stage", self.case.handle, Qin, st, blk.regs[0].view(), blk.regs[regs.Qout].view(),
-                           st, blk.iF.view(), blk.jF.view(), blk.kF.view(), blk.detJinv.view(), rates, len(terms),
-                           src, w, withdt, self.dt.ptr, box, kc)
+                for slab in self._slabs(blk):
+                    self.fluxes(blk, Qin, st, slab)
+                    k0, k1 = slab[:2]
+                    sbox = native.box((self.ng, self.ng, self.ng + k0), (blk.ncell[0], blk.ncell[1], k1 - k0))
+                    self._call("pga_stage", self.case.handle, Qin, st, blk.regs[0].view(),
+                               blk.regs[regs.Qout].view(), st, blk.iF.view(), blk.jF.view(), blk.kF.view(),
+                               blk.detJinv.view(), rates, len(terms), src, w, withdt, self.dt.ptr, sbox, kc)
 
     def step(self, dt):
         start = self.tme.get()

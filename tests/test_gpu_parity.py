@@ -253,7 +253,7 @@ def test_halo_copies(dev):
 
 
 def _run_pair(gas, recon, primary, limiter, chem, integrator, ncell, steps, dt, prims, lengths, fused=True,
-              trans=None, species=None, nstate=8, walls=(), substep=None, mixture=None):
+              trans=None, species=None, nstate=8, walls=(), substep=None, mixture=None, slabs=1):
     """The same case stepped by the solver on the device and by the oracle on the host; walls: the
     faces (0 i-low .. 5 k-high) that are adiabatic slip walls, the rest periodic; substep: the
     sub-stepped chemistry's (max-substeps, bulk-substeps, stiff-bins, equilibrium-tolerance), the
@@ -277,6 +277,7 @@ def _run_pair(gas, recon, primary, limiter, chem, integrator, ncell, steps, dt, 
     oracle_walls = {f: (native.WALL_KINDS.index(kind), tuple(values.get(v, 0.0) for v in "uvw"), values.get("T", 0.0))
                     for f, (kind, values) in spec.items()}
     cfg["backend-mi355x"]["fused-stage"] = fused
+    cfg["backend-mi355x"]["stage-slabs"] = slabs
     backend = pg.get_backend("mi355x", cfg)
     periodic = [not (2 * a in walls or 2 * a + 1 in walls) for a in range(3)]
     mesh = pg.mesher.CubeMesher(blkdims=[n + 1 for n in ncell], lengths=lengths, periodic=periodic,
@@ -365,10 +366,12 @@ def test_fused_and_per_kernel_stages_agree_to_the_bit(dev):
     n, L = 20, (0.01,) * 3
     for integ in ("tvd-rk3", "rk2", "rk1"):
         out = []
-        for fused in (True, False):
+        # the fused stage whole, in three uneven k-slabs (7, 7 and 6 cells), and the per-kernel one
+        for fused, slabs in ((True, 1), (True, 3), (False, 1)):
             cfg = sp.config("h2o2", recon="muscl", primary="hllc", limiter="koren", chem=True, integrator=integ, dt=1e-8)
             cfg["backend-mi355x"]["fused-stage"] = fused
             cfg["backend-mi355x"]["k-chunk"] = 3
+            cfg["backend-mi355x"]["stage-slabs"] = slabs
             mb = pg.get_solver(cfg, pg.mesher.CubeMesher(blkdims=[n + 1] * 3, lengths=L, periodic=[True] * 3),
                                pg.get_backend("mi355x", cfg))
             assert mb.fused == fused
@@ -378,8 +381,9 @@ def test_fused_and_per_kernel_stages_agree_to_the_bit(dev):
             out.append((mb.blocks[0].Q.get(), mb.blocks[0].state.get()))
         g = mb.ng
         inner = np.s_[g:-g, g:-g, g:-g]
-        assert np.array_equal(out[0][0][inner], out[1][0][inner]), integ
-        assert np.array_equal(out[0][1][inner], out[1][1][inner]), integ
+        for other in out[1:]:
+            assert np.array_equal(out[0][0][inner], other[0][inner]), integ
+            assert np.array_equal(out[0][1][inner], other[1][inner]), integ
 
 
 @pytest.mark.parametrize("gas", ["air-tpg", "air-cpg"])
@@ -945,16 +949,20 @@ def test_wall_halos_and_fluxes_match(dev, kind, values):
             assert np.array_equal(dev.get(dF), want), (kind, axis, side)
 
 
-@pytest.mark.parametrize("fused", [True, False], ids=["fused", "per-kernel"])
-def test_channel_between_held_walls_steps_match(dev, fused):
+@pytest.mark.parametrize("fused,slabs,kwalls", [(True, 1, False), (False, 1, False), (True, 3, True)],
+                         ids=["fused", "per-kernel", "fused-slabs-kwalls"])
+def test_channel_between_held_walls_steps_match(dev, fused, slabs, kwalls):
     """A viscous binary mixture between an isothermal no-slip wall (j low, 350 K) and a moving
-    adiabatic wall (j high, 20 m/s along x), periodic in i and k, MUSCL + HLLC, tvd-rk3, 10 steps by
-    the solver against the oracle."""
+    adiabatic wall (j high, 20 m/s along x), periodic in i and k (or walled in k by slip walls, the
+    fused stage in three k-slabs), MUSCL + HLLC, tvd-rk3, 10 steps by the solver against the oracle."""
     ncell = (16, 12, 8)
     L = (0.4, 0.3, 0.2)
     walls = {2: ("no-slp-isot-wall", {"T": 350.0}), 3: ("mov-adia-wall", {"u": 20.0, "v": 0.0, "w": 0.0})}
+    if kwalls:
+        walls.update({4: ("slp-adia-wall", {}), 5: ("no-slp-adia-wall", {})})
     mb, hb = _run_pair("binary", "muscl", "hllc", "koren", False, "tvd-rk3", ncell, 10, 2e-6,
-                       lambda x: sp.smooth_prims(x, 6, ns=2), L, fused=fused, trans="constant", walls=walls)
+                       lambda x: sp.smooth_prims(x, 6, ns=2), L, fused=fused, trans="constant", walls=walls,
+                       slabs=slabs)
     assert mb.fused == fused
     g = hb.ng
     got = mb.blocks[0].Q.get()[g:-g, g:-g, g:-g].transpose(3, 2, 1, 0)

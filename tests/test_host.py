@@ -165,3 +165,22 @@ def test_integrator_weights_give_the_amplification_polynomials():
                         acc += w * v[r]
                     v[c.out] = acc
             assert abs(v[0] - np.polyval(np.flip(want), z)) < 1e-14
+
+
+@pytest.mark.parametrize("nk,n", [(20, 3), (128, 4), (8, 8), (5, 9), (1, 1), (7, 2)])
+def test_stage_slabs_cover_each_cell_and_face_once_before_it_changes(nk, n):
+    """The fused stage's k-slabs: the cells partition 0..nk, the k faces 0..nk, each slab's stage has
+    its own faces, and every face whose stencil (cells k - 2 to k + 1) holds a slab's cell is computed
+    no later than that slab (its cells change in place after it)."""
+    from peregrine_amd.solver import stage_slabs
+
+    slabs = stage_slabs(nk, n)
+    cells = [k for k0, k1, _, _ in slabs for k in range(k0, k1)]
+    faces = [f for _, _, f0, f1 in slabs for f in range(f0, f1 + 1)]
+    assert cells == list(range(nk)) and faces == list(range(nk + 1))
+    done = set()
+    for k0, k1, f0, f1 in slabs:
+        done |= set(range(f0, f1 + 1))
+        assert {k0, k1} <= done
+        for k in range(k0, k1):
+            assert set(range(max(0, k - 1), min(nk, k + 2) + 1)) <= done
