@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define PGA_ABI_VERSION 1
+#define PGA_ABI_VERSION 2
 
 enum {
   PGA_OK = 0,
@@ -133,12 +133,16 @@ int32_t pga_combine(int32_t ne, pga_view out, int32_t own, double wout, int32_t 
 int32_t pga_fill_periodic(pga_view A, int32_t ncomp, const int32_t *ncell, int32_t ng, int32_t depth,
                           int32_t axis, void *stream);
 
-/* a box of `extent` cells of every component copied from src (starting at src_start) to dst
- * (at dst_start): the halo slab of an aligned block interface from its partner's interior planes.
- * Replaces pg_place on an unrotated interface: place.mako:11-37 with
- * peregrine/multiblock/collections/exchange/collection.py:78-96 (haloview / nbrview). */
+/* a box of `extent` cells of every component copied from src to dst (at dst_start): the halo
+ * slab of a block interface from its partner's interior planes. orient (3 digits, or null for
+ * 1 2 3) is the interface's orientation: digit m names the source axis the destination's axis m
+ * runs along, 1 2 3 for +i +j +k and 4 5 6 for -i -j -k; the cell dst_start + (a0, a1, a2) takes
+ * the source cell src_start + sum_m a_m (+-1 along that axis). Replaces pg_place on an
+ * unrotated interface: place.mako:11-37 with
+ * peregrine/multiblock/collections/exchange/collection.py:55-96 (haloview, nbrview, remap_across)
+ * and peregrine/multiblock/interface/base.py:37-45,95-136 (nbrplane, signedaxis, nbrnface). */
 int32_t pga_copy_box(pga_view dst, const int32_t *dst_start, pga_view src, const int32_t *src_start,
-                     const int32_t *extent, int32_t ncomp, void *stream);
+                     const int32_t *extent, int32_t ncomp, const int32_t *orient, void *stream);
 
 /* 1 when the fused stage is compiled for the case's species count, else 0 (use the per-kernel
  * entry points) */
@@ -166,13 +170,14 @@ int32_t pga_stage(const pga_case *c, pga_view Qin, pga_view stin, pga_view Q0, p
  * (peregrine/simulator/chemint/kernels/source.mako:3-10) writing only the species rows. */
 int32_t pga_rates(const pga_case *c, pga_view Q, pga_view state, pga_view rates, pga_box box, void *stream);
 
-/* The halo pass of a stage: for up to 6 slabs, Q copied from src (at src_start) to dst (at
- * dst_start), extent cells each, then the EOS over the slab into stdst, its Newton started from
- * stguess's T at the same cells. Replaces the prerhs slot's pg_place and the halo pg_fromQ
- * (simulator/advec/simulator.py:57-64). */
+/* The halo pass of a stage: for up to 6 slabs, Q copied from src to dst (at dst_start), extent
+ * cells each, the source cells placed by src_start and the slab's 3 orientation digits in orient
+ * (null: all aligned) as in pga_copy_box, then the EOS over the slab into stdst, its Newton
+ * started from stguess's T at the same cells. Replaces the prerhs slot's pg_place and the halo
+ * pg_fromQ (simulator/advec/simulator.py:57-64). */
 int32_t pga_halo(const pga_case *c, int32_t ntask, const pga_view *Qdst, const pga_view *Qsrc,
-                 const int32_t *dst_start, const int32_t *src_start, const int32_t *extent, pga_view stdst,
-                 pga_view stguess, void *stream);
+                 const int32_t *dst_start, const int32_t *src_start, const int32_t *extent, const int32_t *orient,
+                 pga_view stdst, pga_view stguess, void *stream);
 
 /* --- the viscous terms (Navier-Stokes) ---------------------------------------------------- */
 

@@ -313,22 +313,25 @@ extern "C" int32_t pga_production(const pga_case *c, int32_t n, const double *rh
   return launched("pga_production");
 }
 
-// place.mako:11-37 for an aligned interface: a box of every component copied from src to dst
+// place.mako:11-37: a box of every component copied from src to dst, the source cells
+// remapped by the interface's orientation
 namespace pga {
 struct CopyArgs {
-  int d0, d1, d2, s0, s1, s2, e0, e1, e2, ncomp;
+  int d0, d1, d2, e0, e1, e2, ncomp;
+  int64_t sbase, st0, st1, st2;
 };
 
 __global__ __launch_bounds__(256) void k_copy_box(View dst, View src, CopyArgs a) {
   const int ii = blockIdx.x * TI + threadIdx.x, jj = blockIdx.y * TJ + threadIdx.y;
   const int kk = blockIdx.z % a.e2, comp = blockIdx.z / a.e2;
   if (ii >= a.e0 || jj >= a.e1 || comp >= a.ncomp) return;
-  dst.at(dst.off(a.d0 + ii, a.d1 + jj, a.d2 + kk), comp) = src.at(src.off(a.s0 + ii, a.s1 + jj, a.s2 + kk), comp);
+  const int64_t os = a.sbase + ii * a.st0 + jj * a.st1 + kk * a.st2;
+  dst.at(dst.off(a.d0 + ii, a.d1 + jj, a.d2 + kk), comp) = src.at(os, comp);
 }
 }  // namespace pga
 
 extern "C" int32_t pga_copy_box(pga_view dst, const int32_t *dst_start, pga_view src, const int32_t *src_start,
-                                const int32_t *extent, int32_t ncomp, void *stream) {
+                                const int32_t *extent, int32_t ncomp, const int32_t *orient, void *stream) {
   if (!dst_start || !src_start || !extent || ncomp <= 0) {
     set_error("pga_copy_box: null argument or no components");
     return PGA_EINVAL;
@@ -339,17 +342,18 @@ extern "C" int32_t pga_copy_box(pga_view dst, const int32_t *dst_start, pga_view
       return PGA_EINVAL;
     }
   if (!extent[0] || !extent[1] || !extent[2]) return PGA_OK;
+  Placement pl;
   {
     const pga_box bd{{dst_start[0], dst_start[1], dst_start[2]}, {extent[0], extent[1], extent[2]}};
-    const pga_box bs{{src_start[0], src_start[1], src_start[2]}, {extent[0], extent[1], extent[2]}};
-    PGA_REQUIRE(covers(dst, bd, ncomp) && covers(src, bs, ncomp), "pga_copy_box: a box reaches outside its array");
+    PGA_REQUIRE(covers(dst, bd, ncomp), "pga_copy_box: the destination box reaches outside its array");
+    if (!placement("pga_copy_box", src, src_start, orient, extent, ncomp, pl)) return PGA_ERANGE;
   }
   if ((int64_t)extent[2] * ncomp > 65535) {
     set_error("pga_copy_box: too many k-planes times components");
     return PGA_ERANGE;
   }
-  const CopyArgs a{dst_start[0], dst_start[1], dst_start[2], src_start[0], src_start[1], src_start[2],
-                   extent[0],    extent[1],    extent[2],    ncomp};
+  const CopyArgs a{dst_start[0], dst_start[1], dst_start[2], extent[0], extent[1], extent[2], ncomp,
+                   pl.base,      pl.step[0],   pl.step[1],   pl.step[2]};
   const dim3 grid((extent[0] + TI - 1) / TI, (extent[1] + TJ - 1) / TJ, extent[2] * ncomp);
   hipLaunchKernelGGL(k_copy_box, grid, dim3(TI, TJ), 0, stream_of(stream), view(dst), view(src), a);
   return launched("pga_copy_box");

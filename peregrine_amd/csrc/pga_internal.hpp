@@ -67,6 +67,45 @@ inline bool covers(const pga_view &v, const pga_box &b, int ncomp) {
   return covers(v, b, z, z, ncomp);
 }
 
+// Where a placed box reads its source (exchange/collection.py:55-96, the neighbour's window with
+// its axes remapped): the destination cell start + (a0, a1, a2) takes the source cell
+// s + sum_m a_m * (+-1 along source axis A_m), where orient[m] (1..6) names A_m (1, 2, 3 for
+// +i, +j, +k; 4, 5, 6 for -i, -j, -k), the interface orientation digits of the reference; null
+// is 1 2 3. As element offsets: base, the source cell s, and the step per destination axis.
+struct Placement {
+  int64_t base, step[3];
+};
+
+// false (with the error set) when orient is not three distinct axes or the source cells the box
+// maps to reach outside src
+inline bool placement(const char *who, const pga_view &src, const int32_t *s, const int32_t *orient,
+                      const int32_t *ext, int ncomp, Placement &p) {
+  int lo[3] = {0, 0, 0}, n[3] = {0, 0, 0};
+  bool seen[3] = {false, false, false};
+  const int64_t stride[3] = {1, src.sj, src.sk};
+  for (int m = 0; m < 3; m++) {
+    const int code = orient ? orient[m] : m + 1;
+    if (code < 1 || code > 6 || seen[(code - 1) % 3]) {
+      set_error("%s: orientation digits %d %d %d do not name three distinct axes", who, orient ? orient[0] : 1,
+                orient ? orient[1] : 2, orient ? orient[2] : 3);
+      return false;
+    }
+    const int a = (code - 1) % 3;
+    const bool back = code > 3;
+    seen[a] = true;
+    n[a] = ext[m];
+    lo[a] = back ? s[a] - (ext[m] - 1) : s[a];
+    p.step[m] = back ? -stride[a] : stride[a];
+  }
+  const pga_box bs{{lo[0], lo[1], lo[2]}, {n[0], n[1], n[2]}};
+  if (!covers(src, bs, ncomp)) {
+    set_error("%s: the source cells reach outside their array", who);
+    return false;
+  }
+  p.base = s[0] + s[1] * src.sj + s[2] * src.sk;
+  return true;
+}
+
 #define PGA_REQUIRE(cond, ...)      \
   do {                              \
     if (!(cond)) {                  \

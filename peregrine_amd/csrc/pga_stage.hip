@@ -147,8 +147,9 @@ __global__ __launch_bounds__(256) void k_rates(const double *__restrict__ K, con
 // over the slab (its T guess from the state of the previous stage's halo)
 struct HaloTask {
   View Qdst, Qsrc;
-  int d0, d1, d2, s0, s1, s2, e0, e1;
-  int64_t first, cells;  // this slab's first flattened cell index and its cell count
+  int d0, d1, d2, e0, e1;
+  int64_t sbase, st0, st1, st2;  // the source cell of the slab's first cell, the steps of its axes
+  int64_t first, cells;          // this slab's first flattened cell index and its cell count
 };
 
 struct HaloArgs {
@@ -168,7 +169,7 @@ __global__ __launch_bounds__(256) void k_halo(HaloArgs a) {
   const int64_t r = c - t.first;
   if (r >= t.cells) return;
   const int ii = (int)(r % t.e0), jj = (int)((r / t.e0) % t.e1), kk = (int)(r / ((int64_t)t.e0 * t.e1));
-  const int64_t od = t.Qdst.off(t.d0 + ii, t.d1 + jj, t.d2 + kk), os = t.Qsrc.off(t.s0 + ii, t.s1 + jj, t.s2 + kk);
+  const int64_t od = t.Qdst.off(t.d0 + ii, t.d1 + jj, t.d2 + kk), os = t.sbase + ii * t.st0 + jj * t.st1 + kk * t.st2;
   double Q[NS + 4];
 #pragma unroll
   for (int l = 0; l < NS + 4; l++) {
@@ -279,8 +280,8 @@ extern "C" int32_t pga_rates(const pga_case *c, pga_view Q, pga_view state, pga_
 }
 
 extern "C" int32_t pga_halo(const pga_case *c, int32_t ntask, const pga_view *Qdst, const pga_view *Qsrc,
-                            const int32_t *dst_start, const int32_t *src_start, const int32_t *extent, pga_view stdst,
-                            pga_view stguess, void *stream) {
+                            const int32_t *dst_start, const int32_t *src_start, const int32_t *extent,
+                            const int32_t *orient, pga_view stdst, pga_view stguess, void *stream) {
   if (!c || ntask < 0 || ntask > 6 || (ntask && (!Qdst || !Qsrc || !dst_start || !src_start || !extent))) {
     set_error("pga_halo: null argument or more than 6 slabs");
     return PGA_EINVAL;
@@ -293,14 +294,19 @@ extern "C" int32_t pga_halo(const pga_case *c, int32_t ntask, const pga_view *Qd
   int64_t total = 0;
   for (int n = 0; n < ntask; n++) {
     const int *ds = dst_start + 3 * n, *ss = src_start + 3 * n, *e = extent + 3 * n;
-    const pga_box bd{{ds[0], ds[1], ds[2]}, {e[0], e[1], e[2]}}, bs{{ss[0], ss[1], ss[2]}, {e[0], e[1], e[2]}};
-    PGA_REQUIRE(box_ok(bd) && box_ok(bs) && covers(Qdst[n], bd, ne) && covers(Qsrc[n], bs, ne) &&
-                    covers(stdst, bd, nstate(c->desc)) && covers(stguess, bd, nstate(c->desc)),
+    const pga_box bd{{ds[0], ds[1], ds[2]}, {e[0], e[1], e[2]}};
+    PGA_REQUIRE(box_ok(bd) && covers(Qdst[n], bd, ne) && covers(stdst, bd, nstate(c->desc)) &&
+                    covers(stguess, bd, nstate(c->desc)),
                 "pga_halo: slab %d reaches outside its arrays", n);
     PGA_REQUIRE(Qdst[n].sj == stdst.sj && Qdst[n].sk == stdst.sk, "pga_halo: Q and state must share their layout");
+    for (int m = 0; m < 3; m++)
+      PGA_REQUIRE(ss[m] >= 0 && e[m] >= 0, "pga_halo: slab %d has a negative source start or extent", n);
+    Placement pl;
+    if (!placement("pga_halo", Qsrc[n], ss, orient ? orient + 3 * n : nullptr, e, ne, pl)) return PGA_ERANGE;
     HaloTask &t = a.t[n];
     t.Qdst = view(Qdst[n]), t.Qsrc = view(Qsrc[n]);
-    t.d0 = ds[0], t.d1 = ds[1], t.d2 = ds[2], t.s0 = ss[0], t.s1 = ss[1], t.s2 = ss[2];
+    t.d0 = ds[0], t.d1 = ds[1], t.d2 = ds[2];
+    t.sbase = pl.base, t.st0 = pl.step[0], t.st1 = pl.step[1], t.st2 = pl.step[2];
     t.e0 = e[0], t.e1 = e[1];
     t.first = total;
     t.cells = (int64_t)e[0] * e[1] * e[2];

@@ -12,18 +12,55 @@ import numpy as np
 
 
 class Interface:
+    """One of a block's six interfaces. Facing a neighbour it carries the orientation: digit m
+    (of three) names the neighbour's axis our axis m runs along, 1 2 3 for +i +j +k and 4 5 6 for
+    -i -j -k (``multiblock/interface/base.py:95-136``)."""
+
     def __init__(self, nface):
         self.nface = nface
-        self.axis = (nface - 1) // 2
-        self.is_low = nface % 2 == 1
         self.nbr = None
+        self.orient = None
         self.periodic = False
         self.xlate = (0.0, 0.0, 0.0)
         self.bcname = None
 
     @property
+    def axis(self):
+        return (self.nface - 1) // 2
+
+    @property
+    def is_low(self):
+        return self.nface % 2 == 1
+
+    @staticmethod
+    def signedaxis(code):
+        n = int(code)
+        return (n - 1) % 3, n > 3
+
+    @staticmethod
+    def orientcode(axis, flip):
+        return str(axis + 1 + (3 if flip else 0))
+
+    def connect(self, nbr, orient="123"):
+        self.nbr, self.orient, self.bcname = nbr, orient, None
+
+    @property
     def nbrnface(self):
-        return self.nface + 1 if self.is_low else self.nface - 1
+        """The neighbour's interface this one meets: along the neighbour's axis our normal runs
+        along, its high end for our low one unless that axis runs backwards."""
+        axis, flip = self.signedaxis(self.orient[self.axis])
+        return 2 * axis + (1 if self.is_low == flip else 2)
+
+    @property
+    def nbrorient(self):
+        """The neighbour's orientation digits back to us (the inverse signed permutation)."""
+        out = [None] * 3
+        for ours, code in enumerate(self.orient):
+            theirs, flip = self.signedaxis(code)
+            out[theirs] = self.orientcode(ours, flip)
+        if None in out:
+            raise ValueError(f"orientation {self.orient!r} does not name three distinct axes")
+        return "".join(out)
 
 
 class BaseMesher:
@@ -63,10 +100,10 @@ class BaseMesher:
             for nface, end, step in ((2 * axis + 1, 0, -1), (2 * axis + 2, n - 1, 1)):
                 face = blk.intface(nface)
                 if index != end:
-                    face.nbr = nblki + step * stride
+                    face.connect(nblki + step * stride)
                 elif wraps:
                     face.periodic = True
-                    face.nbr = nblki - step * stride * (n - 1)
+                    face.connect(nblki - step * stride * (n - 1))
                     span = -self.lengths[axis] if face.is_low else self.lengths[axis]
                     face.xlate = tuple(span if m == axis else 0.0 for m in range(3))
                 else:

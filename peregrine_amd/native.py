@@ -75,12 +75,13 @@ SIGNATURES = {
     "pga_combine": (_i32, [_i32, View, _i32, C.c_double, _i32, C.POINTER(View), _dp, C.c_void_p, _i32, Box,
                            C.c_void_p]),
     "pga_fill_periodic": (_i32, [View, _i32, _ip, _i32, _i32, _i32, C.c_void_p]),
-    "pga_copy_box": (_i32, [View, _ip, View, _ip, _ip, _i32, C.c_void_p]),
+    "pga_copy_box": (_i32, [View, _ip, View, _ip, _ip, _i32, _ip, C.c_void_p]),
     "pga_stage_supported": (_i32, [C.c_void_p]),
     "pga_stage": (_i32, [C.c_void_p, View, View, View, View, View, View, View, View, View, View, _i32, _ip, _dp, _ip,
                          C.c_void_p, Box, _i32, C.c_void_p]),
     "pga_rates": (_i32, [C.c_void_p, View, View, View, Box, C.c_void_p]),
-    "pga_halo": (_i32, [C.c_void_p, _i32, C.POINTER(View), C.POINTER(View), _ip, _ip, _ip, View, View, C.c_void_p]),
+    "pga_halo": (_i32, [C.c_void_p, _i32, C.POINTER(View), C.POINTER(View), _ip, _ip, _ip, _ip, View, View,
+                        C.c_void_p]),
     "pga_transport": (_i32, [C.c_void_p, View, View, View, View, C.POINTER(Box), _i32, C.c_void_p]),
     "pga_grads": (_i32, [C.c_void_p, View, View, View, View, Box, C.c_void_p]),
     "pga_diffflux": (_i32, [C.c_void_p, _i32, C.c_double, View, View, View, View, View, View, View, View, View, Box,
@@ -104,6 +105,10 @@ class NativeError(RuntimeError):
 _lib = None
 
 
+# the library interface these bindings are written against (PGA_ABI_VERSION)
+ABI_VERSION = 2
+
+
 def lib():
     """The loaded library; raises when it is not built."""
     global _lib
@@ -114,6 +119,9 @@ def lib():
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(handle, name)
             fn.restype, fn.argtypes = res, args
+        if handle.pga_abi_version() != ABI_VERSION:
+            raise NativeError(f"{LIBPATH} has ABI version {handle.pga_abi_version()}, these bindings {ABI_VERSION}: "
+                              "rebuild it")
         _lib = handle
     return _lib
 

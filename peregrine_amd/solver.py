@@ -134,6 +134,34 @@ class Block:
                 xf.set(grid.facecenters(nodes, a))
 
 
+def placement(blk, f, nbr, depth):
+    """(dst_start, src_start, extent, orientation digits) of the halo slab `depth` deep across
+    interface f of blk from nbr's interior (``exchange/collection.py:55-96``): the slab's first
+    cell x (interior-relative: -depth or n along the normal, 0 along the plane) is the neighbour's
+    cell y, along each of its axes A_m = digit m's axis: y = x_m, or n_A - 1 - x_m when it runs
+    backwards, in the plane; and across the interface the neighbour's cells next to the face it
+    meets, counted outward from it as ours are. The device steps from there along the digits."""
+    ng = blk.ng
+    a, n, N = f.axis, blk.ncell, nbr.ncell
+    ext = list(n)
+    ext[a] = depth
+    dst = [ng] * 3
+    dst[a] = ng - depth if f.is_low else ng + n[a]
+    x = [0, 0, 0]
+    x[a] = -depth if f.is_low else n[a]
+    src = [0, 0, 0]
+    for m in range(3):
+        A, back = f.signedaxis(f.orient[m])
+        if m != a:
+            y = N[A] - 1 - x[m] if back else x[m]
+        elif f.is_low:
+            y = -1 - x[m] if back else x[m] + N[A]
+        else:
+            y = N[A] - 1 - (x[m] - n[a]) if back else x[m] - n[a]
+        src[A] = ng + y
+    return dst, src, ext, [int(c) for c in f.orient]
+
+
 def stage_slabs(nk, n):
     """The k-slabs (k0, k1, f0, f1) a fused stage runs a block of nk cells along k in, n of them
     (at most nk): the stage of cells k0 <= k < k1 follows the fluxes it reads and writes Q and the
@@ -230,6 +258,25 @@ class Solver:
                                      f"{sorted(self.cfg['bcvals'])}")
                 wall = self._wall(where, f.bcname, dict(self.cfg["bcvals"][f.bcname]))
                 blk.walls.append((f.axis, 0 if f.is_low else 1, wall))
+            # a neighbour's interface meets this one back, with the inverse orientation, over a plane
+            # of the same cells
+            for f in blk.faces.values():
+                if f.nbr is None:
+                    continue
+                where = f"block {blk.nblki} interface {f.nface}"
+                if not 0 <= f.nbr < len(self.blocks):
+                    raise ValueError(f"{where} faces block {f.nbr}, which the grid does not have")
+                nbr = self.blocks[f.nbr]
+                back = nbr.faces[f.nbrnface]
+                if back.nbr != blk.nblki or back.orient != f.nbrorient:
+                    raise ValueError(f"{where} (orientation {f.orient}) meets block {nbr.nblki} interface "
+                                     f"{f.nbrnface}, which faces block {back.nbr} with orientation {back.orient}, "
+                                     f"not this one with {f.nbrorient}")
+                for m in range(3):
+                    A = f.signedaxis(f.orient[m])[0]
+                    if m != f.axis and blk.ncell[m] != nbr.ncell[A]:
+                        raise ValueError(f"{where}: blocks {blk.nblki} and {nbr.nblki} do not match across it "
+                                         f"({blk.ncell} and {nbr.ncell} cells, orientation {f.orient})")
             # an exchanged interface copies ng interior planes; a wall reflects however thin the block
             for f in blk.faces.values():
                 if f.nbr is not None and blk.ncell[f.axis] < self.ng:
@@ -309,27 +356,14 @@ class Solver:
         self._exchange(lambda b: b.regs[reg], self.sim.ne, self.ng)
 
     def _exchange(self, array, ncomp, depth):
-        ng = self.ng
         for blk in self.blocks:
             for f in blk.faces.values():
                 if f.nbr is None:
                     continue
                 nbr = self.blocks[f.nbr]
-                a = f.axis
-                ext = list(blk.ncell)
-                ext[a] = depth
-                dst = [ng] * 3
-                src = [ng] * 3
-                if f.is_low:
-                    dst[a] = ng - depth
-                    src[a] = ng + nbr.ncell[a] - depth
-                else:
-                    dst[a] = ng + blk.ncell[a]
-                    src[a] = ng
-                if list(nbr.ncell[m] for m in range(3) if m != a) != list(blk.ncell[m] for m in range(3) if m != a):
-                    raise ValueError(f"blocks {blk.nblki} and {nbr.nblki} do not match across interface {f.nface}")
+                dst, src, ext, orient = placement(blk, f, nbr, depth)
                 self._call("pga_copy_box", array(blk).view(), native.ints(dst), array(nbr).view(),
-                           native.ints(src), native.ints(ext), ncomp)
+                           native.ints(src), native.ints(ext), ncomp, native.ints(orient))
 
     def transport(self, Q):
         """The transport state and species enthalpies over the interior and the first halo layer
@@ -477,27 +511,23 @@ class Solver:
     # -- the fused stage ------------------------------------------------------
     def _halo_pass(self, qin):
         """Each block's halo slabs from its partners' interior planes, and the EOS over them."""
-        ng = self.ng
         for blk in self.blocks:
-            dsts, srcs, ds, ss, es = [], [], [], [], []
+            dsts, srcs, ds, ss, es, os = [], [], [], [], [], []
             for f in blk.faces.values():
                 if f.nbr is None:
                     continue
                 nbr = self.blocks[f.nbr]
-                a = f.axis
-                ext = list(blk.ncell)
-                ext[a] = ng
-                dst, src = [ng] * 3, [ng] * 3
-                dst[a] = 0 if f.is_low else ng + blk.ncell[a]
-                src[a] = nbr.ncell[a] if f.is_low else ng
+                dst, src, ext, orient = placement(blk, f, nbr, self.ng)
                 dsts.append(blk.regs[qin].view())
                 srcs.append(nbr.regs[qin].view())
                 ds += dst
                 ss += src
                 es += ext
+                os += orient
             n = len(dsts)
             self._call("pga_halo", self.case.handle, n, (native.View * n)(*dsts), (native.View * n)(*srcs),
-                       native.ints(ds), native.ints(ss), native.ints(es), blk.state.view(), blk.state.view())
+                       native.ints(ds), native.ints(ss), native.ints(es), native.ints(os), blk.state.view(),
+                       blk.state.view())
 
     def _combine_terms(self, n):
         """The stage's combine as (source, weight, times dt) terms in the reference's order: its

@@ -238,3 +238,66 @@ def reacting_flow(x, n, L):
     q[..., 2] = -40.0 * np.cos(X) * np.sin(Y)
     q[..., 3] = 20.0 * np.sin(Z)
     return q
+
+
+# --- interface orientations (tests/communication/test_orientation.py:13-60) ------------------
+
+def orientations():
+    """The 24 signed axis permutations a right-handed grid can meet another in: digit m names the
+    neighbour's axis ours runs along, 1 2 3 forward and 4 5 6 backward."""
+    import itertools
+
+    out = []
+    for axes in itertools.permutations(range(3)):
+        for signs in itertools.product((1, -1), repeat=3):
+            P = np.zeros((3, 3))
+            for m, (a, s) in enumerate(zip(axes, signs)):
+                P[m, a] = s
+            if np.linalg.det(P) > 0:
+                out.append("".join(str(a + 1 + (3 if s < 0 else 0)) for a, s in zip(axes, signs)))
+    assert len(out) == 24
+    return out
+
+
+def signedperm(S):
+    return tuple((int(d) - 1) % 3 for d in S), tuple(int(d) > 3 for d in S)
+
+
+def reorient(a, S):
+    """An array (i, j, k, ...) in the first block's layout as a block met with digits S holds it."""
+    axes, flips = signedperm(S)
+    for m, f in enumerate(flips):
+        if f:
+            a = np.flip(a, axis=m)
+    return np.moveaxis(a, (0, 1, 2), axes)
+
+
+class ReorientedPair:
+    """A cube lattice whose block 1 the partitioner relabels so that block 0 meets it with digits S."""
+
+    def __init__(self, S, **kwargs):
+        from peregrine_amd.mesher import CubeMesher
+
+        self.S, self.cube = S, CubeMesher(**kwargs)
+
+    def __getattr__(self, name):
+        return getattr(self.cube, name)
+
+    def fill(self, mb):
+        from peregrine_amd.partition import Conditioner
+
+        self.cube.fill(mb)
+        axes, flips = signedperm(self.S)
+        # the partitioner's form: which old axis each new one holds, and which new ones run backwards
+        perm = [axes.index(m) for m in range(3)]
+        Conditioner().reorient(mb, mb.blocks[1], perm, [flips[perm[m]] for m in range(3)])
+
+
+def reorient_back(a, S):
+    """The inverse of reorient: a block met with digits S's array in the first block's layout."""
+    axes, flips = signedperm(S)
+    a = np.moveaxis(a, axes, (0, 1, 2))
+    for m, f in enumerate(flips):
+        if f:
+            a = np.flip(a, axis=m)
+    return a

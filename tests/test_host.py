@@ -92,7 +92,7 @@ def test_the_library_exports_every_declared_symbol():
     lib = ctypes.CDLL(str(native.LIBPATH))
     for name in names:
         assert hasattr(lib, name), name
-    assert native.lib().pga_abi_version() == 1
+    assert native.lib().pga_abi_version() == native.ABI_VERSION == 2
 
 
 def test_the_constant_layout():
@@ -184,3 +184,69 @@ def test_stage_slabs_cover_each_cell_and_face_once_before_it_changes(nk, n):
         assert {k0, k1} <= done
         for k in range(k0, k1):
             assert set(range(max(0, k - 1), min(nk, k + 2) + 1)) <= done
+
+
+# --- interface orientations (multiblock/interface/base.py:95-136, partition/orient.py:14-70,
+# tests/communication/test_orientation.py) -------------------------------------------------
+
+class _Lattice:
+    """The multiblock a mesher fills, without device arrays."""
+
+    def __init__(self, ng):
+        self.blocks, self.ng = [], ng
+
+    def add_blk(self):
+        from peregrine_amd.solver import Block
+
+        blk = Block(len(self.blocks), self)
+        self.blocks.append(blk)
+        return blk
+
+
+def _reoriented_pair(S, ng):
+    mb = _Lattice(ng)
+    sp.ReorientedPair(S, mbdims=[2, 1, 1], blkdims=[6, 4, 3], lengths=[2.0, 1.3, 0.7], defaultbc="wall").fill(mb)
+    return mb
+
+
+@pytest.mark.parametrize("S", sp.orientations())
+def test_a_reoriented_neighbour_places_each_halo_cell_where_it_lies(S):
+    """Block 1 of a pair relabelled by each of the 24 orientations: the interfaces meet back with
+    inverse digits, and every halo cell the exchange places (ng = 2 and 1 deep) is the neighbour's
+    cell at the same place: the halo's centre from its own extrapolated nodes is the source cell's
+    centre (the mapping stepped as the device steps it)."""
+    from peregrine_amd import grid
+    from peregrine_amd.solver import placement
+
+    ng = 2
+    mb = _reoriented_pair(S, ng)
+    b0, b1 = mb.blocks
+    f0 = next(f for f in b0.faces.values() if f.nbr == 1)
+    f1 = next(f for f in b1.faces.values() if f.nbr == 0)
+    assert f0.orient == S and f1.orient == f0.nbrorient and f0.nbrnface == f1.nface and f1.nbrnface == f0.nface
+    centres = [grid.cellcenters(grid.halo_nodes(b.interior_nodes, ng, [False] * 3, b.period)) for b in mb.blocks]
+    for blk, f, nbr in ((b0, f0, b1), (b1, f1, b0)):
+        for depth in (ng, 1):
+            dst, src, ext, orient = placement(blk, f, nbr, depth)
+            a = np.stack(np.meshgrid(*map(np.arange, ext), indexing="ij"), axis=-1)
+            s = np.broadcast_to(np.array(src), a.shape).copy()
+            for m, code in enumerate(orient):
+                A, back = (code - 1) % 3, code > 3
+                s[..., A] += -a[..., m] if back else a[..., m]
+            d = a + np.array(dst)
+            got = centres[blk.nblki][d[..., 0], d[..., 1], d[..., 2]]
+            want = centres[nbr.nblki][s[..., 0], s[..., 1], s[..., 2]]
+            assert np.allclose(got, want, rtol=0, atol=1e-12), (S, blk.nblki, depth)
+            # and the source cells are interior cells of the neighbour
+            for A in range(3):
+                assert s[..., A].min() >= ng and s[..., A].max() < ng + nbr.ncell[A]
+
+
+def test_a_block_joined_to_itself_is_not_reoriented():
+    from peregrine_amd import mesher
+    from peregrine_amd.partition import Conditioner
+
+    mb = _Lattice(2)
+    mesher.CubeMesher(blkdims=[5, 5, 5], periodic=[True, False, False], defaultbc="wall").fill(mb)
+    with pytest.raises(ValueError, match="joined to itself"):
+        Conditioner().reorient(mb, mb.blocks[0], [1, 0, 2], [False, False, True])
