@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define PGA_ABI_VERSION 2
+#define PGA_ABI_VERSION 3
 
 enum {
   PGA_OK = 0,
@@ -137,12 +137,16 @@ int32_t pga_fill_periodic(pga_view A, int32_t ncomp, const int32_t *ncell, int32
  * slab of a block interface from its partner's interior planes. orient (3 digits, or null for
  * 1 2 3) is the interface's orientation: digit m names the source axis the destination's axis m
  * runs along, 1 2 3 for +i +j +k and 4 5 6 for -i -j -k; the cell dst_start + (a0, a1, a2) takes
- * the source cell src_start + sum_m a_m (+-1 along that axis). Replaces pg_place on an
- * unrotated interface: place.mako:11-37 with
+ * the source cell src_start + sum_m a_m (+-1 along that axis). rot (9 doubles, row-major, or
+ * null) is a rotational periodic's turn: the components [vec, vec + 3) are turned by it; with
+ * rows = 1 the components are rows of three (a gradient each), every row is turned, and the rows
+ * [vec, vec + 3) (a velocity gradient) are turned down each column too (vec -1: none). An exact
+ * identity rot copies as null does. Replaces pg_place: place.mako:3-37 with
  * peregrine/multiblock/collections/exchange/collection.py:55-96 (haloview, nbrview, remap_across)
  * and peregrine/multiblock/interface/base.py:37-45,95-136 (nbrplane, signedaxis, nbrnface). */
 int32_t pga_copy_box(pga_view dst, const int32_t *dst_start, pga_view src, const int32_t *src_start,
-                     const int32_t *extent, int32_t ncomp, const int32_t *orient, void *stream);
+                     const int32_t *extent, int32_t ncomp, const int32_t *orient, const double *rot, int32_t vec,
+                     int32_t rows, void *stream);
 
 /* 1 when the fused stage is compiled for the case's species count, else 0 (use the per-kernel
  * entry points) */
@@ -172,12 +176,13 @@ int32_t pga_rates(const pga_case *c, pga_view Q, pga_view state, pga_view rates,
 
 /* The halo pass of a stage: for up to 6 slabs, Q copied from src to dst (at dst_start), extent
  * cells each, the source cells placed by src_start and the slab's 3 orientation digits in orient
- * (null: all aligned) as in pga_copy_box, then the EOS over the slab into stdst, its Newton
+ * (null: all aligned) as in pga_copy_box, the momentum turned by the slab's 9 doubles in rot
+ * (null, or an exact identity: not turned), then the EOS over the slab into stdst, its Newton
  * started from stguess's T at the same cells. Replaces the prerhs slot's pg_place and the halo
  * pg_fromQ (simulator/advec/simulator.py:57-64). */
 int32_t pga_halo(const pga_case *c, int32_t ntask, const pga_view *Qdst, const pga_view *Qsrc,
                  const int32_t *dst_start, const int32_t *src_start, const int32_t *extent, const int32_t *orient,
-                 pga_view stdst, pga_view stguess, void *stream);
+                 const double *rot, pga_view stdst, pga_view stguess, void *stream);
 
 /* --- the viscous terms (Navier-Stokes) ---------------------------------------------------- */
 

@@ -328,10 +328,43 @@ __global__ __launch_bounds__(256) void k_copy_box(View dst, View src, CopyArgs a
   const int64_t os = a.sbase + ii * a.st0 + jj * a.st1 + kk * a.st2;
   dst.at(dst.off(a.d0 + ii, a.d1 + jj, a.d2 + kk), comp) = src.at(os, comp);
 }
+
+// the same box across a rotational periodic, a cell per thread: the vector components [vec, vec+3)
+// turned by R; with rows, the components are rows of three (each a gradient, turned), and the rows
+// [vec, vec+3) (a velocity gradient) are turned down each column as well
+__global__ __launch_bounds__(256) void k_copy_turned(View dst, View src, CopyArgs a, Turn t, int vec, int rows) {
+  const int ii = blockIdx.x * TI + threadIdx.x, jj = blockIdx.y * TJ + threadIdx.y, kk = blockIdx.z;
+  if (ii >= a.e0 || jj >= a.e1) return;
+  const int64_t os = a.sbase + ii * a.st0 + jj * a.st1 + kk * a.st2, od = dst.off(a.d0 + ii, a.d1 + jj, a.d2 + kk);
+  if (!rows) {
+    for (int c = 0; c < a.ncomp; c++)
+      if (c < vec || c >= vec + 3) dst.at(od, c) = src.at(os, c);
+    if (vec >= 0)
+      turned(t.R, src.at(os, vec), src.at(os, vec + 1), src.at(os, vec + 2), dst.at(od, vec), dst.at(od, vec + 1),
+             dst.at(od, vec + 2));
+    return;
+  }
+  for (int r = 0; r < a.ncomp / 3; r++)
+    if (r < vec || r >= vec + 3)
+      turned(t.R, src.at(os, 3 * r), src.at(os, 3 * r + 1), src.at(os, 3 * r + 2), dst.at(od, 3 * r),
+             dst.at(od, 3 * r + 1), dst.at(od, 3 * r + 2));
+  if (vec < 0) return;
+  double H[3][3];
+#pragma unroll
+  for (int r = 0; r < 3; r++) {
+    const int c = 3 * (vec + r);
+    turned(t.R, src.at(os, c), src.at(os, c + 1), src.at(os, c + 2), H[r][0], H[r][1], H[r][2]);
+  }
+#pragma unroll
+  for (int e = 0; e < 3; e++)
+    turned(t.R, H[0][e], H[1][e], H[2][e], dst.at(od, 3 * vec + e), dst.at(od, 3 * (vec + 1) + e),
+           dst.at(od, 3 * (vec + 2) + e));
+}
 }  // namespace pga
 
 extern "C" int32_t pga_copy_box(pga_view dst, const int32_t *dst_start, pga_view src, const int32_t *src_start,
-                                const int32_t *extent, int32_t ncomp, const int32_t *orient, void *stream) {
+                                const int32_t *extent, int32_t ncomp, const int32_t *orient, const double *rot,
+                                int32_t vec, int32_t rows, void *stream) {
   if (!dst_start || !src_start || !extent || ncomp <= 0) {
     set_error("pga_copy_box: null argument or no components");
     return PGA_EINVAL;
@@ -354,6 +387,17 @@ extern "C" int32_t pga_copy_box(pga_view dst, const int32_t *dst_start, pga_view
   }
   const CopyArgs a{dst_start[0], dst_start[1], dst_start[2], extent[0], extent[1], extent[2], ncomp,
                    pl.base,      pl.step[0],   pl.step[1],   pl.step[2]};
+  const Turn t = turn_of(rot);
+  if (t.on) {
+    const int units = rows ? ncomp / 3 : ncomp;
+    PGA_REQUIRE((rows == 0 || rows == 1) && (!rows || ncomp % 3 == 0) && vec >= -1 && vec + 3 <= units,
+                "pga_copy_box: a turned copy's vector (%d) must lie within its %d %s", vec, units,
+                rows ? "rows" : "components");
+    const dim3 tgrid((extent[0] + TI - 1) / TI, (extent[1] + TJ - 1) / TJ, extent[2]);
+    hipLaunchKernelGGL(k_copy_turned, tgrid, dim3(TI, TJ), 0, stream_of(stream), view(dst), view(src), a, t, vec,
+                       rows);
+    return launched("pga_copy_box");
+  }
   const dim3 grid((extent[0] + TI - 1) / TI, (extent[1] + TJ - 1) / TJ, extent[2] * ncomp);
   hipLaunchKernelGGL(k_copy_box, grid, dim3(TI, TJ), 0, stream_of(stream), view(dst), view(src), a);
   return launched("pga_copy_box");

@@ -143,6 +143,15 @@ struct View {
 
 __host__ inline View view(const pga_view &v) { return View{v.data, v.sj, v.sk, v.sc}; }
 
+// (x, y, z) turned by the row-major rotation R, each row's sum in the written order
+__device__ __forceinline__ void turned(const double *R, double x, double y, double z, double &ox, double &oy,
+                                       double &oz) {
+#pragma clang fp contract(off)
+  ox = R[0] * x + R[1] * y + R[2] * z;
+  oy = R[3] * x + R[4] * y + R[5] * z;
+  oz = R[6] * x + R[7] * y + R[8] * z;
+}
+
 // cells along i map to the 64 lanes of a wave, 4 waves step along j; blocks tile j and k
 constexpr int TI = 64;
 constexpr int TJ = 4;

@@ -106,6 +106,24 @@ inline bool placement(const char *who, const pga_view &src, const int32_t *s, co
   return true;
 }
 
+// A rotational periodic's turn of the vectors that cross it (place.mako:3-37): R row-major; a
+// task turns only when R is given and is not exactly the identity, so an unturned copy stays a copy
+// to the bit
+struct Turn {
+  double R[9];
+  int on;
+};
+
+inline Turn turn_of(const double *R) {
+  Turn t{};
+  if (!R) return t;
+  for (int n = 0; n < 9; n++) {
+    t.R[n] = R[n];
+    if (R[n] != (n % 4 == 0 ? 1.0 : 0.0)) t.on = 1;
+  }
+  return t;
+}
+
 #define PGA_REQUIRE(cond, ...)      \
   do {                              \
     if (!(cond)) {                  \

@@ -150,6 +150,7 @@ struct HaloTask {
   int d0, d1, d2, e0, e1;
   int64_t sbase, st0, st1, st2;  // the source cell of the slab's first cell, the steps of its axes
   int64_t first, cells;          // this slab's first flattened cell index and its cell count
+  Turn turn;                     // a rotational periodic's turn of the momentum
 };
 
 struct HaloArgs {
@@ -172,10 +173,13 @@ __global__ __launch_bounds__(256) void k_halo(HaloArgs a) {
   const int64_t od = t.Qdst.off(t.d0 + ii, t.d1 + jj, t.d2 + kk), os = t.sbase + ii * t.st0 + jj * t.st1 + kk * t.st2;
   double Q[NS + 4];
 #pragma unroll
-  for (int l = 0; l < NS + 4; l++) {
-    Q[l] = t.Qsrc.at(os, l);
-    t.Qdst.at(od, l) = Q[l];
+  for (int l = 0; l < NS + 4; l++) Q[l] = t.Qsrc.at(os, l);
+  if (t.turn.on) {
+    const double x = Q[1], y = Q[2], z = Q[3];
+    turned(t.turn.R, x, y, z, Q[1], Q[2], Q[3]);
   }
+#pragma unroll
+  for (int l = 0; l < NS + 4; l++) t.Qdst.at(od, l) = Q[l];
   const Gas<NS> g{a.K};
   const int64_t ost = a.stdst.off(t.d0 + ii, t.d1 + jj, t.d2 + kk);
   EosState s;
@@ -281,7 +285,8 @@ extern "C" int32_t pga_rates(const pga_case *c, pga_view Q, pga_view state, pga_
 
 extern "C" int32_t pga_halo(const pga_case *c, int32_t ntask, const pga_view *Qdst, const pga_view *Qsrc,
                             const int32_t *dst_start, const int32_t *src_start, const int32_t *extent,
-                            const int32_t *orient, pga_view stdst, pga_view stguess, void *stream) {
+                            const int32_t *orient, const double *rot, pga_view stdst, pga_view stguess,
+                            void *stream) {
   if (!c || ntask < 0 || ntask > 6 || (ntask && (!Qdst || !Qsrc || !dst_start || !src_start || !extent))) {
     set_error("pga_halo: null argument or more than 6 slabs");
     return PGA_EINVAL;
@@ -307,6 +312,7 @@ extern "C" int32_t pga_halo(const pga_case *c, int32_t ntask, const pga_view *Qd
     t.Qdst = view(Qdst[n]), t.Qsrc = view(Qsrc[n]);
     t.d0 = ds[0], t.d1 = ds[1], t.d2 = ds[2];
     t.sbase = pl.base, t.st0 = pl.step[0], t.st1 = pl.step[1], t.st2 = pl.step[2];
+    t.turn = turn_of(rot ? rot + 9 * n : nullptr);
     t.e0 = e[0], t.e1 = e[1];
     t.first = total;
     t.cells = (int64_t)e[0] * e[1] * e[2];

@@ -75,12 +75,12 @@ SIGNATURES = {
     "pga_combine": (_i32, [_i32, View, _i32, C.c_double, _i32, C.POINTER(View), _dp, C.c_void_p, _i32, Box,
                            C.c_void_p]),
     "pga_fill_periodic": (_i32, [View, _i32, _ip, _i32, _i32, _i32, C.c_void_p]),
-    "pga_copy_box": (_i32, [View, _ip, View, _ip, _ip, _i32, _ip, C.c_void_p]),
+    "pga_copy_box": (_i32, [View, _ip, View, _ip, _ip, _i32, _ip, _dp, _i32, _i32, C.c_void_p]),
     "pga_stage_supported": (_i32, [C.c_void_p]),
     "pga_stage": (_i32, [C.c_void_p, View, View, View, View, View, View, View, View, View, View, _i32, _ip, _dp, _ip,
                          C.c_void_p, Box, _i32, C.c_void_p]),
     "pga_rates": (_i32, [C.c_void_p, View, View, View, Box, C.c_void_p]),
-    "pga_halo": (_i32, [C.c_void_p, _i32, C.POINTER(View), C.POINTER(View), _ip, _ip, _ip, _ip, View, View,
+    "pga_halo": (_i32, [C.c_void_p, _i32, C.POINTER(View), C.POINTER(View), _ip, _ip, _ip, _ip, _dp, View, View,
                         C.c_void_p]),
     "pga_transport": (_i32, [C.c_void_p, View, View, View, View, C.POINTER(Box), _i32, C.c_void_p]),
     "pga_grads": (_i32, [C.c_void_p, View, View, View, View, Box, C.c_void_p]),
@@ -106,7 +106,7 @@ _lib = None
 
 
 # the library interface these bindings are written against (PGA_ABI_VERSION)
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 
 def lib():
@@ -138,6 +138,10 @@ def box(start, extent):
 
 def ints(values):
     return (_i32 * len(values))(*values)
+
+
+def dbls(values):
+    return (C.c_double * len(values))(*values)
 
 
 def mechanisms():

@@ -118,9 +118,9 @@ class Block:
     def Q(self):
         return self.regs[0]
 
-    def make_metrics(self, periodic):
+    def make_metrics(self, nodes):
+        """The metrics of the block's nodes with their halo layers."""
         ng = self.ng
-        nodes = grid.halo_nodes(self.interior_nodes, ng, periodic, self.period)
         self.nodes = nodes
         self.xcc.set(grid.cellcenters(nodes))
         for a, S in enumerate((self.iS, self.jS, self.kS)):
@@ -132,6 +132,49 @@ class Block:
             self.Jinv.set(Jinv.reshape(self.cellshape + (9,)))
             for a, xf in enumerate(self.xf):
                 xf.set(grid.facecenters(nodes, a))
+
+
+def block_nodes(blocks, ng):
+    """Every block's nodes with ng halo layers (``multiblock/grid.py``, ``gen_halo`` then the node
+    exchange): linear extrapolation first, a translated periodic with itself moved by its period;
+    then across every interface the neighbour's node layers placed through the orientation, turned
+    by a rotational periodic and moved by a translation. Corner and edge halos stay extrapolated."""
+    nodes = {}
+    for blk in blocks:
+        moved = [(lambda f: f.periodic and f.nbr == blk.nblki and f.rot is None)(blk.faces[2 * a + 1])
+                 for a in range(3)]
+        nodes[blk.nblki] = grid.halo_nodes(blk.interior_nodes, ng, moved, blk.period)
+    for blk in blocks:
+        for f in blk.faces.values():
+            if f.nbr is not None:
+                place_halo_nodes(nodes[blk.nblki], blk, f, blocks[f.nbr], ng)
+    return nodes
+
+
+def place_halo_nodes(nodes, blk, f, nbr, ng):
+    """The halo node layers across interface f from nbr's interior nodes: our node x (interior-
+    relative; the layers n_a + 1 .. n_a + ng, or -1 .. -ng, over the plane's nodes 0 .. n_m) is the
+    neighbour's node y, y_A = x_m (or N_A - x_m backwards) in the plane, and across it the
+    neighbour's nodes counted inward from the face it meets."""
+    a, n = f.axis, blk.ncell
+    N = nbr.ncell
+    ranges = [np.arange(n[m] + 1) for m in range(3)]
+    ranges[a] = np.arange(-ng, 0) if f.is_low else np.arange(n[a] + 1, n[a] + ng + 1)
+    x = np.meshgrid(*ranges, indexing="ij")
+    y = [None] * 3
+    for m in range(3):
+        A, back = f.signedaxis(f.orient[m])
+        if m != a:
+            y[A] = N[A] - x[m] if back else x[m]
+        elif f.is_low:
+            y[A] = -x[m] if back else N[A] + x[m]
+        else:
+            y[A] = N[A] - (x[m] - n[a]) if back else x[m] - n[a]
+    values = nbr.interior_nodes[y[0], y[1], y[2]]
+    if f.rot is not None:
+        values = values @ f.rot.T
+    values = values + np.array(f.xlate)
+    nodes[ng + x[0], ng + x[1], ng + x[2]] = values
 
 
 def placement(blk, f, nbr, depth):
@@ -160,6 +203,11 @@ def placement(blk, f, nbr, depth):
             y = N[A] - 1 - (x[m] - n[a]) if back else x[m] - n[a]
         src[A] = ng + y
     return dst, src, ext, [int(c) for c in f.orient]
+
+
+def turn_of(f):
+    """The row-major turn a rotational periodic gives what crosses it, or None."""
+    return native.dbls(f.rot.ravel()) if f.rotperiodic else None
 
 
 def stage_slabs(nk, n):
@@ -195,8 +243,9 @@ class Solver:
         self._check_bcs()
         for blk in self.blocks:
             blk.alloc(backend, sim, integ.nregs)
-            periodic = [blk.faces[2 * a + 1].periodic and blk.faces[2 * a + 1].nbr == blk.nblki for a in range(3)]
-            blk.make_metrics(periodic)
+        nodes = block_nodes(self.blocks, self.ng)
+        for blk in self.blocks:
+            blk.make_metrics(nodes[blk.nblki])
         self.case = NativeCase(sim, backend)
         self.fused = self._fused_supported()
         if self.fused:
@@ -352,10 +401,11 @@ class Solver:
 
     # -- the slots -----------------------------------------------------------
     def exchange(self, reg):
-        """Each interface's halo slab of Q from its partner's interior planes (place, per interface)."""
-        self._exchange(lambda b: b.regs[reg], self.sim.ne, self.ng)
+        """Each interface's halo slab of Q from its partner's interior planes (place, per interface),
+        the momentum turned across a rotational periodic."""
+        self._exchange(lambda b: b.regs[reg], self.sim.ne, self.ng, 1, 0)
 
-    def _exchange(self, array, ncomp, depth):
+    def _exchange(self, array, ncomp, depth, vec, rows):
         for blk in self.blocks:
             for f in blk.faces.values():
                 if f.nbr is None:
@@ -363,7 +413,7 @@ class Solver:
                 nbr = self.blocks[f.nbr]
                 dst, src, ext, orient = placement(blk, f, nbr, depth)
                 self._call("pga_copy_box", array(blk).view(), native.ints(dst), array(nbr).view(),
-                           native.ints(src), native.ints(ext), ncomp, native.ints(orient))
+                           native.ints(src), native.ints(ext), ncomp, native.ints(orient), turn_of(f), vec, rows)
 
     def transport(self, Q):
         """The transport state and species enthalpies over the interior and the first halo layer
@@ -380,7 +430,8 @@ class Solver:
         for blk in self.blocks:
             self._call("pga_grads", self.case.handle, blk.regs[Q].view(), blk.state.view(), blk.Jinv.view(),
                        blk.grads.view(), blk.interior_box())
-        self._exchange(lambda b: b.grads, 3 * self.sim.ngrad, 1)
+        # the rows u, v, w, T, Y: each a gradient, the first three a velocity gradient
+        self._exchange(lambda b: b.grads, 3 * self.sim.ngrad, 1, 0, 1)
 
     def fluxes(self, blk, Qv, st, slab=None):
         """The advective flux of each axis, with the viscous flux added for a viscous case, then the
@@ -512,7 +563,7 @@ class Solver:
     def _halo_pass(self, qin):
         """Each block's halo slabs from its partners' interior planes, and the EOS over them."""
         for blk in self.blocks:
-            dsts, srcs, ds, ss, es, os = [], [], [], [], [], []
+            dsts, srcs, ds, ss, es, os, rots = [], [], [], [], [], [], []
             for f in blk.faces.values():
                 if f.nbr is None:
                     continue
@@ -524,9 +575,11 @@ class Solver:
                 ss += src
                 es += ext
                 os += orient
+                rots += list(f.rot.ravel()) if f.rotperiodic else [1.0, 0, 0, 0, 1.0, 0, 0, 0, 1.0]
             n = len(dsts)
+            turns = native.dbls(rots) if any(f.rotperiodic for f in blk.faces.values() if f.nbr is not None) else None
             self._call("pga_halo", self.case.handle, n, (native.View * n)(*dsts), (native.View * n)(*srcs),
-                       native.ints(ds), native.ints(ss), native.ints(es), native.ints(os), blk.state.view(),
+                       native.ints(ds), native.ints(ss), native.ints(es), native.ints(os), turns, blk.state.view(),
                        blk.state.view())
 
     def _combine_terms(self, n):

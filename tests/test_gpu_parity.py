@@ -248,7 +248,7 @@ def test_halo_copies(dev):
             dst[a] = 0 if low else ng + ncell[a]
             src[a] = ncell[a] if low else ng
             native.check(L.pga_copy_box(dev.view(dB), native.ints(dst), dev.view(dB), native.ints(src), native.ints(ext),
-                                        4, None, None), "pga_copy_box")
+                                        4, None, None, -1, 0, None), "pga_copy_box")
     assert np.array_equal(dev.get(dB), want)
 
 

@@ -92,7 +92,7 @@ def test_the_library_exports_every_declared_symbol():
     lib = ctypes.CDLL(str(native.LIBPATH))
     for name in names:
         assert hasattr(lib, name), name
-    assert native.lib().pga_abi_version() == native.ABI_VERSION == 2
+    assert native.lib().pga_abi_version() == native.ABI_VERSION == 3
 
 
 def test_the_constant_layout():
@@ -250,3 +250,48 @@ def test_a_block_joined_to_itself_is_not_reoriented():
     mesher.CubeMesher(blkdims=[5, 5, 5], periodic=[True, False, False], defaultbc="wall").fill(mb)
     with pytest.raises(ValueError, match="joined to itself"):
         Conditioner().reorient(mb, mb.blocks[0], [1, 0, 2], [False, False, True])
+
+
+# --- rotational periodics (mesher/annulus.py, tests/communication/test_xchg.py:142-213) --------
+
+def _annulus(ng, mbdims=(1, 1, 2), sweep=90.0):
+    from peregrine_amd import mesher
+
+    mb = _Lattice(ng)
+    mesher.AnnulusMesher(mbdims=list(mbdims), blkdims=[4, 4, 4], p1=[0, 0, 0], p2=[1, 0, 0], p3=[0, 0.5, 0],
+                         sweep=sweep, thickness=0.5, periodic=True, defaultbc="wall").fill(mb)
+    return mb
+
+
+@pytest.mark.parametrize("ng", [1, 2])
+def test_a_rotational_periodics_halo_continues_the_sweep(ng):
+    """A 90-degree sector in two blocks of three 15-degree cells: the halo nodes across its ends are
+    the far end's turned by the sweep, so the cell centres step by 15 degrees through halo and
+    interior; the ends turn (y, z) -> (z, -y) coming in at the low end."""
+    from peregrine_amd import grid
+    from peregrine_amd.solver import block_nodes
+
+    mb = _annulus(ng)
+    b0, b1 = mb.blocks
+    low, high = b0.faces[5], b1.faces[6]
+    assert low.rotperiodic and high.rotperiodic and low.nbr == 1 and high.nbr == 0
+    assert np.allclose(low.rot @ [1.0, 2.0, 3.0], [1.0, 3.0, -2.0], atol=1e-15)
+    assert not b0.faces[6].periodic and b0.faces[6].nbr == 1
+    nodes = block_nodes(mb.blocks, ng)
+    for blk, first in ((b0, -7.5 - 15 * (ng - 1)), (b1, 45.0 - 7.5 - 15 * (ng - 1))):
+        x = grid.cellcenters(nodes[blk.nblki])[ng, ng]
+        angles = np.degrees(np.arctan2(x[:, 2], x[:, 1]))
+        want = first + 15 * np.arange(3 + 2 * ng)
+        assert np.allclose(angles, np.where(want > 180, want - 360, want), atol=1e-12)
+
+
+def test_a_full_turn_joins_its_ends_as_an_interior_interface():
+    from peregrine_amd import grid
+    from peregrine_amd.solver import block_nodes
+
+    mb = _annulus(2, mbdims=(1, 1, 4), sweep=360.0)
+    ends = mb.blocks[0].faces[5], mb.blocks[3].faces[6]
+    assert all(f.nbr is not None and not f.periodic and f.rot is None for f in ends)
+    nodes = block_nodes(mb.blocks, 2)
+    x = grid.cellcenters(nodes[0])[2, 2]
+    assert np.allclose(np.degrees(np.arctan2(x[:, 2], x[:, 1])), -45.0 + 30 * np.arange(7), atol=1e-12)
