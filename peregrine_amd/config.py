@@ -67,9 +67,9 @@ _DEFAULTS = {
         "constform": "literal",
     },
     # this package's device: the block shape of the cell and face kernels, whether a stage
-    # runs as the fused pass or as the per-kernel sequence, and the k-slabs a fused stage is
-    # run in (a slab's face arrays stay in the 256 MB Infinity Cache between the flux sweeps
-    # and the divergence that reads them)
+    # runs as the fused pass or as the per-kernel sequence, the k-slabs a fused stage is run
+    # in (a slab's face arrays stay in the 256 MB Infinity Cache between the flux sweeps and
+    # the divergence that reads them), and whether fused steps replay one captured HIP graph
     "backend-mi355x": {
         "device": 0,
         "tile-i": 64,

@@ -631,11 +631,33 @@ class Solver:
                                blk.regs[regs.Qout].view(), st, blk.iF.view(), blk.jF.view(), blk.kF.view(),
                                blk.detJinv.view(), rates, len(terms), src, w, withdt, self.dt.ptr, sbox, kc)
 
+    _graph = None
+
+    def _graphed(self):
+        """A fused step replays one captured HIP graph once its launches are the steady ones: after
+        a first step, with no register's halos left ready by from_prims (that step skips its first
+        halo pass) and no launch timing. Everything a step reads that changes between steps is on
+        the device (Q, the state, dt), so the same graph serves every such step."""
+        return (bool(self.cfg["backend-mi355x"]["graphs"]) and self.nrt > 0 and not self.readied
+                and self.timers is None)
+
+    def _replay(self):
+        if self._graph is None:
+            torch = self.backend.torch
+            graph = torch.cuda.CUDAGraph()
+            # captured on torch's capture stream, which the backend's launches follow; nothing runs
+            with torch.cuda.graph(graph):
+                self._fused_step()
+            self._graph = graph
+        self._graph.replay()
+
     def step(self, dt):
         start = self.tme.get()
         self.dt.set(dt)
         times = self.integ.stagetimes(start, dt)
-        if self.fused:
+        if self.fused and self._graphed():
+            self._replay()
+        elif self.fused:
             self._fused_step()
         else:
             for n in range(self.integ.nstages):
