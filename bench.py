@@ -205,7 +205,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"h2o2-9sp-28rx {n}^3 periodic, muscl-koren + hllc, coupled chemistry, tvd-rk3",
                    "cells": ncells, "stages_per_step": nst, "path": "fused" if mb.fused else "per-kernel",
-                   "hip_graph": mb._graph is not None,
+                   "hip_graph_replays": mb.replays,
                    "k_chunk": cfg["backend-mi355x"]["k-chunk"],
                    "stage_slabs": cfg["backend-mi355x"]["stage-slabs"]},
         "roofline": roof,

@@ -632,37 +632,50 @@ class Solver:
                                blk.detJinv.view(), rates, len(terms), src, w, withdt, self.dt.ptr, sbox, kc)
 
     _graph = None
+    replays = 0
+
+    def _standing(self):
+        return tuple(sorted(self.readied.items(), key=str))
 
     def _graphed(self):
-        """A fused step replays one captured HIP graph once its launches are the steady ones: after
-        a first step, with no register's halos left ready by from_prims (that step skips its first
-        halo pass) and no launch timing. Everything a step reads that changes between steps is on
-        the device (Q, the state, dt), so the same graph serves every such step."""
-        return (bool(self.cfg["backend-mi355x"]["graphs"]) and self.nrt > 0 and not self.readied
-                and self.timers is None)
+        """A step replays one captured HIP graph once its launches are the steady ones: after a
+        first step, with the registers' halos standing as they stood when the graph was captured
+        (a step after from_prims skips its first halo pass) and no launch timing. Everything a step
+        reads that changes between steps is on the device (Q, the state, dt), so the same graph
+        serves every such step."""
+        if not self.cfg["backend-mi355x"]["graphs"] or self.nrt == 0 or self.timers is not None:
+            return False
+        return self._graph is None or self._graph[1] == self._standing()
 
-    def _replay(self):
+    def _replay(self, launches):
         if self._graph is None:
             torch = self.backend.torch
             graph = torch.cuda.CUDAGraph()
-            # captured on torch's capture stream, which the backend's launches follow; nothing runs
+            before = self._standing()
+            # captured on torch's capture stream, which the backend's launches follow; nothing runs,
+            # and the host's bookkeeping moves on as the step's would
             with torch.cuda.graph(graph):
-                self._fused_step()
-            self._graph = graph
-        self._graph.replay()
+                launches()
+            self._graph = (graph, before, dict(self.readied))
+        self._graph[0].replay()
+        self.readied = dict(self._graph[2])
+        self.replays += 1
 
     def step(self, dt):
         start = self.tme.get()
         self.dt.set(dt)
         times = self.integ.stagetimes(start, dt)
-        if self.fused and self._graphed():
-            self._replay()
-        elif self.fused:
-            self._fused_step()
+        if self.fused:
+            launches = self._fused_step
         else:
-            for n in range(self.integ.nstages):
-                self.stage(n)
-                self.tme.set(times[n])
+            def launches():
+                for n in range(self.integ.nstages):
+                    self.stage(n)
+                    self.tme.set(times[n])
+        if self._graphed():
+            self._replay(launches)
+        else:
+            launches()
         self.nrt += 1
         self.tme.set(start + dt)
 
