@@ -193,6 +193,14 @@ int32_t pga_halo(const pga_case *c, int32_t ntask, const pga_view *Qdst, const p
 int32_t pga_transport(const pga_case *c, pga_view Q, pga_view state, pga_view trstate, pga_view hsp,
                       const pga_box *boxes, int32_t nboxes, void *stream);
 
+/* The viscous sponge after the transport, per cell of up to 8 boxes: trstate[0] (the viscosity)
+ * times 1 + min(max(0, s (m - 1)), m - 1), s = (xcc . n - start) / length the cell centre's way
+ * along the line, line = (nx, ny, nz, start, length, m) (host memory).
+ * Replaces the viscsponge plugin's kernel: peregrine/plugins/simulator/kernels/viscsponge.mako:3-9,
+ * peregrine/plugins/simulator/viscsponge.py:9-26. */
+int32_t pga_viscsponge(pga_view xcc, pga_view trstate, const double *line, const pga_box *boxes, int32_t nboxes,
+                       void *stream);
+
 /* Per cell of box: grads[3 l + e] = d(row l)/dx_e for u, v, w, T and the carried Y, by central
  * differences in index space mapped by Jinv[3 r + e] = d xi_r / d x_e (one halo layer read).
  * Replaces pg_dqdxyz: peregrine/simulator/navstokes/kernels/dqdxyz.mako:16-41. */

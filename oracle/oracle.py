@@ -67,6 +67,7 @@ class Block(C.Structure):
         ("viscous", C.c_int), ("alpha", C.c_double),
         ("trstate", Arr), ("hsp", Arr), ("grads", Arr), ("Jinv", Arr), ("xcc", Arr), ("xf", Arr * 3),
         ("bc", C.c_int * 6), ("nstate", C.c_int), ("bcwall", (C.c_double * 3) * 6), ("bcT", C.c_double * 6),
+        ("sponge", C.c_int), ("spongeline", C.c_double * 6),
     ]
 
 
@@ -93,6 +94,7 @@ def _load():
     lib.pgo_combine.argtypes = [C.c_int, Arr, C.c_int, C.c_double, C.c_int, P(Arr), _dp, C.c_double, C.c_int, Box]
     lib.pgo_fill_periodic.argtypes = [Arr, C.c_int, P(C.c_int), C.c_int, C.c_int, C.c_int]
     lib.pgo_transport.argtypes = [P(Case), Arr, Arr, Arr, Arr, Box]
+    lib.pgo_viscsponge.argtypes = [Arr, Arr, _dp, Box]
     lib.pgo_wall_mirror.argtypes = [P(Case), C.c_int, C.c_int, C.c_int, _dp, C.c_double, P(C.c_int), C.c_int,
                                     C.c_int, Arr, Arr, Arr]
     lib.pgo_wall_flux.argtypes = [P(Case), C.c_int, C.c_int, C.c_int, _dp, C.c_double, P(C.c_int), C.c_int, C.c_int,

@@ -1162,6 +1162,15 @@ static void binary_diffusion(const pgo_case *c, const double *X, double p, doubl
   }
 }
 
+void pgo_viscsponge(pgo_arr xcc, pgo_arr trstate, const double *line, pgo_box r) {
+  const double nx = line[0], ny = line[1], nz = line[2], start = line[3], length = line[4], m = line[5];
+  FOR_BOX(r) {
+    const double along = (*at(xcc, i, j, k, 0) * nx + *at(xcc, i, j, k, 1) * ny + *at(xcc, i, j, k, 2) * nz - start) /
+                         length;
+    *at(trstate, i, j, k, 0) *= 1.0 + fmin(fmax(0.0, along * (m - 1.0)), m - 1.0);
+  }
+}
+
 /* navstokes/kernels/transport.mako:5-35 with transport/kernels/{constant,kinetic,dilute}.mako,
  * diffusion/kernels/{lewis,binary}.mako and the ideal hspecies (eos/kernels/species.mako:29-36).
  * ln T is taken by the EOS's log (pgo_elog), as everywhere in this path. */
@@ -1563,6 +1572,13 @@ static void prerhs(const pgo_case *c, pgo_block *b, pgo_arr Q, pgo_arr dQ, int w
     for (int a = 0; a < 3; a++)
       for (int side = 0; side < 2; side++)
         if (!b->bc[2 * a + side]) pgo_transport(c, Q, b->state, b->trstate, b->hsp, haloslab(b, a, side, 1));
+    /* the sponge over the cells the transport has just made (viscsponge.py: the "shallow" range) */
+    if (b->sponge) {
+      pgo_viscsponge(b->xcc, b->trstate, b->spongeline, interior(b));
+      for (int a = 0; a < 3; a++)
+        for (int side = 0; side < 2; side++)
+          if (!b->bc[2 * a + side]) pgo_viscsponge(b->xcc, b->trstate, b->spongeline, haloslab(b, a, side, 1));
+    }
   }
 }
 

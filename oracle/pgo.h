@@ -113,7 +113,16 @@ typedef struct {
   int bc[6];
   int nstate;
   double bcwall[6][3], bcT[6];
+  /* the viscous sponge (plugins/simulator/viscsponge.py): on when sponge, its line as
+   * (nx, ny, nz, start, length, multiplier) */
+  int sponge;
+  double spongeline[6];
 } pgo_block;
+
+/* plugins/simulator/kernels/viscsponge.mako: the viscosity trstate[0] raised along the line
+ * (nx, ny, nz, start, length, multiplier): times 1 + min(max(0, s (m - 1)), m - 1), s the cell
+ * centre's distance along the normal from start over length */
+void pgo_viscsponge(pgo_arr xcc, pgo_arr trstate, const double *line, pgo_box r);
 
 /* the walls (simulator/euler/bcs.py:15-27, navstokes/bcs.py:27-53): the halo as the interior's
  * image across the wall face (side 0 the low face of axis, 1 the high), and the wall's own flux

@@ -449,3 +449,42 @@ extern "C" int32_t pga_diffflux(const pga_case *c, int32_t axis, double alpha, p
   }
   return launched("pga_diffflux");
 }
+
+// plugins/simulator/kernels/viscsponge.mako: after the transport, the viscosity raised along a
+// line from its origin to its end, to the multiplier times itself there and beyond
+namespace pga {
+struct SpongeLine {
+  double nx, ny, nz, start, length, mult;
+};
+
+__global__ __launch_bounds__(256) void k_viscsponge(View xcc, View tr, SpongeLine s, Box3 b) {
+#pragma clang fp contract(off)
+  int i, j, k;
+  if (!cell_of(b, i, j, k)) return;
+  const int64_t ox = xcc.off(i, j, k);
+  const double along = (xcc.at(ox, 0) * s.nx + xcc.at(ox, 1) * s.ny + xcc.at(ox, 2) * s.nz - s.start) / s.length;
+  tr.at(tr.off(i, j, k), 0) *= 1.0 + fmin(fmax(0.0, along * (s.mult - 1.0)), s.mult - 1.0);
+}
+}  // namespace pga
+
+extern "C" int32_t pga_viscsponge(pga_view xcc, pga_view trstate, const double *line, const pga_box *boxes,
+                                  int32_t nboxes, void *stream) {
+  if (!line || nboxes < 0 || nboxes > 8 || (nboxes && !boxes)) {
+    set_error("pga_viscsponge: null argument or more than 8 boxes");
+    return PGA_EINVAL;
+  }
+  if (!(line[4] > 0.0)) {
+    set_error("pga_viscsponge: the sponge's length must be positive, not %g", line[4]);
+    return PGA_EINVAL;
+  }
+  for (int n = 0; n < nboxes; n++)
+    PGA_REQUIRE(box_ok(boxes[n]) && covers(xcc, boxes[n], 3) && covers(trstate, boxes[n], 1),
+                "pga_viscsponge: box %d reaches outside xcc or trstate", n);
+  const SpongeLine s{line[0], line[1], line[2], line[3], line[4], line[5]};
+  for (int n = 0; n < nboxes; n++) {
+    if (empty(boxes[n])) continue;
+    const Box3 b = box3(boxes[n]);
+    hipLaunchKernelGGL(k_viscsponge, grid_of(b), dim3(TI, TJ), 0, stream_of(stream), view(xcc), view(trstate), s, b);
+  }
+  return launched("pga_viscsponge");
+}

@@ -3,7 +3,9 @@ protocol and its cadence (niter-out or dt-out).
 
 Mirrors ``peregrine/plugins/solver/base.py:5-26``; ``report`` prints the banner and the
 seconds per iteration per cell (``plugins/solver/report.py:9-60``), ``nancheck`` stops a run
-whose interior holds a non-finite value (``plugins/solver/nancheck.py:10-44``).
+whose interior holds a non-finite value (``plugins/solver/nancheck.py:10-44``). A physics'
+plugin acts with its physics instead: ``viscsponge`` raises the viscosity along a line after
+every transport (``plugins/simulator/viscsponge.py:9-26``), on the device.
 """
 
 import time
@@ -94,11 +96,44 @@ class NanCheck(BasePlugin):
                 raise NanCheckError(f"block {blk.nblki}: a non-finite value in Q at step {solver.nrt}")
 
 
+class ViscSponge:
+    """The viscosity raised along the line from origin to ending, to multiplier times itself at
+    the end and beyond, over the cells the transport has just made (interior and first halo
+    layer); a Navier-Stokes case only."""
+
+    name = "viscsponge"
+    simulators = ("navier-stokes",)
+
+    def __init__(self, cfgsect, solver):
+        from peregrine_amd import native
+
+        cfgsect = dict(cfgsect or {})
+        missing = [k for k in ("origin", "ending", "multiplier") if k not in cfgsect]
+        if missing:
+            raise ValueError(f"plugins.viscsponge needs {missing}")
+        origin, ending = (np.array(cfgsect[k], float) for k in ("origin", "ending"))
+        length = float(np.linalg.norm(ending - origin))
+        if not length > 0.0:
+            raise ValueError("plugins.viscsponge: origin and ending must differ")
+        normal = (ending - origin) / length
+        self.line = native.dbls([*normal, float(origin @ normal), length, float(cfgsect["multiplier"])])
+
+
 PLUGINS = {p.name: p for p in (Report, NanCheck)}
+PHYSICS_PLUGINS = {p.name: p for p in (ViscSponge,)}
 
 
 def claim(cfgsect, solver):
-    unknown = sorted(set(cfgsect) - set(PLUGINS))
+    """The solver's plugins and the physics' plugins the plugins section names."""
+    unknown = sorted(set(cfgsect) - set(PLUGINS) - set(PHYSICS_PLUGINS))
     if unknown:
-        raise ValueError(f"{unknown}: no plugin of that name in this build (have {sorted(PLUGINS)})")
-    return [PLUGINS[name](sect, solver) for name, sect in cfgsect.items()]
+        raise ValueError(f"{unknown}: no plugin of that name in this build (have "
+                         f"{sorted(PLUGINS) + sorted(PHYSICS_PLUGINS)})")
+    physics = []
+    for name, sect in cfgsect.items():
+        if name in PHYSICS_PLUGINS:
+            cls = PHYSICS_PLUGINS[name]
+            if solver.sim.name not in cls.simulators:
+                raise ValueError(f"plugins.{name} is for {', '.join(cls.simulators)}, not {solver.sim.name}")
+            physics.append(cls(sect, solver))
+    return [PLUGINS[name](sect, solver) for name, sect in cfgsect.items() if name in PLUGINS], physics

@@ -335,7 +335,8 @@ class Solver:
     def _plugins(self):
         from peregrine_amd.plugins import claim
 
-        return claim(self.cfg["plugins"], self)
+        plugins, self.sponges = claim(self.cfg["plugins"], self)
+        return plugins
 
     def _fused_supported(self):
         """The fused stage runs when the config asks for it, the integrator takes one combine per
@@ -423,6 +424,8 @@ class Solver:
             arr = (native.Box * len(boxes))(*boxes)
             self._call("pga_transport", self.case.handle, blk.regs[Q].view(), blk.state.view(), blk.trstate.view(),
                        blk.hsp.view(), arr, len(boxes))
+            for sponge in self.sponges:
+                self._call("pga_viscsponge", blk.xcc.view(), blk.trstate.view(), sponge.line, arr, len(boxes))
 
     def gradients(self, Q):
         """The cell gradients over the interior and their one-plane exchange

@@ -253,7 +253,7 @@ def test_halo_copies(dev):
 
 
 def _run_pair(gas, recon, primary, limiter, chem, integrator, ncell, steps, dt, prims, lengths, fused=True,
-              trans=None, species=None, nstate=8, walls=(), substep=None, mixture=None, slabs=1):
+              trans=None, species=None, nstate=8, walls=(), substep=None, mixture=None, slabs=1, sponge=None):
     """The same case stepped by the solver on the device and by the oracle on the host; walls: the
     faces (0 i-low .. 5 k-high) that are adiabatic slip walls, the rest periodic; substep: the
     sub-stepped chemistry's (max-substeps, bulk-substeps, stiff-bins, equilibrium-tolerance), the
@@ -278,6 +278,8 @@ def _run_pair(gas, recon, primary, limiter, chem, integrator, ncell, steps, dt, 
                     for f, (kind, values) in spec.items()}
     cfg["backend-mi355x"]["fused-stage"] = fused
     cfg["backend-mi355x"]["stage-slabs"] = slabs
+    if sponge is not None:
+        cfg["plugins"]["viscsponge"] = sponge
     backend = pg.get_backend("mi355x", cfg)
     periodic = [not (2 * a in walls or 2 * a + 1 in walls) for a in range(3)]
     mesh = pg.mesher.CubeMesher(blkdims=[n + 1 for n in ncell], lengths=lengths, periodic=periodic,
@@ -297,6 +299,10 @@ def _run_pair(gas, recon, primary, limiter, chem, integrator, ncell, steps, dt, 
     O.lib.pgo_fromQ(oc.ref, O.arr(hb.regs[0]), O.arr(hb.state), O.box((hb.ng,) * 3, hb.ncell))
     b = sp.oracle_block(hb, integrator, chem, viscous=sim.viscous, alpha=getattr(sim, "alpha", 1.0),
                         walls=oracle_walls)
+    if sponge is not None:
+        # the oracle takes the line the plugin made of the section
+        b.sponge = 1
+        b.spongeline[:] = list(mb.sponges[0].line)
     if substep is not None:
         b.chemistry, b.subcap, b.subeqtol = 2, substep[0], substep[3]
         hb.counts = np.zeros(int(np.prod(ncell)), dtype=np.int32)
@@ -1012,3 +1018,38 @@ def test_sods_tube_steps_match_and_meet_the_exact_solution(dev, fused):
     rho_err, _, _ = tr.errors(x_o, rho, u_o, p_o, r)
     print(f"sod on the device ({'fused' if fused else 'per-kernel'}): rho error {rho_err:.4e}")
     assert rho_err == pytest.approx(tr.TUBES[case]["rho"][scheme], rel=1e-3)
+
+
+def test_the_viscous_sponge_matches(dev):
+    """pga_viscsponge against the oracle on a jostled block's centres and an oblique line, to the
+    bit; and the walled channel stepped with a sponge (the viscosity raised eightfold along a line
+    across it) against the oracle's steps, which a sponge left out would not match."""
+    rng = np.random.default_rng(11)
+    hb = sp.HostBlock((12, 9, 7), 2, 6, 1, jostle=0.3, seed=2)
+    tr = rng.uniform(1e-5, 2e-5, hb.cshape + (4,))
+    line = O.dbls([0.6, 0.64, 0.48, 0.1, 0.7, 8.0])
+    want = sp.to_dev(tr)
+    box = O.box((1, 1, 1), tuple(n - 2 for n in hb.cshape))
+    O.lib.pgo_viscsponge(O.arr(hb.xccd), O.arr(want), line, box)
+    dtr = dev.put(sp.to_dev(tr))
+    native.check(native.lib().pga_viscsponge(dev.view(dev.put(hb.xccd)), dev.view(dtr), line,
+                                             (native.Box * 1)(native.box((1, 1, 1), tuple(n - 2 for n in hb.cshape))),
+                                             1, None), "pga_viscsponge")
+    assert np.array_equal(dev.get(dtr), want)
+    assert not np.array_equal(want, sp.to_dev(tr))
+
+    ncell = (16, 12, 8)
+    L = (0.4, 0.3, 0.2)
+    walls = {2: ("no-slp-isot-wall", {"T": 350.0}), 3: ("mov-adia-wall", {"u": 20.0, "v": 0.0, "w": 0.0})}
+    runs = {}
+    for name, sponge in (("sponge", {"origin": [0.1, 0.0, 0.0], "ending": [0.3, 0.1, 0.05], "multiplier": 8.0}),
+                         ("none", None)):
+        mb, hb = _run_pair("binary", "muscl", "hllc", "koren", False, "tvd-rk3", ncell, 10, 2e-6,
+                           lambda x: sp.smooth_prims(x, 6, ns=2), L, trans="constant", walls=walls, sponge=sponge)
+        g = hb.ng
+        got = mb.blocks[0].Q.get()[g:-g, g:-g, g:-g].transpose(3, 2, 1, 0)
+        err = _rowerr(got, hb.regs[0][hb.interior()])
+        print(f"channel with {name}: {err:.3e}")
+        assert err < 1e-10
+        runs[name] = got
+    assert _rowerr(runs["sponge"], runs["none"]) > 1e-8

@@ -295,3 +295,23 @@ def test_a_full_turn_joins_its_ends_as_an_interior_interface():
     nodes = block_nodes(mb.blocks, 2)
     x = grid.cellcenters(nodes[0])[2, 2]
     assert np.allclose(np.degrees(np.arctan2(x[:, 2], x[:, 1])), -45.0 + 30 * np.arange(7), atol=1e-12)
+
+
+def test_a_physics_plugin_is_claimed_for_its_physics_only():
+    """plugins.viscsponge is Navier-Stokes' (plugins/simulator/viscsponge.py:9-10): refused on an
+    Euler case saying which it is for, refused without its line, and an unknown name refused."""
+    from types import SimpleNamespace
+
+    from peregrine_amd.plugins import ViscSponge, claim
+
+    sect = {"origin": [0, 0, 0], "ending": [2.0, 0, 0], "multiplier": 4.0}
+    ns, euler = (SimpleNamespace(sim=SimpleNamespace(name=n)) for n in ("navier-stokes", "euler"))
+    solver_plugins, physics = claim({"viscsponge": sect, "nancheck": {}}, ns)
+    assert [p.name for p in solver_plugins] == ["nancheck"] and isinstance(physics[0], ViscSponge)
+    assert list(physics[0].line) == [1.0, 0.0, 0.0, 0.0, 2.0, 4.0]
+    with pytest.raises(ValueError, match="is for navier-stokes, not euler"):
+        claim({"viscsponge": sect}, euler)
+    with pytest.raises(ValueError, match="needs"):
+        claim({"viscsponge": {"origin": [0, 0, 0]}}, ns)
+    with pytest.raises(ValueError, match="no plugin of that name"):
+        claim({"writer": {}}, ns)

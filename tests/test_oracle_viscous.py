@@ -265,3 +265,34 @@ def test_kinetic_theory_gives_the_measured_gas_properties():
         kap = evaluate(k["kappapoly"][n], lnT) * np.sqrt(300.0)
         assert abs(mu / mu_meas - 1) < 0.03, (name, mu)
         assert abs(kap / kap_meas - 1) < 0.03, (name, kap)
+
+
+def test_the_viscous_sponge_ramps_the_viscosity_along_its_line():
+    """plugins/simulator/kernels/viscsponge.mako: before the line's origin the viscosity is as the
+    transport made it, past its end multiplier times it, linear in between (halfway: 1 + (m - 1)/2),
+    whatever the distance across the line; the other transport fields untouched."""
+    from oracle import oracle as O
+
+    n = 21
+    x = np.zeros((n, 3, 2, 3))
+    x[..., 0] = np.linspace(-1.0, 3.0, n)[:, None, None]  # along the line, 0.2 apart
+    x[..., 1] = np.linspace(-5.0, 5.0, 3)[None, :, None]  # across it
+    x[..., 2] = np.array([0.0, 7.0])[None, None, :]
+    rng = np.random.default_rng(3)
+    # the same transport state across the line, so the cells across it can be compared bit for bit
+    tr = np.broadcast_to(rng.uniform(1e-5, 2e-5, (n, 1, 1, 4)), (n, 3, 2, 4)).copy()
+    got = sp.to_dev(tr)
+    # from x = 0 to x = 2 along +x, eightfold
+    O.lib.pgo_viscsponge(O.arr(sp.to_dev(x)), O.arr(got), O.dbls([1.0, 0.0, 0.0, 0.0, 2.0, 8.0]),
+                         O.box((0, 0, 0), (n, 3, 2)))
+    got = sp.to_host(got)
+    factor = got[..., 0] / tr[..., 0]
+    s = x[..., 0] / 2.0
+    assert np.array_equal(got[s <= 0.0], tr[s <= 0.0])
+    assert np.array_equal(got[..., 0][s >= 1.0], tr[..., 0][s >= 1.0] * 8.0)
+    mid = np.isclose(s, 0.5)
+    assert mid.any() and np.array_equal(got[..., 0][mid], tr[..., 0][mid] * 4.5)
+    assert np.allclose(factor, 1.0 + np.clip(s * 7.0, 0.0, 7.0), rtol=1e-15, atol=0)
+    # across the line it is the same
+    assert np.array_equal(got[..., 0], np.broadcast_to(got[:, :1, :1, 0], got.shape[:3]))
+    assert np.array_equal(got[..., 1:], tr[..., 1:])
