@@ -64,20 +64,34 @@ class BasePlugin:
 
 
 class Report(BasePlugin):
+    """At the start the case; after each step it is due on the step number, the time, the step
+    size and the largest CFL numbers (combined, then each speed); at the end the wall time over
+    this run's steps and the seconds per step per cell."""
+
     name = "report"
 
     def start(self, solver):
+        print(" >>> peregrine-mi355x <<<", flush=True)
         print(repr(solver), flush=True)
         self.t0, self.n0 = time.perf_counter(), solver.nrt
 
     def __call__(self, solver):
-        print(f"  step {solver.nrt:8d}  t = {solver.tme.get():.6e}", flush=True)
+        cfl = solver.cflmax()
+        dt = solver.dt.get()
+        speeds = "".join(f"         {n:10s}: {cfl[n] * dt:.3f}\n" for n in solver.sim.cflspeeds if n != "combined")
+        print(f" >>> --------- nrt: {solver.nrt:<6} ---------- <<<\n"
+              f"     tme: {solver.tme.get():.6E} s\n"
+              f"     dt : {dt:.6E} s\n"
+              f"     MAX CFL       : {cfl['combined'] * dt:.3f}\n" + speeds, flush=True)
 
     def finalize(self, solver):
         solver.synchronize()
         steps = max(solver.nrt - self.n0, 1)
-        dt = time.perf_counter() - self.t0
-        print(f"  Seconds/Iteration/Cell: {dt / steps / max(solver.ncells, 1):.3e}", flush=True)
+        elapsed = time.perf_counter() - self.t0
+        hrs, rem = divmod(int(elapsed), 3600)
+        mins, secs = divmod(rem, 60)
+        print(f"Simulation time: {hrs}h : {mins}m : {secs}s over {steps} steps\n"
+              f"Seconds/Iteration/Cell: {elapsed / steps / max(solver.ncells, 1):.3e}", flush=True)
 
 
 class NanCheckError(RuntimeError):
@@ -90,10 +104,20 @@ class NanCheck(BasePlugin):
     def __call__(self, solver):
         torch = solver.backend.torch
         ng = solver.ng
-        for blk in solver.blocks:
-            Q = blk.Q.t[:, ng:-ng, ng:-ng, ng:-ng]
-            if not bool(torch.isfinite(Q).all()):
-                raise NanCheckError(f"block {blk.nblki}: a non-finite value in Q at step {solver.nrt}")
+        bad = [blk for blk in solver.blocks if not bool(torch.isfinite(blk.Q.t[:, ng:-ng, ng:-ng, ng:-ng]).all())]
+        if not bad:
+            return
+        # where: the centres of the cells holding a non-finite value, a file per block
+        for blk in bad:
+            Q = blk.Q.get()[blk.interior]
+            where = np.where((~np.isfinite(Q)).any(axis=-1))
+            cells = blk.xcc.get()[blk.interior][where]
+            with open(f"nans_{blk.nblki}.log", "w") as f:
+                f.write(f"Non-finite state, block {blk.nblki}\n")
+                for x, y, z in cells:
+                    f.write(f"x = {x} y = {y} z = {z}\n")
+        raise NanCheckError(f"non-finite state at step {solver.nrt} in blocks {[b.nblki for b in bad]}; "
+                            "see the nans_<block>.log files")
 
 
 class ViscSponge:
