@@ -120,6 +120,7 @@ def main():
     ap.add_argument("--per-kernel", action="store_true", help="the reference's launch sequence instead of the fused stage")
     ap.add_argument("--k-chunk", type=int, default=None)
     ap.add_argument("--stage-slabs", type=int, default=None, help="k-slabs of the fused stage")
+    ap.add_argument("--no-graphs", action="store_true", help="launch every step eagerly (no HIP graph replay)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only a plain device copy of 512 MiB, five times: the reference for HBM byte counters")
     args = ap.parse_args()
@@ -149,6 +150,7 @@ def main():
         cfg["backend-mi355x"]["k-chunk"] = args.k_chunk
     if args.stage_slabs is not None:
         cfg["backend-mi355x"]["stage-slabs"] = args.stage_slabs
+    cfg["backend-mi355x"]["graphs"] = not args.no_graphs
     backend = pg.get_backend("mi355x", cfg)
     n = args.n
     mesh = pg.mesher.CubeMesher(blkdims=[n + 1] * 3, lengths=L, periodic=[True] * 3)
