@@ -3,8 +3,9 @@ protocol and its cadence (niter-out or dt-out).
 
 Mirrors ``peregrine/plugins/solver/base.py:5-26``; ``report`` prints the banner and the
 seconds per iteration per cell (``plugins/solver/report.py:9-60``), ``nancheck`` stops a run
-whose interior holds a non-finite value (``plugins/solver/nancheck.py:10-44``). A physics'
-plugin acts with its physics instead: ``viscsponge`` raises the viscosity along a line after
+whose interior holds a non-finite value (``plugins/solver/nancheck.py:10-44``), ``trace``
+samples the primitives at chosen points into csv files (``plugins/solver/trace.py:30-140``). A
+physics' plugin acts with its physics instead: ``viscsponge`` raises the viscosity along a line after
 every transport (``plugins/simulator/viscsponge.py:9-26``), on the device.
 """
 
@@ -120,6 +121,82 @@ class NanCheck(BasePlugin):
                             "see the nans_<block>.log files")
 
 
+def _trace_points(spec):
+    """A trace's points: a point p0, a line of n points from p0 to p1, or a plane of n01 by n02
+    points spanned from p0 by p1 and p2 (``plugins/solver/trace.py:12-27``)."""
+    p0 = np.array(spec["p0"], float)
+    if spec["type"] == "point":
+        return p0[None, :]
+    if spec["type"] == "line":
+        return p0 + np.linspace(0.0, 1.0, spec["n"])[:, None] * (np.array(spec["p1"], float) - p0)
+    a = np.linspace(0.0, 1.0, spec["n01"])[:, None, None]
+    b = np.linspace(0.0, 1.0, spec["n02"])[None, :, None]
+    p1, p2 = (np.array(spec[k], float) for k in ("p1", "p2"))
+    return (p0 + a * (p1 - p0) + b * (p2 - p0)).reshape(-1, 3)
+
+
+class Trace(BasePlugin):
+    """The primitive variables (p, u, v, w, T, the carried Y) at the cells whose centres are
+    nearest chosen points, a row per point appended to each trace's csv when it is due, with the
+    time and the cell centre (``plugins/solver/trace.py:30-140``). A trace may carry its own
+    cadence; the others act on the section's."""
+
+    name = "trace"
+
+    def __init__(self, cfgsect, solver):
+        super().__init__(cfgsect, solver)
+        self.specs = dict(self.cfg.get("traces") or {})
+        for tag, spec in self.specs.items():
+            if spec.get("type") not in ("point", "line", "plane"):
+                raise ValueError(f"trace {tag}: type is point, line or plane")
+            if "file" not in spec:
+                raise ValueError(f"trace {tag}: names the file it writes")
+        self.cadences = {tag: Cadence(spec) if {"niter-out", "dt-out"} & set(spec) else self.cadence
+                         for tag, spec in self.specs.items()}
+
+    def due(self, solver):
+        return any(c.due(solver) for c in self.cadences.values())
+
+    def tnext(self, solver):
+        times = [c.next(solver) for c in self.cadences.values()]
+        return min((t for t in times if t is not None), default=None)
+
+    def start(self, solver):
+        from pathlib import Path
+
+        names = ["p", "u", "v", "w", "T", *solver.sim.mixture.spnames[:-1]]
+        centres = [(blk, blk.xcc.get()[blk.interior]) for blk in solver.blocks]
+        self.cells = {}
+        for tag, spec in self.specs.items():
+            cells = []
+            for p in _trace_points(spec):
+                best = None
+                for blk, c in centres:
+                    d = ((c - p) ** 2).sum(axis=-1)
+                    m = np.unravel_index(int(d.argmin()), d.shape)
+                    if best is None or d[m] < best[0]:
+                        best = (d[m], blk, tuple(int(x) + blk.ng for x in m), c[m])
+                cells.append(best[1:])
+            self.cells[tag] = cells
+            if not Path(spec["file"]).exists():
+                Path(spec["file"]).write_text("time, x, y, z, " + ", ".join(names) + "\n")
+
+    def __call__(self, solver):
+        tme = solver.tme.get()
+        for tag, spec in self.specs.items():
+            if not self.cadences[tag].due(solver):
+                continue
+            rows = []
+            for blk, (i, j, k), centre in self.cells[tag]:
+                # the device arrays hold (component, k, j, i)
+                Q, st = blk.Q.t[:, k, j, i].cpu().numpy(), blk.state.t[:, k, j, i].cpu().numpy()
+                rhoinv = 1.0 / Q[0]
+                values = [st[0], Q[1] * rhoinv, Q[2] * rhoinv, Q[3] * rhoinv, st[1], *(Q[5:] * rhoinv)]
+                rows.append(", ".join(f"{v:.8e}" for v in (tme, *centre, *values)))
+            with open(spec["file"], "a") as f:
+                f.write("\n".join(rows) + "\n")
+
+
 class ViscSponge:
     """The viscosity raised along the line from origin to ending, to multiplier times itself at
     the end and beyond, over the cells the transport has just made (interior and first halo
@@ -143,7 +220,7 @@ class ViscSponge:
         self.line = native.dbls([*normal, float(origin @ normal), length, float(cfgsect["multiplier"])])
 
 
-PLUGINS = {p.name: p for p in (Report, NanCheck)}
+PLUGINS = {p.name: p for p in (Report, NanCheck, Trace)}
 PHYSICS_PLUGINS = {p.name: p for p in (ViscSponge,)}
 
 

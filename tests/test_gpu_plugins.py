@@ -53,3 +53,39 @@ def test_nancheck_stops_the_run_and_says_where(gpu, tmp_path, monkeypatch):
     assert log[0] == "Non-finite state, block 0" and len(log) > 1
     x, y, z = blk.xcc.get()[ng + 3, ng + 4, ng + 5]
     assert f"x = {x} y = {y} z = {z}" in log
+
+
+def test_trace_samples_the_primitives_at_the_nearest_cells(gpu, tmp_path, monkeypatch):
+    """A point (its own cadence, every step) and a line of 5 points (the section's, every second
+    step) on the reacting gas: a header naming the primitives, then a row per point per due step,
+    the time, the nearest cell's centre and its p, u, v, w, T and carried Y; a second run appends."""
+    import peregrine_amd as pg
+
+    monkeypatch.chdir(tmp_path)
+    cfg = sp.config("h2o2", recon="muscl", primary="hllc", limiter="koren", chem=True, dt=1e-8)
+    cfg["simulation"]["niter"] = 4
+    cfg["plugins"]["trace"] = {"niter-out": 2, "traces": {
+        "probe": {"type": "point", "p0": [0.0031, 0.0052, 0.0077], "file": "probe.csv", "niter-out": 1},
+        "line": {"type": "line", "p0": [0.0, 0.001, 0.002], "p1": [0.01, 0.009, 0.002], "n": 5, "file": "line.csv"}}}
+    n, L = 12, (0.01,) * 3
+    mb = pg.get_solver(cfg, pg.mesher.CubeMesher(blkdims=[n + 1] * 3, lengths=L, periodic=[True] * 3),
+                       pg.get_backend("mi355x", cfg))
+    mb.set_prims([sp.reacting_flow(mb.blocks[0].xcc.get(), n, L)])
+    mb.run()
+    probe = (tmp_path / "probe.csv").read_text().splitlines()
+    line = (tmp_path / "line.csv").read_text().splitlines()
+    names = ["p", "u", "v", "w", "T", *mb.sim.mixture.spnames[:-1]]
+    assert probe[0] == "time, x, y, z, " + ", ".join(names) == line[0]
+    assert len(probe) == 1 + 4 and len(line) == 1 + 2 * 5
+    # the last probe row: the cell nearest the point, its primitives now
+    ng, blk = mb.ng, mb.blocks[0]
+    xcc = blk.xcc.get()[blk.interior]
+    d = ((xcc - [0.0031, 0.0052, 0.0077]) ** 2).sum(axis=-1)
+    i, j, k = np.unravel_index(d.argmin(), d.shape)
+    Q, st = blk.Q.get()[ng + i, ng + j, ng + k], blk.state.get()[ng + i, ng + j, ng + k]
+    want = [mb.tme.get(), *xcc[i, j, k], st[0], *(Q[1:4] / Q[0]), st[1], *(Q[5:] / Q[0])]
+    got = [float(v) for v in probe[-1].split(",")]
+    assert np.allclose(got, want, rtol=1e-8, atol=0)
+    # a second run appends to the files it finds
+    mb.run()
+    assert len((tmp_path / "probe.csv").read_text().splitlines()) == 1 + 8
