@@ -117,6 +117,10 @@ class NanCheck(BasePlugin):
                 f.write(f"Non-finite state, block {blk.nblki}\n")
                 for x, y, z in cells:
                     f.write(f"x = {x} y = {y} z = {z}\n")
+        # the state it stopped on, by the writer among the plugins if there is one
+        for plugin in solver.plugins:
+            if isinstance(plugin, Writer):
+                plugin(solver)
         raise NanCheckError(f"non-finite state at step {solver.nrt} in blocks {[b.nblki for b in bad]}; "
                             "see the nans_<block>.log files")
 
@@ -197,6 +201,27 @@ class Trace(BasePlugin):
                 f.write("\n".join(rows) + "\n")
 
 
+class Writer(BasePlugin):
+    """Results into a directory as numbered files (``plugins/solver/writer.py:12-36``): dir
+    (made if need be), precision (single or double), basename formatted from the step n and the
+    time t. Each is a complete restart (``peregrine_amd.restart``)."""
+
+    name = "writer"
+
+    def start(self, solver):
+        from pathlib import Path
+
+        from peregrine_amd.restart import RestartWriter
+
+        self.dir = Path(self.cfg.get("dir", "."))
+        self.dir.mkdir(parents=True, exist_ok=True)
+        self.writer = RestartWriter(solver, self.cfg.get("basename", "q.{n:08d}") + ".npz",
+                                    self.cfg.get("precision", "single"), cfg=solver.cfg)
+
+    def __call__(self, solver):
+        self.writer.write(solver, solver.nrt, solver.tme.get(), self.dir)
+
+
 class ViscSponge:
     """The viscosity raised along the line from origin to ending, to multiplier times itself at
     the end and beyond, over the cells the transport has just made (interior and first halo
@@ -220,7 +245,7 @@ class ViscSponge:
         self.line = native.dbls([*normal, float(origin @ normal), length, float(cfgsect["multiplier"])])
 
 
-PLUGINS = {p.name: p for p in (Report, NanCheck, Trace)}
+PLUGINS = {p.name: p for p in (Report, NanCheck, Trace, Writer)}
 PHYSICS_PLUGINS = {p.name: p for p in (ViscSponge,)}
 
 

@@ -356,11 +356,17 @@ class Solver:
 
     # -- state ---------------------------------------------------------------
     def _set_state(self, state):
+        """The case's initial values everywhere, then a restart's primitives over the interior
+        with its step and time (``run.py``, ``readers/restart.py:30-54``)."""
         values = self.sim.ics()
         for blk in self.blocks:
             q = np.empty(blk.cellshape + (self.sim.ne,))
             q[...] = values
             blk.q.set(q)
+        if state is not None:
+            state.fill(self)
+            self.nrt = state.nrt
+            self.tme.set(state.tme)
         self.from_prims()
 
     def set_prims(self, prims):

@@ -89,3 +89,77 @@ def test_trace_samples_the_primitives_at_the_nearest_cells(gpu, tmp_path, monkey
     # a second run appends to the files it finds
     mb.run()
     assert len((tmp_path / "probe.csv").read_text().splitlines()) == 1 + 8
+
+
+def _reacting(tmp_path, writer=None):
+    import peregrine_amd as pg
+
+    cfg = sp.config("h2o2", recon="muscl", primary="hllc", limiter="koren", chem=True, dt=1e-8)
+    if writer is not None:
+        cfg["plugins"]["writer"] = writer
+    n, L = 12, (0.01,) * 3
+    mesh = pg.mesher.CubeMesher(blkdims=[n + 1] * 3, lengths=L, periodic=[True] * 3)
+    mb = pg.get_solver(cfg, mesh, pg.get_backend("mi355x", cfg))
+    mb.set_prims([sp.reacting_flow(mb.blocks[0].xcc.get(), n, L)])
+    return cfg, mesh, mb
+
+
+def _interior(mb):
+    g = mb.ng
+    return [blk.Q.get()[g:-g, g:-g, g:-g] for blk in mb.blocks]
+
+
+def _scaled(a, b):
+    return (np.abs(a - b).reshape(-1, a.shape[-1]).max(axis=0) / np.abs(b).reshape(-1, b.shape[-1]).max(axis=0)).max()
+
+
+def test_a_restart_continues_as_the_case_would_have(gpu, tmp_path):
+    """After tests/io/test_restart.py:79-104: three steps, a double-precision result, two more;
+    the case started again from the result holds the step, the time and the solution (to the
+    primitives' round trip, Q -> p, T -> Q, in which e(T)'s ln T polynomials magnify a last-bit
+    change of T about 1e5-fold: 1e-10) and its next two steps are the first case's (1e-9)."""
+    import peregrine_amd as pg
+    from peregrine_amd.restart import RestartReader, RestartWriter
+
+    cfg, mesh, mb = _reacting(tmp_path)
+    for _ in range(3):
+        mb.step(1e-8)
+    path = RestartWriter(mb, precision="double", cfg=cfg).write(mb, mb.nrt, mb.tme.get(), tmp_path)
+    assert path.name == "q.00000003.npz"
+    written, tme = _interior(mb), mb.tme.get()
+    for _ in range(2):
+        mb.step(1e-8)
+    expect = _interior(mb)
+    reader = RestartReader(path)
+    assert reader.nrt == 3 and reader.privars == ["p", "u", "v", "w", "T", *mb.sim.mixture.spnames[:-1]]
+    again = pg.get_solver(reader.cfg, mesh, pg.get_backend("mi355x", reader.cfg), state=reader)
+    assert again.nrt == 3 and again.tme.get() == tme
+    for a, b in zip(_interior(again), written):
+        assert _scaled(a, b) < 1e-10
+    for _ in range(2):
+        again.step(1e-8)
+    for a, b in zip(_interior(again), expect):
+        assert _scaled(a, b) < 1e-9
+    assert again.tme.get() == mb.tme.get()
+
+
+def test_the_writer_plugins_result_restarts_the_case(gpu, tmp_path):
+    """The plugin writes single precision unless told otherwise, each variable a [k, j, i] array of
+    a block's interior; the species sum to one; its result restarts the case to that rounding."""
+    import peregrine_amd as pg
+    from peregrine_amd.restart import RestartReader
+
+    cfg, mesh, mb = _reacting(tmp_path, writer={"dir": str(tmp_path / "out")})
+    for _ in range(3):
+        mb.step(1e-8)
+    mb.plugins[0](mb)
+    written = _interior(mb)
+    path = tmp_path / "out" / "q.00000003.npz"
+    with np.load(path) as f:
+        assert f["results_000000/p"].dtype == np.float32 and f["results_000000/p"].shape == (12, 12, 12)
+        total = sum(f[f"results_000000/{s}"].astype(np.float64) for s in mb.sim.mixture.spnames)
+        assert np.abs(total - 1.0).max() < 1e-6
+    again = pg.get_solver(cfg, mesh, pg.get_backend("mi355x", cfg), state=RestartReader(path))
+    assert again.nrt == 3
+    for a, b in zip(_interior(again), written):
+        assert _scaled(a, b) < 2e-6

@@ -314,4 +314,4 @@ def test_a_physics_plugin_is_claimed_for_its_physics_only():
     with pytest.raises(ValueError, match="needs"):
         claim({"viscsponge": {"origin": [0, 0, 0]}}, ns)
     with pytest.raises(ValueError, match="no plugin of that name"):
-        claim({"writer": {}}, ns)
+        claim({"catalyst": {}}, ns)
