@@ -66,14 +66,13 @@ _DEFAULTS = {
         "launch-waves": 2,
         "constform": "literal",
     },
-    # this package's device: the block shape of the cell and face kernels, whether a stage
-    # runs as the fused pass or as the per-kernel sequence, the k-slabs a fused stage is run
-    # in (a slab's face arrays stay in the 256 MB Infinity Cache between the flux sweeps and
-    # the divergence that reads them), and whether fused steps replay one captured HIP graph
+    # this package's device: which GPU, whether a stage runs as the fused pass or as the
+    # per-kernel sequence, the fused stage's k-planes per thread, the k-slabs a fused stage is
+    # run in (a slab's face arrays stay in the 256 MB Infinity Cache between the flux sweeps and
+    # the divergence that reads them), and whether steps replay one captured HIP graph (the
+    # kernels' block shapes are compiled in: 64 lanes along i, waves along j or k)
     "backend-mi355x": {
         "device": 0,
-        "tile-i": 64,
-        "tile-j": 4,
         "fused-stage": True,
         "k-chunk": 1,
         "stage-slabs": 1,
