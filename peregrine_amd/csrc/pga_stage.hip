@@ -41,6 +41,7 @@ struct StageArgs {
   const double *dt;
   int s0, s1, s2, n0, n1, n2;
   int kc;
+  int q0;  // whether a combine term reads Q0 (RK3's first stage reads only Qin and dQ)
 };
 
 constexpr int WI = 64, WJ = 4, WT = WI * WJ;
@@ -94,7 +95,8 @@ __global__ __launch_bounds__(WT) void k_stage(StageArgs a) {
         if (l >= 5) d = __builtin_nontemporal_load(&rates.at(orr, l - 5)) + d;
       }
       // the terms' values are selected as values: a select of pointers would put them in memory
-      const double qin = __builtin_nontemporal_load(&Qin.at(o, l)), q0 = __builtin_nontemporal_load(&Q0.at(o0, l));
+      const double qin = __builtin_nontemporal_load(&Qin.at(o, l));
+      const double q0 = a.q0 ? __builtin_nontemporal_load(&Q0.at(o0, l)) : 0.0;
       double acc = 0.0;
 #pragma unroll
       for (int t = 0; t < 4; t++) {
@@ -231,7 +233,10 @@ extern "C" int32_t pga_stage(const pga_case *c, pga_view Qin, pga_view stin, pga
   a.Qin = view(Qin), a.stin = view(stin), a.Q0 = view(Q0), a.Qout = view(Qout), a.stout = view(stout);
   a.iF = view(iF), a.jF = view(jF), a.kF = view(kF), a.det = view(detJinv);
   a.comb.n = nterms;
-  for (int t = 0; t < nterms; t++) a.comb.src[t] = src[t], a.comb.w[t] = w[t], a.comb.withdt[t] = withdt[t];
+  for (int t = 0; t < nterms; t++) {
+    a.comb.src[t] = src[t], a.comb.w[t] = w[t], a.comb.withdt[t] = withdt[t];
+    a.q0 |= src[t] == SRC_Q0;
+  }
   a.dt = dt;
   a.s0 = box.start[0], a.s1 = box.start[1], a.s2 = box.start[2];
   a.n0 = box.extent[0], a.n1 = box.extent[1], a.n2 = box.extent[2];

@@ -186,17 +186,23 @@ class Trace(BasePlugin):
                 Path(spec["file"]).write_text("time, x, y, z, " + ", ".join(names) + "\n")
 
     def __call__(self, solver):
+        torch = solver.backend.torch
         tme = solver.tme.get()
         for tag, spec in self.specs.items():
             if not self.cadences[tag].due(solver):
                 continue
-            rows = []
-            for blk, (i, j, k), centre in self.cells[tag]:
-                # the device arrays hold (component, k, j, i)
+            cells = self.cells[tag]
+            rows = [None] * len(cells)
+            for blk in {id(c[0]): c[0] for c in cells}.values():
+                mine = [n for n, c in enumerate(cells) if c[0] is blk]
+                i, j, k = (torch.tensor([cells[n][1][a] for n in mine], device=blk.Q.t.device) for a in range(3))
+                # one gather per block; the device arrays hold (component, k, j, i)
                 Q, st = blk.Q.t[:, k, j, i].cpu().numpy(), blk.state.t[:, k, j, i].cpu().numpy()
                 rhoinv = 1.0 / Q[0]
-                values = [st[0], Q[1] * rhoinv, Q[2] * rhoinv, Q[3] * rhoinv, st[1], *(Q[5:] * rhoinv)]
-                rows.append(", ".join(f"{v:.8e}" for v in (tme, *centre, *values)))
+                for m, n in enumerate(mine):
+                    r = rhoinv[m]
+                    values = [st[0, m], Q[1, m] * r, Q[2, m] * r, Q[3, m] * r, st[1, m], *(Q[5:, m] * r)]
+                    rows[n] = ", ".join(f"{v:.8e}" for v in (tme, *cells[n][2], *values))
             with open(spec["file"], "a") as f:
                 f.write("\n".join(rows) + "\n")
 

@@ -141,8 +141,8 @@ def block_nodes(blocks, ng):
     by a rotational periodic and moved by a translation. Corner and edge halos stay extrapolated."""
     nodes = {}
     for blk in blocks:
-        moved = [(lambda f: f.periodic and f.nbr == blk.nblki and f.rot is None)(blk.faces[2 * a + 1])
-                 for a in range(3)]
+        lows = [blk.faces[2 * a + 1] for a in range(3)]
+        moved = [f.periodic and f.nbr == blk.nblki and f.rot is None for f in lows]
         nodes[blk.nblki] = grid.halo_nodes(blk.interior_nodes, ng, moved, blk.period)
     for blk in blocks:
         for f in blk.faces.values():
