@@ -108,6 +108,57 @@ def measured_traffic(kernel):
     return None if got is None else got["bytes_per_launch"]
 
 
+def batch(args, pg, support):
+    """The uneven batch: args.batch periodic H2/O2 boxes with edges drawn by
+    default_rng(20240611).integers(16, 97), the reacting case's model and step, stepped together by
+    one solver (each step one HIP graph replay over every block) against the same boxes stepped by
+    a solver each, one after another, launched eagerly: solves/s = cases x steps / wall time."""
+    import torch
+
+    dims = np.random.default_rng(20240611).integers(16, 97, size=(args.batch, 3))
+    cfg = case_config(support)
+
+    def solver(dlist, graphs):
+        c = case_config(support)
+        c["backend-mi355x"]["graphs"] = graphs
+        mb = pg.get_solver(c, pg.mesher.BatchMesher(dlist, lengths=L), pg.get_backend("mi355x", c))
+        mb.set_prims([support.h2_prims(b.xcc.get(), b.ncell, lengths=L) for b in mb.blocks])
+        return mb
+
+    def timed(solvers):
+        for _ in range(2):
+            for mb in solvers:
+                mb.step(DT)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            for mb in solvers:
+                mb.step(DT)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+    together = solver(dims, True)
+    t_batch = timed([together])
+    cells = together.ncells
+    del together
+    one_by_one = [solver([d], False) for d in dims]
+    t_plain = timed(one_by_one)
+    n = len(dims)
+    line = {
+        "metric": "batched solves/s (H2-air 3D uneven batch)", "value": n * args.steps / t_batch, "unit": "solves/s",
+        "n_gpus": 1, "steps": args.steps, "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"{n} periodic h2o2 boxes, edges 16-96 (seed 20240611), muscl-koren + hllc, "
+                               "coupled chemistry, tvd-rk3", "cells": int(cells)},
+        "plain_per_case": n * args.steps / t_plain, "speedup": t_plain / t_batch,
+        "cell_updates_per_s": cells * cfg_stages(cfg) * args.steps / t_batch,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def cfg_stages(cfg):
+    return {"rk1": 1, "rk2": 2, "tvd-rk3": 3, "rk4": 4, "rk34": 4}[cfg["time-integration"]["integrator"]]
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -121,6 +172,9 @@ def main():
     ap.add_argument("--k-chunk", type=int, default=None)
     ap.add_argument("--stage-slabs", type=int, default=None, help="k-slabs of the fused stage")
     ap.add_argument("--no-graphs", action="store_true", help="launch every step eagerly (no HIP graph replay)")
+    ap.add_argument("--batch", type=int, default=0,
+                    help="the uneven batch instead: this many periodic H2/O2 cases of 16-96 cells per axis "
+                         "(seeded) stepped together, against the same cases stepped one by one")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only a plain device copy of 512 MiB, five times: the reference for HBM byte counters")
     args = ap.parse_args()
@@ -144,6 +198,8 @@ def main():
     import peregrine_amd as pg
     import support
 
+    if args.batch:
+        return batch(args, pg, support)
     cfg = case_config(support)
     cfg["backend-mi355x"]["fused-stage"] = not args.per_kernel
     if args.k_chunk:

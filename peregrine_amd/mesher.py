@@ -213,3 +213,34 @@ class AnnulusMesher(BaseMesher):
             return
         # the low end takes its halo from the far end of the sweep, turned back by it
         face.set_periodic(rot=face.rotabout(self.axis, -self.sweep if face.is_low else self.sweep))
+
+
+class BatchMesher(BaseMesher):
+    """Independent cases side by side: block n is its own periodic box of dims[n] cells on
+    lengths[n] (or one length for all), joined to nothing else, so one solver steps them together
+    (the uneven multiblock batch: blocks of different sizes in one step)."""
+
+    name = "batch"
+
+    def __init__(self, dims, lengths=(1.0, 1.0, 1.0), origin=(0.0, 0.0, 0.0), **kwargs):
+        self.dims = [tuple(int(n) for n in d) for d in dims]
+        super().__init__(mbdims=(len(self.dims), 1, 1), **kwargs)
+        L = np.asarray(lengths, float)
+        self.lengths_of = [tuple(L) if L.ndim == 1 else tuple(L[n]) for n in range(len(self.dims))]
+        self.origin = np.asarray(origin, float)
+
+    def fill(self, mb):
+        if mb.blocks:
+            raise ValueError(f"a mesher fills an empty multiblock; this one has {len(mb.blocks)} blocks")
+        for n, (dims, lengths) in enumerate(zip(self.dims, self.lengths_of)):
+            blk = mb.add_blk()
+            lines = [np.linspace(o, o + L, m + 1) for o, L, m in zip(self.origin, lengths, dims)]
+            blk.set_extents(*(m + 1 for m in dims))
+            blk.interior_nodes = np.stack(np.meshgrid(*lines, indexing="ij"), axis=-1)
+            blk.period = list(lengths)
+            for axis in range(3):
+                for nface in (2 * axis + 1, 2 * axis + 2):
+                    face = blk.intface(nface)
+                    face.connect(n)
+                    span = -lengths[axis] if face.is_low else lengths[axis]
+                    face.set_periodic(xlate=tuple(span if m == axis else 0.0 for m in range(3)))

@@ -102,3 +102,29 @@ def test_graph_replays_are_the_eager_steps(gpu, case):
     # the first step follows from_prims (its halos ready) and launches; the rest replay one graph
     assert graphed.replays == 4 and eager.replays == 0
     assert graphed.nrt == eager.nrt == 5
+
+
+def test_an_uneven_batch_steps_as_its_cases_one_by_one(gpu):
+    """Three periodic H2/O2 boxes of different sizes in one solver, each step one graph replay over
+    all of them, against each box stepped by a solver of its own, eagerly: to the bit."""
+    import peregrine_amd as pg
+
+    dims = [(16, 12, 20), (24, 16, 12), (12, 20, 16)]
+    L = (0.01,) * 3
+
+    def solver(dlist, graphs):
+        cfg = sp.config("h2o2", recon="muscl", primary="hllc", limiter="koren", chem=True, dt=1e-8)
+        cfg["backend-mi355x"]["graphs"] = graphs
+        mb = pg.get_solver(cfg, pg.mesher.BatchMesher(dlist, lengths=L), pg.get_backend("mi355x", cfg))
+        mb.set_prims([sp.h2_prims(b.xcc.get(), b.ncell, lengths=L) for b in mb.blocks])
+        return mb
+
+    together = solver(dims, True)
+    alone = [solver([d], False) for d in dims]
+    for _ in range(4):
+        together.step(1e-8)
+        for mb in alone:
+            mb.step(1e-8)
+    assert together.replays == 3
+    for blk, mb in zip(together.blocks, alone):
+        assert np.array_equal(blk.Q.get(), mb.blocks[0].Q.get())

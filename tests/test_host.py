@@ -315,3 +315,26 @@ def test_a_physics_plugin_is_claimed_for_its_physics_only():
         claim({"viscsponge": {"origin": [0, 0, 0]}}, ns)
     with pytest.raises(ValueError, match="no plugin of that name"):
         claim({"catalyst": {}}, ns)
+
+
+def test_a_batch_is_independent_periodic_boxes():
+    """The uneven batch's mesher: each block its own periodic box, joined to nothing else; its
+    halo cells continue its own lattice across each end."""
+    from peregrine_amd import grid, mesher
+    from peregrine_amd.solver import block_nodes
+
+    mb = _Lattice(2)
+    dims = [(16, 20, 24), (30, 17, 16), (16, 16, 96)]
+    mesher.BatchMesher(dims, lengths=(0.01, 0.01, 0.01)).fill(mb)
+    assert [blk.ncell for blk in mb.blocks] == dims
+    for blk in mb.blocks:
+        assert all(f.nbr == blk.nblki and f.periodic and f.rot is None for f in blk.faces.values())
+    nodes = block_nodes(mb.blocks, 2)
+    for blk in mb.blocks:
+        x = grid.cellcenters(nodes[blk.nblki])
+        for a in range(3):
+            line = np.moveaxis(x[..., a], a, 0)[:, 3, 3]
+            step = 0.01 / blk.ncell[a]
+            assert np.allclose(np.diff(line), step, rtol=0, atol=1e-15)
+    q = sp.h2_prims(grid.cellcenters(nodes[1]), mb.blocks[1].ncell, lengths=(0.01,) * 3)
+    assert q.shape == mb.blocks[1].cellshape + (13,) and np.isfinite(q).all()
