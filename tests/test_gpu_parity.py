@@ -1053,3 +1053,38 @@ def test_the_viscous_sponge_matches(dev):
         assert err < 1e-10
         runs[name] = got
     assert _rowerr(runs["sponge"], runs["none"]) > 1e-8
+
+
+def test_taylor_green_at_its_full_size_navier_stokes_rk4_100_steps(dev):
+    """The Taylor-Green case as the reference's CPU-runnable benchmark states it: one 32^3 periodic
+    block on [0, 2 pi]^3, thermally perfect air, Mach 0.1, Re 1600 (mu = rho0 V0 L / 1600 with
+    L = 1, kappa = mu cp / 0.71), KEEP, Navier-Stokes with alpha damping, RK4, dt = 0.2 dx / (c0 + V0),
+    100 steps: the device against the oracle within 1e-10 of each conserved row's largest value."""
+    n = 32
+    L = (2 * np.pi,) * 3
+    R = 8314.46261815324 / sp.AIR_NASA7["Air"]["MW"]
+    cp = 1005.0
+    gamma = cp / (cp - R)
+    c0 = np.sqrt(gamma * R * 300.0)
+    V0 = 0.1 * c0
+    rho0 = 101325.0 / (R * 300.0)
+    mu = rho0 * V0 * 1.0 / 1600.0
+    species = {"Air": {**sp.AIR_NASA7["Air"], "mu0": mu, "kappa0": mu * cp / 0.71}}
+
+    def prims(x):
+        X, Y, Z = x[..., 0], x[..., 1], x[..., 2]
+        q = np.zeros(x.shape[:3] + (5,))
+        q[..., 0] = 101325.0 + rho0 * V0**2 / 16.0 * (np.cos(2 * X) + np.cos(2 * Y)) * (np.cos(2 * Z) + 2.0)
+        q[..., 1] = V0 * np.sin(X) * np.cos(Y) * np.cos(Z)
+        q[..., 2] = -V0 * np.cos(X) * np.sin(Y) * np.cos(Z)
+        q[..., 4] = q[..., 0] / (rho0 * R)
+        return q
+
+    dt = 0.2 * (2 * np.pi / n) / (c0 + V0)
+    mb, hb = _run_pair("air-tpg", "constant", "kepaec", None, False, "rk4", (n,) * 3, 100, dt, prims, L,
+                       trans="constant", species=species)
+    g = hb.ng
+    got = mb.blocks[0].Q.get()[g:-g, g:-g, g:-g].transpose(3, 2, 1, 0)
+    err = _rowerr(got, hb.regs[0][hb.interior()])
+    print(f"taylor-green 32^3 navier-stokes rk4 100 steps: {err:.3e}")
+    assert err < 1e-10
