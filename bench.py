@@ -30,6 +30,10 @@ sys.path.insert(0, str(ROOT / "tests"))
 
 HBM_PEAK = 8000.0  # GB/s, MI355X HBM3E
 FP64_PEAK = 78.6  # TFLOP/s, MI355X FP64 vector
+# FP64 flop per face of the MUSCL-Koren + HLLC sweep on the 9-species thermally perfect gas, from
+# its gfx950 code: 247 multiplies, 209 adds and 317 fused multiply-adds (2 each), the 12
+# divisions' refinement and the min/max selects counted once (1090 in all)
+FLUX_FLOP_PER_FACE = 1090
 DT = 1e-9
 L = (0.01, 0.01, 0.01)
 
@@ -245,6 +249,11 @@ def main():
     roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK, unit="GB/s", frac=round(achieved / HBM_PEAK, 4),
                 traffic=measured_traffic(kname), kernel=dom, kernel_ms=round(ms, 4), launches=count,
                 bytes_per_item=per, items_per_launch=items)
+    if dom == "pga_flux":
+        # the sweep is FP64-VALU and latency bound: its FP64 rate beside the bandwidth
+        tf = FLUX_FLOP_PER_FACE * items / (ms * 1e-3) / 1e12
+        roof["fp64"] = dict(achieved=round(tf, 2), peak=FP64_PEAK, unit="TFLOP/s", frac=round(tf / FP64_PEAK, 4),
+                            flop_per_item=FLUX_FLOP_PER_FACE)
     breakdown = {k: {"launches": c, "mean_ms": round(t, 4), "share": round(c * t / sum(total.values()), 3)}
                  for k, (c, t) in sorted(times.items(), key=lambda kv: -kv[1][0] * kv[1][1])}
     cpu = None if args.no_cpu else cpu_baseline(support, args.cpu_n, args.cpu_seconds)
