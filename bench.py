@@ -31,8 +31,8 @@ sys.path.insert(0, str(ROOT / "tests"))
 HBM_PEAK = 8000.0  # GB/s, MI355X HBM3E
 FP64_PEAK = 78.6  # TFLOP/s, MI355X FP64 vector
 # FP64 flop per face of the MUSCL-Koren + HLLC sweep on the 9-species thermally perfect gas, from
-# its gfx950 code: 247 multiplies, 209 adds and 317 fused multiply-adds (2 each), the 12
-# divisions' refinement and the min/max selects counted once (1090 in all)
+# its gfx950 code: 247 multiplies, 209 adds and 317 fused multiply-adds (2 each; the divisions'
+# refinement steps are among them), 1090 in all; min, max, compares and selects not counted
 FLUX_FLOP_PER_FACE = 1090
 DT = 1e-9
 L = (0.01, 0.01, 0.01)
