@@ -115,8 +115,9 @@ def measured_traffic(kernel):
 def batch(args, pg, support):
     """The uneven batch: args.batch periodic H2/O2 boxes with edges drawn by
     default_rng(20240611).integers(16, 97), the reacting case's model and step, stepped together by
-    one solver (each step one HIP graph replay over every block) against the same boxes stepped by
-    a solver each, one after another, launched eagerly: solves/s = cases x steps / wall time."""
+    one solver launched eagerly with the blocks' stages overlapping on side streams (value; also
+    as one HIP graph replay per step over every block) against the same boxes stepped by a solver
+    each, one after another, launched eagerly: solves/s = cases x steps / wall time."""
     import torch
 
     dims = np.random.default_rng(20240611).integers(16, 97, size=(args.batch, 3))
@@ -145,16 +146,20 @@ def batch(args, pg, support):
     t_batch = timed([together])
     cells = together.ncells
     del together
+    overlapped = solver(dims, False)
+    t_streams = timed([overlapped])
+    del overlapped
     one_by_one = [solver([d], False) for d in dims]
     t_plain = timed(one_by_one)
     n = len(dims)
     line = {
-        "metric": "batched solves/s (H2-air 3D uneven batch)", "value": n * args.steps / t_batch, "unit": "solves/s",
+        "metric": "batched solves/s (H2-air 3D uneven batch)", "value": n * args.steps / t_streams, "unit": "solves/s",
         "n_gpus": 1, "steps": args.steps, "higher_is_better": True, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"{n} periodic h2o2 boxes, edges 16-96 (seed 20240611), muscl-koren + hllc, "
                                "coupled chemistry, tvd-rk3", "cells": int(cells)},
-        "plain_per_case": n * args.steps / t_plain, "speedup": t_plain / t_batch,
-        "cell_updates_per_s": cells * cfg_stages(cfg) * args.steps / t_batch,
+        "plain_per_case": n * args.steps / t_plain, "speedup": t_plain / t_streams,
+        "graph_replay": n * args.steps / t_batch,
+        "cell_updates_per_s": cells * cfg_stages(cfg) * args.steps / t_streams,
     }
     print(json.dumps(line), flush=True)
 

@@ -69,13 +69,15 @@ _DEFAULTS = {
     # this package's device: which GPU, whether a stage runs as the fused pass or as the
     # per-kernel sequence, the fused stage's k-planes per thread, the k-slabs a fused stage is
     # run in (a slab's face arrays stay in the 256 MB Infinity Cache between the flux sweeps and
-    # the divergence that reads them), and whether steps replay one captured HIP graph (the
+    # the divergence that reads them), the side streams several blocks' stages overlap on when
+    # launched eagerly, and whether steps replay one captured HIP graph (the
     # kernels' block shapes are compiled in: 64 lanes along i, waves along j or k)
     "backend-mi355x": {
         "device": 0,
         "fused-stage": True,
         "k-chunk": 1,
         "stage-slabs": 1,
+        "block-streams": 4,
         "graphs": True,
     },
     "halo-exchange": {"kind": "hoststaged"},

@@ -605,6 +605,34 @@ class Solver:
             terms.append((src(r), w, int(c.step and m == len(c.X) - 1)))
         return terms
 
+    _streams = None
+
+    def _block_streams(self):
+        """Each block with the stream its stage work goes on. Launched eagerly (no graph replay),
+        several blocks take side streams (default priority) in turn, forked from the current
+        stream after what it has queued (the halo pass, which reads other blocks) and joined back
+        once every block is queued, so small blocks' sweeps and stages overlap on the device; a
+        block's stage reads and writes only its own arrays. Under graph replay the blocks stay on
+        the one stream: forking side streams inside the captured graph crashed its replay."""
+        import contextlib
+
+        torch = self.backend.torch
+        nside = int(self.cfg["backend-mi355x"]["block-streams"])
+        if len(self.blocks) < 2 or nside < 2 or self.cfg["backend-mi355x"]["graphs"]:
+            for blk in self.blocks:
+                yield blk, contextlib.nullcontext()
+            return
+        if self._streams is None:
+            self._streams = [torch.cuda.Stream(device=self.backend.device) for _ in range(nside)]
+        main = torch.cuda.current_stream(self.backend.device)
+        used = self._streams[:min(nside, len(self.blocks))]
+        for s in used:
+            s.wait_stream(main)
+        for n, blk in enumerate(self.blocks):
+            yield blk, torch.cuda.stream(used[n % len(used)])
+        for s in used:
+            main.wait_stream(s)
+
     def _slabs(self, blk):
         return stage_slabs(blk.ncell[2], int(self.cfg["backend-mi355x"]["stage-slabs"]))
 
@@ -626,19 +654,20 @@ class Solver:
             src = native.ints([t[0] for t in terms])
             w = (ctypes.c_double * len(terms))(*[t[1] for t in terms])
             withdt = native.ints([t[2] for t in terms])
-            for blk in self.blocks:
-                box = blk.interior_box()
-                Qin, st = blk.regs[regs.Q].view(), blk.state.view()
-                if self.sim.chemistry:
-                    self.chemistry_source(blk, Qin, blk.rates.view(), False)
-                rates = blk.rates.view() if self.sim.chemistry else native.View()
-                for slab in self._slabs(blk):
-                    self.fluxes(blk, Qin, st, slab)
-                    k0, k1 = slab[:2]
-                    sbox = native.box((self.ng, self.ng, self.ng + k0), (blk.ncell[0], blk.ncell[1], k1 - k0))
-                    self._call("pga_stage", self.case.handle, Qin, st, blk.regs[0].view(),
-                               blk.regs[regs.Qout].view(), st, blk.iF.view(), blk.jF.view(), blk.kF.view(),
-                               blk.detJinv.view(), rates, len(terms), src, w, withdt, self.dt.ptr, sbox, kc)
+            for blk, stream in self._block_streams():
+                with stream:
+                    box = blk.interior_box()
+                    Qin, st = blk.regs[regs.Q].view(), blk.state.view()
+                    if self.sim.chemistry:
+                        self.chemistry_source(blk, Qin, blk.rates.view(), False)
+                    rates = blk.rates.view() if self.sim.chemistry else native.View()
+                    for slab in self._slabs(blk):
+                        self.fluxes(blk, Qin, st, slab)
+                        k0, k1 = slab[:2]
+                        sbox = native.box((self.ng, self.ng, self.ng + k0), (blk.ncell[0], blk.ncell[1], k1 - k0))
+                        self._call("pga_stage", self.case.handle, Qin, st, blk.regs[0].view(),
+                                   blk.regs[regs.Qout].view(), st, blk.iF.view(), blk.jF.view(), blk.kF.view(),
+                                   blk.detJinv.view(), rates, len(terms), src, w, withdt, self.dt.ptr, sbox, kc)
 
     _graph = None
     replays = 0

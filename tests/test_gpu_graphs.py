@@ -106,7 +106,8 @@ def test_graph_replays_are_the_eager_steps(gpu, case):
 
 def test_an_uneven_batch_steps_as_its_cases_one_by_one(gpu):
     """Three periodic H2/O2 boxes of different sizes in one solver, each step one graph replay over
-    all of them, against each box stepped by a solver of its own, eagerly: to the bit."""
+    all of them (or launched eagerly, the blocks overlapping on side streams), against each box
+    stepped by a solver of its own, eagerly: to the bit."""
     import peregrine_amd as pg
 
     dims = [(16, 12, 20), (24, 16, 12), (12, 20, 16)]
@@ -120,11 +121,15 @@ def test_an_uneven_batch_steps_as_its_cases_one_by_one(gpu):
         return mb
 
     together = solver(dims, True)
+    # launched eagerly, the blocks' stages overlap on side streams
+    overlapped = solver(dims, False)
     alone = [solver([d], False) for d in dims]
     for _ in range(4):
         together.step(1e-8)
+        overlapped.step(1e-8)
         for mb in alone:
             mb.step(1e-8)
-    assert together.replays == 3
-    for blk, mb in zip(together.blocks, alone):
+    assert together.replays == 3 and overlapped._streams is not None
+    for blk, other, mb in zip(together.blocks, overlapped.blocks, alone):
         assert np.array_equal(blk.Q.get(), mb.blocks[0].Q.get())
+        assert np.array_equal(other.Q.get(), mb.blocks[0].Q.get())
