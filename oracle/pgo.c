@@ -580,6 +580,8 @@ static double phi(int kind, double dm, double dp) {
   return p;
 }
 
+double pgo_limit(int kind, double a, double b) { return limit(kind, a, b); }
+
 static double lft(int kind, double vLL, double vL, double vR) { return vL + 0.5 * phi(kind, vL - vLL, vR - vL); }
 static double rgt(int kind, double vL, double vR, double vRR) { return vR - 0.5 * phi(kind, vRR - vR, vR - vL); }
 
@@ -628,6 +630,25 @@ static void sides(const pgo_case *c, pgo_arr Q, pgo_arr st, const int *cl, const
   R->w = rgt(lim, QQ(1, 3) * ri[1], QQ(2, 3) * ri[2], QQ(3, 3) * ri[3]);
 #undef QQ
 #undef SS
+}
+
+/* the two sides of the face whose right cell is (i, j, k) along axis, each written as
+ * p, T, rho, e, c, u, v, w, Y_0 .. Y_{ns-2} (left first): the reconstruction pgo_flux feeds its
+ * flux function */
+void pgo_sides(const pgo_case *c, pgo_arr Q, pgo_arr st, int axis, int i, int j, int k, double *out) {
+  const int cl[3] = {i, j, k};
+  int d[3] = {0, 0, 0};
+  d[axis] = 1;
+  side L, R;
+  sides(c, Q, st, cl, d, &L, &R);
+  const side *X[2] = {&L, &R};
+  const int w = 8 + c->ns - 1;
+  for (int s = 0; s < 2; s++) {
+    double *o = out + s * w;
+    o[0] = X[s]->s.p, o[1] = X[s]->s.T, o[2] = X[s]->s.rho, o[3] = X[s]->s.e, o[4] = X[s]->s.c;
+    o[5] = X[s]->u, o[6] = X[s]->v, o[7] = X[s]->w;
+    for (int n = 0; n < c->ns - 1; n++) o[8 + n] = X[s]->Y[n];
+  }
 }
 
 /* function/kepaec.mako:2-20 (Shima et al. 2021) */

@@ -68,6 +68,10 @@ double pgo_dcbrt(double x);
 void pgo_fromQ(const pgo_case *c, pgo_arr Q, pgo_arr state, pgo_box r);
 void pgo_fromprims(const pgo_case *c, pgo_arr prims, pgo_arr Q, pgo_arr state, pgo_box r);
 void pgo_flux(const pgo_case *c, int axis, pgo_arr Q, pgo_arr state, pgo_arr S, pgo_arr F, pgo_box r);
+/* recon/{koren,minmod,mc,superbee,vanleer}.mako: b phi(a / b) for b > 0, kind 1..5 as the case's limiter */
+double pgo_limit(int kind, double a, double b);
+/* a face's two reconstructed sides (right cell (i, j, k) along axis), each p, T, rho, e, c, u, v, w, Y */
+void pgo_sides(const pgo_case *c, pgo_arr Q, pgo_arr st, int axis, int i, int j, int k, double *out);
 void pgo_applyflux(int ne, pgo_arr dQ, pgo_arr iF, pgo_arr jF, pgo_arr kF, pgo_arr detJinv, int append, pgo_box r);
 void pgo_production(const pgo_case *c, double rho, const double *Y, double T, double *omega);
 /* omega and the destruction rates D (kg/m^3/s) in one fold; D may be null */

@@ -161,3 +161,95 @@ def test_the_deterministic_cube_root_is_within_an_ulp():
     want = np.cbrt(x)
     assert (np.abs(got - want) <= np.spacing(np.abs(want))).all()
     assert O.dcbrt(0.0) == 0.0 and O.dcbrt(8.0) == 2.0 and O.dcbrt(-27.0) == -3.0
+
+
+def _poling_chung(X, T, rho, cvR):
+    """Chung's dense-gas factors for one nonpolar fluid as Poling, Prausnitz and O'Connell (5th ed.)
+    state them, restated from the book and the species' critical constants alone (after
+    tests/dense/test_chung.py:10-62): one-fluid mixing 9-5.24 to 9-5.44, the dense viscosity
+    9-6.16 to 9-6.18 and conductivity 10-5.5, each as a ratio to its dilute limit. numpy's exp,
+    power and cube root throughout; cvR the ideal cv/R of the mixture."""
+    import yaml
+
+    from peregrine_amd.mixture.species import DATA
+
+    db = yaml.safe_load(open(DATA / "species.yaml"))
+    tabs = yaml.safe_load(open(DATA / "chung.yaml"))
+    names = ("O2", "N2")
+    MW = np.array([sum(n * {"O": 15.999, "N": 14.007}[e] for e, n in db[s]["comp"].items()) for s in names])
+    Vc = np.array([db[s]["Vcrit"] for s in names]) * MW * 1e3  # cm^3/mol
+    Tcr = np.array([db[s]["Tcrit"] for s in names])
+    w = np.array([db[s]["acentric"] for s in names])
+    sigma = 0.809 * np.cbrt(Vc)  # 9-4.4: angstrom
+    epsk = Tcr / 1.2593  # 9-4.5
+    xx = np.outer(X, X)
+    sij = np.sqrt(np.outer(sigma, sigma))  # 9-5.33
+    eij = np.sqrt(np.outer(epsk, epsk))  # 9-5.35
+    Mij = 2.0 * np.outer(MW, MW) / np.add.outer(MW, MW)  # 9-5.40
+    s3 = (xx * sij**3).sum()  # 9-5.25
+    em = (xx * eij * sij**3).sum() / s3  # 9-5.26
+    Mm = ((xx * eij * sij**2 * np.sqrt(Mij)).sum() / (em * s3 ** (2.0 / 3.0))) ** 2  # 9-5.28
+    wm = (xx * 0.5 * np.add.outer(w, w) * sij**3).sum() / s3  # 9-5.29
+    Tc, Vcm = 1.2593 * em, s3 / 0.809**3  # 9-5.42, 9-5.43
+    Fc = 1.0 - 0.2756 * wm  # 9-5.44 without the dipole and association terms
+    E = np.array(tabs["viscosity"])[:, 0] + np.array(tabs["viscosity"])[:, 1] * wm  # Table 9-6
+    B = np.array(tabs["conductivity"])[:, 0] + np.array(tabs["conductivity"])[:, 1] * wm  # Table 10-4
+    Ts = T / em
+    Om = 1.16145 * Ts**-0.14874 + 0.52487 * np.exp(-0.77320 * Ts) + 2.16178 * np.exp(-2.43787 * Ts)  # 9-4.3
+    y = rho / (Mm * 1e3) * Vcm / 6.0  # 9-6.18: rho in mol/cm^3
+    G1 = (1.0 - 0.5 * y) / (1.0 - y) ** 3
+
+    def G2(c):
+        return (c[0] * (1.0 - np.exp(-c[3] * y)) / y + c[1] * G1 * np.exp(c[4] * y) + c[2] * G1) / (
+            c[0] * c[3] + c[1] + c[2])
+
+    g = G2(E)
+    eta2 = E[6] * y**2 * g * np.exp(E[7] + E[8] / Ts + E[9] / Ts**2)
+    # eta = eta* 36.344 sqrt(M Tc) / Vc^2/3 with eta* = sqrt(T*)/Omega Fc (1/G2 + E6 y) + eta**; over
+    # the dilute eta0 = 40.785 Fc sqrt(M T) / (Vc^2/3 Omega), which is its y -> 0 limit
+    muratio = (1.0 / g + E[5] * y) + eta2 * Om / (np.sqrt(Ts) * Fc)
+    eta0 = 4.0785e-6 * np.sqrt(Mm * T) / (Vcm ** (2.0 / 3.0) * Om) * Fc  # 9-4.10, Pa s
+    alpha = cvR - 1.5
+    beta = 0.7862 - 0.7109 * wm + 1.3168 * wm**2
+    Zr = 2.0 + 10.5 * (T / Tc) ** 2
+    Psi = 1.0 + alpha * ((0.215 + 0.28288 * alpha - 1.061 * beta + 0.26665 * Zr) / (0.6366 + beta * Zr
+                                                                                     + 1.061 * alpha * beta))
+    lam0 = 31.2 * eta0 * Psi / (Mm * 1e-3)  # 10-3.14
+    h = G2(B)
+    kratio = (1.0 / h + B[5] * y) + 3.586e-3 * np.sqrt(Tc / (Mm * 1e-3)) / Vcm ** (2.0 / 3.0) * B[6] * y**2 * h * \
+        np.sqrt(T / Tc) / lam0  # 10-5.5
+    return muratio, kratio
+
+
+@pytest.mark.parametrize("eos", ["real-gas", "peng-robinson"])
+def test_chungs_dense_factors_are_polings(eos):
+    """The oracle's Chung viscosity and conductivity over its kinetic-theory values at the same
+    dense states (60 bar, 120 to 300 K, O2/N2 from pure to pure, liquid-like to gas-like density)
+    equal Poling's factors restated above. A wrong E or B row, a G2 of the wrong form or a swapped
+    T*/Tr fails by far more than the tolerance."""
+    dense = _mixture(eos, trans="chung")
+    dilute = _mixture(eos, trans="kinetic-theory")
+    T = np.tile(np.linspace(120.0, 300.0, 7), 5)
+    Y = np.repeat(np.array([1.0, 0.8, 0.5, 0.2, 0.0]), 7)
+    p = np.full(T.size, 6e6)
+    n = T.size
+    trs = []
+    for mix in (dense, dilute):
+        oc = O.OracleCase(mix)
+        hb, st, Q = _states(oc, mix, p, T, Y)
+        tr, hs = np.zeros((4, 1, 1, n)), np.zeros((2, 1, 1, n))
+        O.lib.pgo_transport(oc.ref, O.arr(hb.regs[0]), O.arr(hb.state), O.arr(tr), O.arr(hs), O.box((0, 0, 0), (n, 1, 1)))
+        trs.append(tr[:2, 0, 0, :])
+    rho = st[:, 3]
+    assert rho.max() > 900.0 and rho.min() < 100.0  # liquid-like and gas-like states both
+    k = dense.constants()
+    MW = k["MW"]
+    got_mu, got_k = trs[0][0] / trs[1][0], trs[0][1] / trs[1][1]
+    for m in range(n):
+        Yv = np.array([Y[m], 1.0 - Y[m]])
+        X = Yv / MW / (Yv / MW).sum()
+        cvR = sum(X[s] * (np.polyval(k["cppoly"][s][::-1], np.log(T[m])) - 1.0) for s in range(2))
+        mur, kr = _poling_chung(X, T[m], rho[m], cvR)
+        assert mur > 1.2 or rho[m] < 200.0
+        assert got_mu[m] == pytest.approx(mur, rel=1e-12), (m, T[m], Y[m])
+        assert got_k[m] == pytest.approx(kr, rel=1e-12), (m, T[m], Y[m])
