@@ -14,7 +14,7 @@ import numpy as np
 
 from peregrine_amd.mixture.constants import Ru
 from peregrine_amd.mixture.fits import ConstantCp, TempDepCp, evaluate, padded
-from peregrine_amd.mixture.species import AnyOf
+from peregrine_amd.mixture.species import requirement_text
 
 NT = 7
 NG = 8
@@ -35,12 +35,12 @@ class BaseModel:
     def check(self, species):
         missing = {}
         for name, sp in species.items():
-            lack = sp.missing(self.needs)
+            lack = sp.unmet(self.needs)
             if lack:
                 missing[name] = lack
         if missing:
             text = "; ".join(
-                f"{', '.join('|'.join(k) if isinstance(k, AnyOf) else k for k in keys)} for {name}"
+                f"{', '.join(requirement_text(k) for k in keys)} for {name}"
                 for name, keys in missing.items()
             )
             raise ValueError(f"{self.name} needs {text}")
@@ -169,7 +169,7 @@ class RealGas(BaseEos):
     def check(self, species):
         super().check(species)
         lacking = [name for name, sp in species.items()
-                   if sp.missing(self.critical) and sp.missing(("well", "diam"))]
+                   if sp.unmet(self.critical) and sp.unmet(("well", "diam"))]
         if lacking:
             raise ValueError(f"{self.name} needs Tcrit, pcrit, Vcrit, acentric (or well and diam) for {', '.join(lacking)}")
 
@@ -178,7 +178,7 @@ class RealGas(BaseEos):
 
         out = {key: [] for key in self.critical}
         for sp in species.values():
-            if sp.missing(self.critical):
+            if sp.unmet(self.critical):
                 Tc = 1.2593 * sp["well"] / kB
                 # m3/kmol, from Chung's sigma = 0.809 Vc^1/3 (angstrom, cm3/mol)
                 Vcmol = (sp["diam"] * 1e10 / 0.809) ** 3 * 1e-3

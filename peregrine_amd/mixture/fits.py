@@ -19,7 +19,7 @@ Conventions (the kernels rely on them): with u = ln T,
 import numpy as np
 
 from peregrine_amd.mixture.constants import Ru, Tref
-from peregrine_amd.mixture.species import AnyOf, Species
+from peregrine_amd.mixture.species import HALF_ROTATIONAL_DOF, OneOf
 
 
 class ThermoSource:
@@ -129,7 +129,7 @@ def padded(rows, dtype=np.float64, width=None):
 class TempDepCp:
     """cp(T) refit in ln T over the case's T-range."""
 
-    needs = (AnyOf(("NASA7", "janaf")),)
+    needs = (OneOf(("NASA7", "janaf")),)
 
     def __init__(self, cfgsect):
         self.cfgsect = cfgsect
@@ -164,7 +164,7 @@ class TempDepCp:
     @staticmethod
     def _pad_low(sp, T, cpR, Tlow):
         # towards 0 K cp/R ramps to its frozen value: translation plus rotation when the shape is known
-        frozen = 2.5 + Species.rotdof[sp["geometry"]] if "geometry" in sp else cpR[0]
+        frozen = 2.5 + HALF_ROTATIONAL_DOF[sp["geometry"]] if "geometry" in sp else cpR[0]
         dT = T[1] - T[0]
         Tx = np.arange(T[0] - dT, Tlow, -dT)
         if Tx.size == 0 or Tx[-1] > Tlow:

@@ -14,7 +14,7 @@ from peregrine_amd.mixture.constants import Ru
 from peregrine_amd.mixture.eos import EOS_MODELS
 from peregrine_amd.mixture.finiterate import FiniteRate
 from peregrine_amd.mixture.parser import read_mechanism
-from peregrine_amd.mixture.species import DATA, Species
+from peregrine_amd.mixture.species import DATA, build_species
 from peregrine_amd.mixture.transport import TRANSPORT_MODELS, DIFFUSION_MODELS, MIXRULE_MODELS
 
 
@@ -42,7 +42,7 @@ class Mixture:
             raise ValueError("finite-rate needs the thermally perfect caloric model (eos tpg)")
         if cfgsect["trans"] == "chung" and self.eos.kind != 2:
             raise ValueError("chung's dense-gas transport reads the critical point of a real-gas eos")
-        self.species = Species.build(usersp, self.models)
+        self.species = build_species(usersp, self.models)
         self.spnames = list(self.species)
         self.ns = len(self.spnames)
         for model in self.models:

@@ -1,11 +1,18 @@
-"""Reader for Cantera YAML mechanisms: species thermo and the reaction list.
+"""Reader of Cantera's YAML mechanism format: the phase's species (composition, thermo, transport)
+and its reactions with rate constants in SI units (kmol, m, s; activation energies as
+temperatures). The format is the interface the reference reads too
+(``peregrine/simulator/mixture/parser/{base,cantera,units}.py``); the reader is written here from
+the format's own rules:
 
-Restates ``peregrine/simulator/mixture/parser/base.py:9-263``,
-``parser/cantera.py:17-27`` and ``parser/units.py:1-67``: the file is loaded
-with YAML booleans switched off (so a species ``NO`` stays a name), reaction
-sides are split into stoichiometric dicts, third bodies and falloff colliders
-are classified, explicit orders override stoichiometry, and Arrhenius
-parameters are converted to SI (kmol, m, s, activation energy as T in K).
+- a reaction equation is ``side arrow side``, the arrow ``<=>`` or ``=`` (reversible) or ``=>``;
+  a side is terms joined by ``+`` tokens, each ``[coefficient] species``, ``M`` for a generic
+  third body, and ``(+X)`` after both sides for a falloff reaction's collider X;
+- ``three-body`` reactions raise the order by one; a three-body reaction written with an explicit
+  species as its body takes that species' efficiency alone;
+- explicit ``orders`` replace the reactants' coefficients in the forward rate.
+
+The YAML is read with booleans left as strings, so that a species named ``NO`` or ``ON`` stays a
+name.
 """
 
 import re
@@ -13,235 +20,257 @@ import re
 import yaml
 
 from peregrine_amd.mixture.constants import Ru, Tref, avogadro, debye, kB
-from peregrine_amd.mixture.species import Species
+from peregrine_amd.mixture.species import ATOMIC_WEIGHTS, molecular_weight
 
-# J/kmol per unit of energy
+# J/kmol per unit of activation energy; m per unit of length; kmol per unit of quantity
 ENERGY = {"cal/mol": 4184.0, "kcal/mol": 4.184e6, "J/mol": 1.0e3, "kJ/mol": 1.0e6, "J/kmol": 1.0, "K": Ru}
 LENGTH = {"cm": 1.0e-2, "m": 1.0, "angstrom": 1.0e-10}
 QUANTITY = {"mol": 1.0e-3, "kmol": 1.0, "molec": 1.0 / avogadro}
+# a transport entry's keys: (this package's name, SI factor)
 TRANSPORT = {
     "well-depth": ("well", kB),
     "diameter": ("diam", 1.0e-10),
     "dipole": ("dipole", debye),
     "polarizability": ("polarize", 1.0e-30),
 }
-# which reaction kinds carry a third body, a falloff blend, Troe's blend, and the body's share of the order
+# per reaction kind: a third body, a falloff blend, Troe's blend, and the body's share of the order
 KINDS = {
     "elementary": dict(thirdbody=False, falloff=False, troe=False, bodyorder=0),
     "three-body": dict(thirdbody=True, falloff=False, troe=False, bodyorder=1),
     "lindemann": dict(thirdbody=True, falloff=True, troe=False, bodyorder=0),
     "troe": dict(thirdbody=True, falloff=True, troe=True, bodyorder=0),
 }
+ARROWS = {"<=>": True, "=": True, "=>": False}
+_COLLIDER = re.compile(r"\(\s*\+\s*([^\s()]+)\s*\)")
+_NUMBER = re.compile(r"\d+(\.\d*)?([eE][-+]?\d+)?$")
 
 
-class _Loader(yaml.SafeLoader):
-    pass
+def _loader():
+    """A safe YAML loader that resolves no plain scalar to a boolean."""
+
+    class NoBooleans(yaml.SafeLoader):
+        pass
+
+    NoBooleans.yaml_implicit_resolvers = {
+        first: [r for r in rules if r[0] != "tag:yaml.org,2002:bool"]
+        for first, rules in yaml.SafeLoader.yaml_implicit_resolvers.items()
+    }
+    return NoBooleans
 
 
-_Loader.yaml_implicit_resolvers = {
-    k: [(tag, rx) for tag, rx in v if tag != "tag:yaml.org,2002:bool"]
-    for k, v in yaml.SafeLoader.yaml_implicit_resolvers.items()
-}
+def _unit(table, name, what):
+    if name not in table:
+        raise KeyError(f"{what} unit {name!r}: this reader knows {', '.join(table)}")
+    return table[name]
 
 
-def _factor(table, unit):
-    try:
-        return table[unit]
-    except KeyError:
-        raise KeyError(f"unit {unit!r}; one of {', '.join(table)}") from None
+def _flag(raw, key):
+    value = raw.get(key, False)
+    return value.strip().lower() in ("true", "yes", "on") if isinstance(value, str) else bool(value)
 
 
-def _molar(text):
-    value, unit = str(text).split()
-    return float(value) / _factor(QUANTITY, unit.split("/")[1])
+class Equation:
+    """A parsed reaction equation: stoichiometry per side, direction, generic body, collider."""
+
+    def __init__(self, text):
+        self.text = text
+        body = text.split("#", 1)[0]
+        colliders = set(_COLLIDER.findall(body))
+        if len(colliders) > 1:
+            raise ValueError(f"{text!r}: its sides name different falloff colliders {sorted(colliders)}")
+        self.collider = colliders.pop() if colliders else None
+        tokens = _COLLIDER.sub(" ", body).split()
+        arrows = [n for n, t in enumerate(tokens) if t in ARROWS]
+        if len(arrows) != 1:
+            raise ValueError(f"{text!r}: an equation has one arrow (<=>, = or =>)")
+        cut = arrows[0]
+        self.reversible = ARROWS[tokens[cut]]
+        self.reactants, bodyL = self._side(tokens[:cut])
+        self.products, bodyR = self._side(tokens[cut + 1:])
+        if bodyL != bodyR:
+            raise ValueError(f"{text!r}: a generic third body M stands on both sides or on neither")
+        self.generic_body = bodyL
+
+    def _side(self, tokens):
+        terms, term = [], []
+        for tok in tokens + ["+"]:
+            if tok == "+":
+                if not term:
+                    raise ValueError(f"{self.text!r}: an empty term")
+                terms.append(term)
+                term = []
+            else:
+                term.append(tok)
+        stoich, bodies = {}, 0
+        for term in terms:
+            if len(term) == 1:
+                coeff, name = 1.0, term[0]
+            elif len(term) == 2 and _NUMBER.match(term[0]):
+                coeff, name = float(term[0]), term[1]
+            else:
+                raise ValueError(f"{self.text!r}: cannot read the term {' '.join(term)!r}")
+            if name == "M":
+                bodies += 1
+                continue
+            stoich[name] = stoich.get(name, 0.0) + coeff
+        if bodies > 1:
+            raise ValueError(f"{self.text!r}: more than one M on a side")
+        return stoich, bodies == 1
 
 
-def _truthy(sect, key):
-    v = sect.get(key, False)
-    if isinstance(v, str):
-        return v.strip().lower() in ("true", "yes", "on")
-    return bool(v)
+def _arrhenius(sect, order, units):
+    """(A, b, Ea/Ru) in kmol, m, s and K, A carrying concentration^(1 - order)."""
+    A = float(sect["A"]) * (units["length"] ** 3 / units["quantity"]) ** (order - 1)
+    return A, float(sect["b"]), float(sect["Ea"]) * units["energy"] / Ru
 
 
-def _side(text):
-    species, hasm = {}, False
-    text = re.sub(r"\(\s*\+[^)]+\)", "", text).strip()
-    for term in (t.strip() for t in text.split("+")):
-        if not term:
-            continue
-        if term == "M":
-            if hasm:
-                raise ValueError("more than one generic third body M on a side")
-            hasm = True
-            continue
-        m = re.match(r"^(\d+\.?\d*)\s+(.+)$", term)
-        coeff, name = (float(m.group(1)), m.group(2).strip()) if m else (1.0, term)
-        species[name] = species.get(name, 0.0) + coeff
-    return species, hasm
+def _kind(raw, eq):
+    """The reaction's kind and its third-body efficiencies (default, {species: efficiency})."""
+    declared = raw.get("type")
+    effs = {name: float(e) for name, e in (raw.get("efficiencies") or {}).items()}
+    default = float(raw.get("default-efficiency", 1.0))
+    where = repr(eq.text)
+    if declared in ("falloff", "Troe", "Lindemann"):
+        if eq.collider is None:
+            raise ValueError(f"{where}: a falloff reaction names its collider as (+M) or (+species)")
+        if eq.collider != "M":
+            # one named species is the whole bath
+            if ("default-efficiency" in raw and default != 0.0) or (effs and set(effs) != {eq.collider}):
+                raise ValueError(f"{where}: efficiencies other than its own collider's")
+            return ("troe" if "Troe" in raw else "lindemann"), 0.0, effs or {eq.collider: 1.0}
+        return ("troe" if "Troe" in raw else "lindemann"), default, effs
+    if declared in ("three-body", "three-body-Arrhenius") or (declared is None and eq.generic_body):
+        if eq.collider is not None:
+            raise ValueError(f"{where}: a (+{eq.collider}) collider belongs to a falloff reaction")
+        if eq.generic_body:
+            return "three-body", default, effs
+        # the body is a species written on both sides: one of it leaves each side
+        if len(effs) != 1:
+            raise ValueError(f"{where}: a three-body reaction without M names one body by its efficiency")
+        (body,) = effs
+        for side in (eq.reactants, eq.products):
+            if side.get(body, 0.0) < 1.0:
+                raise ValueError(f"{where}: the body {body!r} stands on both sides")
+            side[body] -= 1.0
+            if side[body] == 0.0:
+                del side[body]
+        return "three-body", 0.0, effs
+    if declared in (None, "elementary", "reaction", "Arrhenius"):
+        if eq.generic_body or eq.collider is not None:
+            raise ValueError(f"{where}: an elementary reaction has no third body")
+        if "efficiencies" in raw or "default-efficiency" in raw:
+            raise ValueError(f"{where}: efficiencies without a third body")
+        return "elementary", 1.0, {}
+    raise ValueError(f"{where}: reaction type {declared!r} is not one this reader takes")
 
 
-def _equation(equation):
-    equation = equation.split("#")[0].strip()
-    colliders = {c.strip() for c in re.findall(r"\(\s*\+([^)]+)\)", equation)}
-    if len(colliders) > 1:
-        raise ValueError(f"inconsistent falloff colliders in {equation!r}")
-    collider = colliders.pop() if colliders else None
-    if "<=>" in equation:
-        lhs, rhs = equation.split("<=>")
-        reversible = True
-    elif "=>" in equation:
-        lhs, rhs = equation.split("=>")
-        reversible = False
+def _reaction(raw, units):
+    eq = Equation(raw["equation"])
+    kind, default, effs = _kind(raw, eq)
+    orders = {name: float(v) for name, v in (raw.get("orders") or {}).items()}
+    if orders:
+        if eq.reversible:
+            raise ValueError(f"{eq.text!r}: explicit orders on a reversible reaction")
+        if not _flag(raw, "nonreactant-orders") and set(orders) - set(eq.reactants):
+            raise ValueError(f"{eq.text!r}: an order for a species that does not react")
+        if not _flag(raw, "negative-orders") and min(orders.values()) < 0.0:
+            raise ValueError(f"{eq.text!r}: a negative order")
+    fwd = {**eq.reactants, **orders}
+    order = sum(fwd.values()) + KINDS[kind]["bodyorder"]
+    out = dict(equation=eq.text, type=kind, reversible=eq.reversible, duplicate=_flag(raw, "duplicate"),
+               reactants=eq.reactants, products=eq.products, fwd=fwd, efficiencies=effs, defeff=default)
+    if KINDS[kind]["falloff"]:
+        out["rate"] = _arrhenius(raw["high-P-rate-constant"], order, units)
+        out["lowrate"] = _arrhenius(raw["low-P-rate-constant"], order + 1, units)
+        if kind == "troe":
+            troe = raw["Troe"]
+            out["troe"] = [float(troe["A"]), float(troe["T3"]), float(troe["T1"]), float(troe.get("T2", 0.0))]
     else:
-        raise ValueError(f"cannot parse equation {equation!r}")
-    reactants, lm = _side(lhs)
-    products, rm = _side(rhs)
-    if lm != rm:
-        raise ValueError(f"third body M must be on both sides: {equation!r}")
-    return reactants, products, reversible, lm, collider
+        out["rate"] = _arrhenius(raw["rate-constant"], order, units)
+    return out
+
+
+def _per_mass(text, MW):
+    """A molar quantity 'value unit/quantity' (energy per mole, say) per kg."""
+    value, unit = str(text).split()
+    per = unit.split("/")[1]
+    return float(value) / _unit(QUANTITY, per, "quantity") / MW
+
+
+def _nasa7(thermo, record):
+    edges = [float(t) for t in thermo["temperature-ranges"]]
+    sets = [[float(c) for c in row] for row in thermo["data"]]
+    if len(sets) == 1:
+        record["NASA7"] = {"T-range": [edges[0], edges[-1], edges[-1]], "low": sets[0], "high": sets[0]}
+    else:
+        record["NASA7"] = {"T-range": [edges[0], edges[1], edges[2]], "low": sets[0], "high": sets[1]}
+
+
+def _constant_cp(thermo, record):
+    MW = record["MW"]
+    T0 = float(str(thermo.get("T0", f"{Tref} K")).split()[0])
+    record["cp0"] = _per_mass(thermo["cp0"], MW)
+    h0 = _per_mass(thermo["h0"], MW) * MW if "h0" in thermo else 0.0
+    # the heat of formation at the reference temperature, J/kmol
+    record["dHf298"] = h0 + record["cp0"] * MW * (Tref - T0)
+
+
+THERMO = {"NASA7": _nasa7, "constant-cp": _constant_cp}
+
+
+def _species(raw, weights):
+    record = {}
+    if "composition" in raw:
+        record["comp"] = {el: float(n) for el, n in raw["composition"].items()}
+        record["MW"] = molecular_weight(record["comp"], weights)
+    thermo = raw.get("thermo")
+    if thermo:
+        model = thermo["model"]
+        reader = THERMO.get(model) or THERMO.get(model.upper())
+        if reader is None:
+            raise ValueError(f"species {raw['name']}: thermo model {model!r}; this reader takes {sorted(THERMO)}")
+        reader(thermo, record)
+    tr = raw.get("transport")
+    if tr:
+        record["geometry"] = tr["geometry"]
+        for key, (name, factor) in TRANSPORT.items():
+            record[name] = float(tr.get(key, 0.0)) * factor
+        record["zrot"] = float(tr.get("rotational-relaxation", 0.0))
+        if "acentric-factor" in tr:
+            record["acentric"] = float(tr["acentric-factor"])
+    return record
 
 
 class CanteraMechanism:
+    """The species (in the phase's order, or the order given) and reactions of one file."""
+
     suffixes = (".yaml", ".yml")
-    defunits = {"length": "m", "quantity": "kmol", "activation-energy": "J/kmol"}
 
     def __init__(self, path, species=None):
         with open(path) as f:
-            data = yaml.load(f, Loader=_Loader)
+            doc = yaml.load(f, Loader=_loader())
         if species is None:
-            if "phases" not in data:
-                raise ValueError(f"{path} has no phases section; give the species list")
-            species = list(data["phases"][0]["species"])
-        lookup = {sp["name"]: sp for sp in data.get("species", [])}
-        weights = dict(Species.atomwts)
-        for el in data.get("elements", []):
-            weights[el["symbol"]] = float(el["atomic-weight"])
-        self.species = {}
-        for name in species:
-            if name not in lookup:
-                raise KeyError(f"species {name!r} not in {path}")
-            self.species[name] = self._species(lookup[name], weights)
+            phases = doc.get("phases")
+            if not phases:
+                raise ValueError(f"{path}: no phases section to take the species list from")
+            species = list(phases[0]["species"])
+        weights = {**ATOMIC_WEIGHTS, **{e["symbol"]: float(e["atomic-weight"]) for e in doc.get("elements", [])}}
+        entries = {entry["name"]: entry for entry in doc.get("species", [])}
+        absent = [name for name in species if name not in entries]
+        if absent:
+            raise KeyError(f"{path}: no species entry for {absent}")
+        self.species = {name: _species(entries[name], weights) for name in species}
+        declared = doc.get("units", {})
+        units = dict(length=_unit(LENGTH, declared.get("length", "m"), "length"),
+                     quantity=_unit(QUANTITY, declared.get("quantity", "kmol"), "quantity"),
+                     energy=_unit(ENERGY, declared.get("activation-energy", "J/kmol"), "activation-energy"))
         self.reactions = []
-        units = {**self.defunits, **data.get("units", {})}
-        self.energy = _factor(ENERGY, units["activation-energy"])
-        self.length = _factor(LENGTH, units["length"])
-        self.quantity = _factor(QUANTITY, units["quantity"])
-        for raw in data.get("reactions") or []:
-            rxn = self._reaction(raw)
-            unknown = (set(rxn["reactants"]) | set(rxn["products"]) | set(rxn["efficiencies"])) - set(self.species)
-            if unknown:
-                raise KeyError(f"reaction {rxn['equation']!r} names species not in the phase: {sorted(unknown)}")
+        for raw in doc.get("reactions") or []:
+            rxn = _reaction(raw, units)
+            named = set(rxn["reactants"]) | set(rxn["products"]) | set(rxn["efficiencies"])
+            if named - set(self.species):
+                raise KeyError(f"{rxn['equation']!r} names species outside the phase: {sorted(named - set(self.species))}")
             self.reactions.append(rxn)
-
-    def arrhenius(self, sect, order):
-        A = float(sect["A"]) * (self.length**3 / self.quantity) ** (order - 1)
-        return A, float(sect["b"]), float(sect["Ea"]) * self.energy / Ru
-
-    def _reaction(self, raw):
-        equation = raw["equation"]
-        reactants, products, reversible, hasm, collider = _equation(equation)
-        kind = raw.get("type")
-        effs = {k: float(v) for k, v in raw.get("efficiencies", {}).items()}
-        defeff = float(raw.get("default-efficiency", 1.0))
-        kind = {"Arrhenius": "elementary", "three-body-Arrhenius": "three-body", "Lindemann": "falloff", "Troe": "falloff"}.get(kind, kind)
-        hastroe = "Troe" in raw
-        if kind == "falloff":
-            rtype = "troe" if hastroe else "lindemann"
-            if collider is None:
-                raise ValueError(f"falloff reaction without a (+M) collider: {equation!r}")
-            if collider != "M":
-                if "default-efficiency" in raw and defeff != 0.0:
-                    raise ValueError(f"default efficiency with an explicit collider: {equation!r}")
-                if effs and (len(effs) != 1 or collider not in effs):
-                    raise ValueError(f"incompatible collider definitions: {equation!r}")
-                defeff = 0.0
-                effs = effs or {collider: 1.0}
-        elif kind == "three-body" or (kind is None and hasm):
-            rtype = "three-body"
-            if collider is not None:
-                raise ValueError(f"(+{collider}) needs type falloff: {equation!r}")
-            if not hasm:
-                if len(effs) != 1:
-                    raise ValueError(f"an explicit third body needs one efficiency: {equation!r}")
-                (sp,) = effs
-                for side in (reactants, products):
-                    if side.get(sp, 0) < 1:
-                        raise ValueError(f"collider {sp!r} must be on both sides: {equation!r}")
-                    if side[sp] == 1:
-                        del side[sp]
-                    else:
-                        side[sp] -= 1
-                defeff = 0.0
-        elif kind in (None, "elementary"):
-            rtype = "elementary"
-            if hasm or collider is not None:
-                raise ValueError(f"elementary reaction with a third body: {equation!r}")
-            if "efficiencies" in raw or "default-efficiency" in raw:
-                raise ValueError(f"efficiencies without a third body: {equation!r}")
-        else:
-            raise ValueError(f"unsupported reaction type {kind!r}: {equation!r}")
-        orders = {sp: float(v) for sp, v in raw.get("orders", {}).items()}
-        if orders:
-            if reversible:
-                raise ValueError(f"orders given for a reversible reaction: {equation!r}")
-            if any(sp not in reactants for sp in orders) and not _truthy(raw, "nonreactant-orders"):
-                raise ValueError(f"order for a non-reactant: {equation!r}")
-            if any(v < 0 for v in orders.values()) and not _truthy(raw, "negative-orders"):
-                raise ValueError(f"negative order: {equation!r}")
-        fwd = dict(reactants)
-        fwd.update(orders)
-        order = sum(fwd.values()) + KINDS[rtype]["bodyorder"]
-        rxn = dict(
-            equation=equation,
-            type=rtype,
-            reversible=reversible,
-            duplicate=_truthy(raw, "duplicate"),
-            reactants=reactants,
-            products=products,
-            fwd=fwd,
-            efficiencies=effs,
-            defeff=defeff,
-        )
-        if KINDS[rtype]["falloff"]:
-            rxn["rate"] = self.arrhenius(raw["high-P-rate-constant"], order)
-            rxn["lowrate"] = self.arrhenius(raw["low-P-rate-constant"], order + 1)
-        else:
-            rxn["rate"] = self.arrhenius(raw["rate-constant"], order)
-        if hastroe:
-            troe = {"T2": 0.0, **raw["Troe"]}
-            rxn["troe"] = [float(troe[k]) for k in ("A", "T3", "T1", "T2")]
-        return rxn
-
-    @staticmethod
-    def _species(raw, weights):
-        sp = {}
-        if "composition" in raw:
-            sp["comp"] = {k: float(v) for k, v in raw["composition"].items()}
-            sp["MW"] = Species.molwt(sp["comp"], weights)
-        thermo = raw.get("thermo")
-        if thermo and thermo["model"] == "constant-cp":
-            cp0 = _molar(thermo["cp0"])
-            sp["cp0"] = cp0 / sp["MW"]
-            T0 = float(str(thermo.get("T0", "298.15 K")).split()[0])
-            h0 = _molar(thermo["h0"]) if "h0" in thermo else 0.0
-            sp["dHf298"] = h0 + cp0 * (Tref - T0)
-        elif thermo:
-            if thermo["model"].upper() != "NASA7":
-                raise ValueError(f"unsupported thermo model {thermo['model']} for {raw['name']}")
-            ranges = [float(r) for r in thermo["temperature-ranges"]]
-            data = [list(map(float, c)) for c in thermo["data"]]
-            low, high = (data[0], data[0]) if len(data) == 1 else data[:2]
-            Tmid = ranges[1] if len(data) > 1 else ranges[-1]
-            sp["NASA7"] = {"T-range": [ranges[0], Tmid, ranges[-1]], "high": high, "low": low}
-        if "transport" in raw:
-            tr = raw["transport"]
-            sp["geometry"] = tr["geometry"]
-            for key, (ours, scale) in TRANSPORT.items():
-                sp[ours] = float(tr.get(key, 0.0)) * scale
-            sp["zrot"] = float(tr.get("rotational-relaxation", 0.0))
-            if "acentric-factor" in tr:
-                sp["acentric"] = float(tr["acentric-factor"])
-        return sp
 
 
 def read_mechanism(path, species=None):

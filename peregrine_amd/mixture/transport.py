@@ -71,12 +71,12 @@ class KineticTheory(BaseModel):
     def _params(self, species):
         from peregrine_amd.mixture.constants import avogadro, epsilon0
         from peregrine_amd.mixture.integrals import CollisionIntegrals
-        from peregrine_amd.mixture.species import Species
+        from peregrine_amd.mixture.species import HALF_ROTATIONAL_DOF
 
         MW = self.collect("MW", species)
         well, diam = self.collect("well", species), self.collect("diam", species)
         dipole = self.collect("dipole", species)
-        rotdof = np.array([Species.rotdof[sp["geometry"]] for sp in species.values()])
+        rotdof = np.array([HALF_ROTATIONAL_DOF[sp["geometry"]] for sp in species.values()])
         mass = MW / avogadro
         # the reduced dipole of each species with itself (the pair terms a mixture rule would read
         # are the diagonal here: a species' own viscosity and conductivity)

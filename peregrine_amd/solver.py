@@ -380,6 +380,8 @@ class Solver:
             self._call("pga_fromprims", self.case.handle, blk.q.view(), blk.Q.view(), blk.state.view(), full)
         self.readied.clear()
         self.postrhs(0)
+        # the next step skips its first halo pass: it is not the steady step a graph holds
+        self._fresh = True
 
     # -- launches --------------------------------------------------------------
     timers = None
@@ -670,18 +672,20 @@ class Solver:
                                    blk.detJinv.view(), rates, len(terms), src, w, withdt, self.dt.ptr, sbox, kc)
 
     _graph = None
+    _fresh = True
     replays = 0
 
     def _standing(self):
         return tuple(sorted(self.readied.items(), key=str))
 
     def _graphed(self):
-        """A step replays one captured HIP graph once its launches are the steady ones: after a
-        first step, with the registers' halos standing as they stood when the graph was captured
-        (a step after from_prims skips its first halo pass) and no launch timing. Everything a step
-        reads that changes between steps is on the device (Q, the state, dt), so the same graph
-        serves every such step."""
-        if not self.cfg["backend-mi355x"]["graphs"] or self.nrt == 0 or self.timers is not None:
+        """A step replays one captured HIP graph once its launches are the steady ones: not the
+        first step after from_prims (which skips its first halo pass, and makes the arrays a step
+        creates on first use), with the registers' halos standing as they stood when the graph was
+        captured and no launch timing. Everything a step reads that changes between steps is on the
+        device (Q, the state, dt), so the same graph serves every such step, a restarted case's
+        included."""
+        if not self.cfg["backend-mi355x"]["graphs"] or self._fresh or self.timers is not None:
             return False
         return self._graph is None or self._graph[1] == self._standing()
 
@@ -714,6 +718,7 @@ class Solver:
             self._replay(launches)
         else:
             launches()
+        self._fresh = False
         self.nrt += 1
         self.tme.set(start + dt)
 

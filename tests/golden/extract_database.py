@@ -1,7 +1,8 @@
 """Writes the species, collision-integral and Chung tables this package reads, from the
 reference's database files.
 
-Run once where the reference tree is mounted; the outputs are committed data:
+Run once where the reference tree is mounted, with the destination directory as its argument
+(peregrine_amd/data); the outputs are committed data:
 - species.yaml: the species of the H/O/N mixtures (H2, H, O, O2, OH, H2O, HO2, H2O2, N2, AR)
   with their molecular data as the reference compiles them (NIST-JANAF thermo tables,
   Lennard-Jones and polar transport parameters, critical constants), each entry keeping
@@ -28,7 +29,7 @@ def main(dst):
     with open(dst / "species.yaml", "w") as f:
         f.write("# Molecular data of the H/O/N species: NIST-JANAF thermo tables (J/kmol/K, J/kmol),\n"
                 "# Lennard-Jones and polar transport parameters (SI), critical constants; extracted by\n"
-                "# extract_database.py.\n")
+                "# tests/golden/extract_database.py.\n")
         yaml.safe_dump(sub, f, sort_keys=False, default_flow_style=None, width=100)
     ints = yaml.safe_load(open(SRC / "integrals.yaml"))
     with open(dst / "integrals.yaml", "w") as f:

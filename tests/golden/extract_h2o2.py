@@ -1,36 +1,35 @@
 """Writes h2o2.yaml, the H/O-only subset of GRI-Mech 3.0, from the reference's GRI30.yaml.
 
-Run once where the reference tree is mounted; the output is committed data. The
+Run once where the reference tree is mounted, with the destination directory as its argument
+(peregrine_amd/data/mechanisms); the output is committed data. The
 species kept are H2, H, O, O2, OH, H2O, HO2, H2O2 and N2 (the bath, last); a
 reaction is kept when every species on both sides is among them, and its
 third-body efficiencies are restricted to them.
 """
 
-import re
 import sys
+from pathlib import Path
 
 import yaml
+
+sys.path.insert(0, str(Path(__file__).resolve().parents[2]))
+from peregrine_amd.mixture.parser import Equation  # noqa: E402
 
 SRC = "/root/reference/peregrine/simulator/mixture/database/mechanisms/GRI30.yaml"
 KEEP = "H2 H O O2 OH H2O HO2 H2O2 N2".split()
 
 
-def names(side):
-    side = re.sub(r"\(\s*\+[^)]+\)", "", side)
-    out = []
-    for t in (t.strip() for t in side.split("+")):
-        if t and t != "M":
-            m = re.match(r"^(\d+\.?\d*)\s+(.+)$", t)
-            out.append(m.group(2) if m else t)
-    return out
+def species_of(equation):
+    """Every species an equation names on either side (its third bodies aside)."""
+    eq = Equation(equation)
+    return list(eq.reactants) + list(eq.products)
 
 
 def main(dst):
     d = yaml.safe_load(open(SRC))
     rx = []
     for r in d["reactions"]:
-        lhs, rhs = re.split(r"<=>|=>", r["equation"])
-        if all(x in KEEP for x in names(lhs) + names(rhs)):
+        if all(x in KEEP for x in species_of(r["equation"])):
             r = dict(r)
             if "efficiencies" in r:
                 r["efficiencies"] = {k: v for k, v in r["efficiencies"].items() if k in KEEP}

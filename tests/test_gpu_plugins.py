@@ -141,6 +141,8 @@ def test_a_restart_continues_as_the_case_would_have(gpu, tmp_path):
     for a, b in zip(_interior(again), expect):
         assert _scaled(a, b) < 1e-9
     assert again.tme.get() == mb.tme.get()
+    # the restarted case's steps after its first replay the captured graph, as the first case's do
+    assert again.replays == 1 and mb.replays == 4
 
 
 def test_the_writer_plugins_result_restarts_the_case(gpu, tmp_path):
