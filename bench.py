@@ -164,6 +164,50 @@ def batch(args, pg, support):
     print(json.dumps(line), flush=True)
 
 
+def serve(args, pg, support):
+    """The serving stream: args.serve requests (80% 0D H2-air ignitions to 0.2 ms, 15% Sod tubes of
+    200-4000 cells, 5% Taylor-Green boxes of 16^3-64^3, 20 steps) arriving as a Poisson stream at
+    args.rate per second (default_rng(20240611)), served with continuous batching (value: solves/s
+    over the run's wall time, with p50 and p99 latency), against the same requests served one at a
+    time, one after another (plain per-case solves/s); the largest difference of any result between
+    the two is printed beside them."""
+    from peregrine_amd import serving
+
+    backend = pg.get_backend("mi355x", support.config("air-cpg"))
+    # warm-up: compile caches and first-use allocations on a few requests of each kind
+    warm = serving.Server(backend, reactors=64)
+    warm.run([serving.ignition(1200.0, 1e6, 1.0, tend=1e-6), serving.tube(200), serving.taylor_green(16, steps=2)])
+    reqs = serving.poisson_stream(args.rate, args.serve)
+    srv = serving.Server(backend, reactors=512)
+    done, wall = srv.run(reqs)
+    p50, p99 = serving.percentiles(done)
+    alone = serving.Server(backend, reactors=1, solvers=1)
+    plain = {}
+    t0 = time.perf_counter()
+    for r in reqs:
+        one = serving.Request(r.kind, dict(r.params), rid=r.rid)
+        (d,), _ = alone.run([one])
+        plain[r.rid] = d
+    t_plain = time.perf_counter() - t0
+    gap = 0.0
+    for r in done:
+        a, b = r.result, plain[r.rid].result
+        for k in a:
+            if isinstance(a[k], (float, np.ndarray)) and k != "delay":
+                x, y = np.asarray(a[k], float), np.asarray(b[k], float)
+                gap = max(gap, float(np.abs(x - y).max() / max(np.abs(y).max(), 1e-300)))
+    kinds = {k: sum(1 for r in done if r.kind == k) for k in ("ignition", "tube", "tgv")}
+    line = {
+        "metric": "batched solves/s + p99 latency (serving stream)", "value": len(done) / wall, "unit": "solves/s",
+        "n_gpus": 1, "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"{len(reqs)} requests, Poisson at {args.rate}/s (seed 20240611): {kinds}",
+                   "reactor_slots": 512, "budget_bytes": srv.budget},
+        "p50_s": p50, "p99_s": p99, "wall_s": wall, "plain_per_case": len(reqs) / t_plain,
+        "speedup": t_plain / wall, "max_result_gap_alone_vs_mix": gap,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def cfg_stages(cfg):
     return {"rk1": 1, "rk2": 2, "tvd-rk3": 3, "rk4": 4, "rk34": 4}[cfg["time-integration"]["integrator"]]
 
@@ -184,6 +228,9 @@ def main():
     ap.add_argument("--batch", type=int, default=0,
                     help="the uneven batch instead: this many periodic H2/O2 cases of 16-96 cells per axis "
                          "(seeded) stepped together, against the same cases stepped one by one")
+    ap.add_argument("--serve", type=int, default=0,
+                    help="the serving stream instead: this many requests arriving as a Poisson stream")
+    ap.add_argument("--rate", type=float, default=400.0, help="the serving stream's arrivals per second")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only a plain device copy of 512 MiB, five times: the reference for HBM byte counters")
     args = ap.parse_args()
@@ -209,6 +256,8 @@ def main():
 
     if args.batch:
         return batch(args, pg, support)
+    if args.serve:
+        return serve(args, pg, support)
     cfg = case_config(support)
     cfg["backend-mi355x"]["fused-stage"] = not args.per_kernel
     if args.k_chunk:

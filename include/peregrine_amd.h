@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define PGA_ABI_VERSION 3
+#define PGA_ABI_VERSION 4
 
 enum {
   PGA_OK = 0,
@@ -169,6 +169,17 @@ int32_t pga_stage(const pga_case *c, pga_view Qin, pga_view stin, pga_view Q0, p
                   const int32_t *src, const double *w, const int32_t *withdt, const double *dt, pga_box box,
                   int32_t kchunk, void *stream);
 
+/* A stage with two combines per cell (rk4's): pga_stage's divergence, source and EOS, with a first
+ * combine into QA before the one into Qout. The first's terms read Qin (0), Q0 (1), dQ (2) or QA as it
+ * stood (3); the second's read Qin, Q0, dQ or the value the first just took (4). Replaces, for one rk4
+ * stage, pg_applyflux and the two pg_combine launches of peregrine/integrators/rungekutta.py:84-95 (the
+ * running total, then the next stage or the step's end) and the interior pg_fromQ. */
+int32_t pga_stage2(const pga_case *c, pga_view Qin, pga_view stin, pga_view Q0, pga_view QA, pga_view Qout,
+                   pga_view stout, pga_view iF, pga_view jF, pga_view kF, pga_view detJinv, pga_view rates,
+                   int32_t nA, const int32_t *srcA, const double *wA, const int32_t *withdtA, int32_t nterms,
+                   const int32_t *src, const double *w, const int32_t *withdt, const double *dt, pga_box box,
+                   int32_t kchunk, void *stream);
+
 /* The carried species' production rates (kg/m^3/s) of every cell in box into rates (ns - 1
  * components): the coupled source of a reacting stage. The body of pg_source
  * (peregrine/simulator/chemint/kernels/source.mako:3-10) writing only the species rows. */
@@ -260,6 +271,26 @@ int64_t pga_cflmax_worksize(pga_box box);
 int32_t pga_cflmax(const pga_case *c, pga_view Q, pga_view state, pga_view dijk, pga_view iS, pga_view jS,
                    pga_view kS, pga_box box, double *work, int64_t nwork, double *out, void *stream);
 
+/* --- graphs and the all-finite check ----------------------------------------------------- */
+
+typedef struct pga_graph pga_graph;
+
+/* Capture the launches issued on `stream` (not the null stream) from pga_graph_begin to pga_graph_end
+ * into a graph, instantiated for replay by pga_graph_launch on any stream. The library's launches
+ * record no memset, allocation or host synchronisation, so a step's launches capture whole.
+ * Replaces peregrine/backend/device/graph.py:10-47 and peregrine/backend/hip/driver.py:159-176
+ * (hipStreamBeginCapture, hipStreamEndCapture, hipGraphInstantiate, hipGraphLaunch). */
+int32_t pga_graph_begin(void *stream);
+int32_t pga_graph_end(void *stream, pga_graph **out);
+int32_t pga_graph_launch(const pga_graph *g, void *stream);
+int32_t pga_graph_destroy(pga_graph *g);
+
+/* out[0] (device int32) = 1 when every one of the ncomp components of every cell of box is finite, else
+ * 0; set on the device by two launches (no memset, so graph-capturable). Replaces the nancheck plugin's
+ * kernel and reduction: peregrine/plugins/solver/kernels/allfinite.mako:3-19,
+ * peregrine/plugins/solver/nancheck.py. */
+int32_t pga_allfinite(pga_view A, int32_t ncomp, pga_box box, int32_t *out, void *stream);
+
 /* Sub-stepped chemistry's options: chem-integration.max-substeps, .bulk-substeps, .stiff-bins and
  * .equilibrium-tolerance (peregrine/simulator/chemint/substepped.py:14-33) */
 typedef struct {
@@ -285,6 +316,99 @@ int64_t pga_substep_listsize(const pga_substep_opts *o, pga_box box);
 int32_t pga_substep(const pga_case *c, const pga_substep_opts *o, pga_view Q, pga_view state, pga_view out,
                     int32_t begin_dQ, int32_t *substeps, int64_t nsubsteps, int32_t *list, int64_t nlist,
                     const double *dt, pga_box box, void *stream);
+
+/* Implicit chemistry's options: chem-integration.implicit-parts, .max-sweeps and
+ * .refine-tolerance. No counterpart in the reference, whose chemistry is coupled or sub-stepped
+ * (peregrine/simulator/chemint/{coupled,substepped}.py); this is the stiff alternative to the latter. */
+typedef struct {
+  int32_t parts;      /* linearly implicit Euler steps per step, >= 1 */
+  int32_t max_sweeps; /* refinement corrections per linear solve, >= 1 */
+  double tolerance;   /* a solve is done when every row's residual is within this share of the row's scale */
+} pga_implicit_opts;
+
+/* Doubles of device work pga_implicit needs for `box` (the FP32 factors and FP64 matrices of 16-cell
+ * groups, the FP64 unknowns and rates); -1 for a bad box or more than 11 species. */
+int64_t pga_implicit_worksize(const pga_case *c, pga_box box);
+
+/* Implicit chemistry over the cells of `box` for a step of *dt (device): each cell integrates the
+ * same 0D system the sub-stepped form does (its ns mass fractions and T at constant density and
+ * internal energy, chemint/kernels/substep.mako:14-50) by the linearly implicit Euler method in
+ * o->parts parts, y1 = y0 + h x with (I - h J) x = f(y0), J by forward differences in FP64. Each
+ * matrix is factored in FP32 on the matrix cores, 16 cells per wave, and each solve refined against
+ * the FP64 matrix with FP64 residuals until every row's residual is within o->tolerance of its
+ * scale |f_i| + sum_j |A_ij x_j|. It writes the mean rate over the step, rho (Y_end - Y_start) / dt,
+ * into dQ (rows 0..4 zero) when begin_dQ, else into the ns - 1 rows of `out` as pga_rates does, and
+ * each cell's refinement corrections over its parts into `sweeps` (int32, box-shaped, i fastest;
+ * 1000 more for each solve left outside the tolerance). `work`: pga_implicit_worksize doubles of
+ * device memory. Takes the place of pga_substep (the reference's chemint/substepped.py:52-68 slot)
+ * for chem-integration.kind = implicit. */
+int32_t pga_implicit(const pga_case *c, const pga_implicit_opts *o, pga_view Q, pga_view state, pga_view out,
+                     int32_t begin_dQ, int32_t *sweeps, int64_t nsweeps, double *work, int64_t nwork,
+                     const double *dt, pga_box box, void *stream);
+
+/* Doubles of device work pga_lu_solve needs for n systems of nv unknowns; -1 when nv is not 1..12. */
+int64_t pga_lu_worksize(int64_t n, int32_t nv);
+
+/* n dense systems A x = b (device memory; A row-major [n][nv][nv], b and x [n][nv], FP64), nv <= 12:
+ * each A rounded to FP32 and factored without pivoting by the batched LU on the matrix cores, then
+ * x refined by x += (LU)^-1 (b - A x) with the residual in FP64 until every row's residual is within
+ * `tolerance` of |b_i| + sum_j |A_ij x_j| or max_sweeps corrections are taken; sweeps (or null)
+ * receives each system's corrections (max_sweeps + 1 when still outside the tolerance). The linear
+ * kernel of pga_implicit, exposed on its own. No counterpart in the reference. */
+int32_t pga_lu_solve(int64_t n, int32_t nv, const double *A, const double *b, double *x, int32_t max_sweeps,
+                     double tolerance, int32_t *sweeps, double *work, int64_t nwork, void *stream);
+
+/* --- launches over a table of blocks ------------------------------------------------------ */
+
+/* One block of a batch: its registers (the first nregs of Q), state, face areas and face flux
+ * arrays per axis, 1/J and (reacting) rates, its interior box, and its halo slabs as pga_halo takes
+ * them: slab t fills slab_extent[t] cells at slab_dst[t] of this block from block slab_src[t] of the
+ * batch, read from slab_from[t] with orientation digits slab_orient[t] (1 2 3 aligned). Tile shapes
+ * (0 for the defaults): stage_wi the stage's tile width along i (16, 32 or 64; its tiles are
+ * stage_wi x 256 / stage_wi columns), stage_kc the k-planes a stage thread walks, flux_ki the k-sweep's
+ * tile width along i (16, 32 or 64; 256 / flux_ki faces along k). */
+typedef struct {
+  pga_view Q[4];
+  pga_view state, iS, jS, kS, iF, jF, kF, detJinv, rates;
+  pga_box interior;
+  int32_t nslab;
+  int32_t slab_src[6];
+  int32_t slab_dst[6][3], slab_from[6][3], slab_extent[6][3], slab_orient[6][3];
+  int32_t stage_wi, stage_kc, flux_ki, reserved;
+} pga_batch_block;
+
+typedef struct pga_batch pga_batch;
+
+/* Bytes of the device table a batch of nblocks blocks with nslabs halo slabs in all needs. */
+int64_t pga_batch_bytes(int32_t nblocks, int32_t nslabs);
+
+/* A batch over `table` (device memory of pga_batch_bytes, the caller's, uploaded here once): checks
+ * every block as the block-at-a-time entry points check theirs, and writes a host handle to *out.
+ * The arrays the blocks name must outlive the batch; the library frees nothing of the caller's.
+ * Replaces the reference's per-kernel range tables over a rank's blocks (peregrine/launch/table.py:
+ * 83-106, policy.mako:8-13) for the fused stage's phases. */
+int32_t pga_batch_create(const pga_case *c, int32_t nblocks, const pga_batch_block *blocks, int32_t nregs,
+                         void *table, int64_t nbytes, pga_batch **out);
+int32_t pga_batch_destroy(pga_batch *b);
+
+/* The fused stage's phases over every block in one launch each, each block's result the
+ * block-at-a-time entry point's to the bit: the halo pass of register q (pga_halo without turns),
+ * the coupled rates of register q into each block's rates (pga_rates), the flux sweep of one axis
+ * (pga_flux), and the stage from register q into register qout with the step's start in register 0
+ * (pga_stage: divergence, source, combine, then the EOS into each block's state). */
+int32_t pga_batch_halo(const pga_batch *b, int32_t q, void *stream);
+int32_t pga_batch_rates(const pga_batch *b, int32_t q, void *stream);
+int32_t pga_batch_flux(const pga_batch *b, int32_t q, int32_t axis, void *stream);
+int32_t pga_batch_stage(const pga_batch *b, int32_t q, int32_t qout, int32_t nterms, const int32_t *src,
+                        const double *w, const int32_t *withdt, const double *dt, void *stream);
+
+/* Implicit chemistry (pga_implicit's method and kernels) over every block's cells of register q in one
+ * launch per phase, the mean rates into each block's rates array; sweeps (int32) and work (doubles,
+ * pga_batch_implicit_worksize) are device memory laid out by the batch's cell tiles. No counterpart
+ * in the reference. */
+int64_t pga_batch_implicit_worksize(const pga_batch *b);
+int32_t pga_batch_implicit(const pga_batch *b, const pga_implicit_opts *o, int32_t q, int32_t *sweeps,
+                           int64_t nsweeps, double *work, int64_t nwork, const double *dt, void *stream);
 
 #ifdef __cplusplus
 }

@@ -68,13 +68,14 @@ class Block(C.Structure):
         ("trstate", Arr), ("hsp", Arr), ("grads", Arr), ("Jinv", Arr), ("xcc", Arr), ("xf", Arr * 3),
         ("bc", C.c_int * 6), ("nstate", C.c_int), ("bcwall", (C.c_double * 3) * 6), ("bcT", C.c_double * 6),
         ("sponge", C.c_int), ("spongeline", C.c_double * 6),
+        ("imparts", C.c_int),
     ]
 
 
-def _load():
-    if not LIB.is_file():
-        raise RuntimeError(f"{LIB} is not built; run `make -C {HERE}`")
-    lib = C.CDLL(str(LIB))
+def _load(path=LIB):
+    if not path.is_file():
+        raise RuntimeError(f"{path} is not built; run `make -C {HERE}`")
+    lib = C.CDLL(str(path))
     lib.pgo_tlog.restype = C.c_double
     lib.pgo_tlog.argtypes = [C.c_double]
     lib.pgo_texp.restype = C.c_double
@@ -106,6 +107,14 @@ def _load():
     lib.pgo_dlog1p.restype = C.c_double
     lib.pgo_dlog1p.argtypes = [C.c_double]
     lib.pgo_substep.argtypes = [P(Case), Arr, Arr, Arr, C.c_int, P(C.c_int32), C.c_int, C.c_double, C.c_double, Box]
+    lib.pgo_implicit.argtypes = [P(Case), Arr, Arr, Arr, C.c_int, C.c_int, C.c_double, Box]
+    lib.pgo_implicit_cell.argtypes = [P(Case), C.c_double, _dp, C.c_double, C.c_int]
+    lib.pgo_implicit_rhs.argtypes = [P(Case), C.c_double, _dp, _dp]
+    lib.pgo_implicit_matrix.argtypes = [P(Case), C.c_double, _dp, C.c_double, _dp]
+    lib.pgo_fd_step.restype = C.c_double
+    lib.pgo_fd_step.argtypes = [C.c_double, C.c_int]
+    lib.pgo_lu_nopivot.argtypes = [C.c_int, _dp]
+    lib.pgo_lu_apply.argtypes = [C.c_int, _dp, _dp]
     lib.pgo_step.argtypes = [P(Case), P(Block), C.c_double]
     lib.pgo_stage.argtypes = [P(Case), P(Block), C.c_int, C.c_double]
     lib.pgo_advance.argtypes = [P(Case), P(Block), C.c_int, C.c_double]
@@ -114,6 +123,16 @@ def _load():
 
 
 lib = _load()
+_libm = None
+
+
+def libm():
+    """The oracle built with libm's log, cbrt and log1p and plain products in place of the specified
+    methods (libpgo_libm.so): the same functions, to measure the gap those methods make."""
+    global _libm
+    if _libm is None:
+        _libm = _load(HERE / "libpgo_libm.so")
+    return _libm
 tlog = lib.pgo_tlog
 texp = lib.pgo_texp
 elog = lib.pgo_elog
@@ -217,6 +236,26 @@ class OracleCase:
     @property
     def ref(self):
         return C.byref(self.c)
+
+    def implicit_rhs(self, rho, y):
+        """f of the 0D reactor at constant density: (w_n / rho, dT/dt) at y = (Y, T)."""
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        out = np.zeros(self.ns + 1)
+        lib.pgo_implicit_rhs(self.ref, rho, y.ctypes.data_as(_dp), out.ctypes.data_as(_dp))
+        return out
+
+    def implicit_matrix(self, rho, y, h):
+        """I - h J at y, J by the oracle's forward differences."""
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        A = np.zeros((self.ns + 1, self.ns + 1))
+        lib.pgo_implicit_matrix(self.ref, rho, y.ctypes.data_as(_dp), h, A.ctypes.data_as(_dp))
+        return A
+
+    def implicit_cell(self, rho, y, dt, parts=1):
+        """y advanced over dt by the FP64 linearly implicit form."""
+        y = np.array(y, dtype=np.float64)
+        lib.pgo_implicit_cell(self.ref, rho, y.ctypes.data_as(_dp), dt, parts)
+        return y
 
     def production(self, rho, Y, T):
         Y = np.ascontiguousarray(Y, dtype=np.float64)

@@ -20,6 +20,23 @@
 #include <omp.h>
 #endif
 
+/* The specified methods (ln by elog, cube roots by dcbrt, ln(1 + x) by dlog1p, explicit fused
+ * multiply-adds in every Horner step and mixture sum) that the device takes too; built with
+ * -DPGO_LIBM instead (libpgo_libm.so), the oracle takes the reference's host calls, libm's log, cbrt
+ * and log1p, and plain products (one rounding each, -ffp-contract=off), so the gap between the two
+ * builds measures what the specified methods move */
+#ifdef PGO_LIBM
+#define ELOG(x) log(x)
+#define DCBRT(x) cbrt(x)
+#define DLOG1P(x) log1p(x)
+#define FMA(a, b, c) ((a) * (b) + (c))
+#else
+#define ELOG(x) pgo_elog(x)
+#define DCBRT(x) pgo_dcbrt(x)
+#define DLOG1P(x) pgo_dlog1p(x)
+#define FMA(a, b, c) fma(a, b, c)
+#endif
+
 #define MAXNS 16
 #define MAXNR 64
 #define MAXTB 16
@@ -168,32 +185,32 @@ static void mix(const pgo_case *c, const double *Y, mixmom *m) {
     } else {
       Yn = Ylast;
     }
-    m->R = fma(Yn, c->Rsp[n], m->R);
+    m->R = FMA(Yn, c->Rsp[n], m->R);
     if (c->eos == 0) {
-      m->cp = fma(Yn, c->cp0[n], m->cp);
+      m->cp = FMA(Yn, c->cp0[n], m->cp);
     } else {
-      for (int t = 0; t < c->nt; t++) m->cpcoeff[t] = fma(Yn, c->cppolyR[n * c->nt + t], m->cpcoeff[t]);
+      for (int t = 0; t < c->nt; t++) m->cpcoeff[t] = FMA(Yn, c->cppolyR[n * c->nt + t], m->cpcoeff[t]);
     }
-    m->h298 = fma(Yn, c->h298R[n], m->h298);
+    m->h298 = FMA(Yn, c->h298R[n], m->h298);
     if (c->eos == 2) {
-      m->b = fma(Yn, c->cubicbMW[n], m->b);
-      m->c = fma(Yn, c->vshiftMW[n], m->c);
-      m->da = fma(Yn, c->cubicdaMW[n], m->da);
+      m->b = FMA(Yn, c->cubicbMW[n], m->b);
+      m->c = FMA(Yn, c->vshiftMW[n], m->c);
+      m->da = FMA(Yn, c->cubicdaMW[n], m->da);
     }
   }
 }
 
 /* caloric/tdepcp/kernels/caloric.mako:4-15 */
 static void ideal_cph(const pgo_case *c, const mixmom *m, double T, double *cp, double *h) {
-  const double u = pgo_elog(T);
+  const double u = ELOG(T);
   double cpv = 0.0, hv = 0.0, hc = 0.0;
   for (int t = c->nt - 1; t >= 0; t--) {
-    cpv = fma(cpv, u, m->cpcoeff[t]);
-    hc = fma(-(double)(t + 1), hc, m->cpcoeff[t]);
-    hv = fma(hv, u, hc);
+    cpv = FMA(cpv, u, m->cpcoeff[t]);
+    hc = FMA(-(double)(t + 1), hc, m->cpcoeff[t]);
+    hv = FMA(hv, u, hc);
   }
   *cp = cpv;
-  *h = fma(hv, T, m->h298);
+  *h = FMA(hv, T, m->h298);
 }
 
 /* ------------------------------------------------------------------------
@@ -210,7 +227,7 @@ typedef struct {
 #define CEPS (c->cubic[3])
 
 /* Soave's |1 + m (1 - sqrt(T / Tc))| as |(1 + m) - sqrt(T) m / sqrt(Tc)| */
-static double modulus(const pgo_case *c, int n, double sqrtT) { return fabs(fma(-sqrtT, c->cubicmTc[n], c->cubiconepm[n])); }
+static double modulus(const pgo_case *c, int n, double sqrtT) { return fabs(FMA(-sqrtT, c->cubicmTc[n], c->cubiconepm[n])); }
 
 /* eos.mako:13-18 */
 static void cubicmix(const pgo_case *c, const mixmom *m, cubicpt *q) {
@@ -227,9 +244,9 @@ static void attraction(const pgo_case *c, cubicpt *q, const double *Y, double T)
   for (int n = 0; n < c->ns - 1; n++) {
     const double Yn = Y[n];
     Ylast -= Yn;
-    s1 = fma(Yn, c->cubicsqrtacMW[n] * modulus(c, n, sqrtT), s1);
+    s1 = FMA(Yn, c->cubicsqrtacMW[n] * modulus(c, n, sqrtT), s1);
   }
-  s1 = fma(Ylast, c->cubicsqrtacMW[c->ns - 1] * modulus(c, c->ns - 1, sqrtT), s1);
+  s1 = FMA(Ylast, c->cubicsqrtacMW[c->ns - 1] * modulus(c, c->ns - 1, sqrtT), s1);
   q->S1 = s1 * q->MWmix;
   q->am = q->S1 * q->S1;
   q->dam = -q->S1 * q->S2 / sqrtT;
@@ -277,7 +294,7 @@ static void zsolve(const pgo_case *c, cubicpt *q, double p, double T) {
   const double M = R * R - Q * Q * Q;
   const double z2o3 = z2 * (1.0 / 3.0);
   if (M > 0.0) {
-    const double S = -copysign(pgo_dcbrt(fabs(R) + sqrt(M)), R);
+    const double S = -copysign(DCBRT(fabs(R) + sqrt(M)), R);
     q->Z = S + Q / S - z2o3;
   } else {
     double r0 = 2.0 * sqrt(Q) - z2o3;
@@ -308,7 +325,7 @@ static void zsolve(const pgo_case *c, cubicpt *q, double p, double T) {
       for (int k = 0; k < 3; k++) {
         const double z = roots[k];
         if (z <= B) continue;
-        const double g = z - 1.0 - pgo_elog(z - B) + AoBse * pgo_elog((z + CEPS * B) / (z + CSIG * B));
+        const double g = z - 1.0 - ELOG(z - B) + AoBse * ELOG((z + CEPS * B) / (z + CSIG * B));
         if (!found || g < gmin) {
           gmin = g;
           Z = z;
@@ -337,7 +354,7 @@ static void departures(const pgo_case *c, cubicpt *q, double T, double *hdep, do
   q->dZdT = dZ(c, q, -2.0 * q->Astar * Tinv * (1.0 - 0.5 * T * dlnamdT), -q->Bstar * Tinv);
   const double x = q->Z / q->Bstar, xm1 = x - 1.0;
   const double D = x * x + CU * x + CW;
-  q->logratio = pgo_elog((x + CEPS) / (x + CSIG));
+  q->logratio = ELOG((x + CEPS) / (x + CSIG));
   const double invBmRu = 1.0 / (q->bm * c->Ru);
   const double hres = invBmRu / (CSIG - CEPS) * (q->am * Tinv - q->dam) * q->logratio;
   q->qd = q->dam * invBmRu;
@@ -383,7 +400,7 @@ static void fromrhoE(const pgo_case *c, const mixmom *m, const double *Y, double
     cubicpt q;
     cubicmix(c, m, &q);
     q.v = q.MWmix / rho + q.cm;
-    q.I = pgo_elog((q.v + CSIG * q.bm) / (q.v + CEPS * q.bm)) / ((CSIG - CEPS) * q.bm);
+    q.I = ELOG((q.v + CSIG * q.bm) / (q.v + CEPS * q.bm)) / ((CSIG - CEPS) * q.bm);
     const double invMWmix = 1.0 / q.MWmix;
     double T = (Tguess < 1.0) ? 300.0 : Tguess, cpi = 0.0, hi = 0.0;
     for (int k = 0; k < 12; k++) {
@@ -803,8 +820,8 @@ void pgo_sources(const pgo_case *c, double rho, const double *Y, double T, doubl
     conc[n] = cn;
     ctot += cn;
     double g = 0.0;
-    for (int m = c->ng_ - 1; m >= 0; m--) g = fma(g, logT, c->gpoly[n * c->ng_ + m]);
-    gRT[n] = fma(c->h298[n], Tinv, g);
+    for (int m = c->ng_ - 1; m >= 0; m--) g = FMA(g, logT, c->gpoly[n * c->ng_ + m]);
+    gRT[n] = FMA(c->h298[n], Tinv, g);
     logc[n] = fmax(c->logcfloor, pgo_tlog(cn));
     cG[n] = fmax(cn, 0.0) * pgo_texp(gRT[n]);
   }
@@ -868,7 +885,7 @@ void pgo_sources(const pgo_case *c, double rho, const double *Y, double T, doubl
         if (c->prb[k] != 0.0) logpr += c->prb[k] * logT;
         if (c->prEaR[k] != 0.0) logpr -= c->prEaR[k] * Tinv;
         const double prexp = pgo_texp(-fabs(logpr));
-        lk += fmin(logpr, 0.0) - pgo_dlog1p(prexp);
+        lk += fmin(logpr, 0.0) - DLOG1P(prexp);
       }
       if (troe) lk += troeblend(c, k, logT, logpr);
       double lrp = lk;
@@ -1032,7 +1049,7 @@ static void chung(const pgo_case *c, const double *X, double T, double lnT, doub
     }
   }
   const double epsk = epssigma3 / sigma3;
-  const double cbrtsigma3 = pgo_dcbrt(sigma3);
+  const double cbrtsigma3 = DCBRT(sigma3);
   const double sigma2 = cbrtsigma3 * cbrtsigma3;
   const double sqrtMm = masssum / (epsk * sigma2);
   const double Mm = sqrtMm * sqrtMm;
@@ -1040,7 +1057,7 @@ static void chung(const pgo_case *c, const double *X, double T, double lnT, doub
   const double Tc = 1.2593 * epsk;
   const double Vc = sigma3 / (0.809 * 0.809 * 0.809);
   const double Fc = 1.0 - 0.2756 * omega;
-  const double cbrtVc = pgo_dcbrt(Vc);
+  const double cbrtVc = DCBRT(Vc);
   const double Vc23 = cbrtVc * cbrtVc;
   const double Mprime = Mm * 1.0e-3;
   double E[10], B[7];
@@ -1049,7 +1066,7 @@ static void chung(const pgo_case *c, const double *X, double T, double lnT, doub
   const double Tr = T / Tc;
   const double Tstar = T / epsk, invTstar = epsk / T;
   /* pow and exp by the table methods (the device evaluates the same) */
-  const double Omegav = 1.16145 * pgo_texp(-0.14874 * pgo_elog(Tstar)) + 0.52487 * pgo_texp(-0.77320 * Tstar) +
+  const double Omegav = 1.16145 * pgo_texp(-0.14874 * ELOG(Tstar)) + 0.52487 * pgo_texp(-0.77320 * Tstar) +
                         2.16178 * pgo_texp(-2.43787 * Tstar);
   const double Yy = rho / Mm * 1.0e-3 * Vc / 6.0;
   const double omYy = 1.0 - Yy;
@@ -1086,7 +1103,7 @@ static void hpartial(const pgo_case *c, const double *Yall, double p, double T, 
   rootslope(&q);
   double hdep, cpdep;
   departures(c, &q, T, &hdep, &cpdep);
-  const double lnT = pgo_elog(T), Tinv = c->Ru * q.invRuT, sqrtT = sqrt(T);
+  const double lnT = ELOG(T), Tinv = c->Ru * q.invRuT, sqrtT = sqrt(T);
   const double ZT = q.dZdT, Bs = q.Bstar, BT = -Bs * Tinv;
   const double invZsB = 1.0 / (Z + CSIG * Bs), invZeB = 1.0 / (Z + CEPS * Bs);
   const double Iz = -q.logratio / (CSIG - CEPS);
@@ -1095,7 +1112,7 @@ static void hpartial(const pgo_case *c, const double *Yall, double p, double T, 
   const double invS1 = 1.0 / q.S1, invBm = 1.0 / q.bm, invamsqrtT = invS1 * invS1 / sqrtT;
   for (int n = 0; n < ns; n++) {
     double hRT = 0.0;
-    for (int mm = c->nt - 1; mm >= 0; mm--) hRT = fma(hRT, lnT, c->hpoly[n * c->nt + mm]);
+    for (int mm = c->nt - 1; mm >= 0; mm--) hRT = FMA(hRT, lnT, c->hpoly[n * c->nt + mm]);
     hRT += c->h298[n] * Tinv;
     const double qT = (q.qd - q.q) * Tinv;
     const double sqrtan = c->cubicsqrtac[n] * modulus(c, n, sqrtT);
@@ -1212,7 +1229,7 @@ void pgo_transport(const pgo_case *c, pgo_arr Q, pgo_arr state, pgo_arr trstate,
     const double MWmix = 1.0 / denom;
     double X[MAXNS];
     for (int n = 0; n < ns; n++) X[n] = Y[n] * c->MWinv[n] * MWmix;
-    const double T = *at(state, i, j, k, 1), lnT = pgo_elog(T);
+    const double T = *at(state, i, j, k, 1), lnT = ELOG(T);
     double kappa;
     const double mu = mu_kappa(c, X, T, lnT, rho, &kappa);
     *at(trstate, i, j, k, 0) = mu;
@@ -1234,7 +1251,7 @@ void pgo_transport(const pgo_case *c, pgo_arr Q, pgo_arr state, pgo_arr trstate,
     const double Tinv = 1.0 / T;
     for (int n = 0; n < ns; n++) {
       double hRT = 0.0;
-      for (int m = c->nt - 1; m >= 0; m--) hRT = fma(hRT, lnT, c->hpoly[n * c->nt + m]);
+      for (int m = c->nt - 1; m >= 0; m--) hRT = FMA(hRT, lnT, c->hpoly[n * c->nt + m]);
       hRT += c->h298[n] * Tinv;
       *at(hsp, i, j, k, n) = hRT * T * c->Ru * c->MWinv[n];
     }
@@ -1517,7 +1534,7 @@ void pgo_wall_flux(const pgo_case *c, int axis, int side, int kind, const double
           for (int l = 0; l < c->ns; l++) denom += Y[l] * c->MWinv[l];
           const double MWmix = 1.0 / denom;
           for (int l = 0; l < c->ns; l++) X[l] = Y[l] * c->MWinv[l] * MWmix;
-          mu = mu_kappa(c, X, Tw, pgo_elog(Tw), thermo.rho, &kappa);
+          mu = mu_kappa(c, X, Tw, ELOG(Tw), thermo.rho, &kappa);
           dT = Tw - *at(state, ic[0], ic[1], ic[2], 1);
         } else {
           mu = *at(trstate, ic[0], ic[1], ic[2], 0);
@@ -1588,6 +1605,8 @@ static void prerhs(const pgo_case *c, pgo_block *b, pgo_arr Q, pgo_arr dQ, int w
   if (withdQ && b->chemistry == 1) pgo_source(c, Q, b->state, dQ, interior(b));
   if (withdQ && b->chemistry == 2)
     pgo_substep(c, Q, b->state, dQ, 1, b->substeps, b->subcap, b->subeqtol, b->dt, interior(b));
+  if (withdQ && b->chemistry == 3)
+    pgo_implicit(c, Q, b->state, dQ, 1, b->imparts, b->dt, interior(b));
   if (b->viscous) {
     pgo_transport(c, Q, b->state, b->trstate, b->hsp, interior(b));
     for (int a = 0; a < 3; a++)
@@ -1834,6 +1853,131 @@ void pgo_substep(const pgo_case *c, pgo_arr Q, pgo_arr state, pgo_arr out, int b
       for (int n = 0; n < ns - 1; n++) *at(out, i, j, k, 5 + n) = src[n];
     } else {
       for (int n = 0; n < ns - 1; n++) *at(out, i, j, k, n) = src[n];
+    }
+  }
+}
+
+/* Implicit chemistry (no counterpart in the reference; the stiff alternative to its sub-stepped
+ * form, chemint/substepped.py): the 0D system the sub-stepped form integrates, y = (Y_0 .. Y_{ns-1},
+ * T) at constant density and internal energy with f = (w_n / rho, -sum e_n w_n / (rho cv))
+ * (chemint/kernels/substep.mako:14-50), stepped over dt by the linearly implicit Euler method in
+ * `parts` equal parts: y1 = y0 + h x with (I - h J) x = f(y0), J by forward differences at y0 rounded
+ * to FP32 (the matrix of a state within its last bits is the same), the system solved by an LU in FP64
+ * without pivoting. The mean rate rho (Y_end - Y_start) / dt is written as
+ * pgo_substep writes its own. Every operation is FP64: this is the plain form the device's
+ * FP32-factored, FP64-refined solve is held to. */
+static void im_rhs0d(const pgo_case *c, double rho, double rhoinv, const double *y, double *f) {
+  const int ns = c->ns, nt = c->nt;
+  double rate[MAXNS];
+  const double T = y[ns];
+  pgo_sources(c, rho, y, T, rate, NULL);
+  const double u = pgo_tlog(T), Tinv = 1.0 / T;
+  double cv = 0.0, dTdt = 0.0;
+  for (int n = 0; n < ns; n++) {
+    double cpR = 0.0, hRT = 0.0;
+    for (int m = nt - 1; m >= 0; m--) cpR = cpR * u + c->cppoly[n * nt + m];
+    for (int m = nt - 1; m >= 0; m--) hRT = hRT * u + c->hpoly[n * nt + m];
+    hRT += c->h298[n] * Tinv;
+    const double e = (hRT - 1.0) * T * c->Rsp[n];
+    cv += (cpR - 1.0) * c->Rsp[n] * y[n];
+    dTdt -= e * rate[n];
+  }
+  const double cvrhoinv = 1.0 / (cv * rho);
+  for (int n = 0; n < ns; n++) f[n] = rate[n] * rhoinv;
+  f[ns] = dTdt * cvrhoinv;
+}
+
+/* the forward difference's step for unknown m: 2^-14 of T or of a mass fraction, at least 2^-14 * 1e-3
+ * for a trace species (large on purpose: the rates' rounding over the step is what the difference adds
+ * of its own, and a step of 2^-26 makes a stiff part's result move by 1e-6 with y's last bit), the
+ * step taken being what the rounding leaves */
+double pgo_fd_step(double ym, int temperature) {
+  const double scale = temperature ? ym : fmax(fabs(ym), 1e-3);
+  return (ym + 6.103515625e-05 * scale) - ym;
+}
+
+/* I - h J at y (nv = ns + 1 unknowns), row-major into A */
+void pgo_implicit_matrix(const pgo_case *c, double rho, const double *y, double h, double *A) {
+  const int ns = c->ns, nv = ns + 1;
+  const double rhoinv = 1.0 / rho;
+  double f0[MAXNS + 1], f1[MAXNS + 1], yp[MAXNS + 1];
+  im_rhs0d(c, rho, rhoinv, y, f0);
+  for (int m = 0; m < nv; m++) {
+    const double dlt = pgo_fd_step(y[m], m == ns);
+    for (int r = 0; r < nv; r++) yp[r] = r == m ? y[m] + dlt : y[r];
+    im_rhs0d(c, rho, rhoinv, yp, f1);
+    const double hd = h / dlt;
+    for (int r = 0; r < nv; r++) A[r * nv + m] = (r == m ? 1.0 : 0.0) - (f1[r] - f0[r]) * hd;
+  }
+}
+
+/* f of the 0D system at y, for the tests */
+void pgo_implicit_rhs(const pgo_case *c, double rho, const double *y, double *f) { im_rhs0d(c, rho, 1.0 / rho, y, f); }
+
+/* the right-looking LU without pivoting, in place (L unit lower below the diagonal) */
+void pgo_lu_nopivot(int n, double *A) {
+  for (int p = 0; p < n; p++) {
+    const double rp = 1.0 / A[p * n + p];
+    for (int i = p + 1; i < n; i++) {
+      const double li = A[i * n + p] * rp;
+      A[i * n + p] = li;
+      for (int j = p + 1; j < n; j++) A[i * n + j] -= li * A[p * n + j];
+    }
+  }
+}
+
+void pgo_lu_apply(int n, const double *LU, double *z) {
+  for (int r = 1; r < n; r++)
+    for (int c = 0; c < r; c++) z[r] -= LU[r * n + c] * z[c];
+  for (int r = n - 1; r >= 0; r--) {
+    for (int c = r + 1; c < n; c++) z[r] -= LU[r * n + c] * z[c];
+    z[r] = z[r] / LU[r * n + r];
+  }
+}
+
+/* one cell: y (ns + 1 values) advanced in place over dt in `parts` linearly implicit Euler parts */
+void pgo_implicit_cell(const pgo_case *c, double rho, double *y, double dt, int parts) {
+  const int ns = c->ns, nv = ns + 1;
+  const double h = dt / parts;
+  double A[(MAXNS + 1) * (MAXNS + 1)], x[MAXNS + 1];
+  for (int part = 0; part < parts; part++) {
+    /* the matrix at y rounded to FP32 (a last-bit change of y then leaves it as it was) */
+    double yr[MAXNS + 1];
+    for (int r = 0; r < nv; r++) yr[r] = (double)(float)y[r];
+    pgo_implicit_matrix(c, rho, yr, h, A);
+    im_rhs0d(c, rho, 1.0 / rho, y, x);
+    pgo_lu_nopivot(nv, A);
+    pgo_lu_apply(nv, A, x);
+    for (int r = 0; r < nv; r++) y[r] = y[r] + h * x[r];
+  }
+}
+
+void pgo_implicit(const pgo_case *c, pgo_arr Q, pgo_arr state, pgo_arr out, int begin_dQ, int parts, double dt,
+                  pgo_box r) {
+  const int ns = c->ns;
+  FOR_BOX(r) {
+    const double rho = *at(Q, i, j, k, 0), rhoinv = 1.0 / rho;
+    double y[MAXNS + 1], Ys[MAXNS];
+    /* massfrac (solution.mako:25-35) */
+    double Ylast = 1.0, Ysum = 0.0;
+    for (int n = 0; n < ns - 1; n++) {
+      const double Yn = fmax(fmin(*at(Q, i, j, k, 5 + n) * rhoinv, 1.0), 0.0);
+      Ys[n] = Yn;
+      Ylast -= Yn;
+      Ysum += Yn;
+    }
+    const double Yscale = 1.0 / fmax(Ysum, 1.0);
+    for (int n = 0; n < ns - 1; n++) Ys[n] *= Yscale;
+    Ys[ns - 1] = fmax(Ylast, 0.0);
+    for (int n = 0; n < ns; n++) y[n] = Ys[n];
+    y[ns] = *at(state, i, j, k, 1);
+    pgo_implicit_cell(c, rho, y, dt, parts);
+    const double rdt = rho / dt;
+    if (begin_dQ) {
+      for (int l = 0; l < 5; l++) *at(out, i, j, k, l) = 0.0;
+      for (int n = 0; n < ns - 1; n++) *at(out, i, j, k, 5 + n) = (y[n] - Ys[n]) * rdt;
+    } else {
+      for (int n = 0; n < ns - 1; n++) *at(out, i, j, k, n) = (y[n] - Ys[n]) * rdt;
     }
   }
 }

@@ -82,6 +82,20 @@ double pgo_dlog1p(double x);
 void pgo_substep(const pgo_case *c, pgo_arr Q, pgo_arr state, pgo_arr out, int begin_dQ, int32_t *substeps, int cap,
                  double eqtol, double dt, pgo_box r);
 void pgo_source(const pgo_case *c, pgo_arr Q, pgo_arr state, pgo_arr dQ, pgo_box r);
+/* implicit chemistry (the FP64 plain form): the linearly implicit Euler method in `parts` parts
+ * over dt per cell, its systems solved by an FP64 LU without pivoting; the mean rate into dQ
+ * (begin_dQ) or the ns - 1 rows of out */
+void pgo_implicit(const pgo_case *c, pgo_arr Q, pgo_arr state, pgo_arr out, int begin_dQ, int parts, double dt,
+                  pgo_box r);
+/* its pieces, for the tests: one cell advanced in place, the 0D system's f, the matrix I - h J, the
+ * forward-difference step, and the unpivoted LU and its solve */
+void pgo_implicit_cell(const pgo_case *c, double rho, double *y, double dt, int parts);
+void pgo_implicit_rhs(const pgo_case *c, double rho, const double *y, double *f);
+void pgo_implicit_matrix(const pgo_case *c, double rho, const double *y, double h, double *A);
+double pgo_fd_step(double ym, int temperature);
+void pgo_lu_nopivot(int n, double *A);
+void pgo_lu_apply(int n, const double *LU, double *z);
+
 void pgo_combine(int ne, pgo_arr out, int own, double wout, int nterms, const pgo_arr *X, const double *w,
                  double dt, int step, pgo_box r);
 void pgo_fill_periodic(pgo_arr A, int ncomp, const int *ncell, int ng, int depth, int axis);
@@ -100,7 +114,7 @@ typedef struct {
   int nregs;
   pgo_arr state, iF, jF, kF, iS, jS, kS, detJinv;
   int integrator; /* 0 rk1, 1 rk2, 2 tvd-rk3, 3 rk4 */
-  int chemistry;  /* 1: the coupled source begins dQ; 2: sub-stepped chemistry does */
+  int chemistry;  /* 1: the coupled source begins dQ; 2: sub-stepped chemistry does; 3: implicit chemistry does */
   /* sub-stepped chemistry: the cap, the equilibrium tolerance, the counts per interior cell, and the
    * step's dt (set by pgo_stage; pgo_rhs reads the last one) */
   int subcap;
@@ -121,6 +135,8 @@ typedef struct {
    * (nx, ny, nz, start, length, multiplier) */
   int sponge;
   double spongeline[6];
+  /* implicit chemistry (chemistry 3): linearly implicit Euler parts per step */
+  int imparts;
 } pgo_block;
 
 /* plugins/simulator/kernels/viscsponge.mako: the viscosity trstate[0] raised along the line

@@ -44,6 +44,9 @@ _DEFAULTS = {
         "bulk-substeps": 1,
         "stiff-bins": 8,
         "equilibrium-tolerance": 1e-7,
+        "implicit-parts": 1,
+        "max-sweeps": 8,
+        "refine-tolerance": 1e-13,
     },
     "ics": {"p": 101325.0, "u": 0.0, "v": 0.0, "w": 0.0, "T": 300.0, "Y": {}},
     "time-integration": {"integrator": "tvd-rk3"},
@@ -79,6 +82,9 @@ _DEFAULTS = {
         "stage-slabs": 1,
         "block-streams": 4,
         "graphs": True,
+        "batched": "auto",
+        "autotune": True,
+        "autotune-cache": None,
     },
     "halo-exchange": {"kind": "hoststaged"},
 }

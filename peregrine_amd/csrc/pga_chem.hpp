@@ -19,6 +19,26 @@ __device__ __forceinline__ double gibbs_rt(const Gas<NS> &g, int n, double logT,
   return fma(g.h298(n), Tinv, gr);
 }
 
+// cp/R and the internal energy of species n at T (eos/kernels/species.mako:3-27), u = ln T
+template <int NS>
+__device__ __forceinline__ double cpR_species(const Gas<NS> &g, int n, double u) {
+#pragma clang fp contract(off)
+  double cpR = 0.0;
+#pragma unroll
+  for (int m = NT - 1; m >= 0; m--) cpR = cpR * u + g.cppoly(n, m);
+  return cpR;
+}
+
+template <int NS>
+__device__ __forceinline__ double e_species(const Gas<NS> &g, int n, double T, double u, double Tinv) {
+#pragma clang fp contract(off)
+  double hRT = 0.0;
+#pragma unroll
+  for (int m = NT - 1; m >= 0; m--) hRT = hRT * u + g.hpoly(n, m);
+  hRT += g.h298(n) * Tinv;
+  return (hRT - 1.0) * T * g.Rsp(n);
+}
+
 // euler/kernels/solution.mako:25-35: Y clamped to [0, 1], rescaled when their sum
 // passes one, the last species what they leave (never below zero)
 template <int NS>

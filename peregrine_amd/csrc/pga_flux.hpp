@@ -217,4 +217,22 @@ __device__ __forceinline__ void flux_function(const FaceState<NS> &L, const Face
     rusanov<NS>(L, R, S, F);
 }
 
+// the face (i, j, k) of `axis` (between cells k - 1 and k along it): its sides, its flux into F
+template <int NS, int EOS, int RECON, int PRIMARY, int LIM>
+__device__ __forceinline__ void flux_face(const double *__restrict__ K, int axis, const View &Q, const View &st,
+                                          const View &S, const View &F, int i, int j, int k) {
+  const Gas<NS> g{K};
+  const int64_t o = Q.off(i, j, k);
+  const int64_t d = axis == 0 ? 1 : axis == 1 ? Q.sj : Q.sk;
+  FaceState<NS> L, R;
+  face_sides<NS, EOS, RECON, LIM>(g, Q, st, o, d, L, R);
+  const int64_t of = S.off(i, j, k);
+  const double Sv[3] = {S.at(of, 0), S.at(of, 1), S.at(of, 2)};
+  double Fv[NS + 4];
+  flux_function<NS, PRIMARY>(L, R, Sv, Fv);
+  const int64_t oF = F.off(i, j, k);
+#pragma unroll
+  for (int l = 0; l < NS + 4; l++) F.at(oF, l) = Fv[l];
+}
+
 }  // namespace pga

@@ -33,18 +33,7 @@ __global__ __launch_bounds__(256) void k_flux(const double *__restrict__ K, int 
   } else if (!flat_cell_of(b, i, j, k)) {
     return;
   }
-  const Gas<NS> g{K};
-  const int64_t o = Q.off(i, j, k);
-  const int64_t d = axis == 0 ? 1 : axis == 1 ? Q.sj : Q.sk;
-  FaceState<NS> L, R;
-  face_sides<NS, EOS, RECON, LIM>(g, Q, st, o, d, L, R);
-  const int64_t of = S.off(i, j, k);
-  const double Sv[3] = {S.at(of, 0), S.at(of, 1), S.at(of, 2)};
-  double Fv[NS + 4];
-  flux_function<NS, PRIMARY>(L, R, Sv, Fv);
-  const int64_t oF = F.off(i, j, k);
-#pragma unroll
-  for (int l = 0; l < NS + 4; l++) F.at(oF, l) = Fv[l];
+  flux_face<NS, EOS, RECON, PRIMARY, LIM>(K, axis, Q, st, S, F, i, j, k);
 }
 
 template <int NS, int EOS>

@@ -1,0 +1,398 @@
+// Implicit chemistry (device code, shared by pga_implicit.hip and the launches over a table of
+// blocks in pga_batch.hip): each cell integrates its own finite-rate chemistry over the step by the
+// linearly implicit (Rosenbrock) Euler method in `parts` equal parts, the same 0D system the
+// sub-stepped form integrates explicitly (chemint/kernels/substep.mako:5-83: the ns mass fractions
+// and T at constant density and internal energy, dT/dt = -sum e_n w_n / (rho cv)), and begins dQ
+// with the mean rate over the step.
+//
+// A part of length h from y0 = (Y_0 .. Y_{ns-1}, T) takes y1 = y0 + h x with (I - h J) x = f(y0):
+// backward Euler's first Newton step, first order and stable however stiff the chemistry. J is
+// formed in FP64 by forward differences of f (one more evaluation of the rates per unknown); the
+// matrix is kept in FP64 and, rounded to FP32, factored by a batched LU on the matrix cores. The
+// solve is an iterative refinement: x += (LU)^-1 r with the residual r = f - A x taken in FP64
+// against the FP64 matrix, until every row's residual is within `tolerance` of that row's scale
+// (|f_i| + sum_j |A_ij x_j|, the componentwise backward error), so x is the FP64 solution whatever
+// the factors' rounding. The Arrhenius exponentials, the Gibbs polynomials, the residuals and the
+// update stay FP64.
+//
+// The batched LU takes 16 matrices per wave, padded to 12 x 12 (three 4 x 4 blocks a side), one
+// lane quad per matrix: lane 4b + q holds column q of each of matrix b's blocks, a block's row i in
+// its register i. That is the accumulator layout of v_mfma_f32_4x4x1f32 (16 independent 4 x 4
+// products per instruction), so the Schur-complement updates of the blocked right-looking LU,
+// A_IJ -= L_IK U_KJ, are four rank-one MFMAs each for all 16 matrices at once; the diagonal blocks'
+// factorisation and the two triangular block solves run on the VALU within the quad, with DPP
+// broadcasts. No pivoting: the matrices are I - h J of a chemical source, whose diagonal grows with
+// the destruction rates; the FP64 refinement corrects what the FP32 factors leave.
+//
+// The work array holds the FP32 factors, [group of 16 cells][144 entries][16 cells] (a row of the
+// 16 cells' same entry is 64 contiguous bytes, so both the lane-per-cell kernels and the
+// quad-per-cell LU read and write whole segments), the FP64 matrices the same way with
+// (ns + 1)^2 entries, then y and f at y's FP32 rounding, [ns + 1][cells] doubles each.
+#pragma once
+
+#include <type_traits>
+
+#include "generated/mechanisms.hpp"
+#include "pga_chem.hpp"
+
+namespace pga {
+
+constexpr int IM_G = 16;                   // matrices per MFMA group: a wave's 16 lane quads
+constexpr int IM_N = 12;                   // matrices padded to 12 x 12
+constexpr int IM_E = IM_N * IM_N;          // entries of a padded matrix
+constexpr int IM_GROUP = IM_E * IM_G;      // floats per group
+constexpr int IM_FAIL = 1000;              // added to a cell's count for each part that did not converge
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+// entry (r, c) of matrix `cell` in the group-major factor array
+__device__ __forceinline__ int64_t lu_at(int64_t cell, int r, int c) {
+  return (cell >> 4) * IM_GROUP + (int64_t)(r * IM_N + c) * IM_G + (cell & 15);
+}
+
+template <int B, int E, class F>
+__host__ __device__ __forceinline__ void sfor(F &&f) {
+  if constexpr (B < E) {
+    f.template operator()<B>();
+    sfor<B + 1, E>(f);
+  }
+}
+
+// lane P's x within each quad, to all four lanes of the quad (DPP quad_perm P, P, P, P)
+template <int P>
+__device__ __forceinline__ float qb(float x) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), P * 0x55, 0xf, 0xf, false));
+}
+
+// The blocked LU of 16 matrices per wave, in place in the group-major array W. A wave exits
+// whole (the MFMAs need every lane); the lanes of a group's missing matrices factor the identity.
+template <int NB>
+__global__ __launch_bounds__(256) void k_lu(float *__restrict__ W, int64_t ncell) {
+  const int64_t grp = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (grp * IM_G >= ncell) return;
+  const int lane = threadIdx.x & 63, b = lane >> 2, q = lane & 3;
+  const bool real = grp * IM_G + b < ncell;
+  float *base = W + grp * IM_GROUP + b;
+  v4f blk[NB][NB];
+  sfor<0, NB>([&]<int I>() {
+    sfor<0, NB>([&]<int J>() {
+      sfor<0, 4>([&]<int i>() {
+        const int r = 4 * I + i, c = 4 * J + q;
+        blk[I][J][i] = real ? base[(r * IM_N + c) * IM_G] : (r == c ? 1.0f : 0.0f);
+      });
+    });
+  });
+  sfor<0, NB>([&]<int K>() {
+    v4f &D = blk[K][K];
+    // the diagonal block: lane q holds its column q; L (unit, strictly below) and U overwrite it
+    sfor<0, 4>([&]<int p>() {
+      const float rp = 1.0f / qb<p>(D[p]);
+      sfor<p + 1, 4>([&]<int i>() {
+        const float li = qb<p>(D[i]) * rp;
+        D[i] = q > p ? D[i] - li * D[p] : (q == p ? li : D[i]);
+      });
+    });
+    // L_KK[i][p], broadcast to the quad
+    float Lkk[4][4];
+    sfor<0, 4>([&]<int p>() { sfor<p + 1, 4>([&]<int i>() { Lkk[i][p] = qb<p>(D[i]); }); });
+    // U_KJ = L_KK^-1 A_KJ right of the diagonal, column by column in each lane
+    sfor<K + 1, NB>([&]<int J>() {
+      v4f &X = blk[K][J];
+      sfor<0, 3>([&]<int p>() { sfor<p + 1, 4>([&]<int i>() { X[i] = X[i] - Lkk[i][p] * X[p]; }); });
+    });
+    // L_IK = A_IK U_KK^-1 below it, row by row across the quad; aL[I][p] is the MFMA operand of
+    // column p, lane 4b + i holding -L_IK[i][p]
+    const float ud = q == 0 ? D[0] : q == 1 ? D[1] : q == 2 ? D[2] : D[3];
+    const float rud = 1.0f / ud;
+    float aL[NB][4];
+    sfor<K + 1, NB>([&]<int I>() {
+      v4f &X = blk[I][K];
+      sfor<0, 4>([&]<int p>() { aL[I][p] = 0.0f; });
+      sfor<0, 4>([&]<int i>() {
+        sfor<0, 4>([&]<int p>() {
+          const float xp = qb<p>(X[i] * rud);
+          X[i] = q > p ? X[i] - xp * D[p] : (q == p ? xp : X[i]);
+          aL[I][p] = q == i ? -xp : aL[I][p];
+        });
+      });
+    });
+    // the Schur complement on the matrix cores: A_IJ += sum_p (-L_IK[:, p]) U_KJ[p, :]
+    sfor<K + 1, NB>([&]<int I>() {
+      sfor<K + 1, NB>([&]<int J>() {
+        sfor<0, 4>([&]<int p>() {
+          blk[I][J] = __builtin_amdgcn_mfma_f32_4x4x1f32(aL[I][p], blk[K][J][p], blk[I][J], 0, 0, 0);
+        });
+      });
+    });
+  });
+  if (!real) return;
+  sfor<0, NB>([&]<int I>() {
+    sfor<0, NB>([&]<int J>() {
+      sfor<0, 4>([&]<int i>() { base[((4 * I + i) * IM_N + 4 * J + q) * IM_G] = blk[I][J][i]; });
+    });
+  });
+}
+
+// x = (L U)^-1 z in FP32 from the factors of matrix `cell`, in place (L unit lower)
+template <int NV>
+__device__ __forceinline__ void lu_solve(const float *__restrict__ W, int64_t cell, float (&z)[NV]) {
+#pragma unroll
+  for (int r = 1; r < NV; r++) {
+#pragma unroll
+    for (int c = 0; c < r; c++) z[r] -= W[lu_at(cell, r, c)] * z[c];
+  }
+#pragma unroll
+  for (int r = NV - 1; r >= 0; r--) {
+#pragma unroll
+    for (int c = r + 1; c < NV; c++) z[r] -= W[lu_at(cell, r, c)] * z[c];
+    z[r] = z[r] / W[lu_at(cell, r, r)];
+  }
+}
+
+struct ImArgs {
+  const double *K, *C;
+  View Q, st, out;
+  int begin_dQ, first, last;
+  float *W;      // the FP32 factors
+  double *A;     // the FP64 matrices, group-major with nv^2 entries
+  double *y;     // [ns + 1][ncell]: the part's start, then its end; then f at its FP32 rounding
+  int64_t ncell;
+  int32_t *sweeps;
+  const double *dt;
+  Box3 b;
+  int parts, maxsweeps;
+  double tol;
+};
+
+// entry (r, c) of FP64 matrix `cell` with nv unknowns
+template <int NV>
+__device__ __forceinline__ int64_t a64_at(int64_t cell, int r, int c) {
+  return (cell >> 4) * (NV * NV * IM_G) + (int64_t)(r * NV + c) * IM_G + (cell & 15);
+}
+
+// f(y) of the 0D reactor at constant density: the species' rates over rho and dT/dt at constant
+// volume, as the sub-stepped form takes them (substep.mako:14-50: ln T from the rate state, cv and
+// each species' internal energy by their ln T polynomials)
+template <class M>
+__device__ __forceinline__ void rhs0d(const Gas<M::NS> &g, const double *__restrict__ C, double rho, double rhoinv,
+                                      const double (&y)[M::NS + 1], double (&f)[M::NS + 1]) {
+#pragma clang fp contract(off)
+  constexpr int NS = M::NS;
+  double Y[NS], rate[NS], D[NS];
+#pragma unroll
+  for (int n = 0; n < NS; n++) Y[n] = y[n];
+  const double T = y[NS];
+  typename M::Rates s;
+  M::ratestate(g, C, rho, Y, T, s);
+  M::template fold<false>(C, s, rate, D);
+  double cv = 0.0, dTdt = 0.0;
+#pragma unroll
+  for (int n = 0; n < NS; n++) {
+    const double cpR = cpR_species(g, n, s.logT);
+    const double e = e_species(g, n, T, s.logT, s.Tinv);
+    cv += (cpR - 1.0) * g.Rsp(n) * Y[n];
+    dTdt -= e * rate[n];
+  }
+  const double cvrhoinv = 1.0 / (cv * rho);
+#pragma unroll
+  for (int n = 0; n < NS; n++) f[n] = rate[n] * rhoinv;
+  f[NS] = dTdt * cvrhoinv;
+}
+
+// p (wave-uniform), as a value the compiler cannot see through: the loads behind it stay in the
+// loop that makes them rather than being hoisted out of it (the hundreds of mechanism constants
+// and a cell's 100 factors would otherwise be held in registers across the loop, at one wave per
+// SIMD)
+template <class T>
+__device__ __forceinline__ T *opaque(T *p) {
+  asm volatile("" : "+s"(p));
+  return p;
+}
+
+__device__ __forceinline__ int64_t im_code(const Box3 &b, int i, int j, int k) {
+  return (i - b.s0) + (int64_t)b.e0 * ((j - b.s1) + (int64_t)b.e1 * (k - b.s2));
+}
+
+// the step of unknown m's forward difference: 2^-14 of T or of a mass fraction, at least 2^-14 * 1e-3
+// for a trace species. Far above the square root of the rounding unit on purpose: the rates'
+// rounding divided by the step is what the difference adds of its own, and a step of 2^-26 would
+// turn a last-bit change of y into a 1e-6 change of a stiff part's result and mass drift through
+// the columns' sums; the step's truncation is smooth in y, and the linearly implicit Euler step
+// is first order with any fixed matrix. The step taken is the difference the rounding leaves
+__device__ __forceinline__ double fd_step(double ym, bool temperature) {
+#pragma clang fp contract(off)
+  const double scale = temperature ? ym : fmax(fabs(ym), 1e-3);
+  return (ym + 6.103515625e-05 * scale) - ym;
+}
+
+// A = I - h J at the part's start, J by forward differences, into the FP64 and FP32 arrays; the
+// first part takes y from Q and the state. J is taken at y rounded to FP32, y^: a last-bit change of y
+// (the oracle's FP64 solve against the device's refined one, a part's start after another part)
+// then leaves the matrix as it was, where the differences' own rounding would move it by 1e-12 of
+// itself and a stiff part's result by more; the linearly implicit Euler step is first order with the
+// matrix of any nearby state. y^ and f(y^) wait in the work array while each column's rates are
+// evaluated, so a column holds no more registers than one evaluation of the rates
+template <class M>
+__device__ __forceinline__ void ijac_cell(const ImArgs &a, const View &Q, const View &st, int i, int j, int k,
+                                          int64_t code) {
+#pragma clang fp contract(off)
+  constexpr int NS = M::NS, NV = NS + 1;
+  const int64_t nc = a.ncell;
+  const Gas<NS> g{a.K};
+  const int64_t o = Q.off(i, j, k);
+  const double rho = Q.at(o, 0), rhoinv = 1.0 / rho;
+  double *y = a.y + code, *fh = a.y + NV * nc + code;
+  {
+    double yc[NV], fc[NV];
+    if (a.first) {
+      double Y[NS];
+      massfrac<NS>(Q, o, rhoinv, Y);
+#pragma unroll
+      for (int n = 0; n < NS; n++) yc[n] = Y[n];
+      yc[NS] = st.at(st.off(i, j, k), 1);
+#pragma unroll
+      for (int r = 0; r < NV; r++) y[r * nc] = yc[r];
+    } else {
+#pragma unroll
+      for (int r = 0; r < NV; r++) yc[r] = y[r * nc];
+    }
+#pragma unroll
+    for (int r = 0; r < NV; r++) yc[r] = (double)(float)yc[r];
+    rhs0d<M>(g, a.C, rho, rhoinv, yc, fc);
+#pragma unroll
+    for (int r = 0; r < NV; r++) fh[r * nc] = fc[r];
+  }
+  const double h = *a.dt / a.parts;
+#pragma unroll 1
+  for (int m = 0; m < NV; m++) {
+    double yp[NV], f1[NV], ym = 0.0;
+#pragma unroll
+    for (int r = 0; r < NV; r++) {
+      yp[r] = (double)(float)y[r * nc];
+      ym = r == m ? yp[r] : ym;
+    }
+    const double dlt = fd_step(ym, m == NS);
+#pragma unroll
+    for (int r = 0; r < NV; r++) yp[r] = r == m ? ym + dlt : yp[r];
+    rhs0d<M>(Gas<NS>{opaque(a.K)}, opaque(a.C), rho, rhoinv, yp, f1);
+    const double hd = h / dlt;
+#pragma unroll
+    for (int r = 0; r < NV; r++) {
+      const double Arm = (r == m ? 1.0 : 0.0) - (f1[r] - fh[r * nc]) * hd;
+      a.A[a64_at<NV>(code, r, m)] = Arm;
+      a.W[lu_at(code, r, m)] = (float)Arm;
+    }
+  }
+  // the padding of the FP32 matrix: the identity
+#pragma unroll
+  for (int r = 0; r < IM_N; r++) {
+#pragma unroll
+    for (int c = 0; c < IM_N; c++)
+      if (r >= NV || c >= NV) a.W[lu_at(code, r, c)] = r == c ? 1.0f : 0.0f;
+  }
+}
+
+// x = A^-1 b by iterative refinement on the FP32 factors W, the residual b - A x in FP64 against A
+// (entry (r, c) at A[at(r, c)]); stops when every row's residual is within tol of its scale
+// |b_r| + sum_c |A_rc x_c|, or after maxsweeps corrections. Returns the corrections taken, or
+// maxsweeps + 1 when the last one left a row outside the tolerance
+template <int NV, class At>
+__device__ __forceinline__ int refine(const float *__restrict__ W, const double *__restrict__ A, At at, int64_t cell,
+                                      const double (&bv)[NV], double (&x)[NV], int maxsweeps, double tol) {
+#pragma clang fp contract(off)
+  double r[NV];
+#pragma unroll
+  for (int m = 0; m < NV; m++) {
+    x[m] = 0.0;
+    r[m] = bv[m];
+  }
+  int s = 0;
+  bool done = false;
+#pragma unroll 1
+  while (s < maxsweeps && !done) {
+    const float *Ws = opaque(W);
+    const double *As = opaque(A);
+    float z[NV];
+#pragma unroll
+    for (int m = 0; m < NV; m++) z[m] = (float)r[m];
+    lu_solve<NV>(Ws, cell, z);
+#pragma unroll
+    for (int m = 0; m < NV; m++) x[m] += (double)z[m];
+    done = true;
+#pragma unroll
+    for (int m = 0; m < NV; m++) {
+      double acc = bv[m], scale = fabs(bv[m]);
+#pragma unroll
+      for (int c = 0; c < NV; c++) {
+        const double ax = As[at(cell, m, c)] * x[c];
+        acc = acc - ax;
+        scale = scale + fabs(ax);
+      }
+      r[m] = acc;
+      done = done && fabs(acc) <= tol * scale;
+    }
+    s++;
+  }
+  return done ? s : maxsweeps + 1;
+}
+
+// the part's linearly implicit step y = y0 + h x, (I - h J) x = f(y0); the last part writes the
+// mean rate over the step
+template <class M>
+__device__ __forceinline__ void ros_cell(const ImArgs &a, const View &Q, const View &out, int i, int j, int k,
+                                         int64_t code) {
+#pragma clang fp contract(off)
+  constexpr int NS = M::NS, NV = NS + 1;
+  const int64_t nc = a.ncell;
+  const int64_t o = Q.off(i, j, k);
+  const double rho = Q.at(o, 0), rhoinv = 1.0 / rho;
+  double bv[NV], x[NV];
+  {
+    double y0[NV];
+#pragma unroll
+    for (int r = 0; r < NV; r++) y0[r] = a.y[r * nc + code];
+    rhs0d<M>(Gas<NS>{a.K}, a.C, rho, rhoinv, y0, bv);
+  }
+  const int taken = refine<NV>(a.W, a.A, [](int64_t c, int r, int m) { return a64_at<NV>(c, r, m); }, code, bv, x,
+                               a.maxsweeps, a.tol);
+  const int32_t count = taken > a.maxsweeps ? a.maxsweeps + IM_FAIL : taken;
+  a.sweeps[code] = a.first ? count : a.sweeps[code] + count;
+  const double h = *a.dt / a.parts;
+  double y[NV];
+#pragma unroll
+  for (int r = 0; r < NV; r++) {
+    y[r] = a.y[r * nc + code] + h * x[r];
+    a.y[r * nc + code] = y[r];
+  }
+  if (!a.last) return;
+  // the mean rate over the step, rho (Y_end - Y_start) / dt, as the sub-stepped form begins dQ
+  double Ys[NS];
+  massfrac<NS>(Q, o, rhoinv, Ys);
+  const double rdt = rho / *a.dt;
+  const int64_t oo = out.off(i, j, k);
+  if (a.begin_dQ) {
+#pragma unroll
+    for (int l = 0; l < 5; l++) out.at(oo, l) = 0.0;
+#pragma unroll
+    for (int n = 0; n < NS - 1; n++) out.at(oo, 5 + n) = (y[n] - Ys[n]) * rdt;
+  } else {
+#pragma unroll
+    for (int n = 0; n < NS - 1; n++) out.at(oo, n) = (y[n] - Ys[n]) * rdt;
+  }
+}
+
+template <class M>
+__global__ __launch_bounds__(FLAT) void k_ijac(ImArgs a) {
+  int i, j, k;
+  if (!flat_cell_of(a.b, i, j, k)) return;
+  ijac_cell<M>(a, a.Q, a.st, i, j, k, im_code(a.b, i, j, k));
+}
+
+template <class M>
+__global__ __launch_bounds__(FLAT) void k_ros(ImArgs a) {
+  int i, j, k;
+  if (!flat_cell_of(a.b, i, j, k)) return;
+  ros_cell<M>(a, a.Q, a.out, i, j, k, im_code(a.b, i, j, k));
+}
+
+}  // namespace pga

@@ -17,99 +17,19 @@
 // order.
 #include <type_traits>
 
-#include "generated/mechanisms.hpp"
-#include "pga_chem.hpp"
-#include "pga_flux.hpp"
 #include "pga_internal.hpp"
+#include "pga_stagecore.hpp"
 
 namespace pga {
-
-enum { SRC_QIN = 0, SRC_Q0 = 1, SRC_DQ = 2 };
-
-// out = sum_t w_t x_t in order, a term with withdt taking w_t * dt * x_t
-struct CombineSpec {
-  int n;
-  int src[4];
-  int withdt[4];
-  double w[4];
-};
-
-struct StageArgs {
-  const double *K;
-  View Qin, stin, Q0, Qout, stout, det, rates, iF, jF, kF;
-  CombineSpec comb;
-  const double *dt;
-  int s0, s1, s2, n0, n1, n2;
-  int kc;
-  int q0;  // whether a combine term reads Q0 (RK3's first stage reads only Qin and dQ)
-};
-
-constexpr int WI = 64, WJ = 4, WT = WI * WJ;
-
-// the EOS inversion of a cell's new Q held in registers, from T guess
-template <int NS, int EOS>
-__device__ __forceinline__ void fromQ_regs(const Gas<NS> &g, const double (&Q)[NS + 4], double Tguess, EosState &s) {
-#pragma clang fp contract(off)
-  const double rho = Q[0], rhoinv = 1.0 / rho;
-  const double e = (Q[4] - 0.5 * (Q[1] * Q[1] + Q[2] * Q[2] + Q[3] * Q[3]) * rhoinv) * rhoinv;
-  double Y[NS > 1 ? NS - 1 : 1];
-#pragma unroll
-  for (int n = 0; n < NS - 1; n++) Y[n] = Q[5 + n] * rhoinv;
-  MixMom<NS, EOS> m;
-  mixmom<NS, EOS>(g, [&](int n) { return Y[n]; }, m);
-  fromrhoE<NS, EOS>(g, m, [&](int n) { return Y[n]; }, rho, e, Tguess, s);
-}
 
 // the divergence, the source and the RK combine of a column of cells: memory-bound
 template <int NE, bool CHEM>
 __global__ __launch_bounds__(WT) void k_stage(StageArgs a) {
-  const View Qin = a.Qin, Q0 = a.Q0, Qout = a.Qout, det = a.det;
-  const View iF = a.iF, jF = a.jF, kF = a.kF;
-  const CombineSpec comb = a.comb;
-  const View rates = a.rates;
   const int i = a.s0 + blockIdx.x * WI + (threadIdx.x & (WI - 1));
   const int j = a.s1 + blockIdx.y * WJ + threadIdx.x / WI;
   if (i >= a.s0 + a.n0 || j >= a.s1 + a.n1) return;
   const int kbeg = a.s2 + blockIdx.z * a.kc;
-  const int kend = min(kbeg + a.kc, a.s2 + a.n2);
-  const double dt = *a.dt;
-#pragma unroll 1
-  for (int k = kbeg; k < kend; k++) {
-    const int64_t o = Qin.off(i, j, k), o0 = Q0.off(i, j, k), oo = Qout.off(i, j, k);
-    const int64_t fil = iF.off(i, j, k), fjl = jF.off(i, j, k), fkl = kF.off(i, j, k);
-    const int64_t fih = fil + 1, fjh = fjl + jF.sj, fkh = fkl + kF.sk;
-    const int64_t orr = CHEM ? rates.off(i, j, k) : 0;
-    const double J = __builtin_nontemporal_load(&det.at(det.off(i, j, k), 0));
-    // per conserved row: its six faces in the reference's order of sums, times detJinv, the
-    // source first where there is one (chemint/kernels/source.mako:3-10), then the RK combine in
-    // the reference's order of terms (integrators/kernels/combine.mako:1-13); a row at a time in a
-    // rolled loop keeps ~30 registers, so eight waves per SIMD hide the loads' latency. What only
-    // this cell reads (Q, Q0, the rates, 1/J) and what it writes stream past the caches
-    // (nontemporal); each face value is read by the two cells it lies between and is cached
-#pragma unroll 1
-    for (int l = 0; l < NE; l++) {
-#pragma clang fp contract(off)
-      double d = ((((iF.at(fil, l) + jF.at(fjl, l)) + kF.at(fkl, l)) - iF.at(fih, l)) - jF.at(fjh, l)) - kF.at(fkh, l);
-      d = d * J;
-      if constexpr (CHEM) {
-        if (l >= 5) d = __builtin_nontemporal_load(&rates.at(orr, l - 5)) + d;
-      }
-      // the terms' values are selected as values: a select of pointers would put them in memory
-      const double qin = __builtin_nontemporal_load(&Qin.at(o, l));
-      const double q0 = a.q0 ? __builtin_nontemporal_load(&Q0.at(o0, l)) : 0.0;
-      double acc = 0.0;
-#pragma unroll
-      for (int t = 0; t < 4; t++) {
-        if (t < comb.n) {
-          const int src = comb.src[t];
-          const double x = src == SRC_QIN ? qin : src == SRC_Q0 ? q0 : d;
-          const double term = comb.withdt[t] ? comb.w[t] * dt * x : comb.w[t] * x;
-          acc = t == 0 ? term : acc + term;
-        }
-      }
-      __builtin_nontemporal_store(acc, &Qout.at(oo, l));
-    }
-  }
+  stage_cells<NE, CHEM>(a, i, j, kbeg, min(kbeg + a.kc, a.s2 + a.n2));
 }
 
 // the EOS of the stage's result, T started from the input state (stin may be stout)
@@ -117,14 +37,7 @@ template <int NS, int EOS>
 __global__ __launch_bounds__(256) void k_stage_eos(const double *__restrict__ K, View Q, View stin, View stout, Box3 b) {
   int i, j, k;
   if (!cell_of(b, i, j, k)) return;
-  const Gas<NS> g{K};
-  const int64_t o = Q.off(i, j, k);
-  double Qn[NS + 4];
-#pragma unroll
-  for (int l = 0; l < NS + 4; l++) Qn[l] = Q.at(o, l);
-  EosState s;
-  fromQ_regs<NS, EOS>(g, Qn, stin.at(stin.off(i, j, k), 1), s);
-  store_state<EOS>(stout, stout.off(i, j, k), s);
+  eos_cell<NS, EOS>(K, Q, stin, stout, i, j, k);
 }
 
 // the carried species' production rates of each cell (chemint/kernels/cell.mako:5-20), the
@@ -132,17 +45,9 @@ __global__ __launch_bounds__(256) void k_stage_eos(const double *__restrict__ K,
 template <class M>
 __global__ __launch_bounds__(256) void k_rates(const double *__restrict__ K, const double *__restrict__ C, View Q,
                                                View st, View rates, Box3 b) {
-  constexpr int NS = M::NS;
   int i, j, k;
   if (!cell_of(b, i, j, k)) return;
-  const int64_t o = Q.off(i, j, k);
-  const double rho = Q.at(o, 0), rhoinv = 1.0 / rho;
-  double Y[NS], omega[NS];
-  massfrac<NS>(Q, o, rhoinv, Y);
-  M::production(Gas<NS>{K}, C, rho, Y, st.at(o, 1), omega);
-  const int64_t orr = rates.off(i, j, k);
-#pragma unroll
-  for (int n = 0; n < NS - 1; n++) rates.at(orr, n) = omega[n];
+  rates_cell<M>(K, C, Q, st, rates, i, j, k);
 }
 
 // the halo pass: Q of each interface's halo slab from its partner's interior, then the EOS
@@ -172,21 +77,8 @@ __global__ __launch_bounds__(256) void k_halo(HaloArgs a) {
   const int64_t r = c - t.first;
   if (r >= t.cells) return;
   const int ii = (int)(r % t.e0), jj = (int)((r / t.e0) % t.e1), kk = (int)(r / ((int64_t)t.e0 * t.e1));
-  const int64_t od = t.Qdst.off(t.d0 + ii, t.d1 + jj, t.d2 + kk), os = t.sbase + ii * t.st0 + jj * t.st1 + kk * t.st2;
-  double Q[NS + 4];
-#pragma unroll
-  for (int l = 0; l < NS + 4; l++) Q[l] = t.Qsrc.at(os, l);
-  if (t.turn.on) {
-    const double x = Q[1], y = Q[2], z = Q[3];
-    turned(t.turn.R, x, y, z, Q[1], Q[2], Q[3]);
-  }
-#pragma unroll
-  for (int l = 0; l < NS + 4; l++) t.Qdst.at(od, l) = Q[l];
-  const Gas<NS> g{a.K};
-  const int64_t ost = a.stdst.off(t.d0 + ii, t.d1 + jj, t.d2 + kk);
-  EosState s;
-  fromQ_regs<NS, EOS>(g, Q, a.stguess.at(ost, 1), s);
-  store_state<EOS>(a.stdst, ost, s);
+  const int64_t os = t.sbase + ii * t.st0 + jj * t.st1 + kk * t.st2;
+  halo_cell<NS, EOS>(a.K, t.Qdst, t.Qsrc, os, t.turn, a.stdst, a.stguess, t.d0 + ii, t.d1 + jj, t.d2 + kk);
 }
 
 }  // namespace pga
@@ -203,12 +95,15 @@ extern "C" int32_t pga_stage_supported(const pga_case *c) {
   return ok;
 }
 
-extern "C" int32_t pga_stage(const pga_case *c, pga_view Qin, pga_view stin, pga_view Q0, pga_view Qout,
-                             pga_view stout, pga_view iF, pga_view jF, pga_view kF, pga_view detJinv, pga_view rates,
-                             int32_t nterms, const int32_t *src, const double *w, const int32_t *withdt,
-                             const double *dt, pga_box box, int32_t kchunk, void *stream) {
-  if (!c || !src || !w || !withdt || !dt || nterms < 1 || nterms > 4 || kchunk < 1 || !box_ok(box)) {
-    set_error("pga_stage: null argument, 1 to 4 combine terms, a k-chunk of at least 1");
+// a stage with one combine (nA = 0) or two, the first into QA
+static int32_t stage_impl(const char *who, const pga_case *c, pga_view Qin, pga_view stin, pga_view Q0, pga_view QA,
+                          pga_view Qout, pga_view stout, pga_view iF, pga_view jF, pga_view kF, pga_view detJinv,
+                          pga_view rates, int32_t nA, const int32_t *srcA, const double *wA, const int32_t *withdtA,
+                          int32_t nterms, const int32_t *src, const double *w, const int32_t *withdt, const double *dt,
+                          pga_box box, int32_t kchunk, void *stream) {
+  if (!c || !src || !w || !withdt || !dt || nterms < 1 || nterms > 4 || kchunk < 1 || !box_ok(box) || nA < 0 ||
+      nA > 4 || (nA && (!srcA || !wA || !withdtA))) {
+    set_error("%s: null argument, 1 to 4 combine terms (0 to 4 in the first), a k-chunk of at least 1", who);
     return PGA_EINVAL;
   }
   if (empty(box)) return PGA_OK;
@@ -218,14 +113,19 @@ extern "C" int32_t pga_stage(const pga_case *c, pga_view Qin, pga_view stin, pga
   {
     PGA_REQUIRE(covers(Qin, box, ne) && covers(stin, box, nstate(d)) && covers(Q0, box, ne) && covers(Qout, box, ne) &&
                     covers(stout, box, nstate(d)) && covers(detJinv, box, 1),
-                "pga_stage: the box reaches outside Qin, Q0, Qout, a state or detJinv");
+                "%s: the box reaches outside Qin, Q0, Qout, a state or detJinv", who);
+    if (nA) PGA_REQUIRE(covers(QA, box, ne), "%s: the box reaches outside the first combine's register", who);
     const int z[3] = {0, 0, 0}, fh[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
     PGA_REQUIRE(covers(iF, box, z, fh[0], ne) && covers(jF, box, z, fh[1], ne) && covers(kF, box, z, fh[2], ne),
-                "pga_stage: the box's faces reach outside the face flux arrays");
+                "%s: the box's faces reach outside the face flux arrays", who);
     for (int t = 0; t < nterms; t++)
-      PGA_REQUIRE(src[t] >= 0 && src[t] <= 2, "pga_stage: a combine term reads Qin (0), Q0 (1) or dQ (2)");
+      PGA_REQUIRE(src[t] >= 0 && src[t] <= (nA ? 4 : 2) && src[t] != SRC_OWNA,
+                  "%s: a combine term reads Qin (0), Q0 (1), dQ (2) or the first combine's value (4)", who);
+    for (int t = 0; t < nA; t++)
+      PGA_REQUIRE(srcA[t] >= 0 && srcA[t] <= 3,
+                  "%s: a first-combine term reads Qin (0), Q0 (1), dQ (2) or its own register (3)", who);
     if (chem)
-      PGA_REQUIRE(covers(rates, box, d.ns - 1), "pga_stage: a reacting case needs the rates of every cell (pga_rates)");
+      PGA_REQUIRE(covers(rates, box, d.ns - 1), "%s: a reacting case needs the rates of every cell (pga_rates)", who);
   }
   StageArgs a{};
   a.K = c->K;
@@ -236,6 +136,15 @@ extern "C" int32_t pga_stage(const pga_case *c, pga_view Qin, pga_view stin, pga
   for (int t = 0; t < nterms; t++) {
     a.comb.src[t] = src[t], a.comb.w[t] = w[t], a.comb.withdt[t] = withdt[t];
     a.q0 |= src[t] == SRC_Q0;
+  }
+  if (nA) {
+    a.two = 1;
+    a.QA = view(QA);
+    a.combA.n = nA;
+    for (int t = 0; t < nA; t++) {
+      a.combA.src[t] = srcA[t], a.combA.w[t] = wA[t], a.combA.withdt[t] = withdtA[t];
+      a.q0 |= srcA[t] == SRC_Q0;
+    }
   }
   a.dt = dt;
   a.s0 = box.start[0], a.s1 = box.start[1], a.s2 = box.start[2];
@@ -252,10 +161,31 @@ extern "C" int32_t pga_stage(const pga_case *c, pga_view Qin, pga_view stin, pga
                        a.stout, b);
   });
   if (!done) {
-    set_error("pga_stage: no kernel for %d species with eos %d", d.ns, d.eos);
+    set_error("%s: no kernel for %d species with eos %d", who, d.ns, d.eos);
     return PGA_EUNSUP;
   }
-  return launched("pga_stage");
+  return launched(who);
+}
+
+extern "C" int32_t pga_stage(const pga_case *c, pga_view Qin, pga_view stin, pga_view Q0, pga_view Qout,
+                             pga_view stout, pga_view iF, pga_view jF, pga_view kF, pga_view detJinv, pga_view rates,
+                             int32_t nterms, const int32_t *src, const double *w, const int32_t *withdt,
+                             const double *dt, pga_box box, int32_t kchunk, void *stream) {
+  return stage_impl("pga_stage", c, Qin, stin, Q0, pga_view{}, Qout, stout, iF, jF, kF, detJinv, rates, 0, nullptr,
+                    nullptr, nullptr, nterms, src, w, withdt, dt, box, kchunk, stream);
+}
+
+extern "C" int32_t pga_stage2(const pga_case *c, pga_view Qin, pga_view stin, pga_view Q0, pga_view QA,
+                              pga_view Qout, pga_view stout, pga_view iF, pga_view jF, pga_view kF, pga_view detJinv,
+                              pga_view rates, int32_t nA, const int32_t *srcA, const double *wA,
+                              const int32_t *withdtA, int32_t nterms, const int32_t *src, const double *w,
+                              const int32_t *withdt, const double *dt, pga_box box, int32_t kchunk, void *stream) {
+  if (nA < 1) {
+    set_error("pga_stage2: the first combine needs 1 to 4 terms");
+    return PGA_EINVAL;
+  }
+  return stage_impl("pga_stage2", c, Qin, stin, Q0, QA, Qout, stout, iF, jF, kF, detJinv, rates, nA, srcA, wA,
+                    withdtA, nterms, src, w, withdt, dt, box, kchunk, stream);
 }
 
 extern "C" int32_t pga_rates(const pga_case *c, pga_view Q, pga_view state, pga_view rates, pga_box box,

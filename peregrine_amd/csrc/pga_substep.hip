@@ -30,26 +30,6 @@ struct SubArgs {
   double eqtol, limitinv, binscale;
 };
 
-// cp/R and the internal energy of species n at T (eos/kernels/species.mako:3-27), u = ln T
-template <int NS>
-__device__ __forceinline__ double cpR_species(const Gas<NS> &g, int n, double u) {
-#pragma clang fp contract(off)
-  double cpR = 0.0;
-#pragma unroll
-  for (int m = NT - 1; m >= 0; m--) cpR = cpR * u + g.cppoly(n, m);
-  return cpR;
-}
-
-template <int NS>
-__device__ __forceinline__ double e_species(const Gas<NS> &g, int n, double T, double u, double Tinv) {
-#pragma clang fp contract(off)
-  double hRT = 0.0;
-#pragma unroll
-  for (int m = NT - 1; m >= 0; m--) hRT = hRT * u + g.hpoly(n, m);
-  hRT += g.h298(n) * Tinv;
-  return (hRT - 1.0) * T * g.Rsp(n);
-}
-
 // substep.mako:5-83: up to `limit` substeps from (Y, T); src gathers the mean rate over the step.
 // Returns the substeps taken; `remaining` is the share of the step left
 template <class M, bool EXIT>

@@ -44,6 +44,20 @@ class SubstepOpts(C.Structure):
                 ("equilibrium_tolerance", C.c_double)]
 
 
+class ImplicitOpts(C.Structure):
+    _fields_ = [("parts", _i32), ("max_sweeps", _i32), ("tolerance", C.c_double)]
+
+
+class BatchBlock(C.Structure):
+    _fields_ = [
+        ("Q", View * 4), ("state", View), ("iS", View), ("jS", View), ("kS", View), ("iF", View), ("jF", View),
+        ("kF", View), ("detJinv", View), ("rates", View), ("interior", Box), ("nslab", _i32), ("slab_src", _i32 * 6),
+        ("slab_dst", (_i32 * 3) * 6), ("slab_from", (_i32 * 3) * 6), ("slab_extent", (_i32 * 3) * 6),
+        ("slab_orient", (_i32 * 3) * 6), ("stage_wi", _i32), ("stage_kc", _i32), ("flux_ki", _i32),
+        ("reserved", _i32),
+    ]
+
+
 class CaseDesc(C.Structure):
     _fields_ = [
         ("ns", _i32), ("eos", _i32), ("recon", _i32), ("limiter", _i32), ("primary", _i32), ("mechanism", _i32),
@@ -79,6 +93,8 @@ SIGNATURES = {
     "pga_stage_supported": (_i32, [C.c_void_p]),
     "pga_stage": (_i32, [C.c_void_p, View, View, View, View, View, View, View, View, View, View, _i32, _ip, _dp, _ip,
                          C.c_void_p, Box, _i32, C.c_void_p]),
+    "pga_stage2": (_i32, [C.c_void_p, View, View, View, View, View, View, View, View, View, View, View, _i32, _ip, _dp,
+                          _ip, _i32, _ip, _dp, _ip, C.c_void_p, Box, _i32, C.c_void_p]),
     "pga_rates": (_i32, [C.c_void_p, View, View, View, Box, C.c_void_p]),
     "pga_halo": (_i32, [C.c_void_p, _i32, C.POINTER(View), C.POINTER(View), _ip, _ip, _ip, _ip, _dp, View, View,
                         C.c_void_p]),
@@ -96,6 +112,28 @@ SIGNATURES = {
                            C.c_void_p, C.c_int64, C.c_void_p, Box, C.c_void_p]),
     "pga_cflmax": (_i32, [C.c_void_p, View, View, View, View, View, View, Box, C.c_void_p, C.c_int64, C.c_void_p,
                           C.c_void_p]),
+    "pga_batch_bytes": (C.c_int64, [_i32, _i32]),
+    "pga_batch_create": (_i32, [C.c_void_p, _i32, C.POINTER(BatchBlock), _i32, C.c_void_p, C.c_int64,
+                                C.POINTER(C.c_void_p)]),
+    "pga_batch_destroy": (_i32, [C.c_void_p]),
+    "pga_batch_halo": (_i32, [C.c_void_p, _i32, C.c_void_p]),
+    "pga_batch_rates": (_i32, [C.c_void_p, _i32, C.c_void_p]),
+    "pga_batch_flux": (_i32, [C.c_void_p, _i32, _i32, C.c_void_p]),
+    "pga_batch_stage": (_i32, [C.c_void_p, _i32, _i32, _i32, _ip, _dp, _ip, C.c_void_p, C.c_void_p]),
+    "pga_batch_implicit_worksize": (C.c_int64, [C.c_void_p]),
+    "pga_batch_implicit": (_i32, [C.c_void_p, C.POINTER(ImplicitOpts), _i32, C.c_void_p, C.c_int64, C.c_void_p,
+                                  C.c_int64, C.c_void_p, C.c_void_p]),
+    "pga_graph_begin": (_i32, [C.c_void_p]),
+    "pga_graph_end": (_i32, [C.c_void_p, C.POINTER(C.c_void_p)]),
+    "pga_graph_launch": (_i32, [C.c_void_p, C.c_void_p]),
+    "pga_graph_destroy": (_i32, [C.c_void_p]),
+    "pga_allfinite": (_i32, [View, _i32, Box, C.c_void_p, C.c_void_p]),
+    "pga_implicit_worksize": (C.c_int64, [C.c_void_p, Box]),
+    "pga_implicit": (_i32, [C.c_void_p, C.POINTER(ImplicitOpts), View, View, View, _i32, C.c_void_p, C.c_int64,
+                            C.c_void_p, C.c_int64, C.c_void_p, Box, C.c_void_p]),
+    "pga_lu_worksize": (C.c_int64, [C.c_int64, _i32]),
+    "pga_lu_solve": (_i32, [C.c_int64, _i32, C.c_void_p, C.c_void_p, C.c_void_p, _i32, C.c_double, C.c_void_p,
+                            C.c_void_p, C.c_int64, C.c_void_p]),
 }
 
 
@@ -107,7 +145,7 @@ _lib = None
 
 
 # the library interface these bindings are written against (PGA_ABI_VERSION)
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 
 def lib():

@@ -103,9 +103,14 @@ class NanCheck(BasePlugin):
     name = "nancheck"
 
     def __call__(self, solver):
+        """The all-finite reduction over each block's interior on the device (pga_allfinite,
+        ``plugins/solver/kernels/allfinite.mako:3-19``), one flag per block read back together."""
         torch = solver.backend.torch
-        ng = solver.ng
-        bad = [blk for blk in solver.blocks if not bool(torch.isfinite(blk.Q.t[:, ng:-ng, ng:-ng, ng:-ng]).all())]
+        flags = torch.zeros(len(solver.blocks), dtype=torch.int32, device=solver.backend.device)
+        for n, blk in enumerate(solver.blocks):
+            solver._call("pga_allfinite", blk.Q.view(), solver.sim.ne, blk.interior_box(), flags.data_ptr() + 4 * n)
+        ok = flags.cpu().numpy()
+        bad = [blk for blk, f in zip(solver.blocks, ok) if not f]
         if not bad:
             return
         # where: the centres of the cells holding a non-finite value, a file per block

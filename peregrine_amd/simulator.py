@@ -74,8 +74,8 @@ class EulerSimulator(BaseSimulator):
         self.recon, self.limiter, self.primary = adv["recon"], adv["limiter"], adv["primary"]
         ci = cfg["chem-integration"]
         kind = ci["kind"]
-        if kind not in (None, "coupled", "sub-stepped"):
-            raise ConfigError("chem-integration.kind", "this build has coupled and sub-stepped chemistry")
+        if kind not in (None, "coupled", "sub-stepped", "implicit"):
+            raise ConfigError("chem-integration.kind", "this build has coupled, sub-stepped and implicit chemistry")
         if kind and self.mixture.chemistry is None:
             raise ConfigError("chem-integration.kind", f"{kind} needs mixture.chemistry")
         self.chemistry = bool(kind)
@@ -95,6 +95,18 @@ class EulerSimulator(BaseSimulator):
                 raise ConfigError("chem-integration.equilibrium-tolerance",
                                   f"{tol!r} is not a mass fraction below one, zero for no exit")
             self.substep = dict(cap=cap, bulk=bulk, bins=bins, eqtol=float(tol))
+        self.implicit = None
+        if kind == "implicit":
+            # the linearly implicit Euler form (no counterpart in the reference): its parts per step,
+            # and the refinement of each FP32-factored solve against the FP64 matrix
+            parts, sweeps, rtol = (ci[k] for k in ("implicit-parts", "max-sweeps", "refine-tolerance"))
+            if not isinstance(parts, int) or isinstance(parts, bool) or parts < 1:
+                raise ConfigError("chem-integration.implicit-parts", f"{parts!r} is not a count of one or more")
+            if not isinstance(sweeps, int) or isinstance(sweeps, bool) or sweeps < 1:
+                raise ConfigError("chem-integration.max-sweeps", f"{sweeps!r} is not a count of one or more")
+            if not isinstance(rtol, (int, float)) or isinstance(rtol, bool) or not 0.0 < rtol < 1.0:
+                raise ConfigError("chem-integration.refine-tolerance", f"{rtol!r} is not a share between 0 and 1")
+            self.implicit = dict(parts=parts, sweeps=sweeps, tol=float(rtol))
         self.iovars = IoVars(self)
 
     @property

@@ -30,6 +30,7 @@ class Block:
         self.interior_nodes = None
         self.dijk = None
         self.substeps = None
+        self.sweeps = None
 
     def intface(self, nface):
         return self.faces[nface]
@@ -252,6 +253,9 @@ class Solver:
             for blk in self.blocks:
                 rshape = blk.cellshape + (max(sim.ns - 1, 1),)
                 blk.rates = backend.alloc(rshape, name="rates", lead=self.ng) if sim.chemistry else None
+        self.batched = self._batch_supported()
+        self.batch = None
+        self.tiles = None
         self.plugins = self._plugins()
         self._set_state(state)
         for p in self.plugins:
@@ -338,11 +342,25 @@ class Solver:
         plugins, self.sponges = claim(self.cfg["plugins"], self)
         return plugins
 
+    def _batch_supported(self):
+        """The fused stage's phases run as one launch over every block (pga_batch_*) when the case has
+        more than one block, each an inviscid block with no walls and no rotational periodic
+        (``backend-mi355x.batched``: auto, true or false)."""
+        want = self.cfg["backend-mi355x"]["batched"]
+        if want is False or want == "false" or not self.fused or len(self.blocks) < 2 or self.integ.name == "rk4":
+            return False
+        if self.sim.viscous or int(self.cfg["backend-mi355x"]["stage-slabs"]) != 1:
+            return False
+        for blk in self.blocks:
+            if blk.walls or any(f.nbr is not None and f.rotperiodic for f in blk.faces.values()):
+                return False
+        return True
+
     def _fused_supported(self):
         """The fused stage runs when the config asks for it, the integrator takes one combine per
         stage, and the library has the case's variant compiled."""
         sect = self.cfg["backend-mi355x"]
-        if not sect["fused-stage"] or self.integ.name not in ("rk1", "rk2", "tvd-rk3"):
+        if not sect["fused-stage"] or self.integ.name not in ("rk1", "rk2", "tvd-rk3", "rk4"):
             return False
         return bool(native.lib().pga_stage_supported(self.case.handle))
 
@@ -498,9 +516,10 @@ class Solver:
 
     def chemistry_source(self, blk, Qv, out, begin_dQ):
         """The chemistry's term of the derivative over the interior: the coupled source (the
-        production rates at the stage's state) or the sub-stepped mean rate over the step
-        (``chemint/coupled.py:10-13``, ``chemint/substepped.py:52-68``); into dQ, rows 0..4 zero,
-        when begin_dQ, else into the rates array the fused stage adds."""
+        production rates at the stage's state), the sub-stepped mean rate over the step
+        (``chemint/coupled.py:10-13``, ``chemint/substepped.py:52-68``) or the implicit form's mean
+        rate (linearly implicit Euler, ``pga_implicit``, in the sub-stepped form's slot); into dQ,
+        rows 0..4 zero, when begin_dQ, else into the rates array the fused stage adds."""
         box = blk.interior_box()
         if self.sim.chemkind == "coupled":
             if begin_dQ:
@@ -508,7 +527,19 @@ class Solver:
             else:
                 self._call("pga_rates", self.case.handle, Qv, blk.state.view(), out, box)
             return
-        opts, torch = self.sim.substep, self.backend.torch
+        torch = self.backend.torch
+        if self.sim.chemkind == "implicit":
+            opts = self.sim.implicit
+            if blk.sweeps is None:
+                blk.imopts = native.ImplicitOpts(opts["parts"], opts["sweeps"], opts["tol"])
+                blk.sweeps = torch.zeros(blk.ncells, dtype=torch.int32, device=self.backend.device)
+                n = int(native.lib().pga_implicit_worksize(self.case.handle, box))
+                blk.imwork = torch.zeros(max(n, 1), dtype=torch.float64, device=self.backend.device)
+            self._call("pga_implicit", self.case.handle, ctypes.byref(blk.imopts), Qv, blk.state.view(), out,
+                       int(begin_dQ), blk.sweeps.data_ptr(), blk.ncells, blk.imwork.data_ptr(), blk.imwork.numel(),
+                       self.dt.ptr, box)
+            return
+        opts = self.sim.substep
         if blk.substeps is None:
             o = native.SubstepOpts(opts["cap"], opts["bulk"], opts["bins"], 0, opts["eqtol"])
             blk.subopts = o
@@ -595,17 +626,27 @@ class Solver:
 
     def _combine_terms(self, n):
         """The stage's combine as (source, weight, times dt) terms in the reference's order: its
-        registers read as the step's start (Q0), the stage's input (Qin) or its derivative."""
-        (c,) = self.integ.stagekerns(n, self.regs)
+        registers read as the step's start (Q0), the stage's input (Qin) or its derivative. An rk4
+        stage has two combines, [first, second]: the first's own register reads as 3, and the second
+        reads the first's output (the running total) as the value the first just took, 4."""
+        kerns = self.integ.stagekerns(n, self.regs)
         dQ = self.regs[-1]
+        first = kerns[0].out if len(kerns) == 2 else None
 
-        def src(r):
+        def src(r, own=False):
+            if own:
+                return 3
+            if r == first:
+                return 4
             return 2 if r == dQ else (1 if r == 0 else 0)
 
-        terms = [(src(c.out), c.wout, 0)] if c.own else []
-        for m, (r, w) in enumerate(zip(c.X, c.w)):
-            terms.append((src(r), w, int(c.step and m == len(c.X) - 1)))
-        return terms
+        out = []
+        for m, c in enumerate(kerns):
+            terms = [(src(c.out, own=m == 0 and len(kerns) == 2), c.wout, 0)] if c.own else []
+            for k, (r, w) in enumerate(zip(c.X, c.w)):
+                terms.append((src(r), w, int(c.step and k == len(c.X) - 1)))
+            out.append(terms)
+        return out
 
     _streams = None
 
@@ -635,6 +676,109 @@ class Solver:
         for s in used:
             main.wait_stream(s)
 
+    class _Batch:
+        """A table of blocks on the device (pga_batch_create) and the buffer it lives in."""
+
+        def __init__(self, handle, table, keep):
+            self.handle, self.table, self.keep = handle, table, keep
+            self.imwork = None
+
+        def close(self):
+            if self.handle:
+                native.lib().pga_batch_destroy(self.handle)
+                self.handle = None
+
+        def __del__(self):
+            self.close()
+
+    def _make_batch(self, tiles, regs=None, states=None):
+        """The table of every block: its registers (those named by regs, in order, else all), state
+        (or the given stand-ins), faces, 1/J, rates, interior and halo slabs, and the tile shapes of its
+        size class (tiles: {class: {"stage": (wi, kc), "fluxk": ki}})."""
+        from peregrine_amd import autotune
+
+        torch = self.backend.torch
+        regs = tuple(range(self.integ.nregs)) if regs is None else tuple(regs)
+        recs = (native.BatchBlock * len(self.blocks))()
+        nslab = 0
+        for n, blk in enumerate(self.blocks):
+            r = recs[n]
+            for m, q in enumerate(regs):
+                r.Q[m] = blk.regs[q].view()
+            r.state = (states[n] if states is not None else blk.state).view()
+            r.iS, r.jS, r.kS = blk.iS.view(), blk.jS.view(), blk.kS.view()
+            r.iF, r.jF, r.kF = blk.iF.view(), blk.jF.view(), blk.kF.view()
+            r.detJinv = blk.detJinv.view()
+            r.rates = blk.rates.view() if blk.rates is not None else native.View()
+            r.interior = blk.interior_box()
+            t = 0
+            for f in blk.faces.values():
+                if f.nbr is None:
+                    continue
+                dst, src, ext, orient = placement(blk, f, self.blocks[f.nbr], self.ng)
+                r.slab_src[t] = f.nbr
+                for d in range(3):
+                    r.slab_dst[t][d], r.slab_from[t][d] = dst[d], src[d]
+                    r.slab_extent[t][d], r.slab_orient[t][d] = ext[d], orient[d]
+                t += 1
+            r.nslab = t
+            nslab += t
+            choice = (tiles or {}).get(autotune.size_class(blk.ncell[0]), autotune.DEFAULT)
+            r.stage_wi, r.stage_kc = choice["stage"]
+            r.flux_ki = choice["fluxk"]
+        L = native.lib()
+        nbytes = int(L.pga_batch_bytes(len(self.blocks), nslab))
+        table = torch.zeros(nbytes, dtype=torch.uint8, device=self.backend.device)
+        handle = ctypes.c_void_p()
+        native.check(L.pga_batch_create(self.case.handle, len(self.blocks), recs, len(regs), table.data_ptr(), nbytes,
+                                        ctypes.byref(handle)), "pga_batch_create")
+        return self._Batch(handle, table, recs)
+
+    def _batched_step(self):
+        """The fused step with each phase one launch over every block: the halo pass (unless from_prims
+        left the halos ready), the chemistry rates, the three flux sweeps, and the stage with its EOS."""
+        if self.batch is None:
+            if self.tiles is None:
+                from peregrine_amd import autotune
+
+                self.tiles = autotune.tune(self) if self.cfg["backend-mi355x"]["autotune"] else {}
+            self.batch = self._make_batch(self.tiles)
+        h = self.batch.handle
+        for n, regs in enumerate(self.integ.stageregs(self.regs)):
+            if regs.Q not in self.readied:
+                self._call("pga_batch_halo", h, regs.Q)
+            self.readied.clear()
+            (terms,) = self._combine_terms(n)
+            src = native.ints([t[0] for t in terms])
+            w = (ctypes.c_double * len(terms))(*[t[1] for t in terms])
+            withdt = native.ints([t[2] for t in terms])
+            if self.sim.chemistry:
+                if self.sim.chemkind == "coupled":
+                    self._call("pga_batch_rates", h, regs.Q)
+                elif self.sim.chemkind == "implicit":
+                    self._batch_implicit(regs.Q)
+                else:
+                    for blk in self.blocks:
+                        self.chemistry_source(blk, blk.regs[regs.Q].view(), blk.rates.view(), False)
+            for a in range(3):
+                self._call("pga_batch_flux", h, regs.Q, a)
+            self._call("pga_batch_stage", h, regs.Q, regs.Qout, len(terms), src, w, withdt, self.dt.ptr)
+
+    def _batch_implicit(self, q):
+        """Implicit chemistry over every block in one launch per phase, into each block's rates."""
+        torch = self.backend.torch
+        L = native.lib()
+        if getattr(self.batch, "imwork", None) is None:
+            opts = self.sim.implicit
+            self.batch.imopts = native.ImplicitOpts(opts["parts"], opts["sweeps"], opts["tol"])
+            n = int(L.pga_batch_implicit_worksize(self.batch.handle))
+            self.batch.imwork = torch.zeros(max(n, 1), dtype=torch.float64, device=self.backend.device)
+            cells = int(sum(-(-blk.ncells // 256) for blk in self.blocks)) * 256
+            self.batch.sweeps = torch.zeros(cells, dtype=torch.int32, device=self.backend.device)
+        b = self.batch
+        self._call("pga_batch_implicit", b.handle, ctypes.byref(b.imopts), q, b.sweeps.data_ptr(), b.sweeps.numel(),
+                   b.imwork.data_ptr(), b.imwork.numel(), self.dt.ptr)
+
     def _slabs(self, blk):
         return stage_slabs(blk.ncell[2], int(self.cfg["backend-mi355x"]["stage-slabs"]))
 
@@ -652,10 +796,17 @@ class Solver:
             self.readied.clear()
             if self.sim.viscous:
                 self.gradients(regs.Q)
-            terms = self._combine_terms(n)
+            combines = self._combine_terms(n)
+            terms = combines[-1]
             src = native.ints([t[0] for t in terms])
             w = (ctypes.c_double * len(terms))(*[t[1] for t in terms])
             withdt = native.ints([t[2] for t in terms])
+            if len(combines) == 2:
+                tA = combines[0]
+                first = (len(tA), native.ints([t[0] for t in tA]), (ctypes.c_double * len(tA))(*[t[1] for t in tA]),
+                         native.ints([t[2] for t in tA]), self.integ.stagekerns(n, self.regs)[0].out)
+            else:
+                first = None
             for blk, stream in self._block_streams():
                 with stream:
                     box = blk.interior_box()
@@ -667,9 +818,16 @@ class Solver:
                         self.fluxes(blk, Qin, st, slab)
                         k0, k1 = slab[:2]
                         sbox = native.box((self.ng, self.ng, self.ng + k0), (blk.ncell[0], blk.ncell[1], k1 - k0))
-                        self._call("pga_stage", self.case.handle, Qin, st, blk.regs[0].view(),
-                                   blk.regs[regs.Qout].view(), st, blk.iF.view(), blk.jF.view(), blk.kF.view(),
-                                   blk.detJinv.view(), rates, len(terms), src, w, withdt, self.dt.ptr, sbox, kc)
+                        if first is None:
+                            self._call("pga_stage", self.case.handle, Qin, st, blk.regs[0].view(),
+                                       blk.regs[regs.Qout].view(), st, blk.iF.view(), blk.jF.view(), blk.kF.view(),
+                                       blk.detJinv.view(), rates, len(terms), src, w, withdt, self.dt.ptr, sbox, kc)
+                        else:
+                            nA, srcA, wA, dtA, outA = first
+                            self._call("pga_stage2", self.case.handle, Qin, st, blk.regs[0].view(),
+                                       blk.regs[outA].view(), blk.regs[regs.Qout].view(), st, blk.iF.view(),
+                                       blk.jF.view(), blk.kF.view(), blk.detJinv.view(), rates, nA, srcA, wA, dtA,
+                                       len(terms), src, w, withdt, self.dt.ptr, sbox, kc)
 
     _graph = None
     _fresh = True
@@ -708,7 +866,7 @@ class Solver:
         self.dt.set(dt)
         times = self.integ.stagetimes(start, dt)
         if self.fused:
-            launches = self._fused_step
+            launches = self._batched_step if self.batched else self._fused_step
         else:
             def launches():
                 for n in range(self.integ.nstages):

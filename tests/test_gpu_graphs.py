@@ -105,31 +105,40 @@ def test_graph_replays_are_the_eager_steps(gpu, case):
 
 
 def test_an_uneven_batch_steps_as_its_cases_one_by_one(gpu):
-    """Three periodic H2/O2 boxes of different sizes in one solver, each step one graph replay over
-    all of them (or launched eagerly, the blocks overlapping on side streams), against each box
-    stepped by a solver of its own, eagerly: to the bit."""
+    """Three periodic H2/O2 boxes of different sizes in one solver, against each box stepped by a
+    solver of its own, eagerly: to the bit. The batch runs each phase as one launch over the three
+    (pga_batch_*), with the autotuner's tiles and replayed from one graph per step, with tiles forced
+    to the narrowest candidates, and block at a time with the blocks overlapping on side streams."""
     import peregrine_amd as pg
 
     dims = [(16, 12, 20), (24, 16, 12), (12, 20, 16)]
     L = (0.01,) * 3
 
-    def solver(dlist, graphs):
+    def solver(dlist, graphs, batched="auto", tiles=None):
         cfg = sp.config("h2o2", recon="muscl", primary="hllc", limiter="koren", chem=True, dt=1e-8)
         cfg["backend-mi355x"]["graphs"] = graphs
+        cfg["backend-mi355x"]["batched"] = batched
         mb = pg.get_solver(cfg, pg.mesher.BatchMesher(dlist, lengths=L), pg.get_backend("mi355x", cfg))
+        if tiles is not None:
+            mb.tiles = tiles
         mb.set_prims([sp.h2_prims(b.xcc.get(), b.ncell, lengths=L) for b in mb.blocks])
         return mb
 
+    narrow = {c: {"stage": (16, 2), "fluxk": 16} for c in ("narrow", "small", "medium", "wide")}
     together = solver(dims, True)
-    # launched eagerly, the blocks' stages overlap on side streams
-    overlapped = solver(dims, False)
+    forced = solver(dims, False, tiles=narrow)
+    # block at a time, launched eagerly, the blocks' stages overlapping on side streams
+    overlapped = solver(dims, False, batched=False)
     alone = [solver([d], False) for d in dims]
+    assert together.batched and forced.batched and not overlapped.batched and not alone[0].batched
     for _ in range(4):
-        together.step(1e-8)
-        overlapped.step(1e-8)
-        for mb in alone:
+        for mb in (together, forced, overlapped, *alone):
             mb.step(1e-8)
+    print("tiles", together.tiles)
     assert together.replays == 3 and overlapped._streams is not None
-    for blk, other, mb in zip(together.blocks, overlapped.blocks, alone):
-        assert np.array_equal(blk.Q.get(), mb.blocks[0].Q.get())
-        assert np.array_equal(other.Q.get(), mb.blocks[0].Q.get())
+    assert set(together.tiles) == {"narrow"}
+    for blocks in zip(together.blocks, forced.blocks, overlapped.blocks, alone):
+        want = blocks[-1].blocks[0]
+        for blk in blocks[:-1]:
+            assert np.array_equal(blk.Q.get(), want.Q.get())
+            assert np.array_equal(blk.state.get(), want.state.get())

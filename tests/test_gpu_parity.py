@@ -253,11 +253,13 @@ def test_halo_copies(dev):
 
 
 def _run_pair(gas, recon, primary, limiter, chem, integrator, ncell, steps, dt, prims, lengths, fused=True,
-              trans=None, species=None, nstate=8, walls=(), substep=None, mixture=None, slabs=1, sponge=None):
+              trans=None, species=None, nstate=8, walls=(), substep=None, mixture=None, slabs=1, sponge=None,
+              implicit=None):
     """The same case stepped by the solver on the device and by the oracle on the host; walls: the
     faces (0 i-low .. 5 k-high) that are adiabatic slip walls, the rest periodic; substep: the
     sub-stepped chemistry's (max-substeps, bulk-substeps, stiff-bins, equilibrium-tolerance), the
-    oracle taking the one-pass form."""
+    oracle taking the one-pass form; implicit: the implicit chemistry's (parts, max-sweeps,
+    refine-tolerance), the oracle taking the FP64 form."""
     import peregrine_amd as pg
 
     cfg = sp.config(gas, recon=recon, primary=primary, limiter=limiter, chem=chem, integrator=integrator, dt=dt,
@@ -266,6 +268,10 @@ def _run_pair(gas, recon, primary, limiter, chem, integrator, ncell, steps, dt, 
         ci = cfg["chem-integration"]
         ci["kind"] = "sub-stepped"
         ci["max-substeps"], ci["bulk-substeps"], ci["stiff-bins"], ci["equilibrium-tolerance"] = substep
+    if implicit is not None:
+        ci = cfg["chem-integration"]
+        ci["kind"] = "implicit"
+        ci["implicit-parts"], ci["max-sweeps"], ci["refine-tolerance"] = implicit
     if species is not None:
         cfg["mixture"]["species"] = species
     for key, value in (mixture or {}).items():
@@ -307,6 +313,8 @@ def _run_pair(gas, recon, primary, limiter, chem, integrator, ncell, steps, dt, 
         b.chemistry, b.subcap, b.subeqtol = 2, substep[0], substep[3]
         hb.counts = np.zeros(int(np.prod(ncell)), dtype=np.int32)
         b.substeps = hb.counts.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+    if implicit is not None:
+        b.chemistry, b.imparts = 3, implicit[0]
     for _ in range(steps):
         mb.step(dt)
         O.lib.pgo_step(oc.ref, ctypes.byref(b), dt)

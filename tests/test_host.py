@@ -87,12 +87,12 @@ def test_expintegral_is_the_exponential_integral():
 
 def test_the_library_exports_every_declared_symbol():
     header = (ROOT / "include" / "peregrine_amd.h").read_text()
-    names = re.findall(r"^\s*(?:int32_t|const char \*)\s*(pga_\w+)\(", header, re.M)
+    names = re.findall(r"^\s*(?:int32_t|int64_t|const char \*)\s*(pga_\w+)\(", header, re.M)
     assert len(names) >= 20
     lib = ctypes.CDLL(str(native.LIBPATH))
     for name in names:
         assert hasattr(lib, name), name
-    assert native.lib().pga_abi_version() == native.ABI_VERSION == 3
+    assert native.lib().pga_abi_version() == native.ABI_VERSION == 4
 
 
 def test_the_constant_layout():
