@@ -142,3 +142,67 @@ def test_an_uneven_batch_steps_as_its_cases_one_by_one(gpu):
         for blk in blocks[:-1]:
             assert np.array_equal(blk.Q.get(), want.Q.get())
             assert np.array_equal(blk.state.get(), want.state.get())
+
+
+def test_a_step_captured_and_replayed_through_the_c_abi(gpu):
+    """A reacting step's launches captured on a stream of their own by pga_graph_begin/end and replayed
+    three times by pga_graph_launch: to the bit of three steps launched eagerly."""
+    import ctypes
+
+    import torch
+
+    import peregrine_amd as pg
+    from peregrine_amd import native
+
+    L = (0.01,) * 3
+
+    def solver():
+        cfg = sp.config("h2o2", recon="muscl", primary="hllc", limiter="koren", chem=True, dt=1e-8)
+        cfg["backend-mi355x"]["graphs"] = False
+        mb = pg.get_solver(cfg, pg.mesher.CubeMesher(blkdims=[17] * 3, lengths=L, periodic=[True] * 3),
+                           pg.get_backend("mi355x", cfg))
+        mb.set_prims([sp.h2_prims(b.xcc.get(), b.ncell, lengths=L) for b in mb.blocks])
+        return mb
+
+    eager, replayed = solver(), solver()
+    for mb in (eager, replayed):
+        mb.step(1e-8)
+    lib = native.lib()
+    stream = torch.cuda.Stream()
+    graph = ctypes.c_void_p()
+    torch.cuda.synchronize()
+    with torch.cuda.stream(stream):
+        native.check(lib.pga_graph_begin(ctypes.c_void_p(stream.cuda_stream)), "pga_graph_begin")
+        replayed._fused_step()
+        native.check(lib.pga_graph_end(ctypes.c_void_p(stream.cuda_stream), ctypes.byref(graph)), "pga_graph_end")
+    try:
+        for _ in range(3):
+            native.check(lib.pga_graph_launch(graph, ctypes.c_void_p(stream.cuda_stream)), "pga_graph_launch")
+            eager.step(1e-8)
+        stream.synchronize()
+    finally:
+        lib.pga_graph_destroy(graph)
+    assert np.array_equal(replayed.blocks[0].Q.get(), eager.blocks[0].Q.get())
+    assert np.array_equal(replayed.blocks[0].state.get(), eager.blocks[0].state.get())
+
+
+def test_allfinite_finds_a_planted_nan(gpu):
+    import torch
+
+    from peregrine_amd import native
+
+    lib = native.lib()
+    a = torch.ones((5, 6, 7, 9), dtype=torch.float64, device="cuda")
+    out = torch.full((2,), 7, dtype=torch.int32, device="cuda")
+    view = native.view(a.data_ptr(), 9, 7, 6, 5)
+    box = native.box((1, 1, 1), (7, 5, 4))
+    native.check(lib.pga_allfinite(view, 5, box, out.data_ptr(), None), "pga_allfinite")
+    assert out.tolist() == [1, 7]
+    a[3, 2, 3, 4] = float("nan")  # component 3 of cell (4, 3, 2)
+    native.check(lib.pga_allfinite(view, 5, box, out.data_ptr(), None), "pga_allfinite")
+    assert out.tolist() == [0, 7]
+    # outside the box, or past the components checked, it is not seen
+    a[3, 2, 3, 4] = 1.0
+    a[4, 0, 0, 0] = float("inf")
+    native.check(lib.pga_allfinite(view, 4, box, out.data_ptr(), None), "pga_allfinite")
+    assert out.tolist() == [1, 7]
