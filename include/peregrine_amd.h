@@ -291,6 +291,16 @@ int32_t pga_graph_destroy(pga_graph *g);
  * peregrine/plugins/solver/nancheck.py. */
 int32_t pga_allfinite(pga_view A, int32_t ncomp, pga_box box, int32_t *out, void *stream);
 
+/* pga_cflmax with no work argument and no allocation: the partials go to a 6 MiB array the library
+ * holds on each GPU for the life of the process (its address looked up on the first call on that
+ * device). No counterpart in the reference: for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). Boxes up to 2^26 cells
+ * (262144 partials); a larger one is refused with PGA_ERANGE. Calls on one GPU share the array, so
+ * calls that overlap on one GPU (different streams, or threads) race; the array keeps the last
+ * call's partials, each call writing every partial it reads. */
+int32_t pga_cflmax_noalloc(const pga_case *c, pga_view Q, pga_view state, pga_view dijk, pga_view iS, pga_view jS,
+                           pga_view kS, pga_box box, double *out, void *stream);
+
 /* Sub-stepped chemistry's options: chem-integration.max-substeps, .bulk-substeps, .stiff-bins and
  * .equilibrium-tolerance (peregrine/simulator/chemint/substepped.py:14-33) */
 typedef struct {

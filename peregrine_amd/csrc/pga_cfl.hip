@@ -3,6 +3,8 @@
 // (codegen/grammar.py:34-42). One flat launch writes a partial per workgroup into the caller's
 // work array; one workgroup folds the partials into out[3]. fmax is exact and order-free, so the
 // result does not depend on the launch shape.
+#include <mutex>
+
 #include "pga_common.hpp"
 #include "pga_internal.hpp"
 
@@ -76,6 +78,36 @@ __global__ __launch_bounds__(FLAT) void k_cflfold(const double *__restrict__ par
     for (int n = 0; n < 3; n++) out[n] = best[n];
 }
 
+// The allocation-free entry point's partials: one array per GPU for the process's life, addressed on
+// the host only through the pointer hipGetSymbolAddress gives for the current device (the array's
+// name in host code is a stand-in, not a device address). Every call on one GPU shares it.
+constexpr int64_t CFL_SCRATCH_PARTS = 262144;  // 6 MiB: boxes up to 2^26 cells (a 406^3 block)
+static __device__ __align__(256) double scratch[3 * CFL_SCRATCH_PARTS];
+
+constexpr int CFL_MAX_DEVICES = 64;
+static std::mutex scratch_lock;
+static double *scratch_ptr[CFL_MAX_DEVICES];
+
+// the current device's scratch array, its address looked up once per device; null with the error set
+static double *device_scratch() {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= CFL_MAX_DEVICES) {
+    set_error("pga_cflmax_noalloc: no current device below %d", CFL_MAX_DEVICES);
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> hold(scratch_lock);
+  if (!scratch_ptr[dev]) {
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+      set_error("pga_cflmax_noalloc: the scratch array's address on device %d: %s", dev, hipGetErrorString(e));
+      return nullptr;
+    }
+    scratch_ptr[dev] = static_cast<double *>(ptr);
+  }
+  return scratch_ptr[dev];
+}
+
 // the partials one call writes: a workgroup per FLAT cells of each slab
 static int64_t cfl_parts(const pga_box &box) {
   int64_t parts = 0;
@@ -123,4 +155,20 @@ extern "C" int32_t pga_cflmax(const pga_case *c, pga_view Q, pga_view state, pga
     });
   hipLaunchKernelGGL(k_cflfold, dim3(1), dim3(FLAT), 0, stream_of(stream), work, parts, out);
   return launched("pga_cflmax");
+}
+
+extern "C" int32_t pga_cflmax_noalloc(const pga_case *c, pga_view Q, pga_view state, pga_view dijk, pga_view iS,
+                                      pga_view jS, pga_view kS, pga_box box, double *out, void *stream) {
+  if (!c || !box_ok(box) || !out) {
+    set_error("pga_cflmax_noalloc: null case or out, or a bad box");
+    return PGA_EINVAL;
+  }
+  if (!flat_fits(box)) return PGA_ERANGE;
+  const int64_t parts = cfl_parts(box);
+  PGA_REQUIRE(parts <= CFL_SCRATCH_PARTS,
+              "pga_cflmax_noalloc: the box needs %lld partials, the scratch array holds %lld (boxes up to %lld cells)",
+              (long long)parts, (long long)CFL_SCRATCH_PARTS, (long long)(CFL_SCRATCH_PARTS * FLAT));
+  double *work = device_scratch();
+  if (!work) return PGA_EHIP;
+  return pga_cflmax(c, Q, state, dijk, iS, jS, kS, box, work, 3 * CFL_SCRATCH_PARTS, out, stream);
 }

@@ -112,6 +112,7 @@ SIGNATURES = {
                            C.c_void_p, C.c_int64, C.c_void_p, Box, C.c_void_p]),
     "pga_cflmax": (_i32, [C.c_void_p, View, View, View, View, View, View, Box, C.c_void_p, C.c_int64, C.c_void_p,
                           C.c_void_p]),
+    "pga_cflmax_noalloc": (_i32, [C.c_void_p, View, View, View, View, View, View, Box, C.c_void_p, C.c_void_p]),
     "pga_batch_bytes": (C.c_int64, [_i32, _i32]),
     "pga_batch_create": (_i32, [C.c_void_p, _i32, C.POINTER(BatchBlock), _i32, C.c_void_p, C.c_int64,
                                 C.POINTER(C.c_void_p)]),
