@@ -42,6 +42,13 @@ def case_config(support):
     return support.config("h2o2", recon="muscl", primary="hllc", limiter="koren", chem=True, integrator="tvd-rk3", dt=DT)
 
 
+def batch_config(support):
+    c = case_config(support)
+    c["chem-integration"]["kind"] = "implicit"
+    c.validate()
+    return c
+
+
 def b_alg(ne, rbar=5.0 / 3.0, wbar=1.0, tguess=2, G=10):
     """Compulsory bytes per cell per stage of a fused pass (tvd-rk3, Euler): the stage input read
     and the step's start read in two of three stages (rbar ne), the result written (wbar ne), T read
@@ -114,24 +121,29 @@ def measured_traffic(kernel):
 
 def batch(args, pg, support):
     """The uneven batch: args.batch periodic H2/O2 boxes with edges drawn by
-    default_rng(20240611).integers(16, 97), the reacting case's model and step, stepped together by
-    one solver launched eagerly with the blocks' stages overlapping on side streams (value; also
-    as one HIP graph replay per step over every block) against the same boxes stepped by a solver
-    each, one after another, launched eagerly: solves/s = cases x steps / wall time."""
+    default_rng(20240611).integers(16, 97), MUSCL-Koren + HLLC, implicit chemistry (linearly implicit
+    Euler, FP32 matrix-core LU with FP64 refinement), tvd-rk3. value: one solver stepping every box,
+    each phase one launch over the table of blocks, each step replayed from its HIP graph; solves/s =
+    boxes x steps / wall time. Beside it the same boxes stepped by a solver each, one after another
+    (the plain per-case form, launched eagerly and replayed from a graph each), the table launched
+    eagerly, and a launch per block (one graph, or eagerly on side streams); the largest gap of any
+    box's Y and T between the batch and the box alone is printed beside them."""
     import torch
 
     dims = np.random.default_rng(20240611).integers(16, 97, size=(args.batch, 3))
-    cfg = case_config(support)
+    cfg = batch_config(support)
 
-    def solver(dlist, graphs):
-        c = case_config(support)
+    def solver(dlist, graphs, batched):
+        c = batch_config(support)
         c["backend-mi355x"]["graphs"] = graphs
+        c["backend-mi355x"]["batched"] = batched
         mb = pg.get_solver(c, pg.mesher.BatchMesher(dlist, lengths=L), pg.get_backend("mi355x", c))
         mb.set_prims([support.h2_prims(b.xcc.get(), b.ncell, lengths=L) for b in mb.blocks])
         return mb
 
     def timed(solvers):
-        for _ in range(2):
+        # the warm-up's second step tunes the tiles and captures each graph
+        for _ in range(max(args.warmup, 2)):
             for mb in solvers:
                 mb.step(DT)
         torch.cuda.synchronize()
@@ -142,24 +154,48 @@ def batch(args, pg, support):
         torch.cuda.synchronize()
         return time.perf_counter() - t0
 
-    together = solver(dims, True)
-    t_batch = timed([together])
-    cells = together.ncells
-    del together
-    overlapped = solver(dims, False)
-    t_streams = timed([overlapped])
-    del overlapped
-    one_by_one = [solver([d], False) for d in dims]
-    t_plain = timed(one_by_one)
+    def YT(mb):
+        out = []
+        for blk in mb.blocks:
+            Q = blk.regs[0].get()[blk.interior]
+            T = blk.state.get()[blk.interior][..., 1]
+            out.append((Q[..., 5:] / Q[..., :1], T))
+        return out
+
     n = len(dims)
+    times, results = {}, {}
+    for name, (graphs, batched) in (("table_graph", (True, True)), ("table_eager", (False, True)),
+                                    ("per_block_graph", (True, False)), ("per_block_side_streams", (False, False))):
+        mb = solver(dims, graphs, batched)
+        assert mb.batched == batched, name
+        times[name] = timed([mb])
+        print(f"{name}: {args.steps} steps in {times[name]:.3f} s", file=sys.stderr, flush=True)
+        if name == "table_graph":
+            cells = mb.ncells
+            results["batch"] = YT(mb)
+        del mb
+    for name, graphs in (("plain_eager", False), ("plain_graph", True)):
+        each = [solver([d], graphs, False) for d in dims]
+        times[name] = timed(each)
+        print(f"{name}: {args.steps} steps in {times[name]:.3f} s", file=sys.stderr, flush=True)
+        if name == "plain_eager":
+            results["alone"] = [YT(mb)[0] for mb in each]
+        del each
+    gap = 0.0
+    for (Yb, Tb), (Ya, Ta) in zip(results["batch"], results["alone"]):
+        gap = max(gap, float(np.abs(Yb - Ya).max()), float(np.abs(Tb - Ta).max() / np.abs(Ta).max()))
+    rate = {k: n * args.steps / t for k, t in times.items()}
     line = {
-        "metric": "batched solves/s (H2-air 3D uneven batch)", "value": n * args.steps / t_streams, "unit": "solves/s",
-        "n_gpus": 1, "steps": args.steps, "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+        "metric": "batched solves/s (H2-air 3D uneven batch)", "value": rate["table_graph"], "unit": "solves/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": times["table_graph"] / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"{n} periodic h2o2 boxes, edges 16-96 (seed 20240611), muscl-koren + hllc, "
-                               "coupled chemistry, tvd-rk3", "cells": int(cells)},
-        "plain_per_case": n * args.steps / t_plain, "speedup": t_plain / t_streams,
-        "graph_replay": n * args.steps / t_batch,
-        "cell_updates_per_s": cells * cfg_stages(cfg) * args.steps / t_streams,
+                               "implicit chemistry (fp32 mfma lu + fp64 refinement), tvd-rk3", "cells": int(cells)},
+        "plain_per_case": rate["plain_eager"], "speedup": times["plain_eager"] / times["table_graph"],
+        "speedup_vs_plain_graph": times["plain_graph"] / times["table_graph"],
+        "variants": {k: round(v, 2) for k, v in rate.items()},
+        "cell_updates_per_s": cells * cfg_stages(cfg) * args.steps / times["table_graph"],
+        "max_gap_Y_T_batch_vs_alone": gap,
     }
     print(json.dumps(line), flush=True)
 
@@ -208,6 +244,90 @@ def serve(args, pg, support):
     print(json.dumps(line), flush=True)
 
 
+def tgv_case(support, n):
+    """Taylor-Green at Mach 0.1 and Re 1600 on [0, 2 pi]^3: thermally perfect NASA-7 air, KEEP,
+    Navier-Stokes with constant transport (Pr 0.71), RK4, dt at CFL 0.2 on the sound speed plus V0."""
+    L3 = (2 * np.pi,) * 3
+    R = 8314.46261815324 / support.AIR_NASA7["Air"]["MW"]
+    cp = 1005.0
+    c0 = np.sqrt(cp / (cp - R) * R * 300.0)
+    V0 = 0.1 * c0
+    rho0 = 101325.0 / (R * 300.0)
+    mu = rho0 * V0 / 1600.0
+    cfg = support.config("air-tpg", integrator="rk4", trans="constant")
+    cfg["mixture"]["species"] = {"Air": {**support.AIR_NASA7["Air"], "mu0": mu, "kappa0": mu * cp / 0.71}}
+
+    def prims(x):
+        X, Y, Z = x[..., 0], x[..., 1], x[..., 2]
+        q = np.zeros(x.shape[:3] + (5,))
+        q[..., 0] = 101325.0 + rho0 * V0**2 / 16.0 * (np.cos(2 * X) + np.cos(2 * Y)) * (np.cos(2 * Z) + 2.0)
+        q[..., 1] = V0 * np.sin(X) * np.cos(Y) * np.cos(Z)
+        q[..., 2] = -V0 * np.cos(X) * np.sin(Y) * np.cos(Z)
+        q[..., 4] = q[..., 0] / (rho0 * R)
+        return q
+
+    return cfg, L3, prims, 0.2 * (2 * np.pi / n) / (c0 + V0)
+
+
+def tgv(args, pg, support):
+    """Taylor-Green 32^3 (the case the reference's CPU path runs), 100 rk4 steps: value = the steps'
+    cell-updates/s per stage on the device (fused rk4 stage, each step replayed from its HIP graph);
+    beside it the C oracle running the same 100 steps on the host's cores (the whole case, not a
+    sample) and the largest gap between the two solutions, row-normalised."""
+    import torch
+    from oracle import oracle as O
+    from peregrine_amd.simulator import get_simulator
+
+    n, nsteps = 32, 100
+    cfg, L3, prims, dt = tgv_case(support, n)
+    mb = pg.get_solver(cfg, pg.mesher.CubeMesher(blkdims=[n + 1] * 3, lengths=L3, periodic=[True] * 3),
+                       pg.get_backend("mi355x", cfg))
+    blk = mb.blocks[0]
+    mb.set_prims([prims(blk.xcc.get())])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(nsteps):
+        mb.step(dt)
+    torch.cuda.synchronize()
+    first = time.perf_counter() - t0
+    got = blk.regs[0].get()[blk.interior]
+    # the steady rate: as many more steps, every one replayed from the graph
+    t0 = time.perf_counter()
+    for _ in range(nsteps):
+        mb.step(dt)
+    torch.cuda.synchronize()
+    steady = time.perf_counter() - t0
+    sim = get_simulator(cfg)
+    oc = O.OracleCase(sim.mixture, "constant", None, "kepaec")
+    hb = support.HostBlock((n,) * 3, sim.ng, sim.ne, 4, lengths=L3)
+    O.lib.pgo_fromprims(oc.ref, O.arr(support.to_dev(prims(hb.xcc))), O.arr(hb.regs[0]), O.arr(hb.state),
+                        O.box((0, 0, 0), hb.cshape))
+    O.lib.pgo_fromQ(oc.ref, O.arr(hb.regs[0]), O.arr(hb.state), O.box((hb.ng,) * 3, hb.ncell))
+    b = support.oracle_block(hb, "rk4", 0, viscous=True, alpha=getattr(sim, "alpha", 1.0))
+    t0 = time.perf_counter()
+    for _ in range(nsteps):
+        O.lib.pgo_step(oc.ref, ctypes.byref(b), dt)
+    cpu = time.perf_counter() - t0
+    want = hb.regs[0][hb.interior()].transpose(3, 2, 1, 0)
+    scale = np.abs(want).reshape(-1, want.shape[-1]).max(axis=0)
+    gap = float((np.abs(got - want).reshape(-1, want.shape[-1]).max(axis=0) / scale).max())
+    cells = n**3
+    line = {
+        "metric": "cell-updates/s per RK stage (FP64, Taylor-Green 32^3)", "value": cells * 4 * nsteps / steady,
+        "unit": "cell-updates/s", "n_gpus": 1, "steps": nsteps, "warmup": nsteps, "ms_per_step": steady / nsteps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "taylor-green 32^3, air nasa-7 tpg, navier-stokes constant transport, keep, rk4, "
+                               "100 steps", "path": "fused" if mb.fused else "per-kernel",
+                   "hip_graph_replays": mb.replays},
+        "first_100_steps_ms_per_step": first / nsteps * 1e3,
+        "cpu_baseline": dict(value=cells * 4 * nsteps / cpu, unit="cell-updates/s", cores=O.threads(), kind="port",
+                             sample="the whole case: 100 rk4 steps of 32^3, C oracle with OpenMP",
+                             ms_per_step=cpu / nsteps * 1e3),
+        "max_row_gap_vs_oracle_100_steps": gap,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def cfg_stages(cfg):
     return {"rk1": 1, "rk2": 2, "tvd-rk3": 3, "rk4": 4, "rk34": 4}[cfg["time-integration"]["integrator"]]
 
@@ -225,11 +345,15 @@ def main():
     ap.add_argument("--k-chunk", type=int, default=None)
     ap.add_argument("--stage-slabs", type=int, default=None, help="k-slabs of the fused stage")
     ap.add_argument("--no-graphs", action="store_true", help="launch every step eagerly (no HIP graph replay)")
+    ap.add_argument("--implicit", action="store_true",
+                    help="implicit chemistry (FP32 matrix-core LU, FP64 refinement) in place of the coupled source")
     ap.add_argument("--batch", type=int, default=0,
                     help="the uneven batch instead: this many periodic H2/O2 cases of 16-96 cells per axis "
                          "(seeded) stepped together, against the same cases stepped one by one")
     ap.add_argument("--serve", type=int, default=0,
                     help="the serving stream instead: this many requests arriving as a Poisson stream")
+    ap.add_argument("--tgv", action="store_true",
+                    help="the Taylor-Green 32^3 rk4 case instead, 100 steps, against the oracle's same steps on the host")
     ap.add_argument("--rate", type=float, default=400.0, help="the serving stream's arrivals per second")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only a plain device copy of 512 MiB, five times: the reference for HBM byte counters")
@@ -258,7 +382,9 @@ def main():
         return batch(args, pg, support)
     if args.serve:
         return serve(args, pg, support)
-    cfg = case_config(support)
+    if args.tgv:
+        return tgv(args, pg, support)
+    cfg = batch_config(support) if args.implicit else case_config(support)
     cfg["backend-mi355x"]["fused-stage"] = not args.per_kernel
     if args.k_chunk:
         cfg["backend-mi355x"]["k-chunk"] = args.k_chunk
@@ -324,7 +450,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"h2o2-9sp-28rx {n}^3 periodic, muscl-koren + hllc, coupled chemistry, tvd-rk3",
+        "config": {"workload": f"h2o2-9sp-28rx {n}^3 periodic, muscl-koren + hllc, "
+                               f"{'implicit' if args.implicit else 'coupled'} chemistry, tvd-rk3",
                    "cells": ncells, "stages_per_step": nst, "path": "fused" if mb.fused else "per-kernel",
                    "hip_graph_replays": mb.replays,
                    "k_chunk": cfg["backend-mi355x"]["k-chunk"],

@@ -336,17 +336,17 @@ typedef struct {
   double tolerance;   /* a solve is done when every row's residual is within this share of the row's scale */
 } pga_implicit_opts;
 
-/* Doubles of device work pga_implicit needs for `box` (the FP32 factors and FP64 matrices of 16-cell
- * groups, the FP64 unknowns and rates); -1 for a bad box or more than 11 species. */
+/* Doubles of device work pga_implicit needs for `box` (the FP64 matrices of 16-cell groups, the FP64
+ * unknowns and two right-hand sides); -1 for a bad box or more than 11 species. */
 int64_t pga_implicit_worksize(const pga_case *c, pga_box box);
 
 /* Implicit chemistry over the cells of `box` for a step of *dt (device): each cell integrates the
  * same 0D system the sub-stepped form does (its ns mass fractions and T at constant density and
  * internal energy, chemint/kernels/substep.mako:14-50) by the linearly implicit Euler method in
  * o->parts parts, y1 = y0 + h x with (I - h J) x = f(y0), J by forward differences in FP64. Each
- * matrix is factored in FP32 on the matrix cores, 16 cells per wave, and each solve refined against
- * the FP64 matrix with FP64 residuals until every row's residual is within o->tolerance of its
- * scale |f_i| + sum_j |A_ij x_j|. It writes the mean rate over the step, rho (Y_end - Y_start) / dt,
+ * matrix is factored in FP32 on the matrix cores, 16 cells per wave, and in the same registers
+ * each solve is refined against the FP64 matrix with FP64 residuals until every row's residual is
+ * within o->tolerance of its scale |f_i| + sum_j |A_ij x_j|. It writes the mean rate over the step, rho (Y_end - Y_start) / dt,
  * into dQ (rows 0..4 zero) when begin_dQ, else into the ns - 1 rows of `out` as pga_rates does, and
  * each cell's refinement corrections over its parts into `sweeps` (int32, box-shaped, i fastest;
  * 1000 more for each solve left outside the tolerance). `work`: pga_implicit_worksize doubles of

@@ -179,23 +179,46 @@ __global__ __launch_bounds__(FLAT) void kb_stage(BArgs a, BStage s) {
 // implicit chemistry over every block's cells into their rates: a cell's place in the work arrays is
 // its workgroup's flat number times 256 plus its lane, the same in both launches
 template <class M>
-__global__ __launch_bounds__(FLAT) void kb_ijac(BArgs a, ImArgs im) {
+__global__ __launch_bounds__(FLAT) void kb_ibase(BArgs a, ImArgs im) {
   const int64_t g = xcd_block();
   const int n = bfind(a.prefix, a.nblocks, g);
   const BBlock &B = a.blocks[n];
   int i, j, k;
   if (!flat_at(B.b, g - a.prefix[n], i, j, k)) return;
-  ijac_cell<M>(im, B.Q[a.q], B.st, i, j, k, g * FLAT + threadIdx.x);
+  ijac_base<M>(im, B.Q[a.q], B.st, i, j, k, g * FLAT + threadIdx.x);
+}
+
+// blockIdx.y is the slice (a column, or ns + 1: f(y))
+template <class M>
+__global__ __launch_bounds__(FLAT) void kb_icol(BArgs a, ImArgs im) {
+  const int64_t g = xcd_block();
+  const int n = bfind(a.prefix, a.nblocks, g);
+  const BBlock &B = a.blocks[n];
+  int i, j, k;
+  if (!flat_at(B.b, g - a.prefix[n], i, j, k)) return;
+  ijac_column<M>(im, B.Q[a.q], i, j, k, g * FLAT + threadIdx.x, (int)blockIdx.y);
+}
+
+// the places of a tile's overhang (no cell): the identity and a zero right-hand side, once a call,
+// so the quad-per-cell solve finds every place of its groups defined and settles them in one sweep
+template <int NV>
+__global__ __launch_bounds__(FLAT) void kb_ivoid(BArgs a, ImArgs im) {
+  const int64_t g = xcd_block();
+  const int n = bfind(a.prefix, a.nblocks, g);
+  const BBlock &B = a.blocks[n];
+  int i, j, k;
+  if (flat_at(B.b, g - a.prefix[n], i, j, k)) return;
+  for (int m = 0; m <= NV; m++) ijac_void<NV>(im, g * FLAT + threadIdx.x, m);
 }
 
 template <class M>
-__global__ __launch_bounds__(FLAT) void kb_ros(BArgs a, ImArgs im) {
+__global__ __launch_bounds__(FLAT) void kb_iend(BArgs a, ImArgs im) {
   const int64_t g = xcd_block();
   const int n = bfind(a.prefix, a.nblocks, g);
   const BBlock &B = a.blocks[n];
   int i, j, k;
   if (!flat_at(B.b, g - a.prefix[n], i, j, k)) return;
-  ros_cell<M>(im, B.Q[a.q], B.rates, i, j, k, g * FLAT + threadIdx.x);
+  iend_cell<M>(im, B.Q[a.q], B.rates, i, j, k, g * FLAT + threadIdx.x);
 }
 
 // the tiles of each kind a block takes
@@ -510,7 +533,7 @@ static int64_t batch_cells(const pga_batch *h) { return h->total[BK_CELLS] * FLA
 extern "C" int64_t pga_batch_implicit_worksize(const pga_batch *h) {
   if (!h || h->c->desc.ns + 1 > IM_N) return -1;
   const int64_t cells = batch_cells(h), groups = (cells + IM_G - 1) / IM_G, nv = h->c->desc.ns + 1;
-  return groups * (IM_GROUP / 2) + groups * nv * nv * IM_G + 2 * nv * cells;
+  return groups * nv * nv * IM_G + 3 * nv * cells;
 }
 
 extern "C" int32_t pga_batch_implicit(const pga_batch *h, const pga_implicit_opts *o, int32_t q, int32_t *sweeps,
@@ -536,8 +559,7 @@ extern "C" int32_t pga_batch_implicit(const pga_batch *h, const pga_implicit_opt
   ImArgs im{};
   im.K = h->c->K, im.C = h->c->chem;
   im.begin_dQ = 0;
-  im.W = reinterpret_cast<float *>(work);
-  im.A = work + groups * (IM_GROUP / 2);
+  im.A = work;
   im.y = im.A + groups * nv * nv * IM_G;
   im.ncell = cells, im.sweeps = sweeps, im.dt = dt;
   im.parts = o->parts, im.maxsweeps = o->max_sweeps, im.tol = o->tolerance;
@@ -547,13 +569,15 @@ extern "C" int32_t pga_batch_implicit(const pga_batch *h, const pga_implicit_opt
   bool done = false;
 #define X(NAME, IDX)                                                                             \
   if (d.mechanism == IDX) {                                                                      \
-    constexpr int NB = (Mech_##NAME::NS + 1 + 3) / 4;                                            \
+    constexpr int NV = Mech_##NAME::NS + 1, NB = (NV + 3) / 4;                                   \
+    hipLaunchKernelGGL((kb_ivoid<NV>), grid, dim3(FLAT), 0, s, a, im);                           \
     for (int part = 0; part < o->parts; part++) {                                                \
       im.first = part == 0, im.last = part == o->parts - 1;                                      \
-      hipLaunchKernelGGL((kb_ijac<Mech_##NAME>), grid, dim3(FLAT), 0, s, a, im);                \
-      hipLaunchKernelGGL((k_lu<NB>), dim3((unsigned)((groups + 3) / 4)), dim3(256), 0, s, im.W, cells); \
-      hipLaunchKernelGGL((kb_ros<Mech_##NAME>), grid, dim3(FLAT), 0, s, a, im);                 \
+      hipLaunchKernelGGL((kb_ibase<Mech_##NAME>), grid, dim3(FLAT), 0, s, a, im);               \
+      hipLaunchKernelGGL((kb_icol<Mech_##NAME>), dim3(grid.x, NV + 1), dim3(FLAT), 0, s, a, im); \
+      hipLaunchKernelGGL((k_lusolve<NB, NV>), dim3((unsigned)((groups + 3) / 4)), dim3(256), 0, s, im); \
     }                                                                                            \
+    hipLaunchKernelGGL((kb_iend<Mech_##NAME>), grid, dim3(FLAT), 0, s, a, im);                  \
     done = true;                                                                                 \
   }
   PGA_MECHANISMS(X)

@@ -48,7 +48,7 @@ using namespace pga;
 extern "C" int64_t pga_implicit_worksize(const pga_case *c, pga_box box) {
   if (!c || !box_ok(box) || c->desc.ns + 1 > IM_N) return -1;
   const int64_t cells = im_cells(box), groups = im_groups(cells), nv = c->desc.ns + 1;
-  return groups * (IM_GROUP / 2) + groups * nv * nv * IM_G + 2 * nv * cells;
+  return groups * nv * nv * IM_G + 3 * nv * cells;
 }
 
 extern "C" int32_t pga_implicit(const pga_case *c, const pga_implicit_opts *o, pga_view Q, pga_view state,
@@ -86,8 +86,7 @@ extern "C" int32_t pga_implicit(const pga_case *c, const pga_implicit_opts *o, p
   a.K = c->K, a.C = c->chem;
   a.Q = view(Q), a.st = view(state), a.out = view(out);
   a.begin_dQ = begin_dQ != 0;
-  a.W = reinterpret_cast<float *>(work);
-  a.A = work + groups * (IM_GROUP / 2);
+  a.A = work;
   a.y = a.A + groups * nv * nv * IM_G;
   a.ncell = cells, a.sweeps = sweeps, a.dt = dt, a.b = box3(box);
   a.parts = o->parts, a.maxsweeps = o->max_sweeps, a.tol = o->tolerance;
@@ -96,13 +95,14 @@ extern "C" int32_t pga_implicit(const pga_case *c, const pga_implicit_opts *o, p
   bool done = false;
 #define X(NAME, IDX)                                                                              \
   if (d.mechanism == IDX) {                                                                       \
-    constexpr int NB = (Mech_##NAME::NS + 1 + 3) / 4;                                             \
+    constexpr int NV = Mech_##NAME::NS + 1, NB = (NV + 3) / 4;                                    \
     for (int part = 0; part < o->parts; part++) {                                                 \
       a.first = part == 0, a.last = part == o->parts - 1;                                         \
-      hipLaunchKernelGGL((k_ijac<Mech_##NAME>), grid, dim3(FLAT), 0, s, a);                      \
-      launch_lu<NB>(a.W, cells, s);                                                               \
-      hipLaunchKernelGGL((k_ros<Mech_##NAME>), grid, dim3(FLAT), 0, s, a);                       \
+      hipLaunchKernelGGL((k_ibase<Mech_##NAME>), grid, dim3(FLAT), 0, s, a);                     \
+      hipLaunchKernelGGL((k_icol<Mech_##NAME>), dim3(grid.x, NV + 1), dim3(FLAT), 0, s, a);      \
+      hipLaunchKernelGGL((k_lusolve<NB, NV>), dim3((unsigned)((groups + 3) / 4)), dim3(256), 0, s, a); \
     }                                                                                             \
+    hipLaunchKernelGGL((k_iend<Mech_##NAME>), grid, dim3(FLAT), 0, s, a);                        \
     done = true;                                                                                  \
   }
   PGA_MECHANISMS(X)

@@ -24,10 +24,12 @@
 // broadcasts. No pivoting: the matrices are I - h J of a chemical source, whose diagonal grows with
 // the destruction rates; the FP64 refinement corrects what the FP32 factors leave.
 //
-// The work array holds the FP32 factors, [group of 16 cells][144 entries][16 cells] (a row of the
-// 16 cells' same entry is 64 contiguous bytes, so both the lane-per-cell kernels and the
-// quad-per-cell LU read and write whole segments), the FP64 matrices the same way with
-// (ns + 1)^2 entries, then y and f at y's FP32 rounding, [ns + 1][cells] doubles each.
+// The implicit form's work array holds the FP64 matrices, [group of 16 cells][(ns + 1)^2 entries]
+// [16 cells] (a row of the 16 cells' same entry is 128 contiguous bytes, so both the lane-per-cell
+// kernels and the quad-per-cell solve move whole segments), then y, f at y's FP32 rounding and f at
+// y, [ns + 1][cells] doubles each. The FP32 factors never leave the registers of the solve. The
+// batched solve of caller-given systems keeps its FP32 factors in a work array the same way,
+// [group][144 entries][16 cells].
 #pragma once
 
 #include <type_traits>
@@ -64,24 +66,27 @@ __device__ __forceinline__ float qb(float x) {
   return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), P * 0x55, 0xf, 0xf, false));
 }
 
-// The blocked LU of 16 matrices per wave, in place in the group-major array W. A wave exits
-// whole (the MFMAs need every lane); the lanes of a group's missing matrices factor the identity.
+// a double moved within each quad: lane q takes lane PERM's pick of the quad (DPP quad_perm on both
+// halves; 0xB1 swaps neighbours, 0x4E swaps pairs)
+template <int PERM>
+__device__ __forceinline__ double qperm(double x) {
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(x), PERM, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(x), PERM, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+
+// the sum of x over the quad, the same bits on its four lanes ((x0 + x1) + (x2 + x3), each add
+// commuted alike)
+__device__ __forceinline__ double qsum(double x) {
+  const double s = x + qperm<0xB1>(x);
+  return s + qperm<0x4E>(s);
+}
+
+// The blocked LU of one lane quad's matrix per quad, 16 per wave, in registers: lane q holds column q
+// of each 4 x 4 block, block (I, J)'s row i in blk[I][J][i]. Every lane of the wave takes part (the
+// MFMAs need every lane).
 template <int NB>
-__global__ __launch_bounds__(256) void k_lu(float *__restrict__ W, int64_t ncell) {
-  const int64_t grp = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (grp * IM_G >= ncell) return;
-  const int lane = threadIdx.x & 63, b = lane >> 2, q = lane & 3;
-  const bool real = grp * IM_G + b < ncell;
-  float *base = W + grp * IM_GROUP + b;
-  v4f blk[NB][NB];
-  sfor<0, NB>([&]<int I>() {
-    sfor<0, NB>([&]<int J>() {
-      sfor<0, 4>([&]<int i>() {
-        const int r = 4 * I + i, c = 4 * J + q;
-        blk[I][J][i] = real ? base[(r * IM_N + c) * IM_G] : (r == c ? 1.0f : 0.0f);
-      });
-    });
-  });
+__device__ __forceinline__ void lu_factor(v4f (&blk)[NB][NB], int q) {
   sfor<0, NB>([&]<int K>() {
     v4f &D = blk[K][K];
     // the diagonal block: lane q holds its column q; L (unit, strictly below) and U overwrite it
@@ -125,6 +130,27 @@ __global__ __launch_bounds__(256) void k_lu(float *__restrict__ W, int64_t ncell
       });
     });
   });
+}
+
+// The batched LU of 16 matrices per wave, in place in the group-major array W. A wave exits
+// whole; the lanes of a group's missing matrices factor the identity.
+template <int NB>
+__global__ __launch_bounds__(256) void k_lu(float *__restrict__ W, int64_t ncell) {
+  const int64_t grp = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (grp * IM_G >= ncell) return;
+  const int lane = threadIdx.x & 63, b = lane >> 2, q = lane & 3;
+  const bool real = grp * IM_G + b < ncell;
+  float *base = W + grp * IM_GROUP + b;
+  v4f blk[NB][NB];
+  sfor<0, NB>([&]<int I>() {
+    sfor<0, NB>([&]<int J>() {
+      sfor<0, 4>([&]<int i>() {
+        const int r = 4 * I + i, c = 4 * J + q;
+        blk[I][J][i] = real ? base[(r * IM_N + c) * IM_G] : (r == c ? 1.0f : 0.0f);
+      });
+    });
+  });
+  lu_factor<NB>(blk, q);
   if (!real) return;
   sfor<0, NB>([&]<int I>() {
     sfor<0, NB>([&]<int J>() {
@@ -153,9 +179,8 @@ struct ImArgs {
   const double *K, *C;
   View Q, st, out;
   int begin_dQ, first, last;
-  float *W;      // the FP32 factors
   double *A;     // the FP64 matrices, group-major with nv^2 entries
-  double *y;     // [ns + 1][ncell]: the part's start, then its end; then f at its FP32 rounding
+  double *y;     // [ns + 1][ncell] three times: the part's start (then its end), f at its FP32 rounding, f at it
   int64_t ncell;
   int32_t *sweeps;
   const double *dt;
@@ -225,75 +250,9 @@ __device__ __forceinline__ double fd_step(double ym, bool temperature) {
   return (ym + 6.103515625e-05 * scale) - ym;
 }
 
-// A = I - h J at the part's start, J by forward differences, into the FP64 and FP32 arrays; the
-// first part takes y from Q and the state. J is taken at y rounded to FP32, y^: a last-bit change of y
-// (the oracle's FP64 solve against the device's refined one, a part's start after another part)
-// then leaves the matrix as it was, where the differences' own rounding would move it by 1e-12 of
-// itself and a stiff part's result by more; the linearly implicit Euler step is first order with the
-// matrix of any nearby state. y^ and f(y^) wait in the work array while each column's rates are
-// evaluated, so a column holds no more registers than one evaluation of the rates
-template <class M>
-__device__ __forceinline__ void ijac_cell(const ImArgs &a, const View &Q, const View &st, int i, int j, int k,
-                                          int64_t code) {
-#pragma clang fp contract(off)
-  constexpr int NS = M::NS, NV = NS + 1;
-  const int64_t nc = a.ncell;
-  const Gas<NS> g{a.K};
-  const int64_t o = Q.off(i, j, k);
-  const double rho = Q.at(o, 0), rhoinv = 1.0 / rho;
-  double *y = a.y + code, *fh = a.y + NV * nc + code;
-  {
-    double yc[NV], fc[NV];
-    if (a.first) {
-      double Y[NS];
-      massfrac<NS>(Q, o, rhoinv, Y);
-#pragma unroll
-      for (int n = 0; n < NS; n++) yc[n] = Y[n];
-      yc[NS] = st.at(st.off(i, j, k), 1);
-#pragma unroll
-      for (int r = 0; r < NV; r++) y[r * nc] = yc[r];
-    } else {
-#pragma unroll
-      for (int r = 0; r < NV; r++) yc[r] = y[r * nc];
-    }
-#pragma unroll
-    for (int r = 0; r < NV; r++) yc[r] = (double)(float)yc[r];
-    rhs0d<M>(g, a.C, rho, rhoinv, yc, fc);
-#pragma unroll
-    for (int r = 0; r < NV; r++) fh[r * nc] = fc[r];
-  }
-  const double h = *a.dt / a.parts;
-#pragma unroll 1
-  for (int m = 0; m < NV; m++) {
-    double yp[NV], f1[NV], ym = 0.0;
-#pragma unroll
-    for (int r = 0; r < NV; r++) {
-      yp[r] = (double)(float)y[r * nc];
-      ym = r == m ? yp[r] : ym;
-    }
-    const double dlt = fd_step(ym, m == NS);
-#pragma unroll
-    for (int r = 0; r < NV; r++) yp[r] = r == m ? ym + dlt : yp[r];
-    rhs0d<M>(Gas<NS>{opaque(a.K)}, opaque(a.C), rho, rhoinv, yp, f1);
-    const double hd = h / dlt;
-#pragma unroll
-    for (int r = 0; r < NV; r++) {
-      const double Arm = (r == m ? 1.0 : 0.0) - (f1[r] - fh[r * nc]) * hd;
-      a.A[a64_at<NV>(code, r, m)] = Arm;
-      a.W[lu_at(code, r, m)] = (float)Arm;
-    }
-  }
-  // the padding of the FP32 matrix: the identity
-#pragma unroll
-  for (int r = 0; r < IM_N; r++) {
-#pragma unroll
-    for (int c = 0; c < IM_N; c++)
-      if (r >= NV || c >= NV) a.W[lu_at(code, r, c)] = r == c ? 1.0f : 0.0f;
-  }
-}
-
-// x = A^-1 b by iterative refinement on the FP32 factors W, the residual b - A x in FP64 against A
-// (entry (r, c) at A[at(r, c)]); stops when every row's residual is within tol of its scale
+// x = A^-1 b by iterative refinement on the FP32 factors W (the batched solve of caller-given
+// systems), the residual b - A x in FP64 against A (entry (r, c) at A[at(r, c)]); a lane per system;
+// stops when every row's residual is within tol of its scale
 // |b_r| + sum_c |A_rc x_c|, or after maxsweeps corrections. Returns the corrections taken, or
 // maxsweeps + 1 when the last one left a row outside the tolerance
 template <int NV, class At>
@@ -336,36 +295,207 @@ __device__ __forceinline__ int refine(const float *__restrict__ W, const double 
   return done ? s : maxsweeps + 1;
 }
 
-// the part's linearly implicit step y = y0 + h x, (I - h J) x = f(y0); the last part writes the
-// mean rate over the step
+// A = I - h J at the part's start, J by forward differences, in two launches: the base (a lane per
+// cell) stores y, and f at y rounded to FP32 (y^); the columns (a lane per cell and unknown) each one
+// column of A, and one more slice f(y) itself, the solve's right-hand side. J is taken at y^: a
+// last-bit change of y (the oracle's FP64 solve against the device's refined one, a part's start
+// after another part) then leaves the matrix as it was, where the differences' own rounding would
+// move it by 1e-12 of itself and a stiff part's result by more; the linearly implicit Euler step is
+// first order with the matrix of any nearby state. A lane makes one evaluation of the rates, so the
+// lanes hold the registers of one evaluation and the chip's waves are as many as for the rates alone.
+
+// the base: the first part takes y from Q and the state; f(y^) into the work array
 template <class M>
-__device__ __forceinline__ void ros_cell(const ImArgs &a, const View &Q, const View &out, int i, int j, int k,
-                                         int64_t code) {
+__device__ __forceinline__ void ijac_base(const ImArgs &a, const View &Q, const View &st, int i, int j, int k,
+                                          int64_t code) {
+#pragma clang fp contract(off)
+  constexpr int NS = M::NS, NV = NS + 1;
+  const int64_t nc = a.ncell;
+  const Gas<NS> g{a.K};
+  const int64_t o = Q.off(i, j, k);
+  const double rho = Q.at(o, 0), rhoinv = 1.0 / rho;
+  double *y = a.y + code, *fh = a.y + NV * nc + code;
+  // y from Q and the state (the first part) or as the last part left it, selected rather than
+  // branched (one copy of the rates in the code)
+  double y0[NV], yc[NV], fc[NV], Y[NS];
+  massfrac<NS>(Q, o, rhoinv, Y);
+#pragma unroll
+  for (int n = 0; n < NS; n++) y0[n] = a.first ? Y[n] : y[n * nc];
+  y0[NS] = a.first ? st.at(st.off(i, j, k), 1) : y[NS * nc];
+#pragma unroll
+  for (int r = 0; r < NV; r++) yc[r] = (double)(float)y0[r];
+  rhs0d<M>(g, a.C, rho, rhoinv, yc, fc);
+  // every store after the rates, whose constants are then read through the scalar cache (a later
+  // part stores the y it read)
+#pragma unroll
+  for (int r = 0; r < NV; r++) y[r * nc] = y0[r];
+#pragma unroll
+  for (int r = 0; r < NV; r++) fh[r * nc] = fc[r];
+}
+
+// slice m (wave-uniform) of the columns' launch: for m < ns + 1, column m of A, the rates at
+// y^ + dlt e_m less f(y^), over dlt, times -h, plus the identity's column; for m = ns + 1, f(y)
+template <class M>
+__device__ __forceinline__ void ijac_column(const ImArgs &a, const View &Q, int i, int j, int k, int64_t code,
+                                            int m) {
 #pragma clang fp contract(off)
   constexpr int NS = M::NS, NV = NS + 1;
   const int64_t nc = a.ncell;
   const int64_t o = Q.off(i, j, k);
   const double rho = Q.at(o, 0), rhoinv = 1.0 / rho;
-  double bv[NV], x[NV];
-  {
-    double y0[NV];
-#pragma unroll
-    for (int r = 0; r < NV; r++) y0[r] = a.y[r * nc + code];
-    rhs0d<M>(Gas<NS>{a.K}, a.C, rho, rhoinv, y0, bv);
-  }
-  const int taken = refine<NV>(a.W, a.A, [](int64_t c, int r, int m) { return a64_at<NV>(c, r, m); }, code, bv, x,
-                               a.maxsweeps, a.tol);
-  const int32_t count = taken > a.maxsweeps ? a.maxsweeps + IM_FAIL : taken;
-  a.sweeps[code] = a.first ? count : a.sweeps[code] + count;
-  const double h = *a.dt / a.parts;
-  double y[NV];
+  const double *y = a.y + code, *fh = a.y + NV * nc + code;
+  const bool column = m < NV;
+  double yp[NV], f1[NV], ym = 0.0;
 #pragma unroll
   for (int r = 0; r < NV; r++) {
-    y[r] = a.y[r * nc + code] + h * x[r];
-    a.y[r * nc + code] = y[r];
+    const double yr = y[r * nc];
+    yp[r] = column ? (double)(float)yr : yr;
+    ym = r == m ? yp[r] : ym;
   }
-  if (!a.last) return;
-  // the mean rate over the step, rho (Y_end - Y_start) / dt, as the sub-stepped form begins dQ
+  const double dlt = fd_step(ym, m == NS);
+#pragma unroll
+  for (int r = 0; r < NV; r++) yp[r] = r == m ? ym + dlt : yp[r];
+  rhs0d<M>(Gas<NS>{a.K}, a.C, rho, rhoinv, yp, f1);
+  if (!column) {
+    double *bv = a.y + 2 * NV * nc + code;
+#pragma unroll
+    for (int r = 0; r < NV; r++) bv[r * nc] = f1[r];
+    return;
+  }
+  const double h = *a.dt / a.parts;
+  const double hd = h / dlt;
+#pragma unroll
+  for (int r = 0; r < NV; r++) a.A[a64_at<NV>(code, r, m)] = (r == m ? 1.0 : 0.0) - (f1[r] - fh[r * nc]) * hd;
+}
+
+// what a slice of the columns' launch leaves for a work place that holds no cell (a tile's overhang in
+// a table of blocks): the identity's column, or a zero right-hand side, which the solve takes in one
+// sweep
+template <int NV>
+__device__ __forceinline__ void ijac_void(const ImArgs &a, int64_t code, int m) {
+  if (m < NV) {
+#pragma unroll
+    for (int r = 0; r < NV; r++) a.A[a64_at<NV>(code, r, m)] = r == m ? 1.0 : 0.0;
+  } else {
+#pragma unroll
+    for (int r = 0; r < NV; r++) a.y[2 * NV * a.ncell + r * a.ncell + code] = 0.0;
+  }
+}
+
+// The part's solve and step, a lane quad per cell, 16 cells per wave: (I - h J) x = f(y) by iterative
+// refinement, then y += h x. The quad loads its FP64 matrix once (lane q: columns q, q + 4, q + 8)
+// and keeps it, and the FP32 factors of its rounding (lu_factor), in registers; each sweep solves
+// L U z = r in FP32 with the residual broadcast to the quad, adds z to x and takes r = f - A x in
+// FP64 (each lane's columns, summed over the quad) with the row scales |f_m| + sum_c |A_mc x_c|. A
+// cell stops when every row's residual is within tol of its scale (its x then stays as it is), or
+// after maxsweeps; the wave runs until its last cell stops. Lane q keeps rows q, q + 4, q + 8 of x, f
+// and r. A cell's x depends only on its own matrix and f, whichever cells share its wave.
+template <int NB, int NV>
+__device__ __forceinline__ void lu_refine_group(const ImArgs &a, int64_t grp, int lane) {
+#pragma clang fp contract(off)
+  const int b = lane >> 2, q = lane & 3;
+  const int64_t nc = a.ncell, cell = grp * IM_G + b;
+  const bool real = cell < nc;
+  const int64_t at = real ? cell : grp * IM_G;
+  double Ad[NB][NV];
+  sfor<0, NB>([&]<int J>() {
+    const int c = 4 * J + q;
+    const int cc = c < NV ? c : 0;
+    sfor<0, NV>([&]<int m>() {
+      const double v = a.A[a64_at<NV>(at, m, cc)];
+      Ad[J][m] = !real ? (m == c ? 1.0 : 0.0) : (c < NV ? v : 0.0);
+    });
+  });
+  v4f blk[NB][NB];
+  sfor<0, NB>([&]<int I>() {
+    sfor<0, NB>([&]<int J>() {
+      sfor<0, 4>([&]<int i>() {
+        constexpr int r = 4 * I + i;
+        if constexpr (r < NV)
+          blk[I][J][i] = (float)Ad[J][r];
+        else
+          blk[I][J][i] = r == 4 * J + q ? 1.0f : 0.0f;
+      });
+    });
+  });
+  lu_factor<NB>(blk, q);
+  // the diagonal's reciprocals, to the quad
+  float rdiag[NV];
+  sfor<0, NV>([&]<int c>() { rdiag[c] = 1.0f / qb<c % 4>(blk[c / 4][c / 4][c % 4]); });
+  const double *bvp = a.y + 2 * NV * nc + at;
+  double bv[NB], x[NB], r[NB];
+  sfor<0, NB>([&]<int t>() {
+    const int m = 4 * t + q;
+    const double v = bvp[(m < NV ? m : 0) * nc];
+    bv[t] = real && m < NV ? v : 0.0;
+    x[t] = 0.0;
+    r[t] = bv[t];
+  });
+  // the flags are ints and a cell's count is set inside the loop: no lane mask lives out of it
+  int done = 0, count = a.maxsweeps + IM_FAIL;
+  int s = 0;
+#pragma unroll 1
+  while (s < a.maxsweeps && !__all(done)) {
+    // z = (L U)^-1 r in FP32, replicated over the quad, in lu_solve's order
+    float z[NV];
+    sfor<0, NV>([&]<int m>() { z[m] = qb<m % 4>((float)r[m / 4]); });
+    sfor<1, NV>([&]<int m>() { sfor<0, m>([&]<int c>() { z[m] -= qb<c % 4>(blk[m / 4][c / 4][m % 4]) * z[c]; }); });
+    sfor<0, NV>([&]<int n>() {
+      constexpr int m = NV - 1 - n;
+      sfor<m + 1, NV>([&]<int c>() { z[m] -= qb<c % 4>(blk[m / 4][c / 4][m % 4]) * z[c]; });
+      z[m] = z[m] * rdiag[m];
+    });
+    // x += z on the lanes' own rows, for the cells still refining
+    sfor<0, NB>([&]<int t>() {
+      float zq = 0.0f;
+      sfor<0, 4>([&]<int p>() {
+        if constexpr (4 * t + p < NV) zq = q == p ? z[4 * t + p] : zq;
+      });
+      x[t] = done ? x[t] : x[t] + (double)zq;
+    });
+    // r = f - A x and the rows' scales, the products over the lane's columns summed over the quad
+    int ok = 1;
+    sfor<0, NV>([&]<int m>() {
+      double p = 0.0, sc = 0.0;
+      sfor<0, NB>([&]<int J>() {
+        const double ax = Ad[J][m] * x[J];
+        p = p + ax;
+        sc = sc + fabs(ax);
+      });
+      p = qsum(p);
+      sc = qsum(sc);
+      if (q == m % 4) {
+        const double acc = bv[m / 4] - p, scale = fabs(bv[m / 4]) + sc;
+        r[m / 4] = done ? r[m / 4] : acc;
+        ok = ok & (fabs(acc) <= a.tol * scale ? 1 : 0);
+      }
+    });
+    // the cell's rows are spread over its quad: done when all four lanes' rows are
+    ok = ok & __builtin_amdgcn_mov_dpp(ok, 0xB1, 0xf, 0xf, false);
+    ok = ok & __builtin_amdgcn_mov_dpp(ok, 0x4E, 0xf, 0xf, false);
+    s++;
+    count = !done && ok ? s : count;
+    done = done | ok;
+  }
+  if (!real) return;
+  if (q == 0) a.sweeps[cell] = a.first ? count : a.sweeps[cell] + count;
+  const double h = *a.dt / a.parts;
+  sfor<0, NB>([&]<int t>() {
+    const int m = 4 * t + q;
+    if (m < NV) a.y[m * nc + cell] = a.y[m * nc + cell] + h * x[t];
+  });
+}
+
+// the last part's end: the mean rate over the step, rho (Y_end - Y_start) / dt, into out as the
+// sub-stepped form begins dQ (or into the rates)
+template <class M>
+__device__ __forceinline__ void iend_cell(const ImArgs &a, const View &Q, const View &out, int i, int j, int k,
+                                          int64_t code) {
+#pragma clang fp contract(off)
+  constexpr int NS = M::NS;
+  const int64_t nc = a.ncell;
+  const int64_t o = Q.off(i, j, k);
+  const double rho = Q.at(o, 0), rhoinv = 1.0 / rho;
   double Ys[NS];
   massfrac<NS>(Q, o, rhoinv, Ys);
   const double rdt = rho / *a.dt;
@@ -374,25 +504,41 @@ __device__ __forceinline__ void ros_cell(const ImArgs &a, const View &Q, const V
 #pragma unroll
     for (int l = 0; l < 5; l++) out.at(oo, l) = 0.0;
 #pragma unroll
-    for (int n = 0; n < NS - 1; n++) out.at(oo, 5 + n) = (y[n] - Ys[n]) * rdt;
+    for (int n = 0; n < NS - 1; n++) out.at(oo, 5 + n) = (a.y[n * nc + code] - Ys[n]) * rdt;
   } else {
 #pragma unroll
-    for (int n = 0; n < NS - 1; n++) out.at(oo, n) = (y[n] - Ys[n]) * rdt;
+    for (int n = 0; n < NS - 1; n++) out.at(oo, n) = (a.y[n * nc + code] - Ys[n]) * rdt;
   }
 }
 
 template <class M>
-__global__ __launch_bounds__(FLAT) void k_ijac(ImArgs a) {
+__global__ __launch_bounds__(FLAT) void k_ibase(ImArgs a) {
   int i, j, k;
   if (!flat_cell_of(a.b, i, j, k)) return;
-  ijac_cell<M>(a, a.Q, a.st, i, j, k, im_code(a.b, i, j, k));
+  ijac_base<M>(a, a.Q, a.st, i, j, k, im_code(a.b, i, j, k));
+}
+
+// a flat launch per slice: blockIdx.y is the column (or ns + 1, f(y))
+template <class M>
+__global__ __launch_bounds__(FLAT) void k_icol(ImArgs a) {
+  int i, j, k;
+  if (!flat_cell_of(a.b, i, j, k)) return;
+  ijac_column<M>(a, a.Q, i, j, k, im_code(a.b, i, j, k), (int)blockIdx.y);
+}
+
+// four waves of 16 cells per workgroup
+template <int NB, int NV>
+__global__ __launch_bounds__(256) void k_lusolve(ImArgs a) {
+  const int64_t grp = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (grp * IM_G >= a.ncell) return;
+  lu_refine_group<NB, NV>(a, grp, threadIdx.x & 63);
 }
 
 template <class M>
-__global__ __launch_bounds__(FLAT) void k_ros(ImArgs a) {
+__global__ __launch_bounds__(FLAT) void k_iend(ImArgs a) {
   int i, j, k;
   if (!flat_cell_of(a.b, i, j, k)) return;
-  ros_cell<M>(a, a.Q, a.out, i, j, k, im_code(a.b, i, j, k));
+  iend_cell<M>(a, a.Q, a.out, i, j, k, im_code(a.b, i, j, k));
 }
 
 }  // namespace pga

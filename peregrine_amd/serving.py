@@ -130,7 +130,7 @@ class ReactorPool:
         """Device bytes of a pool of `capacity` slots (Q, prims, state, rates, T history, the
         implicit form's work), per slot."""
         ne, nv = ns + 4, ns + 1
-        work = 144 * 4 + nv * nv * 8 + 2 * nv * 8
+        work = nv * nv * 8 + 3 * nv * 8
         return 8 * (2 * ne + 8 + ns - 1 + 4096) + work + 4
 
     def _set(self, s, T0, p0, Y):
@@ -295,6 +295,7 @@ class SolverSlot:
             inner = Q[g:-g, g:-g, g:-g]
             ke = 0.5 * (inner[..., 1] ** 2 + inner[..., 2] ** 2 + inner[..., 3] ** 2) / inner[..., 0]
             self.req.result = {"ke": float(ke.mean()), "Q": inner.copy()}
+        self.mb.release()
         self.mb = None
 
 

@@ -11,6 +11,7 @@ each slot a sequence of launches of the HIP library on one stream.
 """
 
 import ctypes
+import gc
 
 import numpy as np
 
@@ -853,13 +854,29 @@ class Solver:
             graph = torch.cuda.CUDAGraph()
             before = self._standing()
             # captured on torch's capture stream, which the backend's launches follow; nothing runs,
-            # and the host's bookkeeping moves on as the step's would
-            with torch.cuda.graph(graph):
-                launches()
+            # and the host's bookkeeping moves on as the step's would. The cyclic collector is held off
+            # meanwhile: a collection inside the capture could free another solver's graph, and
+            # destroying a graph is refused while a stream captures.
+            collecting = gc.isenabled()
+            gc.disable()
+            try:
+                with torch.cuda.graph(graph):
+                    launches()
+            finally:
+                if collecting:
+                    gc.enable()
             self._graph = (graph, before, dict(self.readied))
         self._graph[0].replay()
         self.readied = dict(self._graph[2])
         self.replays += 1
+
+    def release(self):
+        """Drop the captured graph and the table of blocks now, on the host's schedule, rather than
+        whenever the collector frees the solver (which may fall inside another solver's capture)."""
+        self._graph = None
+        if self.batch is not None:
+            self.batch.close()
+            self.batch = None
 
     def step(self, dt):
         start = self.tme.get()
