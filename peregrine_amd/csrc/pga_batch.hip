@@ -78,16 +78,6 @@ struct BArgs {
   int q, qout, axis;
 };
 
-// the cell of a flat tile t of box b, false past the box
-__device__ __forceinline__ bool flat_at(const Box3 &b, int64_t t, int &i, int &j, int &k) {
-  const int64_t c = t * FLAT + threadIdx.x;
-  const int64_t row = c / b.e0;
-  i = b.s0 + (int)(c - row * b.e0);
-  j = b.s1 + (int)(row % b.e1);
-  k = b.s2 + (int)(row / b.e1);
-  return row < (int64_t)b.e1 * b.e2;
-}
-
 template <class M>
 __global__ __launch_bounds__(FLAT) void kb_rates(BArgs a) {
   const int64_t g = xcd_block();
@@ -188,15 +178,16 @@ __global__ __launch_bounds__(FLAT) void kb_ibase(BArgs a, ImArgs im) {
   ijac_base<M>(im, B.Q[a.q], B.st, i, j, k, g * FLAT + threadIdx.x);
 }
 
-// blockIdx.y is the slice (a column, or ns + 1: f(y))
+// a tile's slices (the ns + 1 columns and f(y)) in workgroups numbered one after another, as k_icol
 template <class M>
 __global__ __launch_bounds__(FLAT) void kb_icol(BArgs a, ImArgs im) {
-  const int64_t g = xcd_block();
+  constexpr int64_t S = M::NS + 2;
+  const int64_t gs = xcd_block(), g = gs / S;
   const int n = bfind(a.prefix, a.nblocks, g);
   const BBlock &B = a.blocks[n];
   int i, j, k;
   if (!flat_at(B.b, g - a.prefix[n], i, j, k)) return;
-  ijac_column<M>(im, B.Q[a.q], i, j, k, g * FLAT + threadIdx.x, (int)blockIdx.y);
+  ijac_column<M>(im, B.Q[a.q], i, j, k, g * FLAT + threadIdx.x, (int)(gs - g * S));
 }
 
 // the places of a tile's overhang (no cell): the identity and a zero right-hand side, once a call,
@@ -574,7 +565,7 @@ extern "C" int32_t pga_batch_implicit(const pga_batch *h, const pga_implicit_opt
     for (int part = 0; part < o->parts; part++) {                                                \
       im.first = part == 0, im.last = part == o->parts - 1;                                      \
       hipLaunchKernelGGL((kb_ibase<Mech_##NAME>), grid, dim3(FLAT), 0, s, a, im);               \
-      hipLaunchKernelGGL((kb_icol<Mech_##NAME>), dim3(grid.x, NV + 1), dim3(FLAT), 0, s, a, im); \
+      hipLaunchKernelGGL((kb_icol<Mech_##NAME>), dim3(grid.x * (NV + 1)), dim3(FLAT), 0, s, a, im); \
       hipLaunchKernelGGL((k_lusolve<NB, NV>), dim3((unsigned)((groups + 3) / 4)), dim3(256), 0, s, im); \
     }                                                                                            \
     hipLaunchKernelGGL((kb_iend<Mech_##NAME>), grid, dim3(FLAT), 0, s, a, im);                  \

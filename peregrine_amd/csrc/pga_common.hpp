@@ -210,6 +210,16 @@ __device__ __forceinline__ bool flat_cell_of(const Box3 &b, int &i, int &j, int 
   return row < (unsigned)b.e1 * (unsigned)b.e2;
 }
 
+// the cell of a flat tile t of box b, false past the box
+__device__ __forceinline__ bool flat_at(const Box3 &b, int64_t t, int &i, int &j, int &k) {
+  const int64_t c = t * FLAT + threadIdx.x;
+  const int64_t row = c / b.e0;
+  i = b.s0 + (int)(c - row * b.e0);
+  j = b.s1 + (int)(row % b.e1);
+  k = b.s2 + (int)(row / b.e1);
+  return row < (int64_t)b.e1 * b.e2;
+}
+
 // the cell a thread takes, false when the block overhangs the box
 __device__ __forceinline__ bool cell_of(const Box3 &b, int &i, int &j, int &k) {
   const int ii = blockIdx.x * TI + threadIdx.x, jj = blockIdx.y * TJ + threadIdx.y;

@@ -99,7 +99,7 @@ extern "C" int32_t pga_implicit(const pga_case *c, const pga_implicit_opts *o, p
     for (int part = 0; part < o->parts; part++) {                                                 \
       a.first = part == 0, a.last = part == o->parts - 1;                                         \
       hipLaunchKernelGGL((k_ibase<Mech_##NAME>), grid, dim3(FLAT), 0, s, a);                     \
-      hipLaunchKernelGGL((k_icol<Mech_##NAME>), dim3(grid.x, NV + 1), dim3(FLAT), 0, s, a);      \
+      hipLaunchKernelGGL((k_icol<Mech_##NAME>), dim3(grid.x * (NV + 1)), dim3(FLAT), 0, s, a);   \
       hipLaunchKernelGGL((k_lusolve<NB, NV>), dim3((unsigned)((groups + 3) / 4)), dim3(256), 0, s, a); \
     }                                                                                             \
     hipLaunchKernelGGL((k_iend<Mech_##NAME>), grid, dim3(FLAT), 0, s, a);                        \

@@ -518,12 +518,15 @@ __global__ __launch_bounds__(FLAT) void k_ibase(ImArgs a) {
   ijac_base<M>(a, a.Q, a.st, i, j, k, im_code(a.b, i, j, k));
 }
 
-// a flat launch per slice: blockIdx.y is the column (or ns + 1, f(y))
+// every slice of a tile of cells (the ns + 1 columns and f(y)) in workgroups numbered one after
+// another, so one XCD runs a tile's slices and its L2 serves y and f(y^) to all of them
 template <class M>
 __global__ __launch_bounds__(FLAT) void k_icol(ImArgs a) {
+  constexpr unsigned S = M::NS + 2;
+  const unsigned g = xcd_block(), tile = g / S;
   int i, j, k;
-  if (!flat_cell_of(a.b, i, j, k)) return;
-  ijac_column<M>(a, a.Q, i, j, k, im_code(a.b, i, j, k), (int)blockIdx.y);
+  if (!flat_at(a.b, tile, i, j, k)) return;
+  ijac_column<M>(a, a.Q, i, j, k, im_code(a.b, i, j, k), (int)(g - tile * S));
 }
 
 // four waves of 16 cells per workgroup
