@@ -11,7 +11,7 @@ Alongside: the dominant kernel's achieved rate against its roofline (device even
 launch stream around each launch, over as many more steps launched eagerly), and the CPU oracle
 timed on the host's cores on a bounded sample of the same case.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 128] [--cpu-n 64] [--cpu-seconds 10] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 128] [--cpu-n 128] [--cpu-seconds 10] [--no-cpu]
 """
 
 import argparse
@@ -61,6 +61,9 @@ def algorithmic_bytes(name, ne, ng):
     """Compulsory HBM bytes per item of one launch of an entry point (doubles x 8)."""
     if name == "pga_stage":
         return b_alg(ne)
+    if name == "pga_implicit":
+        # per cell: Q and T in, the carried species' mean rates out (the matrices are the call's own work)
+        return 8 * (ne + 1 + ne - 5)
     if name == "pga_rates":
         # per cell: Q and T in, the carried species' rates out
         return 8 * (ne + 1 + ne - 5)
@@ -80,9 +83,20 @@ def algorithmic_bytes(name, ne, ng):
     return 0
 
 
+def cpu_model():
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(support, n, seconds):
-    """The oracle (C, OpenMP over the host's cores) on an n^3 block of the same case: stages in
-    turn (RK3's three, repeated) until `seconds` of them have run, the median stage reported."""
+    """The oracle (C, OpenMP over the host's cores) on an n^3 block of the same case (the full 128^3
+    by default): stages in turn (RK3's three, repeated) until `seconds` of them have run, the
+    median stage reported."""
     from oracle import oracle as O
     from peregrine_amd.simulator import get_simulator
 
@@ -103,9 +117,9 @@ def cpu_baseline(support, n, seconds):
         times.append(time.perf_counter() - t0)
         s += 1
     med = float(np.median(times))
-    return dict(value=n**3 / med, unit="cell-updates/s", cores=O.threads(), kind="port",
+    return dict(value=n**3 / med, unit="cell-updates/s", cores=O.threads(), kind="port", cpu=cpu_model(),
                 sample=f"{n}^3 block of the same case, median of {len(times)} RK3 stages ({sum(times):.1f} s), "
-                       "C oracle with OpenMP")
+                       "C oracle with OpenMP, one thread per core the process may run on")
 
 
 def measured_traffic(kernel):
@@ -338,7 +352,7 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=128)
-    ap.add_argument("--cpu-n", type=int, default=64)
+    ap.add_argument("--cpu-n", type=int, default=128)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--per-kernel", action="store_true", help="the reference's launch sequence instead of the fused stage")
@@ -426,14 +440,24 @@ def main():
     per = algorithmic_bytes(dom, ne, mb.ng)
     achieved = per * items / (ms * 1e-3) / 1e9
     kname = {"pga_flux": "k_flux", "pga_stage": "k_stage"}.get(dom, dom)
+    # the flux sweep is bound by FP64 VALU issue and latency (3 waves per SIMD), not by HBM: its
+    # bytes against HBM and its flops against FP64 are both reported, and the bound names the one
+    # whose peak it is nearer
     roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK, unit="GB/s", frac=round(achieved / HBM_PEAK, 4),
-                traffic=measured_traffic(kname), kernel=dom, kernel_ms=round(ms, 4), launches=count,
-                bytes_per_item=per, items_per_launch=items)
+                traffic=measured_traffic(kname), traffic_source="profiles/traffic.json: rocprofv3 FETCH_SIZE and "
+                "WRITE_SIZE passes of this benchmark, scaled by a plain copy's readings",
+                kernel=dom, kernel_ms=round(ms, 4), launches=count, bytes_per_item=per, items_per_launch=items)
     if dom == "pga_flux":
-        # the sweep is FP64-VALU and latency bound: its FP64 rate beside the bandwidth
         tf = FLUX_FLOP_PER_FACE * items / (ms * 1e-3) / 1e12
         roof["fp64"] = dict(achieved=round(tf, 2), peak=FP64_PEAK, unit="TFLOP/s", frac=round(tf / FP64_PEAK, 4),
                             flop_per_item=FLUX_FLOP_PER_FACE)
+        roof["limiter"] = "FP64 VALU issue and latency (3 waves per SIMD at 152 VGPRs); neither peak is reached"
+    # the whole stage against the compulsory bytes of a fused pass (b_alg per cell and stage)
+    if not mb.sim.viscous:
+        ba = b_alg(ne)
+        st = ba * ncells * nst * args.steps / elapsed / 1e9
+        roof["stage"] = dict(achieved=round(st, 1), peak=HBM_PEAK, unit="GB/s", frac=round(st / HBM_PEAK, 4),
+                             bytes_per_cell=ba, stage_ms=round(elapsed / args.steps / nst * 1e3, 4))
     breakdown = {k: {"launches": c, "mean_ms": round(t, 4), "share": round(c * t / sum(total.values()), 3)}
                  for k, (c, t) in sorted(times.items(), key=lambda kv: -kv[1][0] * kv[1][1])}
     cpu = None if args.no_cpu else cpu_baseline(support, args.cpu_n, args.cpu_seconds)

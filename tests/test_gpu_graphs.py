@@ -206,3 +206,46 @@ def test_allfinite_finds_a_planted_nan(gpu):
     a[4, 0, 0, 0] = float("inf")
     native.check(lib.pga_allfinite(view, 4, box, out.data_ptr(), None), "pga_allfinite")
     assert out.tolist() == [1, 7]
+
+
+@pytest.mark.skip(reason="forking the blocks' side streams inside a captured HIP graph crashed the host process "
+                         "(segmentation fault) in the graph's replay on a 2x2x1 lattice of the reacting case; no cause "
+                         "is shown, so under graph replay the solver keeps every block on the launch stream")
+def test_side_streams_forked_inside_a_captured_graph(gpu):
+    """A 2x2x1 lattice of the reacting case, each block's sweeps and stage on one of four side streams
+    forked from the launch stream after the halo pass and joined before the next, captured into the
+    step's graph and replayed three times: to the bit of the same steps launched eagerly."""
+    import peregrine_amd as pg
+    from peregrine_amd.solver import Solver
+
+    class Forked(Solver):
+        def _block_streams(self):
+            torch = self.backend.torch
+            if self._streams is None:
+                self._streams = [torch.cuda.Stream(device=self.backend.device) for _ in range(4)]
+            main = torch.cuda.current_stream(self.backend.device)
+            for s in self._streams:
+                s.wait_stream(main)
+            for n, blk in enumerate(self.blocks):
+                yield blk, torch.cuda.stream(self._streams[n % 4])
+            for s in self._streams:
+                main.wait_stream(s)
+
+    def lattice(graphs):
+        cfg = sp.config("h2o2", recon="muscl", primary="hllc", limiter="koren", chem=True, dt=1e-8)
+        cfg["backend-mi355x"]["graphs"] = graphs
+        cfg["backend-mi355x"]["batched"] = False
+        n, L = 16, (0.01,) * 3
+        mesh = pg.mesher.CubeMesher(mbdims=(2, 2, 1), blkdims=[n // 2 + 1, n // 2 + 1, n + 1], lengths=L,
+                                    periodic=[True] * 3)
+        mb = pg.get_solver(cfg, mesh, pg.get_backend("mi355x", cfg), cls=Forked)
+        mb.set_prims([sp.reacting_flow(b.xcc.get(), n, L) for b in mb.blocks])
+        return mb
+
+    eager, replayed = lattice(False), lattice(True)
+    for _ in range(5):
+        eager.step(1e-8)
+        replayed.step(1e-8)
+    assert replayed.replays == 3
+    for a, b in zip(eager.blocks, replayed.blocks):
+        assert np.array_equal(a.Q.get(), b.Q.get())
