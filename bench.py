@@ -358,6 +358,8 @@ def main():
     ap.add_argument("--per-kernel", action="store_true", help="the reference's launch sequence instead of the fused stage")
     ap.add_argument("--k-chunk", type=int, default=None)
     ap.add_argument("--stage-slabs", type=int, default=None, help="k-slabs of the fused stage")
+    ap.add_argument("--k-march", type=int, default=None,
+                    help="k-planes a thread of the march walks (the k sweep inside the stage's launch; 0: split)")
     ap.add_argument("--no-graphs", action="store_true", help="launch every step eagerly (no HIP graph replay)")
     ap.add_argument("--implicit", action="store_true",
                     help="implicit chemistry (FP32 matrix-core LU, FP64 refinement) in place of the coupled source")
@@ -404,6 +406,8 @@ def main():
         cfg["backend-mi355x"]["k-chunk"] = args.k_chunk
     if args.stage_slabs is not None:
         cfg["backend-mi355x"]["stage-slabs"] = args.stage_slabs
+    if args.k_march is not None:
+        cfg["backend-mi355x"]["k-march"] = args.k_march
     cfg["backend-mi355x"]["graphs"] = not args.no_graphs
     backend = pg.get_backend("mi355x", cfg)
     n = args.n
@@ -479,7 +483,8 @@ def main():
                    "cells": ncells, "stages_per_step": nst, "path": "fused" if mb.fused else "per-kernel",
                    "hip_graph_replays": mb.replays,
                    "k_chunk": cfg["backend-mi355x"]["k-chunk"],
-                   "stage_slabs": cfg["backend-mi355x"]["stage-slabs"]},
+                   "stage_slabs": cfg["backend-mi355x"]["stage-slabs"],
+                   "k_march": cfg["backend-mi355x"]["k-march"]},
         "roofline": roof,
         "cpu_baseline": cpu,
         "kernels": breakdown,

@@ -180,6 +180,20 @@ int32_t pga_stage2(const pga_case *c, pga_view Qin, pga_view stin, pga_view Q0, 
                    const int32_t *src, const double *w, const int32_t *withdt, const double *dt, pga_box box,
                    int32_t kchunk, void *stream);
 
+/* pga_stage with its k-face fluxes computed in the same launch (the march): each thread walks a
+ * chunk of kchunk cells of one (i, j) column, takes each k face's advective flux once (pga_flux's
+ * arithmetic, from Qin, stin and the face areas kS) and keeps it for the two cells it lies between;
+ * the i and j faces come from iF and jF (pga_flux along i and j first). No k-face array is written
+ * or read. Results are pga_stage's after pga_flux along k, to the bit. The output must not be the
+ * input (a chunk's last face reads cells the next chunk writes): PGA_EINVAL. For a block without
+ * walls or viscous terms (their hooks change the k faces' fluxes). Replaces, for one stage, the
+ * reference's k-axis pg_flux launch (euler/kernels/flux/single.mako:4-9) with the launches
+ * pga_stage replaces. */
+int32_t pga_stage_march(const pga_case *c, pga_view Qin, pga_view stin, pga_view Q0, pga_view Qout, pga_view stout,
+                        pga_view iF, pga_view jF, pga_view kS, pga_view detJinv, pga_view rates, int32_t nterms,
+                        const int32_t *src, const double *w, const int32_t *withdt, const double *dt, pga_box box,
+                        int32_t kchunk, void *stream);
+
 /* The carried species' production rates (kg/m^3/s) of every cell in box into rates (ns - 1
  * components): the coupled source of a reacting stage. The body of pg_source
  * (peregrine/simulator/chemint/kernels/source.mako:3-10) writing only the species rows. */

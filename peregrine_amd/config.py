@@ -73,8 +73,9 @@ _DEFAULTS = {
     # per-kernel sequence, the fused stage's k-planes per thread, the k-slabs a fused stage is
     # run in (a slab's face arrays stay in the 256 MB Infinity Cache between the flux sweeps and
     # the divergence that reads them), the side streams several blocks' stages overlap on when
-    # launched eagerly, and whether steps replay one captured HIP graph (the
-    # kernels' block shapes are compiled in: 64 lanes along i, waves along j or k)
+    # launched eagerly, whether steps replay one captured HIP graph, and the march: the k sweep
+    # done inside the stage's launch, a thread walking k-march planes of its column (0: off)
+    # (the kernels' block shapes are compiled in: 64 lanes along i, waves along j or k)
     "backend-mi355x": {
         "device": 0,
         "fused-stage": True,
@@ -82,6 +83,7 @@ _DEFAULTS = {
         "stage-slabs": 1,
         "block-streams": 4,
         "graphs": True,
+        "k-march": 0,
         "batched": "auto",
         "autotune": True,
         "autotune-cache": None,

@@ -210,6 +210,15 @@ __device__ __forceinline__ bool flat_cell_of(const Box3 &b, int &i, int &j, int 
   return row < (unsigned)b.e1 * (unsigned)b.e2;
 }
 
+// p (wave-uniform), as a value the compiler cannot see through: the loads behind it stay in the
+// loop that makes them rather than being hoisted out of it (a mechanism's or a gas's hundreds of
+// constants would otherwise be held in registers across the loop, at one wave per SIMD)
+template <class T>
+__device__ __forceinline__ T *opaque(T *p) {
+  asm volatile("" : "+s"(p));
+  return p;
+}
+
 // the cell of a flat tile t of box b, false past the box
 __device__ __forceinline__ bool flat_at(const Box3 &b, int64_t t, int &i, int &j, int &k) {
   const int64_t c = t * FLAT + threadIdx.x;

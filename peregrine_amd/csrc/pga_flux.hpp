@@ -217,10 +217,10 @@ __device__ __forceinline__ void flux_function(const FaceState<NS> &L, const Face
     rusanov<NS>(L, R, S, F);
 }
 
-// the face (i, j, k) of `axis` (between cells k - 1 and k along it): its sides, its flux into F
+// the face (i, j, k) of `axis` (between cells k - 1 and k along it): its sides, its flux into Fv
 template <int NS, int EOS, int RECON, int PRIMARY, int LIM>
-__device__ __forceinline__ void flux_face(const double *__restrict__ K, int axis, const View &Q, const View &st,
-                                          const View &S, const View &F, int i, int j, int k) {
+__device__ __forceinline__ void flux_at(const double *__restrict__ K, int axis, const View &Q, const View &st,
+                                        const View &S, int i, int j, int k, double (&Fv)[NS + 4]) {
   const Gas<NS> g{K};
   const int64_t o = Q.off(i, j, k);
   const int64_t d = axis == 0 ? 1 : axis == 1 ? Q.sj : Q.sk;
@@ -228,8 +228,15 @@ __device__ __forceinline__ void flux_face(const double *__restrict__ K, int axis
   face_sides<NS, EOS, RECON, LIM>(g, Q, st, o, d, L, R);
   const int64_t of = S.off(i, j, k);
   const double Sv[3] = {S.at(of, 0), S.at(of, 1), S.at(of, 2)};
-  double Fv[NS + 4];
   flux_function<NS, PRIMARY>(L, R, Sv, Fv);
+}
+
+// the face's flux into the face array
+template <int NS, int EOS, int RECON, int PRIMARY, int LIM>
+__device__ __forceinline__ void flux_face(const double *__restrict__ K, int axis, const View &Q, const View &st,
+                                          const View &S, const View &F, int i, int j, int k) {
+  double Fv[NS + 4];
+  flux_at<NS, EOS, RECON, PRIMARY, LIM>(K, axis, Q, st, S, i, j, k, Fv);
   const int64_t oF = F.off(i, j, k);
 #pragma unroll
   for (int l = 0; l < NS + 4; l++) F.at(oF, l) = Fv[l];

@@ -224,16 +224,6 @@ __device__ __forceinline__ void rhs0d(const Gas<M::NS> &g, const double *__restr
   f[NS] = dTdt * cvrhoinv;
 }
 
-// p (wave-uniform), as a value the compiler cannot see through: the loads behind it stay in the
-// loop that makes them rather than being hoisted out of it (the hundreds of mechanism constants
-// and a cell's 100 factors would otherwise be held in registers across the loop, at one wave per
-// SIMD)
-template <class T>
-__device__ __forceinline__ T *opaque(T *p) {
-  asm volatile("" : "+s"(p));
-  return p;
-}
-
 __device__ __forceinline__ int64_t im_code(const Box3 &b, int i, int j, int k) {
   return (i - b.s0) + (int64_t)b.e0 * ((j - b.s1) + (int64_t)b.e1 * (k - b.s2));
 }

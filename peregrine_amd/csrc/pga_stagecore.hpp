@@ -54,67 +54,111 @@ __device__ __forceinline__ void fromQ_regs(const Gas<NS> &g, const double (&Q)[N
   fromrhoE<NS, EOS>(g, m, [&](int n) { return Y[n]; }, rho, e, Tguess, s);
 }
 
+// Row l of one cell's stage: its six faces in the reference's order of sums (the k faces given), times
+// detJinv, the source first where there is one (chemint/kernels/source.mako:3-10), then the RK combine
+// in the reference's order of terms (integrators/kernels/combine.mako:1-13). What only this cell reads
+// (Q, Q0, the rates, 1/J) and what it writes stream past the caches (nontemporal); each face value is
+// read by the two cells it lies between and is cached
+template <bool CHEM>
+__device__ __forceinline__ void stage_row(const StageArgs &a, int l, double kfl, double kfh, int64_t o, int64_t o0,
+                                          int64_t oo, int64_t fil, int64_t fjl, int64_t orr, double J, double dt,
+                                          int i, int j, int k) {
+#pragma clang fp contract(off)
+  const View &iF = a.iF, &jF = a.jF;
+  const int64_t fih = fil + 1, fjh = fjl + jF.sj;
+  double d = ((((iF.at(fil, l) + jF.at(fjl, l)) + kfl) - iF.at(fih, l)) - jF.at(fjh, l)) - kfh;
+  d = d * J;
+  if constexpr (CHEM) {
+    if (l >= 5) d = __builtin_nontemporal_load(&a.rates.at(orr, l - 5)) + d;
+  }
+  // the terms' values are selected as values: a select of pointers would put them in memory
+  const double qin = __builtin_nontemporal_load(&a.Qin.at(o, l));
+  const double q0 = a.q0 ? __builtin_nontemporal_load(&a.Q0.at(o0, l)) : 0.0;
+  double va = 0.0;
+  if (a.two) {
+    // the first combine, its own register read as it stood, written before the second reads it
+    const int64_t oa = a.QA.off(i, j, k);
+    const double own = a.QA.at(oa, l);
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+      if (t < a.combA.n) {
+        const int src = a.combA.src[t];
+        const double x = src == SRC_QIN ? qin : src == SRC_Q0 ? q0 : src == SRC_OWNA ? own : d;
+        const double term = a.combA.withdt[t] ? a.combA.w[t] * dt * x : a.combA.w[t] * x;
+        va = t == 0 ? term : va + term;
+      }
+    }
+    a.QA.at(oa, l) = va;
+  }
+  double acc = 0.0;
+#pragma unroll
+  for (int t = 0; t < 4; t++) {
+    if (t < a.comb.n) {
+      const int src = a.comb.src[t];
+      const double x = src == SRC_QIN ? qin : src == SRC_Q0 ? q0 : src == SRC_A ? va : d;
+      const double term = a.comb.withdt[t] ? a.comb.w[t] * dt * x : a.comb.w[t] * x;
+      acc = t == 0 ? term : acc + term;
+    }
+  }
+  __builtin_nontemporal_store(acc, &a.Qout.at(oo, l));
+}
+
 // The divergence, the source and the RK combine of the cells (i, j, kbeg..kend-1) of one column:
-// memory-bound
+// memory-bound. A row at a time in a rolled loop keeps ~30 registers, so eight waves per SIMD hide
+// the loads' latency
 template <int NE, bool CHEM>
 __device__ __forceinline__ void stage_cells(const StageArgs &a, int i, int j, int kbeg, int kend) {
-  const View Qin = a.Qin, Q0 = a.Q0, Qout = a.Qout, det = a.det;
-  const View iF = a.iF, jF = a.jF, kF = a.kF;
-  const CombineSpec comb = a.comb;
-  const View rates = a.rates;
+  const View &kF = a.kF;
   const double dt = *a.dt;
 #pragma unroll 1
   for (int k = kbeg; k < kend; k++) {
-    const int64_t o = Qin.off(i, j, k), o0 = Q0.off(i, j, k), oo = Qout.off(i, j, k);
-    const int64_t fil = iF.off(i, j, k), fjl = jF.off(i, j, k), fkl = kF.off(i, j, k);
-    const int64_t fih = fil + 1, fjh = fjl + jF.sj, fkh = fkl + kF.sk;
-    const int64_t orr = CHEM ? rates.off(i, j, k) : 0;
-    const double J = __builtin_nontemporal_load(&det.at(det.off(i, j, k), 0));
-    // per conserved row: its six faces in the reference's order of sums, times detJinv, the
-    // source first where there is one (chemint/kernels/source.mako:3-10), then the RK combine in
-    // the reference's order of terms (integrators/kernels/combine.mako:1-13); a row at a time in a
-    // rolled loop keeps ~30 registers, so eight waves per SIMD hide the loads' latency. What only
-    // this cell reads (Q, Q0, the rates, 1/J) and what it writes stream past the caches
-    // (nontemporal); each face value is read by the two cells it lies between and is cached
+    const int64_t o = a.Qin.off(i, j, k), o0 = a.Q0.off(i, j, k), oo = a.Qout.off(i, j, k);
+    const int64_t fil = a.iF.off(i, j, k), fjl = a.jF.off(i, j, k), fkl = kF.off(i, j, k);
+    const int64_t fkh = fkl + kF.sk;
+    const int64_t orr = CHEM ? a.rates.off(i, j, k) : 0;
+    const double J = __builtin_nontemporal_load(&a.det.at(a.det.off(i, j, k), 0));
 #pragma unroll 1
-    for (int l = 0; l < NE; l++) {
-#pragma clang fp contract(off)
-      double d = ((((iF.at(fil, l) + jF.at(fjl, l)) + kF.at(fkl, l)) - iF.at(fih, l)) - jF.at(fjh, l)) - kF.at(fkh, l);
-      d = d * J;
-      if constexpr (CHEM) {
-        if (l >= 5) d = __builtin_nontemporal_load(&rates.at(orr, l - 5)) + d;
-      }
-      // the terms' values are selected as values: a select of pointers would put them in memory
-      const double qin = __builtin_nontemporal_load(&Qin.at(o, l));
-      const double q0 = a.q0 ? __builtin_nontemporal_load(&Q0.at(o0, l)) : 0.0;
-      double va = 0.0;
-      if (a.two) {
-        // the first combine, its own register read as it stood, written before the second reads it
-        const int64_t oa = a.QA.off(i, j, k);
-        const double own = a.QA.at(oa, l);
+    for (int l = 0; l < NE; l++)
+      stage_row<CHEM>(a, l, kF.at(fkl, l), kF.at(fkh, l), o, o0, oo, fil, fjl, orr, J, dt, i, j, k);
+  }
+}
+
+// The k sweep and the stage in one pass (the march): the thread of column (i, j) takes the k faces of
+// its chunk one after another, each face's flux computed once into the workgroup's LDS (lo and hi
+// slots, [slot][row][thread]) and read by the two cells it lies between, so no k-face array goes to
+// HBM; the i and j faces come from their sweeps' arrays. The cells' arithmetic is stage_cells' to the
+// bit. The output must not be the input: a chunk's last face reads cells the next chunk writes
+template <int NS, int EOS, int RECON, int PRIMARY, int LIM, bool CHEM>
+__device__ __forceinline__ void march_cells(const StageArgs &a, const View &kS, double *lds, int i, int j, int kbeg,
+                                            int kend) {
+  constexpr int NE = NS + 4;
+  const int tid = threadIdx.x;
+  const double dt = *a.dt;
+  int lo = 0;
+  {
+    double Fv[NE];
+    flux_at<NS, EOS, RECON, PRIMARY, LIM>(a.K, 2, a.Qin, a.stin, kS, i, j, kbeg, Fv);
 #pragma unroll
-        for (int t = 0; t < 4; t++) {
-          if (t < a.combA.n) {
-            const int src = a.combA.src[t];
-            const double x = src == SRC_QIN ? qin : src == SRC_Q0 ? q0 : src == SRC_OWNA ? own : d;
-            const double term = a.combA.withdt[t] ? a.combA.w[t] * dt * x : a.combA.w[t] * x;
-            va = t == 0 ? term : va + term;
-          }
-        }
-        a.QA.at(oa, l) = va;
-      }
-      double acc = 0.0;
+    for (int l = 0; l < NE; l++) lds[(lo * NE + l) * WT + tid] = Fv[l];
+  }
+#pragma unroll 1
+  for (int k = kbeg; k < kend; k++) {
+    const int hi = lo ^ 1;
+    {
+      double Fv[NE];
+      flux_at<NS, EOS, RECON, PRIMARY, LIM>(opaque(a.K), 2, a.Qin, a.stin, kS, i, j, k + 1, Fv);
 #pragma unroll
-      for (int t = 0; t < 4; t++) {
-        if (t < comb.n) {
-          const int src = comb.src[t];
-          const double x = src == SRC_QIN ? qin : src == SRC_Q0 ? q0 : src == SRC_A ? va : d;
-          const double term = comb.withdt[t] ? comb.w[t] * dt * x : comb.w[t] * x;
-          acc = t == 0 ? term : acc + term;
-        }
-      }
-      __builtin_nontemporal_store(acc, &Qout.at(oo, l));
+      for (int l = 0; l < NE; l++) lds[(hi * NE + l) * WT + tid] = Fv[l];
     }
+    const int64_t o = a.Qin.off(i, j, k), o0 = a.Q0.off(i, j, k), oo = a.Qout.off(i, j, k);
+    const int64_t fil = a.iF.off(i, j, k), fjl = a.jF.off(i, j, k);
+    const int64_t orr = CHEM ? a.rates.off(i, j, k) : 0;
+    const double J = __builtin_nontemporal_load(&a.det.at(a.det.off(i, j, k), 0));
+#pragma unroll 1
+    for (int l = 0; l < NE; l++)
+      stage_row<CHEM>(a, l, lds[(lo * NE + l) * WT + tid], lds[(hi * NE + l) * WT + tid], o, o0, oo, fil, fjl, orr, J,
+                      dt, i, j, k);
+    lo = hi;
   }
 }
 

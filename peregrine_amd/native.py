@@ -95,6 +95,8 @@ SIGNATURES = {
                          C.c_void_p, Box, _i32, C.c_void_p]),
     "pga_stage2": (_i32, [C.c_void_p, View, View, View, View, View, View, View, View, View, View, View, _i32, _ip, _dp,
                           _ip, _i32, _ip, _dp, _ip, C.c_void_p, Box, _i32, C.c_void_p]),
+    "pga_stage_march": (_i32, [C.c_void_p, View, View, View, View, View, View, View, View, View, View, _i32, _ip, _dp,
+                               _ip, C.c_void_p, Box, _i32, C.c_void_p]),
     "pga_rates": (_i32, [C.c_void_p, View, View, View, Box, C.c_void_p]),
     "pga_halo": (_i32, [C.c_void_p, _i32, C.POINTER(View), C.POINTER(View), _ip, _ip, _ip, _ip, _dp, View, View,
                         C.c_void_p]),

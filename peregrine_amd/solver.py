@@ -463,11 +463,13 @@ class Solver:
         # the rows u, v, w, T, Y: each a gradient, the first three a velocity gradient
         self._exchange(lambda b: b.grads, 3 * self.sim.ngrad, 1, 0, 1)
 
-    def fluxes(self, blk, Qv, st, slab=None):
+    def fluxes(self, blk, Qv, st, slab=None, axes=(0, 1, 2)):
         """The advective flux of each axis, with the viscous flux added for a viscous case, then the
         walls' own flux through their faces (the postflux hook). A slab (k0, k1, f0, f1) limits the
         i and j faces to the cells k0 <= k < k1 and the k faces to f0 <= k <= f1."""
         for a, (S, F) in enumerate(((blk.iS, blk.iF), (blk.jS, blk.jF), (blk.kS, blk.kF))):
+            if a not in axes:
+                continue
             box = blk.face_box(a)
             if slab is not None:
                 k0, k1, f0, f1 = slab
@@ -780,6 +782,17 @@ class Solver:
         self._call("pga_batch_implicit", b.handle, ctypes.byref(b.imopts), q, b.sweeps.data_ptr(), b.sweeps.numel(),
                    b.imwork.data_ptr(), b.imwork.numel(), self.dt.ptr)
 
+    def _march(self, blk, regs, first):
+        """The k-planes a thread of the march walks for this block and stage, or 0 for the split
+        k sweep: the march takes an inviscid block without walls (whose hooks change the k faces'
+        fluxes), a stage with one combine whose output is not its input, and one slab."""
+        n = int(self.cfg["backend-mi355x"]["k-march"])
+        if n < 1 or first is not None or regs.Qout == regs.Q or self.sim.viscous or blk.walls:
+            return 0
+        if int(self.cfg["backend-mi355x"]["stage-slabs"]) != 1:
+            return 0
+        return n
+
     def _slabs(self, blk):
         return stage_slabs(blk.ncell[2], int(self.cfg["backend-mi355x"]["stage-slabs"]))
 
@@ -815,6 +828,14 @@ class Solver:
                     if self.sim.chemistry:
                         self.chemistry_source(blk, Qin, blk.rates.view(), False)
                     rates = blk.rates.view() if self.sim.chemistry else native.View()
+                    march = self._march(blk, regs, first)
+                    if march:
+                        # the i and j sweeps, then the k sweep inside the stage's launch
+                        self.fluxes(blk, Qin, st, axes=(0, 1))
+                        self._call("pga_stage_march", self.case.handle, Qin, st, blk.regs[0].view(),
+                                   blk.regs[regs.Qout].view(), st, blk.iF.view(), blk.jF.view(), blk.kS.view(),
+                                   blk.detJinv.view(), rates, len(terms), src, w, withdt, self.dt.ptr, box, march)
+                        continue
                     for slab in self._slabs(blk):
                         self.fluxes(blk, Qin, st, slab)
                         k0, k1 = slab[:2]

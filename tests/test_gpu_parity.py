@@ -373,19 +373,22 @@ def test_reacting_steps_match(dev, fused):
 
 
 def test_fused_and_per_kernel_stages_agree_to_the_bit(dev):
-    """The fused stage against the per-kernel sequence on the reacting case, RK3 and RK2: the same
-    device functions in the same order give the same bits."""
+    """The fused stage against the per-kernel sequence on the reacting case, RK3, RK2 and RK1: the same
+    device functions in the same order give the same bits; and the march (the k sweep inside the
+    stage's launch, in chunks of 7, 7 and 6 planes and of 20) gives the split k sweep's bits."""
     import peregrine_amd as pg
 
     n, L = 20, (0.01,) * 3
     for integ in ("tvd-rk3", "rk2", "rk1"):
         out = []
-        # the fused stage whole, in three uneven k-slabs (7, 7 and 6 cells), and the per-kernel one
-        for fused, slabs in ((True, 1), (True, 3), (False, 1)):
+        # the fused stage whole, in three uneven k-slabs (7, 7 and 6 cells), marching, and the
+        # per-kernel one
+        for fused, slabs, march in ((True, 1, 0), (True, 3, 0), (True, 1, 7), (True, 1, 20), (False, 1, 0)):
             cfg = sp.config("h2o2", recon="muscl", primary="hllc", limiter="koren", chem=True, integrator=integ, dt=1e-8)
             cfg["backend-mi355x"]["fused-stage"] = fused
             cfg["backend-mi355x"]["k-chunk"] = 3
             cfg["backend-mi355x"]["stage-slabs"] = slabs
+            cfg["backend-mi355x"]["k-march"] = march
             mb = pg.get_solver(cfg, pg.mesher.CubeMesher(blkdims=[n + 1] * 3, lengths=L, periodic=[True] * 3),
                                pg.get_backend("mi355x", cfg))
             assert mb.fused == fused
@@ -398,6 +401,24 @@ def test_fused_and_per_kernel_stages_agree_to_the_bit(dev):
         for other in out[1:]:
             assert np.array_equal(out[0][0][inner], other[0][inner]), integ
             assert np.array_equal(out[0][1][inner], other[1][inner]), integ
+
+
+def test_the_march_refuses_to_write_its_input(dev):
+    """pga_stage_march with Qout = Qin: refused before any launch (a chunk's last k face would read
+    cells the next chunk has written)."""
+    cfg, sim, nc, oc = _case("air-tpg", recon="muscl", primary="hllc", limiter="koren")
+    L = native.lib()
+    hb = sp.HostBlock((8, 8, 8), sim.ng, sim.ne, 1)
+    Q, st = dev.put(hb.regs[0]), dev.put(hb.state)
+    faces = [dev.put(f) for f in hb.F]
+    S = [dev.put(x) for x in hb.S]
+    det = dev.put(hb.detJinv)
+    ddt = dev.put(np.array([1e-6]))
+    box = native.box((hb.ng,) * 3, (8, 8, 8))
+    rc = L.pga_stage_march(nc.handle, dev.view(Q), dev.view(st), dev.view(Q), dev.view(Q), dev.view(st),
+                           dev.view(faces[0]), dev.view(faces[1]), dev.view(S[2]), dev.view(det), native.View(), 1,
+                           native.ints([2]), (ctypes.c_double * 1)(1.0), native.ints([1]), ddt.data_ptr(), box, 4, None)
+    assert rc == -1 and b"output is the input" in L.pga_last_error()
 
 
 @pytest.mark.parametrize("gas", ["air-tpg", "air-cpg"])
