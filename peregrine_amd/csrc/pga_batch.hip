@@ -169,13 +169,23 @@ __global__ __launch_bounds__(FLAT) void kb_stage(BArgs a, BStage s) {
 // implicit chemistry over every block's cells into their rates: a cell's place in the work arrays is
 // its workgroup's flat number times 256 plus its lane, the same in both launches
 template <class M>
+__global__ __launch_bounds__(FLAT) void kb_itc(BArgs a, ImArgs im) {
+  const int64_t g = xcd_block();
+  const int n = bfind(a.prefix, a.nblocks, g);
+  const BBlock &B = a.blocks[n];
+  int i, j, k;
+  if (!flat_at(B.b, g - a.prefix[n], i, j, k)) return;
+  ijac_tcache<M>(im, B.Q[a.q], B.st, i, j, k, g * FLAT + threadIdx.x);
+}
+
+template <class M>
 __global__ __launch_bounds__(FLAT) void kb_ibase(BArgs a, ImArgs im) {
   const int64_t g = xcd_block();
   const int n = bfind(a.prefix, a.nblocks, g);
   const BBlock &B = a.blocks[n];
   int i, j, k;
   if (!flat_at(B.b, g - a.prefix[n], i, j, k)) return;
-  ijac_base<M>(im, B.Q[a.q], B.st, i, j, k, g * FLAT + threadIdx.x);
+  ijac_base<M>(im, B.Q[a.q], i, j, k, g * FLAT + threadIdx.x);
 }
 
 // a tile's slices (the ns + 1 columns and f(y)) in workgroups numbered one after another, as k_icol
@@ -524,7 +534,9 @@ static int64_t batch_cells(const pga_batch *h) { return h->total[BK_CELLS] * FLA
 extern "C" int64_t pga_batch_implicit_worksize(const pga_batch *h) {
   if (!h || h->c->desc.ns + 1 > IM_N) return -1;
   const int64_t cells = batch_cells(h), groups = (cells + IM_G - 1) / IM_G, nv = h->c->desc.ns + 1;
-  return groups * nv * nv * IM_G + 3 * nv * cells;
+  const int64_t nt = im_cache_doubles(h->c->desc);
+  if (nt < 0) return -1;
+  return groups * nv * nv * IM_G + (3 * nv + nt) * cells;
 }
 
 extern "C" int32_t pga_batch_implicit(const pga_batch *h, const pga_implicit_opts *o, int32_t q, int32_t *sweeps,
@@ -552,6 +564,7 @@ extern "C" int32_t pga_batch_implicit(const pga_batch *h, const pga_implicit_opt
   im.begin_dQ = 0;
   im.A = work;
   im.y = im.A + groups * nv * nv * IM_G;
+  im.tc = im.y + 3 * nv * cells;
   im.ncell = cells, im.sweeps = sweeps, im.dt = dt;
   im.parts = o->parts, im.maxsweeps = o->max_sweeps, im.tol = o->tolerance;
   const BArgs a = bargs(h, q, q, 0, h->prefix[BK_CELLS]);
@@ -564,6 +577,7 @@ extern "C" int32_t pga_batch_implicit(const pga_batch *h, const pga_implicit_opt
     hipLaunchKernelGGL((kb_ivoid<NV>), grid, dim3(FLAT), 0, s, a, im);                           \
     for (int part = 0; part < o->parts; part++) {                                                \
       im.first = part == 0, im.last = part == o->parts - 1;                                      \
+      hipLaunchKernelGGL((kb_itc<Mech_##NAME>), grid, dim3(FLAT), 0, s, a, im);                 \
       hipLaunchKernelGGL((kb_ibase<Mech_##NAME>), grid, dim3(FLAT), 0, s, a, im);               \
       hipLaunchKernelGGL((kb_icol<Mech_##NAME>), dim3(grid.x * (NV + 1)), dim3(FLAT), 0, s, a, im); \
       hipLaunchKernelGGL((k_lusolve<NB, NV>), dim3((unsigned)((groups + 3) / 4)), dim3(256), 0, s, im); \

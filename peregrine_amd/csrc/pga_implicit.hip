@@ -33,6 +33,16 @@ __global__ __launch_bounds__(256) void k_refine(const double *__restrict__ A, co
 }
 
 static int64_t im_cells(const pga_box &b) { return (int64_t)b.extent[0] * b.extent[1] * b.extent[2]; }
+
+// doubles of a cell's cache of f at one T for the case's mechanism (ImCache), -1 for none
+int64_t im_cache_doubles(const pga_case_desc &d) {
+  int64_t n = -1;
+#define X(NAME, IDX) \
+  if (d.mechanism == IDX) n = im_cache_n<Mech_##NAME>;
+  PGA_MECHANISMS(X)
+#undef X
+  return n;
+}
 static int64_t im_groups(int64_t cells) { return (cells + IM_G - 1) / IM_G; }
 
 template <int NB>
@@ -48,7 +58,9 @@ using namespace pga;
 extern "C" int64_t pga_implicit_worksize(const pga_case *c, pga_box box) {
   if (!c || !box_ok(box) || c->desc.ns + 1 > IM_N) return -1;
   const int64_t cells = im_cells(box), groups = im_groups(cells), nv = c->desc.ns + 1;
-  return groups * nv * nv * IM_G + 3 * nv * cells;
+  const int64_t nt = im_cache_doubles(c->desc);
+  if (nt < 0) return -1;
+  return groups * nv * nv * IM_G + (3 * nv + nt) * cells;
 }
 
 extern "C" int32_t pga_implicit(const pga_case *c, const pga_implicit_opts *o, pga_view Q, pga_view state,
@@ -88,6 +100,7 @@ extern "C" int32_t pga_implicit(const pga_case *c, const pga_implicit_opts *o, p
   a.begin_dQ = begin_dQ != 0;
   a.A = work;
   a.y = a.A + groups * nv * nv * IM_G;
+  a.tc = a.y + 3 * nv * cells;
   a.ncell = cells, a.sweeps = sweeps, a.dt = dt, a.b = box3(box);
   a.parts = o->parts, a.maxsweeps = o->max_sweeps, a.tol = o->tolerance;
   const dim3 grid = flat_grid(a.b);
@@ -98,6 +111,7 @@ extern "C" int32_t pga_implicit(const pga_case *c, const pga_implicit_opts *o, p
     constexpr int NV = Mech_##NAME::NS + 1, NB = (NV + 3) / 4;                                    \
     for (int part = 0; part < o->parts; part++) {                                                 \
       a.first = part == 0, a.last = part == o->parts - 1;                                         \
+      hipLaunchKernelGGL((k_itc<Mech_##NAME>), grid, dim3(FLAT), 0, s, a);                       \
       hipLaunchKernelGGL((k_ibase<Mech_##NAME>), grid, dim3(FLAT), 0, s, a);                     \
       hipLaunchKernelGGL((k_icol<Mech_##NAME>), dim3(grid.x * (NV + 1)), dim3(FLAT), 0, s, a);   \
       hipLaunchKernelGGL((k_lusolve<NB, NV>), dim3((unsigned)((groups + 3) / 4)), dim3(256), 0, s, a); \

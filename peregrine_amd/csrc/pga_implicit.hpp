@@ -181,6 +181,7 @@ struct ImArgs {
   int begin_dQ, first, last;
   double *A;     // the FP64 matrices, group-major with nv^2 entries
   double *y;     // [ns + 1][ncell] three times: the part's start (then its end), f at its FP32 rounding, f at it
+  double *tc;    // [im_cache_n][ncell]: the cache of f at the FP32 rounding's T
   int64_t ncell;
   int32_t *sweeps;
   const double *dt;
@@ -222,6 +223,72 @@ __device__ __forceinline__ void rhs0d(const Gas<M::NS> &g, const double *__restr
 #pragma unroll
   for (int n = 0; n < NS; n++) f[n] = rate[n] * rhoinv;
   f[NS] = dTdt * cvrhoinv;
+}
+
+// What f(y) shares over every composition at one T: the rates' T-only values (M::TCache: ln T, 1/T,
+// the Gibbs terms and their exponentials, each reaction's cached rate constant, ln k or Troe fit) and
+// each species' cv share (cp/R - 1) R_n and internal energy e_n. A forward-difference column of a
+// species changes the composition at the same T, so it takes f from this and the composition alone;
+// every value is the expression rhs0d evaluates, so the column keeps rhs0d's bits.
+template <class M>
+struct ImCache {
+  typename M::TCache t;
+  double ea[M::NS], ee[M::NS];
+};
+
+template <class M>
+constexpr int im_cache_n = sizeof(ImCache<M>) / sizeof(double);
+
+// the cache of f at T into tc[entry * nc] in ImCache's order, each value stored as it is made (C, K and
+// tc in one restrict scope, so the constants keep their scalar loads and nothing waits in a register)
+template <class M>
+__device__ __forceinline__ void im_cache_write(const double *__restrict__ K, const double *__restrict__ C, double T,
+                                               double *__restrict__ tc, int64_t nc) {
+#pragma clang fp contract(off)
+  constexpr int NS = M::NS, NT = sizeof(typename M::TCache) / sizeof(double);
+  const Gas<NS> g{K};
+  M::tstate(g, C, T, tc, nc);
+  const double logT = tlog(T), Tinv = 1.0 / T;
+#pragma unroll
+  for (int n = 0; n < NS; n++) {
+    const double cpR = cpR_species(g, n, logT);
+    tc[(NT + n) * nc] = (cpR - 1.0) * g.Rsp(n);
+    tc[(NT + NS + n) * nc] = e_species(g, n, T, logT, Tinv);
+  }
+}
+
+// rhs0d at a composition whose T is the cache's
+template <class M>
+__device__ __forceinline__ void rhs0d_cached(const Gas<M::NS> &g, const double *__restrict__ C, double rho,
+                                             double rhoinv, const double (&y)[M::NS + 1], const ImCache<M> &c,
+                                             double (&f)[M::NS + 1]) {
+#pragma clang fp contract(off)
+  constexpr int NS = M::NS;
+  double Y[NS], rate[NS], D[NS];
+#pragma unroll
+  for (int n = 0; n < NS; n++) Y[n] = y[n];
+  typename M::Rates s;
+  M::cstate(g, C, rho, Y, c.t, s);
+  M::template fold_cached<false>(C, s, c.t, rate, D);
+  double cv = 0.0, dTdt = 0.0;
+#pragma unroll
+  for (int n = 0; n < NS; n++) {
+    cv += c.ea[n] * Y[n];
+    dTdt -= c.ee[n] * rate[n];
+  }
+  const double cvrhoinv = 1.0 / (cv * rho);
+#pragma unroll
+  for (int n = 0; n < NS; n++) f[n] = rate[n] * rhoinv;
+  f[NS] = dTdt * cvrhoinv;
+}
+
+// the cache of work place `code` from the work array's [entry][ncell] region
+template <class M>
+__device__ __forceinline__ void im_cache_load(const double *__restrict__ tc, int64_t nc, int64_t code, ImCache<M> &c) {
+  double v[im_cache_n<M>];
+#pragma unroll
+  for (int e = 0; e < im_cache_n<M>; e++) v[e] = tc[e * nc + code];
+  __builtin_memcpy(&c, v, sizeof(v));
 }
 
 __device__ __forceinline__ int64_t im_code(const Box3 &b, int i, int j, int k) {
@@ -285,40 +352,54 @@ __device__ __forceinline__ int refine(const float *__restrict__ W, const double 
   return done ? s : maxsweeps + 1;
 }
 
-// A = I - h J at the part's start, J by forward differences, in two launches: the base (a lane per
-// cell) stores y, and f at y rounded to FP32 (y^); the columns (a lane per cell and unknown) each one
-// column of A, and one more slice f(y) itself, the solve's right-hand side. J is taken at y^: a
+// A = I - h J at the part's start, J by forward differences, in three launches: the cache (a lane per
+// cell) stores y and what f shares over the compositions of y^'s T (y^: y rounded to FP32); the base
+// f(y^); the columns (a lane per cell and unknown) each one column of A, a species column from the
+// cache, the T column by two whole evaluations, and one more slice f(y) itself, the solve's
+// right-hand side. J is taken at y^: a
 // last-bit change of y (the oracle's FP64 solve against the device's refined one, a part's start
 // after another part) then leaves the matrix as it was, where the differences' own rounding would
 // move it by 1e-12 of itself and a stiff part's result by more; the linearly implicit Euler step is
 // first order with the matrix of any nearby state. A lane makes one evaluation of the rates, so the
 // lanes hold the registers of one evaluation and the chip's waves are as many as for the rates alone.
 
-// the base: the first part takes y from Q and the state; f(y^) into the work array
+// the part's start: the first part takes y from Q and the state, a later one the y the last part left
+// (selected rather than branched, one copy of the code); y and the cache of f at y^'s T into the work
+// array, every store after the constants' loads (which then go through the scalar cache)
 template <class M>
-__device__ __forceinline__ void ijac_base(const ImArgs &a, const View &Q, const View &st, int i, int j, int k,
-                                          int64_t code) {
+__device__ __forceinline__ void ijac_tcache(const ImArgs &a, const View &Q, const View &st, int i, int j, int k,
+                                            int64_t code) {
 #pragma clang fp contract(off)
   constexpr int NS = M::NS, NV = NS + 1;
   const int64_t nc = a.ncell;
-  const Gas<NS> g{a.K};
   const int64_t o = Q.off(i, j, k);
-  const double rho = Q.at(o, 0), rhoinv = 1.0 / rho;
-  double *y = a.y + code, *fh = a.y + NV * nc + code;
-  // y from Q and the state (the first part) or as the last part left it, selected rather than
-  // branched (one copy of the rates in the code)
-  double y0[NV], yc[NV], fc[NV], Y[NS];
+  const double rhoinv = 1.0 / Q.at(o, 0);
+  double *y = a.y + code;
+  double y0[NV], Y[NS];
   massfrac<NS>(Q, o, rhoinv, Y);
 #pragma unroll
   for (int n = 0; n < NS; n++) y0[n] = a.first ? Y[n] : y[n * nc];
   y0[NS] = a.first ? st.at(st.off(i, j, k), 1) : y[NS * nc];
-#pragma unroll
-  for (int r = 0; r < NV; r++) yc[r] = (double)(float)y0[r];
-  rhs0d<M>(g, a.C, rho, rhoinv, yc, fc);
-  // every store after the rates, whose constants are then read through the scalar cache (a later
-  // part stores the y it read)
+  im_cache_write<M>(a.K, a.C, (double)(float)y0[NS], a.tc + code, nc);
 #pragma unroll
   for (int r = 0; r < NV; r++) y[r * nc] = y0[r];
+}
+
+// the base: f at y rounded to FP32 (y^), from the cache, into the work array
+template <class M>
+__device__ __forceinline__ void ijac_base(const ImArgs &a, const View &Q, int i, int j, int k, int64_t code) {
+#pragma clang fp contract(off)
+  constexpr int NS = M::NS, NV = NS + 1;
+  const int64_t nc = a.ncell;
+  const double rho = Q.at(Q.off(i, j, k), 0), rhoinv = 1.0 / rho;
+  const double *y = a.y + code;
+  double yc[NV], fc[NV];
+#pragma unroll
+  for (int r = 0; r < NV; r++) yc[r] = (double)(float)y[r * nc];
+  ImCache<M> cc;
+  im_cache_load<M>(a.tc, nc, code, cc);
+  rhs0d_cached<M>(Gas<NS>{a.K}, a.C, rho, rhoinv, yc, cc, fc);
+  double *fh = a.y + NV * nc + code;
 #pragma unroll
   for (int r = 0; r < NV; r++) fh[r * nc] = fc[r];
 }
@@ -345,7 +426,14 @@ __device__ __forceinline__ void ijac_column(const ImArgs &a, const View &Q, int 
   const double dlt = fd_step(ym, m == NS);
 #pragma unroll
   for (int r = 0; r < NV; r++) yp[r] = r == m ? ym + dlt : yp[r];
-  rhs0d<M>(Gas<NS>{a.K}, a.C, rho, rhoinv, yp, f1);
+  if (m < NS) {
+    // a species column: T is the base's, whose cache the base left
+    ImCache<M> cc;
+    im_cache_load<M>(a.tc, nc, code, cc);
+    rhs0d_cached<M>(Gas<NS>{a.K}, a.C, rho, rhoinv, yp, cc, f1);
+  } else {
+    rhs0d<M>(Gas<NS>{a.K}, a.C, rho, rhoinv, yp, f1);
+  }
   if (!column) {
     double *bv = a.y + 2 * NV * nc + code;
 #pragma unroll
@@ -502,10 +590,17 @@ __device__ __forceinline__ void iend_cell(const ImArgs &a, const View &Q, const 
 }
 
 template <class M>
+__global__ __launch_bounds__(FLAT) void k_itc(ImArgs a) {
+  int i, j, k;
+  if (!flat_cell_of(a.b, i, j, k)) return;
+  ijac_tcache<M>(a, a.Q, a.st, i, j, k, im_code(a.b, i, j, k));
+}
+
+template <class M>
 __global__ __launch_bounds__(FLAT) void k_ibase(ImArgs a) {
   int i, j, k;
   if (!flat_cell_of(a.b, i, j, k)) return;
-  ijac_base<M>(a, a.Q, a.st, i, j, k, im_code(a.b, i, j, k));
+  ijac_base<M>(a, a.Q, i, j, k, im_code(a.b, i, j, k));
 }
 
 // every slice of a tile of cells (the ns + 1 columns and f(y)) in workgroups numbered one after

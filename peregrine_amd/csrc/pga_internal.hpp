@@ -134,6 +134,10 @@ inline Turn turn_of(const double *R) {
 
 #define PGA_NS_CASES(X) X(1) X(2) X(3) X(9)
 
+// doubles of a cell's cache of the implicit chemistry's f at one T for the case's mechanism, -1 for
+// none (pga_implicit.hip)
+int64_t im_cache_doubles(const pga_case_desc &d);
+
 // the species counts the cubic real gas is compiled for (its kernels are the heaviest to build)
 constexpr bool realgas_compiled(int ns) { return ns == 1 || ns == 2; }
 

@@ -62,6 +62,59 @@ def test_batched_lu_solve_refines_to_fp64(dev, nv):
         assert errs[1] > 1e-10 and (counts >= 2).any()
 
 
+def _device_y(q_regs, sim):
+    """The part's start as the device reads it from Q and the state: the carried mass fractions
+    clamped to [0, 1] and scaled when they sum past one, the last species the rest (massfrac)."""
+    Q, T = q_regs
+    rinv = 1.0 / Q[0]
+    Y, last, total = [], np.ones_like(Q[0]), np.zeros_like(Q[0])
+    for n in range(sim.ns - 1):
+        Yn = np.maximum(np.minimum(Q[5 + n] * rinv, 1.0), 0.0)
+        Y.append(Yn)
+        last, total = last - Yn, total + Yn
+    scale = 1.0 / np.maximum(total, 1.0)
+    return np.stack([v * scale for v in Y] + [np.maximum(last, 0.0), T]), Q[0]
+
+
+def test_the_matrices_are_the_oracles(dev):
+    """The FP64 matrices I - h J the device assembles (a species column's rates from the cache of the
+    T-only values, the T column and f(y) whole) read back from the work array, against
+    pgo_implicit_matrix at each cell's y rounded to FP32: within 1e-12 of each column's largest entry
+    (the same rates, the same steps). A cached value taken for the wrong reaction or species, or at
+    the wrong T, moves a column by far more."""
+    import torch
+
+    cfg, sim, nc, oc = _case("h2o2", chem=True)
+    ncell = (20, 6, 5)
+    q = _hot_and_cold(sim, ncell, radicals=1e-3)
+    hb = sp.HostBlock(ncell, 0, sim.ne, 1)
+    full = O.box((0, 0, 0), ncell)
+    O.lib.pgo_fromprims(oc.ref, O.arr(sp.to_dev(q)), O.arr(hb.regs[0]), O.arr(hb.state), full)
+    L = native.lib()
+    box = native.box((0, 0, 0), ncell)
+    cells, nv, dt = int(np.prod(ncell)), sim.ns + 1, 1e-7
+    groups = -(-cells // 16)
+    nwork = int(L.pga_implicit_worksize(nc.handle, box))
+    work = torch.zeros(nwork, dtype=torch.float64, device="cuda")
+    sweeps = torch.zeros(cells, dtype=torch.int32, device="cuda")
+    out = dev.put(np.zeros((sim.ns - 1,) + hb.regs[0].shape[1:]))
+    ddt = dev.put(np.array([dt]))
+    opts = native.ImplicitOpts(1, 8, 1e-13)
+    native.check(L.pga_implicit(nc.handle, ctypes.byref(opts), dev.view(dev.put(hb.regs[0])), dev.view(dev.put(hb.state)),
+                                dev.view(out), 0, sweeps.data_ptr(), cells, work.data_ptr(), nwork, ddt.data_ptr(), box,
+                                None), "pga_implicit")
+    w = work.cpu().numpy()
+    A = w[:groups * nv * nv * 16].reshape(groups, nv, nv, 16).transpose(0, 3, 1, 2).reshape(-1, nv, nv)[:cells]
+    y, rho = _device_y((hb.regs[0].reshape(sim.ne, -1), hb.state[1].reshape(-1)), sim)
+    worst = 0.0
+    for c in range(cells):
+        want = oc.implicit_matrix(rho[c], y[:, c].astype(np.float32).astype(np.float64), dt)
+        scale = np.abs(want).max(axis=0)
+        worst = max(worst, float((np.abs(A[c] - want).max(axis=0) / scale).max()))
+    print(f"device matrices against the oracle's: {worst:.2e} of a column's largest entry")
+    assert worst <= 1e-12
+
+
 @pytest.mark.parametrize("parts", [1, 3])
 @pytest.mark.parametrize("begin_dQ", [True, False], ids=["dQ", "rates"])
 def test_implicit_source_matches_the_fp64_form(dev, parts, begin_dQ):
