@@ -148,6 +148,16 @@ int32_t pga_copy_box(pga_view dst, const int32_t *dst_start, pga_view src, const
                      const int32_t *extent, int32_t ncomp, const int32_t *orient, const double *rot, int32_t vec,
                      int32_t rows, void *stream);
 
+/* pga_copy_box for up to 6 slabs in one launch: slab n copies ncomp components of extent[3n..3n+2]
+ * cells from src[n] (placed by src_start and the orientation digits orient[3n..], null: aligned) to
+ * dst[n] at dst_start[3n..], turned by rot[9n..9n+8] (null, or an exact identity: a copy) as
+ * pga_copy_box turns with vec and rows. The one-plane exchange of the gradients across every
+ * interface of a block in one launch. Replaces, for those slabs, the reference's pg_place launches
+ * (place.mako:3-37) that pga_copy_box replaces one by one. */
+int32_t pga_copy_slabs(int32_t ntask, const pga_view *dst, const pga_view *src, const int32_t *dst_start,
+                       const int32_t *src_start, const int32_t *extent, int32_t ncomp, const int32_t *orient,
+                       const double *rot, int32_t vec, int32_t rows, void *stream);
+
 /* 1 when the fused stage is compiled for the case's species count, else 0 (use the per-kernel
  * entry points) */
 int32_t pga_stage_supported(const pga_case *c);

@@ -329,22 +329,20 @@ __global__ __launch_bounds__(256) void k_copy_box(View dst, View src, CopyArgs a
   dst.at(dst.off(a.d0 + ii, a.d1 + jj, a.d2 + kk), comp) = src.at(os, comp);
 }
 
-// the same box across a rotational periodic, a cell per thread: the vector components [vec, vec+3)
-// turned by R; with rows, the components are rows of three (each a gradient, turned), and the rows
-// [vec, vec+3) (a velocity gradient) are turned down each column as well
-__global__ __launch_bounds__(256) void k_copy_turned(View dst, View src, CopyArgs a, Turn t, int vec, int rows) {
-  const int ii = blockIdx.x * TI + threadIdx.x, jj = blockIdx.y * TJ + threadIdx.y, kk = blockIdx.z;
-  if (ii >= a.e0 || jj >= a.e1) return;
-  const int64_t os = a.sbase + ii * a.st0 + jj * a.st1 + kk * a.st2, od = dst.off(a.d0 + ii, a.d1 + jj, a.d2 + kk);
+// one cell of a box across a rotational periodic: the vector components [vec, vec+3) turned by R;
+// with rows, the components are rows of three (each a gradient, turned), and the rows [vec, vec+3)
+// (a velocity gradient) are turned down each column as well
+__device__ __forceinline__ void copy_turned_cell(const View &dst, const View &src, int64_t od, int64_t os, int ncomp,
+                                                 const Turn &t, int vec, int rows) {
   if (!rows) {
-    for (int c = 0; c < a.ncomp; c++)
+    for (int c = 0; c < ncomp; c++)
       if (c < vec || c >= vec + 3) dst.at(od, c) = src.at(os, c);
     if (vec >= 0)
       turned(t.R, src.at(os, vec), src.at(os, vec + 1), src.at(os, vec + 2), dst.at(od, vec), dst.at(od, vec + 1),
              dst.at(od, vec + 2));
     return;
   }
-  for (int r = 0; r < a.ncomp / 3; r++)
+  for (int r = 0; r < ncomp / 3; r++)
     if (r < vec || r >= vec + 3)
       turned(t.R, src.at(os, 3 * r), src.at(os, 3 * r + 1), src.at(os, 3 * r + 2), dst.at(od, 3 * r),
              dst.at(od, 3 * r + 1), dst.at(od, 3 * r + 2));
@@ -359,6 +357,45 @@ __global__ __launch_bounds__(256) void k_copy_turned(View dst, View src, CopyArg
   for (int e = 0; e < 3; e++)
     turned(t.R, H[0][e], H[1][e], H[2][e], dst.at(od, 3 * vec + e), dst.at(od, 3 * (vec + 1) + e),
            dst.at(od, 3 * (vec + 2) + e));
+}
+
+// the same box across a rotational periodic, a cell per thread
+__global__ __launch_bounds__(256) void k_copy_turned(View dst, View src, CopyArgs a, Turn t, int vec, int rows) {
+  const int ii = blockIdx.x * TI + threadIdx.x, jj = blockIdx.y * TJ + threadIdx.y, kk = blockIdx.z;
+  if (ii >= a.e0 || jj >= a.e1) return;
+  const int64_t os = a.sbase + ii * a.st0 + jj * a.st1 + kk * a.st2, od = dst.off(a.d0 + ii, a.d1 + jj, a.d2 + kk);
+  copy_turned_cell(dst, src, od, os, a.ncomp, t, vec, rows);
+}
+
+// up to six slabs in one launch, a cell per thread over the slabs back to back (i fastest in a
+// slab), every component copied, or turned where the slab's turn is on
+struct SlabCopy {
+  View dst, src;
+  CopyArgs a;
+  Turn t;
+  int64_t first, cells;
+};
+
+struct SlabsArgs {
+  SlabCopy s[6];
+  int n, ncomp, vec, rows;
+};
+
+__global__ __launch_bounds__(256) void k_copy_slabs(SlabsArgs a) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int n = 0;
+  while (n + 1 < a.n && c >= a.s[n + 1].first) n++;
+  const SlabCopy &sl = a.s[n];
+  const int64_t r = c - sl.first;
+  if (r >= sl.cells) return;
+  const int ii = (int)(r % sl.a.e0), jj = (int)((r / sl.a.e0) % sl.a.e1), kk = (int)(r / ((int64_t)sl.a.e0 * sl.a.e1));
+  const int64_t os = sl.a.sbase + ii * sl.a.st0 + jj * sl.a.st1 + kk * sl.a.st2;
+  const int64_t od = sl.dst.off(sl.a.d0 + ii, sl.a.d1 + jj, sl.a.d2 + kk);
+  if (sl.t.on) {
+    copy_turned_cell(sl.dst, sl.src, od, os, a.ncomp, sl.t, a.vec, a.rows);
+    return;
+  }
+  for (int comp = 0; comp < a.ncomp; comp++) sl.dst.at(od, comp) = sl.src.at(os, comp);
 }
 }  // namespace pga
 
@@ -401,4 +438,43 @@ extern "C" int32_t pga_copy_box(pga_view dst, const int32_t *dst_start, pga_view
   const dim3 grid((extent[0] + TI - 1) / TI, (extent[1] + TJ - 1) / TJ, extent[2] * ncomp);
   hipLaunchKernelGGL(k_copy_box, grid, dim3(TI, TJ), 0, stream_of(stream), view(dst), view(src), a);
   return launched("pga_copy_box");
+}
+
+extern "C" int32_t pga_copy_slabs(int32_t ntask, const pga_view *dst, const pga_view *src, const int32_t *dst_start,
+                                  const int32_t *src_start, const int32_t *extent, int32_t ncomp, const int32_t *orient,
+                                  const double *rot, int32_t vec, int32_t rows, void *stream) {
+  if (ntask < 0 || ntask > 6 || ncomp <= 0 || (ntask && (!dst || !src || !dst_start || !src_start || !extent))) {
+    set_error("pga_copy_slabs: 0 to 6 slabs with their views, starts and extents, and components");
+    return PGA_EINVAL;
+  }
+  SlabsArgs a{};
+  a.ncomp = ncomp, a.vec = vec, a.rows = rows;
+  int64_t total = 0;
+  for (int n = 0; n < ntask; n++) {
+    const int32_t *ds = dst_start + 3 * n, *ss = src_start + 3 * n, *ex = extent + 3 * n;
+    for (int m = 0; m < 3; m++)
+      PGA_REQUIRE(ds[m] >= 0 && ss[m] >= 0 && ex[m] >= 0, "pga_copy_slabs: slab %d has a negative start or extent", n);
+    if (!ex[0] || !ex[1] || !ex[2]) continue;
+    const pga_box bd{{ds[0], ds[1], ds[2]}, {ex[0], ex[1], ex[2]}};
+    PGA_REQUIRE(covers(dst[n], bd, ncomp), "pga_copy_slabs: slab %d's destination reaches outside its array", n);
+    Placement pl;
+    if (!placement("pga_copy_slabs", src[n], ss, orient ? orient + 3 * n : nullptr, ex, ncomp, pl)) return PGA_ERANGE;
+    SlabCopy &sl = a.s[a.n++];
+    sl.dst = view(dst[n]), sl.src = view(src[n]);
+    sl.a = CopyArgs{ds[0], ds[1], ds[2], ex[0], ex[1], ex[2], ncomp, pl.base, pl.step[0], pl.step[1], pl.step[2]};
+    sl.t = turn_of(rot ? rot + 9 * n : nullptr);
+    if (sl.t.on) {
+      const int units = rows ? ncomp / 3 : ncomp;
+      PGA_REQUIRE((rows == 0 || rows == 1) && (!rows || ncomp % 3 == 0) && vec >= -1 && vec + 3 <= units,
+                  "pga_copy_slabs: a turned copy's vector (%d) must lie within its %d %s", vec, units,
+                  rows ? "rows" : "components");
+    }
+    sl.first = total;
+    sl.cells = (int64_t)ex[0] * ex[1] * ex[2];
+    total += sl.cells;
+  }
+  if (!total) return PGA_OK;
+  PGA_REQUIRE(total < FLAT_MAX, "pga_copy_slabs: %lld cells; at most 2^31", (long long)total);
+  hipLaunchKernelGGL(k_copy_slabs, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, stream_of(stream), a);
+  return launched("pga_copy_slabs");
 }

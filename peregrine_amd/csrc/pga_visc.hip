@@ -183,11 +183,23 @@ __device__ __forceinline__ void transport_cell(const Gas<NS> &g, const View &Q, 
   }
 }
 
+// the boxes of one transport call back to back, a cell per thread (i fastest in a box)
+struct BoxList {
+  Box3 b[8];
+  int64_t first[8];
+  int n;
+};
+
 template <int NS, int TRANS, int EOS>
-__global__ __launch_bounds__(256) void k_transport(const double *K, View Q, View st, View tr, View hsp, Box3 b,
-                                                   int binary) {
-  int i, j, k;
-  if (!cell_of(b, i, j, k)) return;
+__global__ __launch_bounds__(256) void k_transport(const double *K, View Q, View st, View tr, View hsp, BoxList L,
+                                                   int64_t total, int binary) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= total) return;
+  int n = 0;
+  while (n + 1 < L.n && c >= L.first[n + 1]) n++;
+  const Box3 &b = L.b[n];
+  const int64_t r = c - L.first[n];
+  const int i = b.s0 + (int)(r % b.e0), j = b.s1 + (int)((r / b.e0) % b.e1), k = b.s2 + (int)(r / ((int64_t)b.e0 * b.e1));
   const Gas<NS> g{K};
   transport_cell<NS, TRANS, EOS>(g, Q, st, Q.off(i, j, k), tr, hsp, tr.off(i, j, k), hsp.off(i, j, k), binary != 0);
 }
@@ -350,24 +362,33 @@ extern "C" int32_t pga_transport(const pga_case *c, pga_view Q, pga_view state, 
                     covers(hsp, boxes[n], d.ns),
                 "pga_transport: box %d reaches outside its arrays", n);
   }
+  // every box in one launch
+  BoxList L{};
+  int64_t total = 0;
   for (int n = 0; n < nboxes; n++) {
     if (empty(boxes[n])) continue;
-    const Box3 b = box3(boxes[n]);
-    const bool done = dispatch_eos(d, [&]<int N, int E>() {
-      if (d.trans == TRANS_CONSTANT)
-        hipLaunchKernelGGL((k_transport<N, TRANS_CONSTANT, E>), grid_of(b), dim3(TI, TJ), 0, stream_of(stream), c->K,
-                           view(Q), view(state), view(trstate), view(hsp), b, d.diffusion);
-      else if (d.trans == TRANS_KINETIC)
-        hipLaunchKernelGGL((k_transport<N, TRANS_KINETIC, E>), grid_of(b), dim3(TI, TJ), 0, stream_of(stream), c->K,
-                           view(Q), view(state), view(trstate), view(hsp), b, d.diffusion);
-      else if constexpr (E == EOS_REAL)
-        hipLaunchKernelGGL((k_transport<N, TRANS_CHUNG, E>), grid_of(b), dim3(TI, TJ), 0, stream_of(stream), c->K,
-                           view(Q), view(state), view(trstate), view(hsp), b, d.diffusion);
-    });
-    if (!done) {
-      set_error("pga_transport: no kernel for %d species with eos %d", d.ns, d.eos);
-      return PGA_EUNSUP;
-    }
+    L.b[L.n] = box3(boxes[n]);
+    L.first[L.n] = total;
+    total += (int64_t)boxes[n].extent[0] * boxes[n].extent[1] * boxes[n].extent[2];
+    L.n++;
+  }
+  if (!total) return PGA_OK;
+  PGA_REQUIRE(total < FLAT_MAX, "pga_transport: %lld cells; at most 2^31", (long long)total);
+  const dim3 grid((unsigned)((total + 255) / 256));
+  const bool done = dispatch_eos(d, [&]<int N, int E>() {
+    if (d.trans == TRANS_CONSTANT)
+      hipLaunchKernelGGL((k_transport<N, TRANS_CONSTANT, E>), grid, dim3(256), 0, stream_of(stream), c->K, view(Q),
+                         view(state), view(trstate), view(hsp), L, total, d.diffusion);
+    else if (d.trans == TRANS_KINETIC)
+      hipLaunchKernelGGL((k_transport<N, TRANS_KINETIC, E>), grid, dim3(256), 0, stream_of(stream), c->K, view(Q),
+                         view(state), view(trstate), view(hsp), L, total, d.diffusion);
+    else if constexpr (E == EOS_REAL)
+      hipLaunchKernelGGL((k_transport<N, TRANS_CHUNG, E>), grid, dim3(256), 0, stream_of(stream), c->K, view(Q),
+                         view(state), view(trstate), view(hsp), L, total, d.diffusion);
+  });
+  if (!done) {
+    set_error("pga_transport: no kernel for %d species with eos %d", d.ns, d.eos);
+    return PGA_EUNSUP;
   }
   return launched("pga_transport");
 }

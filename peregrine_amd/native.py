@@ -90,6 +90,8 @@ SIGNATURES = {
                            C.c_void_p]),
     "pga_fill_periodic": (_i32, [View, _i32, _ip, _i32, _i32, _i32, C.c_void_p]),
     "pga_copy_box": (_i32, [View, _ip, View, _ip, _ip, _i32, _ip, _dp, _i32, _i32, C.c_void_p]),
+    "pga_copy_slabs": (_i32, [_i32, C.POINTER(View), C.POINTER(View), _ip, _ip, _ip, _i32, _ip, _dp, _i32, _i32,
+                              C.c_void_p]),
     "pga_stage_supported": (_i32, [C.c_void_p]),
     "pga_stage": (_i32, [C.c_void_p, View, View, View, View, View, View, View, View, View, View, _i32, _ip, _dp, _ip,
                          C.c_void_p, Box, _i32, C.c_void_p]),

@@ -434,14 +434,25 @@ class Solver:
         self._exchange(lambda b: b.regs[reg], self.sim.ne, self.ng, 1, 0)
 
     def _exchange(self, array, ncomp, depth, vec, rows):
+        """The halo slabs of `array` across every interface, a block's slabs in one launch."""
+        eye = np.eye(3).ravel()
         for blk in self.blocks:
+            slabs = []
             for f in blk.faces.values():
                 if f.nbr is None:
                     continue
                 nbr = self.blocks[f.nbr]
                 dst, src, ext, orient = placement(blk, f, nbr, depth)
-                self._call("pga_copy_box", array(blk).view(), native.ints(dst), array(nbr).view(),
-                           native.ints(src), native.ints(ext), ncomp, native.ints(orient), turn_of(f), vec, rows)
+                rot = f.rot.ravel() if f.rotperiodic else eye
+                slabs.append((array(blk).view(), array(nbr).view(), dst, src, ext, orient, rot))
+            if not slabs:
+                continue
+            n = len(slabs)
+            dv = (native.View * n)(*[t[0] for t in slabs])
+            sv = (native.View * n)(*[t[1] for t in slabs])
+            dst, src, ext, orient = (native.ints([v for t in slabs for v in t[m]]) for m in (2, 3, 4, 5))
+            rots = native.dbls([v for t in slabs for v in t[6]])
+            self._call("pga_copy_slabs", n, dv, sv, dst, src, ext, ncomp, orient, rots, vec, rows)
 
     def transport(self, Q):
         """The transport state and species enthalpies over the interior and the first halo layer
