@@ -67,7 +67,7 @@ def algorithmic_bytes(name, ne, ng):
     if name == "pga_rates":
         # per cell: Q and T in, the carried species' rates out
         return 8 * (ne + 1 + ne - 5)
-    if name == "pga_flux":
+    if name in ("pga_flux", "pga_faces"):
         # per face: Q and p, T, rhoinv of one cell (the stencil's others are neighbours' reads), S, F out
         return 8 * (ne + 3 + 3 + ne)
     if name == "pga_source":
@@ -439,11 +439,11 @@ def main():
     count, ms = times[dom]
     ne = mb.sim.ne
     # items per launch: a stage's faces (three axes) or cells over the launches a stage makes
-    per_stage = {"pga_flux": 3 * (n + 1) * n * n}.get(dom, n**3)
+    per_stage = {"pga_flux": 3 * (n + 1) * n * n, "pga_faces": 3 * (n + 1) * n * n}.get(dom, n**3)
     items = per_stage * nst * args.steps / count
     per = algorithmic_bytes(dom, ne, mb.ng)
     achieved = per * items / (ms * 1e-3) / 1e9
-    kname = {"pga_flux": "k_flux", "pga_stage": "k_stage"}.get(dom, dom)
+    kname = {"pga_flux": "k_flux", "pga_faces": "k_faces", "pga_stage": "k_stage"}.get(dom, dom)
     # the flux sweep is bound by FP64 VALU issue and latency (3 waves per SIMD), not by HBM: its
     # bytes against HBM and its flops against FP64 are both reported, and the bound names the one
     # whose peak it is nearer
@@ -451,7 +451,7 @@ def main():
                 traffic=measured_traffic(kname), traffic_source="profiles/traffic.json: rocprofv3 FETCH_SIZE and "
                 "WRITE_SIZE passes of this benchmark, scaled by a plain copy's readings",
                 kernel=dom, kernel_ms=round(ms, 4), launches=count, bytes_per_item=per, items_per_launch=items)
-    if dom == "pga_flux":
+    if dom in ("pga_flux", "pga_faces"):
         tf = FLUX_FLOP_PER_FACE * items / (ms * 1e-3) / 1e12
         roof["fp64"] = dict(achieved=round(tf, 2), peak=FP64_PEAK, unit="TFLOP/s", frac=round(tf / FP64_PEAK, 4),
                             flop_per_item=FLUX_FLOP_PER_FACE)

@@ -107,6 +107,16 @@ int32_t pga_fromprims(const pga_case *c, pga_view prims, pga_view Q, pga_view st
 int32_t pga_flux(const pga_case *c, int32_t axis, pga_view Q, pga_view state, pga_view S, pga_view F,
                  pga_box box, void *stream);
 
+/* Every face of a block in one launch: for each axis a (0 i, 1 j, 2 k), the faces boxes[a] of S[a]
+ * and F[a], each face's advective flux as pga_flux gives it and, when viscous, the alpha-damped viscous
+ * flux added as pga_diffflux adds it (alpha, trstate, hsp, grads, xcc and the face centres xf[a]), so
+ * F[a] holds what pga_flux then pga_diffflux leave, to the bit. Replaces, for a stage, the reference's
+ * three pg_flux launches (euler/kernels/flux/single.mako:4-9) and the three viscous ones
+ * (navstokes/kernels/flux/alpha-damping.mako:3-99). */
+int32_t pga_faces(const pga_case *c, pga_view Q, pga_view state, const pga_view *S, const pga_view *F,
+                  const pga_box *boxes, int32_t viscous, double alpha, pga_view trstate, pga_view hsp, pga_view grads,
+                  pga_view xcc, const pga_view *xf, void *stream);
+
 /* dQ (= or +=) (iF_low + jF_low + kF_low - iF_high - jF_high - kF_high) * detJinv over the cells in box.
  * Replaces pg_applyflux: peregrine/simulator/advec/kernels/applyflux.mako:2-11. */
 int32_t pga_applyflux(int32_t ne, pga_view dQ, pga_view iF, pga_view jF, pga_view kF, pga_view detJinv,

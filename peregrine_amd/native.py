@@ -83,6 +83,8 @@ SIGNATURES = {
     "pga_fromQ": (_i32, [C.c_void_p, View, View, C.POINTER(Box), _i32, C.c_void_p]),
     "pga_fromprims": (_i32, [C.c_void_p, View, View, View, Box, C.c_void_p]),
     "pga_flux": (_i32, [C.c_void_p, _i32, View, View, View, View, Box, C.c_void_p]),
+    "pga_faces": (_i32, [C.c_void_p, View, View, C.POINTER(View), C.POINTER(View), C.POINTER(Box), _i32, C.c_double,
+                         View, View, View, View, C.POINTER(View), C.c_void_p]),
     "pga_applyflux": (_i32, [_i32, View, View, View, View, View, _i32, Box, C.c_void_p]),
     "pga_source": (_i32, [C.c_void_p, View, View, View, Box, C.c_void_p]),
     "pga_production": (_i32, [C.c_void_p, _i32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),

@@ -474,13 +474,15 @@ class Solver:
         # the rows u, v, w, T, Y: each a gradient, the first three a velocity gradient
         self._exchange(lambda b: b.grads, 3 * self.sim.ngrad, 1, 0, 1)
 
-    def fluxes(self, blk, Qv, st, slab=None, axes=(0, 1, 2)):
+    def fluxes(self, blk, Qv, st, slab=None, axes=(0, 1, 2), one_launch=False):
         """The advective flux of each axis, with the viscous flux added for a viscous case, then the
         walls' own flux through their faces (the postflux hook). A slab (k0, k1, f0, f1) limits the
-        i and j faces to the cells k0 <= k < k1 and the k faces to f0 <= k <= f1."""
-        for a, (S, F) in enumerate(((blk.iS, blk.iF), (blk.jS, blk.jF), (blk.kS, blk.kF))):
-            if a not in axes:
-                continue
+        i and j faces to the cells k0 <= k < k1 and the k faces to f0 <= k <= f1. one_launch: every
+        axis's faces, advective and viscous, in one launch (pga_faces, the same bits), else a launch
+        per axis and kind as the reference sequences them."""
+        sides = ((blk.iS, blk.iF), (blk.jS, blk.jF), (blk.kS, blk.kF))
+        boxes = []
+        for a in range(3):
             box = blk.face_box(a)
             if slab is not None:
                 k0, k1, f0, f1 = slab
@@ -488,11 +490,27 @@ class Solver:
                 start, ext = list(box.start), list(box.extent)
                 start[2], ext[2] = self.ng + lo, max(0, hi - lo)
                 box = native.box(start, ext)
-            self._call("pga_flux", self.case.handle, a, Qv, st, S.view(), F.view(), box)
+            boxes.append(box)
+        if one_launch and tuple(axes) == (0, 1, 2):
+            S = (native.View * 3)(*[sd[0].view() for sd in sides])
+            F = (native.View * 3)(*[sd[1].view() for sd in sides])
+            B = (native.Box * 3)(*boxes)
             if self.sim.viscous:
-                self._call("pga_diffflux", self.case.handle, a, self.sim.alpha, Qv, st, blk.trstate.view(),
-                           blk.hsp.view(), blk.grads.view(), blk.xcc.view(), blk.xf[a].view(), S.view(), F.view(),
-                           box)
+                xf = (native.View * 3)(*[blk.xf[a].view() for a in range(3)])
+                self._call("pga_faces", self.case.handle, Qv, st, S, F, B, 1, self.sim.alpha, blk.trstate.view(),
+                           blk.hsp.view(), blk.grads.view(), blk.xcc.view(), xf)
+            else:
+                self._call("pga_faces", self.case.handle, Qv, st, S, F, B, 0, 0.0, native.View(), native.View(),
+                           native.View(), native.View(), None)
+        else:
+            for a, (S, F) in enumerate(sides):
+                if a not in axes:
+                    continue
+                self._call("pga_flux", self.case.handle, a, Qv, st, S.view(), F.view(), boxes[a])
+                if self.sim.viscous:
+                    self._call("pga_diffflux", self.case.handle, a, self.sim.alpha, Qv, st, blk.trstate.view(),
+                               blk.hsp.view(), blk.grads.view(), blk.xcc.view(), blk.xf[a].view(), S.view(),
+                               F.view(), boxes[a])
         for axis, side, wall in blk.walls:
             S, F = (blk.iS, blk.jS, blk.kS)[axis], (blk.iF, blk.jF, blk.kF)[axis]
             visc = self.sim.viscous
@@ -848,7 +866,7 @@ class Solver:
                                    blk.detJinv.view(), rates, len(terms), src, w, withdt, self.dt.ptr, box, march)
                         continue
                     for slab in self._slabs(blk):
-                        self.fluxes(blk, Qin, st, slab)
+                        self.fluxes(blk, Qin, st, slab, one_launch=True)
                         k0, k1 = slab[:2]
                         sbox = native.box((self.ng, self.ng, self.ng + k0), (blk.ncell[0], blk.ncell[1], k1 - k0))
                         if first is None:
