@@ -32,6 +32,19 @@ __global__ __launch_bounds__(WT) void k_stage(StageArgs a) {
   stage_cells<NE, CHEM>(a, i, j, kbeg, min(kbeg + a.kc, a.s2 + a.n2));
 }
 
+// a small block's stage and the EOS of its result in one launch: each thread's cells after its
+// column's combine (a block of a few thousand cells is bound by launches, not by the registers the
+// two phases hold at once, which at 128^3 favour the two launches apart)
+template <int NS, int EOS, bool CHEM>
+__global__ __launch_bounds__(WT) void k_stage_whole(StageArgs a) {
+  const int i = a.s0 + blockIdx.x * WI + (threadIdx.x & (WI - 1));
+  const int j = a.s1 + blockIdx.y * WJ + threadIdx.x / WI;
+  if (i >= a.s0 + a.n0 || j >= a.s1 + a.n1) return;
+  const int kbeg = a.s2 + blockIdx.z * a.kc, kend = min(kbeg + a.kc, a.s2 + a.n2);
+  stage_cells<NS + 4, CHEM>(a, i, j, kbeg, kend);
+  for (int k = kbeg; k < kend; k++) eos_cell<NS, EOS>(a.K, a.Qout, a.stin, a.stout, i, j, k);
+}
+
 // the EOS of the stage's result, T started from the input state (stin may be stout)
 template <int NS, int EOS>
 __global__ __launch_bounds__(256) void k_stage_eos(const double *__restrict__ K, View Q, View stin, View stout, Box3 b) {
@@ -152,7 +165,16 @@ static int32_t stage_impl(const char *who, const pga_case *c, pga_view Qin, pga_
   a.kc = kchunk;
   const dim3 grid((a.n0 + WI - 1) / WI, (a.n1 + WJ - 1) / WJ, (a.n2 + kchunk - 1) / kchunk);
   const Box3 b = box3(box);
+  // a block of at most 2^16 cells (up to 40^3) in one launch, a larger one in two
+  const bool whole = (int64_t)a.n0 * a.n1 * a.n2 <= (int64_t(1) << 16);
   const bool done = dispatch_eos(d, [&]<int N, int E>() {
+    if (whole) {
+      if (chem)
+        hipLaunchKernelGGL((k_stage_whole<N, E, true>), grid, dim3(WT), 0, stream_of(stream), a);
+      else
+        hipLaunchKernelGGL((k_stage_whole<N, E, false>), grid, dim3(WT), 0, stream_of(stream), a);
+      return;
+    }
     if (chem)
       hipLaunchKernelGGL((k_stage<N + 4, true>), grid, dim3(WT), 0, stream_of(stream), a);
     else
