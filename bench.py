@@ -342,6 +342,88 @@ def tgv(args, pg, support):
     print(json.dumps(line), flush=True)
 
 
+def layer_case(support):
+    """The transcritical mixing layer: O2 at 120 K under N2 at 300 K across a tanh layer in y, 60 bar,
+    Peng-Robinson with Chung transport, MUSCL-Koren + Rusanov, Navier-Stokes, tvd-rk3; 10 um cells,
+    periodic in x and z, adiabatic slip walls at both y ends; a shear and seeded perturbations."""
+    cfg = support.config("o2n2-pr", recon="muscl", primary="rusanov", limiter="koren", trans="chung", dt=2e-9)
+    for f in (2, 3):
+        cfg["bcvals"][f"wall{f}"] = {"kind": "slp-adia-wall"}
+
+    def prims(xcc, ne, seed=7):
+        rng = np.random.default_rng(seed)
+        y = xcc[..., 1]
+        Ly = xcc[..., 1].max() + xcc[..., 1].min()
+        s = np.tanh((y - 0.5 * Ly) / (0.1 * Ly))
+        q = np.zeros(xcc.shape[:3] + (ne,))
+        q[..., 0] = 6e6
+        q[..., 1] = 10.0 * s + 0.5 * np.sin(2 * np.pi * xcc[..., 0] / Ly + rng.uniform(0, 6.3))
+        q[..., 2] = 0.5 * np.sin(2 * np.pi * xcc[..., 2] / Ly + rng.uniform(0, 6.3))
+        q[..., 4] = 210.0 - 90.0 * s
+        q[..., 5] = 0.5 * (1.0 - s) * 0.999 + 0.0005
+        return q
+
+    return cfg, prims
+
+
+def layer(args, pg, support):
+    """The transcritical mixing layer at 256 x 128 x 64: cell-updates/s per stage over the timed
+    steps (each replayed from its HIP graph), and the C oracle on the host's cores on a bounded
+    64 x 32 x 16 sample of the same case."""
+    import torch
+    from oracle import oracle as O
+    from peregrine_amd.simulator import get_simulator
+
+    ncell = (256, 128, 64)
+    lengths = tuple(1e-5 * n for n in ncell)
+    cfg, prims = layer_case(support)
+    mesh = pg.mesher.CubeMesher(blkdims=[n + 1 for n in ncell], lengths=lengths, periodic=[True, False, True],
+                                bcnames={3: "wall2", 4: "wall3"})
+    mb = pg.get_solver(cfg, mesh, pg.get_backend("mi355x", cfg))
+    blk = mb.blocks[0]
+    mb.set_prims([prims(blk.xcc.get(), mb.sim.ne)])
+    for _ in range(max(args.warmup, 2)):
+        mb.step(2e-9)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        mb.step(2e-9)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    cells = int(np.prod(ncell))
+    cpu = None
+    if not args.no_cpu:
+        small = (64, 32, 16)
+        sim = get_simulator(cfg)
+        oc = O.OracleCase(sim.mixture, "muscl", "koren", "rusanov")
+        hb = support.HostBlock(small, sim.ng, sim.ne, 3, lengths=tuple(1e-5 * n for n in small), nstate=9)
+        q = prims(hb.xcc, sim.ne)
+        O.lib.pgo_fromprims(oc.ref, O.arr(support.to_dev(q)), O.arr(hb.regs[0]), O.arr(hb.state),
+                            O.box((0, 0, 0), hb.cshape))
+        O.lib.pgo_fromQ(oc.ref, O.arr(hb.regs[0]), O.arr(hb.state), O.box((hb.ng,) * 3, hb.ncell))
+        b = support.oracle_block(hb, "tvd-rk3", 0, viscous=True, alpha=getattr(sim, "alpha", 1.0), walls=(2, 3))
+        times, start = [], time.perf_counter()
+        while time.perf_counter() - start < args.cpu_seconds or len(times) < 3:
+            t1 = time.perf_counter()
+            O.lib.pgo_step(oc.ref, ctypes.byref(b), 2e-9)
+            times.append(time.perf_counter() - t1)
+        med = float(np.median(times))
+        cpu = dict(value=int(np.prod(small)) * 3 / med, unit="cell-updates/s", cores=O.threads(), kind="port",
+                   cpu=cpu_model(), sample=f"64 x 32 x 16 block of the same case, median of {len(times)} tvd-rk3 steps, "
+                                           "C oracle with OpenMP")
+    line = {
+        "metric": "cell-updates/s per RK stage (FP64, transcritical O2/N2 layer)",
+        "value": cells * 3 * args.steps / elapsed, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": elapsed / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "o2/n2 mixing layer 256x128x64, 60 bar, peng-robinson, chung, muscl-koren + rusanov, "
+                               "navier-stokes, slip walls in y, tvd-rk3", "cells": cells,
+                   "path": "fused" if mb.fused else "per-kernel", "hip_graph_replays": mb.replays},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def cfg_stages(cfg):
     return {"rk1": 1, "rk2": 2, "tvd-rk3": 3, "rk4": 4, "rk34": 4}[cfg["time-integration"]["integrator"]]
 
@@ -368,6 +450,8 @@ def main():
                          "(seeded) stepped together, against the same cases stepped one by one")
     ap.add_argument("--serve", type=int, default=0,
                     help="the serving stream instead: this many requests arriving as a Poisson stream")
+    ap.add_argument("--layer", action="store_true",
+                    help="the transcritical O2/N2 mixing layer at 256x128x64 instead")
     ap.add_argument("--tgv", action="store_true",
                     help="the Taylor-Green 32^3 rk4 case instead, 100 steps, against the oracle's same steps on the host")
     ap.add_argument("--rate", type=float, default=400.0, help="the serving stream's arrivals per second")
@@ -400,6 +484,8 @@ def main():
         return serve(args, pg, support)
     if args.tgv:
         return tgv(args, pg, support)
+    if args.layer:
+        return layer(args, pg, support)
     cfg = batch_config(support) if args.implicit else case_config(support)
     cfg["backend-mi355x"]["fused-stage"] = not args.per_kernel
     if args.k_chunk:
