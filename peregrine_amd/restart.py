@@ -8,6 +8,12 @@ block (``results_<block>/<variable>``, the interior only, each variable an array
 as the reference's datasets are) and the attributes ``nrt``, ``tme``, ``privars``, ``variables``
 and ``config`` (the case's config as YAML). A result is a complete restart: the solution is all an
 integrator carries between steps.
+
+Two departures from the reference's files: a block's group is keyed by its number in this
+partition and its arrays are written in the block's current axes (the reference keys by the
+grid's original block and undoes a reorientation), so a result restarts the partition that wrote
+it, reoriented blocks included, and not another partition of the grid; and a carried species the
+file lacks starts at zero, where the reference takes the variable's default.
 """
 
 import numpy as np
