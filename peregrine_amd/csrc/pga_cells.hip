@@ -368,7 +368,9 @@ __global__ __launch_bounds__(256) void k_copy_turned(View dst, View src, CopyArg
 }
 
 // up to six slabs in one launch, a cell per thread over the slabs back to back (i fastest in a
-// slab), every component copied, or turned where the slab's turn is on
+// slab), every component copied, or turned where the slab's turn is on. Each slab begins on a
+// workgroup's boundary, so a workgroup's slab is a scalar choice read from the kernel arguments
+// once, not a per-lane one that would read them through vector loads
 struct SlabCopy {
   View dst, src;
   CopyArgs a;
@@ -382,20 +384,31 @@ struct SlabsArgs {
 };
 
 __global__ __launch_bounds__(256) void k_copy_slabs(SlabsArgs a) {
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t c0 = (int64_t)blockIdx.x * 256;
   int n = 0;
-  while (n + 1 < a.n && c >= a.s[n + 1].first) n++;
+  while (n + 1 < a.n && c0 >= a.s[n + 1].first) n++;
   const SlabCopy &sl = a.s[n];
-  const int64_t r = c - sl.first;
+  // a slab holds fewer than 2^31 cells
+  const unsigned r = (unsigned)(c0 - sl.first) + threadIdx.x;
   if (r >= sl.cells) return;
-  const int ii = (int)(r % sl.a.e0), jj = (int)((r / sl.a.e0) % sl.a.e1), kk = (int)(r / ((int64_t)sl.a.e0 * sl.a.e1));
+  const unsigned e0 = sl.a.e0, e1 = sl.a.e1;
+  const int ii = (int)(r % e0), jj = (int)((r / e0) % e1), kk = (int)(r / (e0 * e1));
   const int64_t os = sl.a.sbase + ii * sl.a.st0 + jj * sl.a.st1 + kk * sl.a.st2;
   const int64_t od = sl.dst.off(sl.a.d0 + ii, sl.a.d1 + jj, sl.a.d2 + kk);
   if (sl.t.on) {
     copy_turned_cell(sl.dst, sl.src, od, os, a.ncomp, sl.t, a.vec, a.rows);
     return;
   }
-  for (int comp = 0; comp < a.ncomp; comp++) sl.dst.at(od, comp) = sl.src.at(os, comp);
+  // eight components read before any is written, so a thread waits on memory once per eight
+  for (int c0 = 0; c0 < a.ncomp; c0 += 8) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++)
+      if (c0 + u < a.ncomp) v[u] = sl.src.at(os, c0 + u);
+#pragma unroll
+    for (int u = 0; u < 8; u++)
+      if (c0 + u < a.ncomp) sl.dst.at(od, c0 + u) = v[u];
+  }
 }
 }  // namespace pga
 
@@ -471,10 +484,10 @@ extern "C" int32_t pga_copy_slabs(int32_t ntask, const pga_view *dst, const pga_
     }
     sl.first = total;
     sl.cells = (int64_t)ex[0] * ex[1] * ex[2];
-    total += sl.cells;
+    total += (sl.cells + 255) / 256 * 256;
   }
   if (!total) return PGA_OK;
   PGA_REQUIRE(total < FLAT_MAX, "pga_copy_slabs: %lld cells; at most 2^31", (long long)total);
-  hipLaunchKernelGGL(k_copy_slabs, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, stream_of(stream), a);
+  hipLaunchKernelGGL(k_copy_slabs, dim3((unsigned)(total / 256)), dim3(256), 0, stream_of(stream), a);
   return launched("pga_copy_slabs");
 }

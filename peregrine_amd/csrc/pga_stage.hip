@@ -41,8 +41,22 @@ __global__ __launch_bounds__(WT) void k_stage_whole(StageArgs a) {
   const int j = a.s1 + blockIdx.y * WJ + threadIdx.x / WI;
   if (i >= a.s0 + a.n0 || j >= a.s1 + a.n1) return;
   const int kbeg = a.s2 + blockIdx.z * a.kc, kend = min(kbeg + a.kc, a.s2 + a.n2);
-  stage_cells<NS + 4, CHEM>(a, i, j, kbeg, kend);
-  for (int k = kbeg; k < kend; k++) eos_cell<NS, EOS>(a.K, a.Qout, a.stin, a.stout, i, j, k);
+  if constexpr (NS + 4 <= 8) {
+    // a few rows: each cell's reads at once, its EOS from the new Q in registers
+    const Gas<NS> g{a.K};
+    const double dt = *a.dt;
+    for (int k = kbeg; k < kend; k++) {
+      const double Tguess = a.stin.at(a.stin.off(i, j, k), 1);
+      double Qn[NS + 4];
+      stage_cell_regs<NS + 4, CHEM>(a, i, j, k, dt, Qn);
+      EosState s;
+      fromQ_regs<NS, EOS>(g, Qn, Tguess, s);
+      store_state<EOS>(a.stout, a.stout.off(i, j, k), s);
+    }
+  } else {
+    stage_cells<NS + 4, CHEM>(a, i, j, kbeg, kend);
+    for (int k = kbeg; k < kend; k++) eos_cell<NS, EOS>(a.K, a.Qout, a.stin, a.stout, i, j, k);
+  }
 }
 
 // the EOS of the stage's result, T started from the input state (stin may be stout)
@@ -80,16 +94,19 @@ struct HaloArgs {
   HaloTask t[6];
 };
 
-// one thread per halo cell over the slabs back to back, i fastest within a slab
+// one thread per halo cell over the slabs back to back, i fastest within a slab; each slab begins on
+// a workgroup's boundary, so a workgroup's slab is a scalar choice
 template <int NS, int EOS>
 __global__ __launch_bounds__(256) void k_halo(HaloArgs a) {
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t c0 = (int64_t)blockIdx.x * 256;
   int n = 0;
-  while (n + 1 < a.ntask && c >= a.t[n + 1].first) n++;
+  while (n + 1 < a.ntask && c0 >= a.t[n + 1].first) n++;
   const HaloTask &t = a.t[n];
-  const int64_t r = c - t.first;
+  // a slab holds fewer than 2^31 cells
+  const unsigned r = (unsigned)(c0 - t.first) + threadIdx.x;
   if (r >= t.cells) return;
-  const int ii = (int)(r % t.e0), jj = (int)((r / t.e0) % t.e1), kk = (int)(r / ((int64_t)t.e0 * t.e1));
+  const unsigned e0 = t.e0, e1 = t.e1;
+  const int ii = (int)(r % e0), jj = (int)((r / e0) % e1), kk = (int)(r / (e0 * e1));
   const int64_t os = t.sbase + ii * t.st0 + jj * t.st1 + kk * t.st2;
   halo_cell<NS, EOS>(a.K, t.Qdst, t.Qsrc, os, t.turn, a.stdst, a.stguess, t.d0 + ii, t.d1 + jj, t.d2 + kk);
 }
@@ -273,11 +290,12 @@ extern "C" int32_t pga_halo(const pga_case *c, int32_t ntask, const pga_view *Qd
     t.e0 = e[0], t.e1 = e[1];
     t.first = total;
     t.cells = (int64_t)e[0] * e[1] * e[2];
-    total += t.cells;
+    PGA_REQUIRE(t.cells < FLAT_MAX, "pga_halo: slab %d has %lld cells; at most 2^31", n, (long long)t.cells);
+    total += (t.cells + 255) / 256 * 256;
   }
   a.ntask = ntask;
   if (!total) return PGA_OK;
-  const int64_t nblk = (total + 255) / 256;
+  const int64_t nblk = total / 256;
   PGA_REQUIRE(nblk <= 0x7fffffff, "pga_halo: too many halo cells");
   const dim3 grid((unsigned)nblk);
   const bool done = dispatch_eos(c->desc, [&]<int N, int E>() {

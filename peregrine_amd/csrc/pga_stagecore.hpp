@@ -54,6 +54,38 @@ __device__ __forceinline__ void fromQ_regs(const Gas<NS> &g, const double (&Q)[N
   fromrhoE<NS, EOS>(g, m, [&](int n) { return Y[n]; }, rho, e, Tguess, s);
 }
 
+// The RK combines of row l from its derivative d (the faces' sum times detJinv, the source added):
+// the first combine into va (its own register's value `own` as it stood) where the stage has two,
+// then the combine returned, each in the reference's order of terms
+// (integrators/kernels/combine.mako:1-13)
+__device__ __forceinline__ double row_combine(const StageArgs &a, double d, double qin, double q0, double own,
+                                              double dt, double &va) {
+#pragma clang fp contract(off)
+  va = 0.0;
+  if (a.two) {
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+      if (t < a.combA.n) {
+        const int src = a.combA.src[t];
+        const double x = src == SRC_QIN ? qin : src == SRC_Q0 ? q0 : src == SRC_OWNA ? own : d;
+        const double term = a.combA.withdt[t] ? a.combA.w[t] * dt * x : a.combA.w[t] * x;
+        va = t == 0 ? term : va + term;
+      }
+    }
+  }
+  double acc = 0.0;
+#pragma unroll
+  for (int t = 0; t < 4; t++) {
+    if (t < a.comb.n) {
+      const int src = a.comb.src[t];
+      const double x = src == SRC_QIN ? qin : src == SRC_Q0 ? q0 : src == SRC_A ? va : d;
+      const double term = a.comb.withdt[t] ? a.comb.w[t] * dt * x : a.comb.w[t] * x;
+      acc = t == 0 ? term : acc + term;
+    }
+  }
+  return acc;
+}
+
 // Row l of one cell's stage: its six faces in the reference's order of sums (the k faces given), times
 // detJinv, the source first where there is one (chemint/kernels/source.mako:3-10), then the RK combine
 // in the reference's order of terms (integrators/kernels/combine.mako:1-13). What only this cell reads
@@ -74,33 +106,55 @@ __device__ __forceinline__ void stage_row(const StageArgs &a, int l, double kfl,
   // the terms' values are selected as values: a select of pointers would put them in memory
   const double qin = __builtin_nontemporal_load(&a.Qin.at(o, l));
   const double q0 = a.q0 ? __builtin_nontemporal_load(&a.Q0.at(o0, l)) : 0.0;
-  double va = 0.0;
+  double va;
   if (a.two) {
     // the first combine, its own register read as it stood, written before the second reads it
     const int64_t oa = a.QA.off(i, j, k);
-    const double own = a.QA.at(oa, l);
-#pragma unroll
-    for (int t = 0; t < 4; t++) {
-      if (t < a.combA.n) {
-        const int src = a.combA.src[t];
-        const double x = src == SRC_QIN ? qin : src == SRC_Q0 ? q0 : src == SRC_OWNA ? own : d;
-        const double term = a.combA.withdt[t] ? a.combA.w[t] * dt * x : a.combA.w[t] * x;
-        va = t == 0 ? term : va + term;
-      }
-    }
+    const double acc = row_combine(a, d, qin, q0, a.QA.at(oa, l), dt, va);
     a.QA.at(oa, l) = va;
+    __builtin_nontemporal_store(acc, &a.Qout.at(oo, l));
+    return;
   }
-  double acc = 0.0;
+  __builtin_nontemporal_store(row_combine(a, d, qin, q0, 0.0, dt, va), &a.Qout.at(oo, l));
+}
+
+// One cell's stage with everything it reads loaded before anything is written, its new Q returned in
+// Qn: for a few rows (NE <= 8), where the rows' loads fit registers, so a thread waits on memory once
+// a cell rather than once a row. stage_row's arithmetic to the bit; the caller also loads stin's T
+// with the rest
+template <int NE, bool CHEM>
+__device__ __forceinline__ void stage_cell_regs(const StageArgs &a, int i, int j, int k, double dt,
+                                                double (&Qn)[NE]) {
+#pragma clang fp contract(off)
+  static_assert(NE <= 8, "stage_cell_regs holds every row of a cell in registers");
+  const View &iF = a.iF, &jF = a.jF, &kF = a.kF;
+  const int64_t o = a.Qin.off(i, j, k), o0 = a.Q0.off(i, j, k), oo = a.Qout.off(i, j, k);
+  const int64_t fil = iF.off(i, j, k), fjl = jF.off(i, j, k), fkl = kF.off(i, j, k);
+  const int64_t fih = fil + 1, fjh = fjl + jF.sj, fkh = fkl + kF.sk;
+  const int64_t orr = CHEM ? a.rates.off(i, j, k) : 0, oa = a.two ? a.QA.off(i, j, k) : 0;
+  const double J = __builtin_nontemporal_load(&a.det.at(a.det.off(i, j, k), 0));
+  double d[NE], qin[NE], q0[NE], own[NE];
 #pragma unroll
-  for (int t = 0; t < 4; t++) {
-    if (t < a.comb.n) {
-      const int src = a.comb.src[t];
-      const double x = src == SRC_QIN ? qin : src == SRC_Q0 ? q0 : src == SRC_A ? va : d;
-      const double term = a.comb.withdt[t] ? a.comb.w[t] * dt * x : a.comb.w[t] * x;
-      acc = t == 0 ? term : acc + term;
-    }
+  for (int l = 0; l < NE; l++) {
+    d[l] = ((((iF.at(fil, l) + jF.at(fjl, l)) + kF.at(fkl, l)) - iF.at(fih, l)) - jF.at(fjh, l)) - kF.at(fkh, l);
+    qin[l] = __builtin_nontemporal_load(&a.Qin.at(o, l));
+    q0[l] = a.q0 ? __builtin_nontemporal_load(&a.Q0.at(o0, l)) : 0.0;
+    own[l] = a.two ? a.QA.at(oa, l) : 0.0;
   }
-  __builtin_nontemporal_store(acc, &a.Qout.at(oo, l));
+  double rate[NE];
+#pragma unroll
+  for (int l = 0; l < NE; l++) rate[l] = CHEM && l >= 5 ? __builtin_nontemporal_load(&a.rates.at(orr, l - 5)) : 0.0;
+#pragma unroll
+  for (int l = 0; l < NE; l++) {
+    double dl = d[l] * J;
+    if constexpr (CHEM) {
+      if (l >= 5) dl = rate[l] + dl;
+    }
+    double va;
+    Qn[l] = row_combine(a, dl, qin[l], q0[l], own[l], dt, va);
+    if (a.two) a.QA.at(oa, l) = va;
+    __builtin_nontemporal_store(Qn[l], &a.Qout.at(oo, l));
+  }
 }
 
 // The divergence, the source and the RK combine of the cells (i, j, kbeg..kend-1) of one column:
@@ -198,10 +252,12 @@ template <int NS, int EOS>
 __device__ __forceinline__ void halo_cell(const double *__restrict__ K, const View &Qdst, const View &Qsrc, int64_t os,
                                           const Turn &turn, const View &stdst, const View &stguess, int i, int j,
                                           int k) {
-  const int64_t od = Qdst.off(i, j, k);
+  const int64_t od = Qdst.off(i, j, k), ost = stdst.off(i, j, k);
   double Q[NS + 4];
 #pragma unroll
   for (int l = 0; l < NS + 4; l++) Q[l] = Qsrc.at(os, l);
+  // the T guess read with Q, before the stores, so the thread waits on memory once
+  const double Tguess = stguess.at(ost, 1);
   if (turn.on) {
     const double x = Q[1], y = Q[2], z = Q[3];
     turned(turn.R, x, y, z, Q[1], Q[2], Q[3]);
@@ -209,9 +265,8 @@ __device__ __forceinline__ void halo_cell(const double *__restrict__ K, const Vi
 #pragma unroll
   for (int l = 0; l < NS + 4; l++) Qdst.at(od, l) = Q[l];
   const Gas<NS> g{K};
-  const int64_t ost = stdst.off(i, j, k);
   EosState s;
-  fromQ_regs<NS, EOS>(g, Q, stguess.at(ost, 1), s);
+  fromQ_regs<NS, EOS>(g, Q, Tguess, s);
   store_state<EOS>(stdst, ost, s);
 }
 

@@ -194,13 +194,16 @@ struct BoxList {
 template <int NS, int TRANS, int EOS>
 __global__ __launch_bounds__(256) void k_transport(const double *K, View Q, View st, View tr, View hsp, BoxList L,
                                                    int64_t total, int binary) {
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= total) return;
+  // each box begins on a workgroup's boundary, so a workgroup's box is a scalar choice
+  const int64_t c0 = (int64_t)blockIdx.x * 256;
   int n = 0;
-  while (n + 1 < L.n && c >= L.first[n + 1]) n++;
+  while (n + 1 < L.n && c0 >= L.first[n + 1]) n++;
   const Box3 &b = L.b[n];
-  const int64_t r = c - L.first[n];
-  const int i = b.s0 + (int)(r % b.e0), j = b.s1 + (int)((r / b.e0) % b.e1), k = b.s2 + (int)(r / ((int64_t)b.e0 * b.e1));
+  // fewer than 2^31 cells in all
+  const unsigned r = (unsigned)(c0 - L.first[n]) + threadIdx.x;
+  const unsigned e0 = b.e0, e1 = b.e1;
+  if (r >= e0 * e1 * (unsigned)b.e2) return;
+  const int i = b.s0 + (int)(r % e0), j = b.s1 + (int)((r / e0) % e1), k = b.s2 + (int)(r / (e0 * e1));
   const Gas<NS> g{K};
   transport_cell<NS, TRANS, EOS>(g, Q, st, Q.off(i, j, k), tr, hsp, tr.off(i, j, k), hsp.off(i, j, k), binary != 0);
 }
@@ -227,21 +230,25 @@ __global__ __launch_bounds__(256) void k_grads(View Q, View st, View Jinv, View 
   for (int a = 0; a < 3; a++)
 #pragma unroll
     for (int e = 0; e < 3; e++) J[a][e] = Jinv.at(oj, 3 * a + e);
+  // every difference before the first store, so a thread waits on memory once
+  double dd[NGRAD][3];
 #pragma unroll
   for (int l = 0; l < NGRAD; l++) {
-    double dd[3];
 #pragma unroll
     for (int a = 0; a < 3; a++) {
       if (l == 3) {
-        dd[a] = 0.5 * (st.at(o + d[a], 1) - st.at(o - d[a], 1));
+        dd[l][a] = 0.5 * (st.at(o + d[a], 1) - st.at(o - d[a], 1));
       } else {
         const int m = l + 1;
-        dd[a] = 0.5 * (Q.at(o + d[a], m) * rm[a] - Q.at(o - d[a], m) * rp[a]) * inv[a];
+        dd[l][a] = 0.5 * (Q.at(o + d[a], m) * rm[a] - Q.at(o - d[a], m) * rp[a]) * inv[a];
       }
     }
-#pragma unroll
-    for (int e = 0; e < 3; e++) gr.at(og, 3 * l + e) = dd[0] * J[0][e] + dd[1] * J[1][e] + dd[2] * J[2][e];
   }
+#pragma unroll
+  for (int l = 0; l < NGRAD; l++)
+#pragma unroll
+    for (int e = 0; e < 3; e++)
+      gr.at(og, 3 * l + e) = dd[l][0] * J[0][e] + dd[l][1] * J[1][e] + dd[l][2] * J[2][e];
 }
 
 struct DiffArgs {
@@ -294,7 +301,7 @@ extern "C" int32_t pga_transport(const pga_case *c, pga_view Q, pga_view state, 
     if (empty(boxes[n])) continue;
     L.b[L.n] = box3(boxes[n]);
     L.first[L.n] = total;
-    total += (int64_t)boxes[n].extent[0] * boxes[n].extent[1] * boxes[n].extent[2];
+    total += ((int64_t)boxes[n].extent[0] * boxes[n].extent[1] * boxes[n].extent[2] + 255) / 256 * 256;
     L.n++;
   }
   if (!total) return PGA_OK;
