@@ -252,6 +252,16 @@ int32_t pga_viscsponge(pga_view xcc, pga_view trstate, const double *line, const
 int32_t pga_grads(const pga_case *c, pga_view Q, pga_view state, pga_view Jinv, pga_view grads, pga_box box,
                   void *stream);
 
+/* pga_transport over its boxes and pga_grads over gbox in one launch, the same values as the two
+ * calls (neither reads what the other writes; both read the halo pass's Q and state). For a step
+ * of small blocks, where a launch costs more than its work. An empty gbox computes no gradients.
+ * No counterpart of its own in the reference: the two kernels it replaces are
+ * peregrine/simulator/navstokes/kernels/transport.mako:5-35 and dqdxyz.mako:16-41, launched one
+ * after the other by peregrine/simulator/navstokes/simulator.py. */
+int32_t pga_transport_grads(const pga_case *c, pga_view Q, pga_view state, pga_view trstate, pga_view hsp,
+                            const pga_box *boxes, int32_t nboxes, pga_view Jinv, pga_view grads, pga_box gbox,
+                            void *stream);
+
 /* The alpha-damped viscous flux of the faces of one axis in box (face indices), added to F:
  * Newtonian stress, Fourier heat, Fick species fluxes with the correction velocity, their
  * enthalpy flux and the stress work; xcc the cell centres, xf the face centres.

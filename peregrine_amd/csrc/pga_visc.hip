@@ -191,30 +191,36 @@ struct BoxList {
   int n;
 };
 
-template <int NS, int TRANS, int EOS>
-__global__ __launch_bounds__(256) void k_transport(const double *K, View Q, View st, View tr, View hsp, BoxList L,
-                                                   int64_t total, int binary) {
-  // each box begins on a workgroup's boundary, so a workgroup's box is a scalar choice
-  const int64_t c0 = (int64_t)blockIdx.x * 256;
+// the transport of the cell the thread takes in a box list (a workgroup's box a scalar choice, each
+// box beginning on a workgroup's boundary); false past its box
+__device__ __forceinline__ bool boxlist_cell(const BoxList &L, unsigned blk, int &i, int &j, int &k) {
+  const int64_t c0 = (int64_t)blk * 256;
   int n = 0;
   while (n + 1 < L.n && c0 >= L.first[n + 1]) n++;
   const Box3 &b = L.b[n];
   // fewer than 2^31 cells in all
   const unsigned r = (unsigned)(c0 - L.first[n]) + threadIdx.x;
   const unsigned e0 = b.e0, e1 = b.e1;
-  if (r >= e0 * e1 * (unsigned)b.e2) return;
-  const int i = b.s0 + (int)(r % e0), j = b.s1 + (int)((r / e0) % e1), k = b.s2 + (int)(r / (e0 * e1));
+  if (r >= e0 * e1 * (unsigned)b.e2) return false;
+  i = b.s0 + (int)(r % e0), j = b.s1 + (int)((r / e0) % e1), k = b.s2 + (int)(r / (e0 * e1));
+  return true;
+}
+
+template <int NS, int TRANS, int EOS>
+__global__ __launch_bounds__(256) void k_transport(const double *K, View Q, View st, View tr, View hsp, BoxList L,
+                                                   int binary) {
+  int i, j, k;
+  if (!boxlist_cell(L, blockIdx.x, i, j, k)) return;
   const Gas<NS> g{K};
   transport_cell<NS, TRANS, EOS>(g, Q, st, Q.off(i, j, k), tr, hsp, tr.off(i, j, k), hsp.off(i, j, k), binary != 0);
 }
 
-// navstokes/kernels/dqdxyz.mako:16-41
+// navstokes/kernels/dqdxyz.mako:16-41: the cell's gradients of u, v, w, T and the carried Y
 template <int NS>
-__global__ __launch_bounds__(256) void k_grads(View Q, View st, View Jinv, View gr, Box3 b) {
+__device__ __forceinline__ void grads_cell(const View &Q, const View &st, const View &Jinv, const View &gr, int i, int j,
+                                           int k) {
 #pragma clang fp contract(off)
   constexpr int NGRAD = NS + 3;
-  int i, j, k;
-  if (!cell_of(b, i, j, k)) return;
   const int64_t o = Q.off(i, j, k);
   const int64_t d[3] = {1, Q.sj, Q.sk};
   double rp[3], rm[3], inv[3];
@@ -251,6 +257,30 @@ __global__ __launch_bounds__(256) void k_grads(View Q, View st, View Jinv, View 
       gr.at(og, 3 * l + e) = dd[l][0] * J[0][e] + dd[l][1] * J[1][e] + dd[l][2] * J[2][e];
 }
 
+template <int NS>
+__global__ __launch_bounds__(256) void k_grads(View Q, View st, View Jinv, View gr, Box3 b) {
+  int i, j, k;
+  if (!cell_of(b, i, j, k)) return;
+  grads_cell<NS>(Q, st, Jinv, gr, i, j, k);
+}
+
+// the transport over a box list and the gradients over a box in one launch: the transport's
+// workgroups first, then the gradients' (flat, i fastest); neither reads what the other writes
+template <int NS, int TRANS, int EOS>
+__global__ __launch_bounds__(256) void k_transport_grads(const double *K, View Q, View st, View tr, View hsp,
+                                                         BoxList L, unsigned tblocks, int binary, View Jinv, View gr,
+                                                         Box3 gb) {
+  int i, j, k;
+  if (blockIdx.x < tblocks) {
+    if (!boxlist_cell(L, blockIdx.x, i, j, k)) return;
+    const Gas<NS> g{K};
+    transport_cell<NS, TRANS, EOS>(g, Q, st, Q.off(i, j, k), tr, hsp, tr.off(i, j, k), hsp.off(i, j, k), binary != 0);
+    return;
+  }
+  if (!flat_at(gb, blockIdx.x - tblocks, i, j, k)) return;
+  grads_cell<NS>(Q, st, Jinv, gr, i, j, k);
+}
+
 struct DiffArgs {
   View Q, st, tr, hsp, gr, xcc, xf, S, F;
   double alpha;
@@ -276,27 +306,28 @@ __global__ __launch_bounds__(256) void k_diffflux(DiffArgs a) {
 
 using namespace pga;
 
-extern "C" int32_t pga_transport(const pga_case *c, pga_view Q, pga_view state, pga_view trstate, pga_view hsp,
-                                 const pga_box *boxes, int32_t nboxes, void *stream) {
+// the transport call's checks and its box list (each box on a workgroup's boundary); total the
+// list's padded cell count
+static int32_t transport_list(const char *who, const pga_case *c, pga_view Q, pga_view state, pga_view trstate,
+                              pga_view hsp, const pga_box *boxes, int32_t nboxes, BoxList &L, int64_t &total) {
   if (!c || nboxes < 0 || nboxes > 8 || (nboxes && !boxes)) {
-    set_error("pga_transport: null argument or more than 8 boxes");
+    set_error("%s: null argument or more than 8 boxes", who);
     return PGA_EINVAL;
   }
   const pga_case_desc &d = c->desc;
   if (d.trans < 0) {
-    set_error("pga_transport: the case has no transport model");
+    set_error("%s: the case has no transport model", who);
     return PGA_EUNSUP;
   }
   const int ne = d.ns + 4;
   for (int n = 0; n < nboxes; n++) {
-    PGA_REQUIRE(box_ok(boxes[n]), "pga_transport: box %d has a negative extent", n);
+    PGA_REQUIRE(box_ok(boxes[n]), "%s: box %d has a negative extent", who, n);
     PGA_REQUIRE(covers(Q, boxes[n], ne) && covers(state, boxes[n], nstate(d)) && covers(trstate, boxes[n], 2 + d.ns) &&
                     covers(hsp, boxes[n], d.ns),
-                "pga_transport: box %d reaches outside its arrays", n);
+                "%s: box %d reaches outside its arrays", who, n);
   }
-  // every box in one launch
-  BoxList L{};
-  int64_t total = 0;
+  L = BoxList{};
+  total = 0;
   for (int n = 0; n < nboxes; n++) {
     if (empty(boxes[n])) continue;
     L.b[L.n] = box3(boxes[n]);
@@ -304,25 +335,91 @@ extern "C" int32_t pga_transport(const pga_case *c, pga_view Q, pga_view state, 
     total += ((int64_t)boxes[n].extent[0] * boxes[n].extent[1] * boxes[n].extent[2] + 255) / 256 * 256;
     L.n++;
   }
-  if (!total) return PGA_OK;
-  PGA_REQUIRE(total < FLAT_MAX, "pga_transport: %lld cells; at most 2^31", (long long)total);
-  const dim3 grid((unsigned)((total + 255) / 256));
+  PGA_REQUIRE(total < FLAT_MAX, "%s: %lld cells; at most 2^31", who, (long long)total);
+  return PGA_OK;
+}
+
+extern "C" int32_t pga_transport(const pga_case *c, pga_view Q, pga_view state, pga_view trstate, pga_view hsp,
+                                 const pga_box *boxes, int32_t nboxes, void *stream) {
+  BoxList L;
+  int64_t total;
+  const int32_t rc = transport_list("pga_transport", c, Q, state, trstate, hsp, boxes, nboxes, L, total);
+  if (rc != PGA_OK || !total) return rc;
+  const pga_case_desc &d = c->desc;
+  // every box in one launch
+  const dim3 grid((unsigned)(total / 256));
   const bool done = dispatch_eos(d, [&]<int N, int E>() {
     if (d.trans == TRANS_CONSTANT)
       hipLaunchKernelGGL((k_transport<N, TRANS_CONSTANT, E>), grid, dim3(256), 0, stream_of(stream), c->K, view(Q),
-                         view(state), view(trstate), view(hsp), L, total, d.diffusion);
+                         view(state), view(trstate), view(hsp), L, d.diffusion);
     else if (d.trans == TRANS_KINETIC)
       hipLaunchKernelGGL((k_transport<N, TRANS_KINETIC, E>), grid, dim3(256), 0, stream_of(stream), c->K, view(Q),
-                         view(state), view(trstate), view(hsp), L, total, d.diffusion);
+                         view(state), view(trstate), view(hsp), L, d.diffusion);
     else if constexpr (E == EOS_REAL)
       hipLaunchKernelGGL((k_transport<N, TRANS_CHUNG, E>), grid, dim3(256), 0, stream_of(stream), c->K, view(Q),
-                         view(state), view(trstate), view(hsp), L, total, d.diffusion);
+                         view(state), view(trstate), view(hsp), L, d.diffusion);
   });
   if (!done) {
     set_error("pga_transport: no kernel for %d species with eos %d", d.ns, d.eos);
     return PGA_EUNSUP;
   }
   return launched("pga_transport");
+}
+
+// the gradient call's checks
+static int32_t grads_check(const char *who, const pga_case *c, pga_view Q, pga_view state, pga_view Jinv,
+                           pga_view grads, pga_box box) {
+  const pga_case_desc &d = c->desc;
+  const int ne = d.ns + 4;
+  const int one[3] = {1, 1, 1};
+  PGA_REQUIRE(covers(Q, box, one, one, ne) && covers(state, box, one, one, nstate(d)),
+              "%s: the stencil reaches outside Q or the state (one halo layer needed)", who);
+  PGA_REQUIRE(covers(Jinv, box, 9) && covers(grads, box, 3 * (ne - 1)), "%s: the box reaches outside Jinv or grads", who);
+  PGA_REQUIRE(Q.sj == state.sj && Q.sk == state.sk, "%s: Q and state must share their layout", who);
+  return PGA_OK;
+}
+
+extern "C" int32_t pga_transport_grads(const pga_case *c, pga_view Q, pga_view state, pga_view trstate, pga_view hsp,
+                                       const pga_box *boxes, int32_t nboxes, pga_view Jinv, pga_view grads,
+                                       pga_box gbox, void *stream) {
+  const char *who = "pga_transport_grads";
+  if (!box_ok(gbox)) {
+    set_error("%s: the gradients' box has a negative extent", who);
+    return PGA_EINVAL;
+  }
+  BoxList L;
+  int64_t total;
+  int32_t rc = transport_list(who, c, Q, state, trstate, hsp, boxes, nboxes, L, total);
+  if (rc != PGA_OK) return rc;
+  const bool grad = !empty(gbox);
+  if (grad && (rc = grads_check(who, c, Q, state, Jinv, grads, gbox)) != PGA_OK) return rc;
+  const int64_t gcells = grad ? (int64_t)gbox.extent[0] * gbox.extent[1] * gbox.extent[2] : 0;
+  PGA_REQUIRE(total + gcells < FLAT_MAX - FLAT, "%s: %lld cells in one launch; at most 2^31", who,
+              (long long)(total + gcells));
+  const unsigned tblocks = (unsigned)(total / 256), gblocks = (unsigned)((gcells + FLAT - 1) / FLAT);
+  if (!tblocks && !gblocks) return PGA_OK;
+  const pga_case_desc &d = c->desc;
+  const Box3 gb = grad ? box3(gbox) : Box3{};
+  const dim3 grid(tblocks + gblocks);
+  const bool done = dispatch_eos(d, [&]<int N, int E>() {
+    if (d.trans == TRANS_CONSTANT)
+      hipLaunchKernelGGL((k_transport_grads<N, TRANS_CONSTANT, E>), grid, dim3(256), 0, stream_of(stream), c->K,
+                         view(Q), view(state), view(trstate), view(hsp), L, tblocks, d.diffusion, view(Jinv),
+                         view(grads), gb);
+    else if (d.trans == TRANS_KINETIC)
+      hipLaunchKernelGGL((k_transport_grads<N, TRANS_KINETIC, E>), grid, dim3(256), 0, stream_of(stream), c->K,
+                         view(Q), view(state), view(trstate), view(hsp), L, tblocks, d.diffusion, view(Jinv),
+                         view(grads), gb);
+    else if constexpr (E == EOS_REAL)
+      hipLaunchKernelGGL((k_transport_grads<N, TRANS_CHUNG, E>), grid, dim3(256), 0, stream_of(stream), c->K,
+                         view(Q), view(state), view(trstate), view(hsp), L, tblocks, d.diffusion, view(Jinv),
+                         view(grads), gb);
+  });
+  if (!done) {
+    set_error("%s: no kernel for %d species with eos %d", who, d.ns, d.eos);
+    return PGA_EUNSUP;
+  }
+  return launched(who);
 }
 
 extern "C" int32_t pga_grads(const pga_case *c, pga_view Q, pga_view state, pga_view Jinv, pga_view grads,
@@ -333,12 +430,8 @@ extern "C" int32_t pga_grads(const pga_case *c, pga_view Q, pga_view state, pga_
   }
   if (empty(box)) return PGA_OK;
   const pga_case_desc &d = c->desc;
-  const int ne = d.ns + 4;
-  const int one[3] = {1, 1, 1};
-  PGA_REQUIRE(covers(Q, box, one, one, ne) && covers(state, box, one, one, nstate(d)),
-              "pga_grads: the stencil reaches outside Q or the state (one halo layer needed)");
-  PGA_REQUIRE(covers(Jinv, box, 9) && covers(grads, box, 3 * (ne - 1)), "pga_grads: the box reaches outside Jinv or grads");
-  PGA_REQUIRE(Q.sj == state.sj && Q.sk == state.sk, "pga_grads: Q and state must share their layout");
+  const int32_t rc = grads_check("pga_grads", c, Q, state, Jinv, grads, box);
+  if (rc != PGA_OK) return rc;
   const Box3 b = box3(box);
   bool done = false;
 #define X(N)                                                                                                    \

@@ -106,6 +106,8 @@ SIGNATURES = {
                         C.c_void_p]),
     "pga_transport": (_i32, [C.c_void_p, View, View, View, View, C.POINTER(Box), _i32, C.c_void_p]),
     "pga_grads": (_i32, [C.c_void_p, View, View, View, View, Box, C.c_void_p]),
+    "pga_transport_grads": (_i32, [C.c_void_p, View, View, View, View, C.POINTER(Box), _i32, View, View, Box,
+                                   C.c_void_p]),
     "pga_viscsponge": (_i32, [View, View, _dp, C.POINTER(Box), _i32, C.c_void_p]),
     "pga_diffflux": (_i32, [C.c_void_p, _i32, C.c_double, View, View, View, View, View, View, View, View, View, Box,
                             C.c_void_p]),

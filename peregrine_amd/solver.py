@@ -474,6 +474,20 @@ class Solver:
         # the rows u, v, w, T, Y: each a gradient, the first three a velocity gradient
         self._exchange(lambda b: b.grads, 3 * self.sim.ngrad, 1, 0, 1)
 
+    def transport_gradients(self, Q):
+        """transport() and gradients() with each block's two kernels in one launch
+        (pga_transport_grads, the same values): for the fused step, whose small blocks pay more for
+        a launch than for its work."""
+        for blk in self.blocks:
+            boxes = [blk.interior_box()] + blk.halo_boxes(1)
+            arr = (native.Box * len(boxes))(*boxes)
+            self._call("pga_transport_grads", self.case.handle, blk.regs[Q].view(), blk.state.view(),
+                       blk.trstate.view(), blk.hsp.view(), arr, len(boxes), blk.Jinv.view(), blk.grads.view(),
+                       blk.interior_box())
+            for sponge in self.sponges:
+                self._call("pga_viscsponge", blk.xcc.view(), blk.trstate.view(), sponge.line, arr, len(boxes))
+        self._exchange(lambda b: b.grads, 3 * self.sim.ngrad, 1, 0, 1)
+
     def fluxes(self, blk, Qv, st, slab=None, axes=(0, 1, 2), one_launch=False):
         """The advective flux of each axis, with the viscous flux added for a viscous case, then the
         walls' own flux through their faces (the postflux hook). A slab (k0, k1, f0, f1) limits the
@@ -835,10 +849,10 @@ class Solver:
                 self._halo_pass(regs.Q)
                 self.wall_halos(regs.Q)
                 if self.sim.viscous:
-                    self.transport(regs.Q)
-            self.readied.clear()
-            if self.sim.viscous:
+                    self.transport_gradients(regs.Q)
+            elif self.sim.viscous:
                 self.gradients(regs.Q)
+            self.readied.clear()
             combines = self._combine_terms(n)
             terms = combines[-1]
             src = native.ints([t[0] for t in terms])

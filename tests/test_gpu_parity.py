@@ -532,6 +532,16 @@ def test_transport_gradients_and_viscous_flux(dev, gas, trans, diffusion, bath):
     native.check(L.pga_grads(nc.handle, dev.view(dQv), dev.view(dSt), dev.view(dev.put(hb.Jinv)), dev.view(dG),
                              native.box((1, 1, 1), tuple(n - 2 for n in hb.cshape)), None), "pga_grads")
     assert np.array_equal(dev.get(dG), hb.grads)
+    # the two in one launch, the transport's cells as two boxes (k < 2 and the rest)
+    ni, nj, nk = hb.cshape
+    two = (native.Box * 2)(native.box((0, 0, 0), (ni, nj, 2)), native.box((0, 0, 2), (ni, nj, nk - 2)))
+    dTr2, dH2, dG2 = (dev.put(np.zeros_like(x)) for x in (hb.trstate, hb.hsp, hb.grads))
+    native.check(L.pga_transport_grads(nc.handle, dev.view(dQv), dev.view(dSt), dev.view(dTr2), dev.view(dH2), two, 2,
+                                       dev.view(dev.put(hb.Jinv)), dev.view(dG2),
+                                       native.box((1, 1, 1), (ni - 2, nj - 2, nk - 2)), None), "pga_transport_grads")
+    assert np.array_equal(dev.get(dTr2), hb.trstate)
+    assert np.array_equal(dev.get(dH2), hb.hsp)
+    assert np.array_equal(dev.get(dG2), hb.grads)
     dXc = dev.put(hb.xccd)
     for a in range(3):
         # faces whose two cells both hold gradients
