@@ -92,7 +92,7 @@ def tune(solver):
     src = native.ints([t[0] for t in terms])
     w = (ctypes.c_double * len(terms))(*[t[1] for t in terms])
     withdt = native.ints([t[2] for t in terms])
-    L = native.lib()
+    L = solver.case.L
 
     def measure(kind, trial):
         batch = solver._make_batch(trial, regs=(0, out), states=scratch)
@@ -100,9 +100,9 @@ def tune(solver):
             if kind == "stage":
                 return _time(solver, lambda: native.check(
                     L.pga_batch_stage(batch.handle, 0, 1, len(terms), src, w, withdt, solver.dt.ptr,
-                                      solver.backend.stream), "pga_batch_stage"))
+                                      solver.backend.stream), "pga_batch_stage", L))
             return _time(solver, lambda: native.check(L.pga_batch_flux(batch.handle, 0, 2, solver.backend.stream),
-                                                      "pga_batch_flux"))
+                                                      "pga_batch_flux", L))
         finally:
             batch.close()
 

@@ -74,7 +74,9 @@ _DEFAULTS = {
     # run in (a slab's face arrays stay in the 256 MB Infinity Cache between the flux sweeps and
     # the divergence that reads them), the side streams several blocks' stages overlap on when
     # launched eagerly, whether steps replay one captured HIP graph, and the march: the k sweep
-    # done inside the stage's launch, a thread walking k-march planes of its column (0: off)
+    # done inside the stage's launch, a thread walking k-march planes of its column (0: off),
+    # and jit: a case the main library is not compiled for (species count, mechanism structure)
+    # gets a library of its own, built on first use and cached (0: such a case is refused)
     # (the kernels' block shapes are compiled in: 64 lanes along i, waves along j or k)
     "backend-mi355x": {
         "device": 0,
@@ -87,6 +89,7 @@ _DEFAULTS = {
         "batched": "auto",
         "autotune": True,
         "autotune-cache": None,
+        "jit": True,
     },
     "halo-exchange": {"kind": "hoststaged"},
 }

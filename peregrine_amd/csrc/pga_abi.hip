@@ -23,12 +23,16 @@ struct MechInfo {
   int ns, table;
 };
 
-static const MechInfo mechs[] = {
+// the compiled mechanisms, after a sentinel so a library built for an inert case (none) still has
+// an array
+static const MechInfo mechs_[] = {
+    {"", "", 0, 0},
 #define X(NAME, IDX) {Mech_##NAME::NAME_, Mech_##NAME::SIGNATURE, Mech_##NAME::NS, Mech_##NAME::TABLE},
     PGA_MECHANISMS(X)
 #undef X
 };
-static const int nmechs = sizeof mechs / sizeof mechs[0];
+static const MechInfo *const mechs = mechs_ + 1;
+static const int nmechs = sizeof mechs_ / sizeof mechs_[0] - 1;
 
 }  // namespace pga
 
@@ -98,7 +102,8 @@ extern "C" int32_t pga_case_create(const pga_case_desc *desc, const double *dev_
     return PGA_EUNSUP;
   }
   if (desc->eos == 2 && !realgas_compiled(desc->ns)) {
-    set_error("pga_case_create: the real gas is compiled for 1 or 2 species, not %d", desc->ns);
+    set_error("pga_case_create: the real gas is not compiled for %d species here (a library built for the case "
+              "has it)", desc->ns);
     return PGA_EUNSUP;
   }
   if (desc->trans < -1 || desc->trans > 2 || (desc->trans == 2 && desc->eos != 2)) {

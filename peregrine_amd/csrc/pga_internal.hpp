@@ -132,14 +132,24 @@ inline Turn turn_of(const double *R) {
     }                               \
   } while (0)
 
+// a library built for one case (peregrine_amd/build.py, build_variant) names its own species counts
+// and real-gas counts in generated/variant.hpp; the main library's are these
+#if __has_include("generated/variant.hpp")
+#include "generated/variant.hpp"
+#endif
+#ifndef PGA_NS_CASES
 #define PGA_NS_CASES(X) X(1) X(2) X(3) X(9)
+#endif
+#ifndef PGA_REALGAS_CASES
+#define PGA_REALGAS_CASES(ns) ((ns) == 1 || (ns) == 2)
+#endif
 
 // doubles of a cell's cache of the implicit chemistry's f at one T for the case's mechanism, -1 for
 // none (pga_implicit.hip)
 int64_t im_cache_doubles(const pga_case_desc &d);
 
 // the species counts the cubic real gas is compiled for (its kernels are the heaviest to build)
-constexpr bool realgas_compiled(int ns) { return ns == 1 || ns == 2; }
+constexpr bool realgas_compiled(int ns) { return PGA_REALGAS_CASES(ns); }
 
 // the state fields a case stores: the real gas adds its compressibility Z
 inline int nstate(const pga_case_desc &d) { return d.eos == 2 ? 9 : 8; }

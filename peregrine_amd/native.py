@@ -65,6 +65,7 @@ class CaseDesc(C.Structure):
     ]
 
 
+PGA_EINVAL, PGA_EUNSUP, PGA_ELAUNCH, PGA_ERANGE = -1, -2, -3, -4
 ERRORS = {-1: "invalid argument", -2: "unsupported case", -3: "HIP launch failure", -4: "size out of range"}
 
 # name -> (restype, argtypes); every launching entry point takes the stream last
@@ -150,33 +151,44 @@ class NativeError(RuntimeError):
     pass
 
 
-_lib = None
+_libs = {}
 
 
 # the library interface these bindings are written against (PGA_ABI_VERSION)
 ABI_VERSION = 4
 
 
-def lib():
-    """The loaded library; raises when it is not built."""
-    global _lib
-    if _lib is None:
-        if not LIBPATH.is_file():
-            raise NativeError(f"{LIBPATH} is not built: run peregrine_amd.build (or __graft_entry__.build())")
-        handle = C.CDLL(str(LIBPATH))
+def _load(path):
+    path = Path(path)
+    if str(path) not in _libs:
+        if not path.is_file():
+            raise NativeError(f"{path} is not built: run peregrine_amd.build (or __graft_entry__.build())")
+        handle = C.CDLL(str(path))
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(handle, name)
             fn.restype, fn.argtypes = res, args
         if handle.pga_abi_version() != ABI_VERSION:
-            raise NativeError(f"{LIBPATH} has ABI version {handle.pga_abi_version()}, these bindings {ABI_VERSION}: "
+            raise NativeError(f"{path} has ABI version {handle.pga_abi_version()}, these bindings {ABI_VERSION}: "
                               "rebuild it")
-        _lib = handle
-    return _lib
+        _libs[str(path)] = handle
+    return _libs[str(path)]
 
 
-def check(rc, what):
+def lib():
+    """The main library, loaded; raises when it is not built."""
+    return _load(LIBPATH)
+
+
+def lib_at(path):
+    """A library built for one case (peregrine_amd.build.build_variant), loaded with the same
+    bindings as the main one."""
+    return _load(path)
+
+
+def check(rc, what, L=None):
+    """Raise on a failed call, with the message of the library that ran it (L, else the main one)."""
     if rc != 0:
-        msg = lib().pga_last_error().decode(errors="replace")
+        msg = (L or lib()).pga_last_error().decode(errors="replace")
         raise NativeError(f"{what}: {ERRORS.get(rc, rc)}: {msg}")
 
 
@@ -192,17 +204,17 @@ def dbls(values):
     return (C.c_double * len(values))(*values)
 
 
-def mechanisms():
-    L = lib()
+def mechanisms(L=None):
+    L = L or lib()
     return [
         (L.pga_mechanism_name(i).decode(), L.pga_mechanism_signature(i).decode(), L.pga_mechanism_table_size(i))
         for i in range(L.pga_mechanism_count())
     ]
 
 
-def const_layout(ns):
+def const_layout(ns, L=None):
     """{section: (offset, count)} of the species constant table, and its size."""
-    L = lib()
+    L = L or lib()
     names = ("MW", "MWinv", "Rsp", "cppolyR", "h298R", "cp0", "gpoly", "h298", "mu0", "kappa0", "lewis", "hpoly",
              "MWqinv", "wilkepair", "mupoly", "kappapoly", "cppoly", "cubic", "cubicsqrtac", "cubicb", "cubicda",
              "cubicmTc", "cubiconepm", "cubicsqrtacMW", "cubicbMW", "vshift", "vshiftMW", "cubicdaMW", "acentric",

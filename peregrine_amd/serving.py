@@ -108,7 +108,7 @@ class ReactorPool:
         self.rates = backend.alloc((capacity, 1, 1, ns - 1), name="reactor-rates")
         self.prims = backend.alloc((capacity, 1, 1, ne), name="reactor-prims")
         self.box = native.box((0, 0, 0), (capacity, 1, 1))
-        n = int(native.lib().pga_implicit_worksize(self.case.handle, self.box))
+        n = int(self.case.L.pga_implicit_worksize(self.case.handle, self.box))
         self.work = torch.zeros(max(n, 1), dtype=torch.float64, device=backend.device)
         self.sweeps = torch.zeros(capacity, dtype=torch.int32, device=backend.device)
         self.dt = backend.scalar("dt", dt)
@@ -122,8 +122,8 @@ class ReactorPool:
         inert = np.zeros(ne)
         inert[0], inert[4] = 101325.0, 300.0
         self.prims.t.copy_(torch.from_numpy(inert).to(self.prims.t.device)[:, None, None, None].expand_as(self.prims.t))
-        native.check(native.lib().pga_fromprims(self.case.handle, self.prims.view(), self.Q.view(), self.state.view(),
-                                                self.box, backend.stream), "pga_fromprims")
+        native.check(self.case.L.pga_fromprims(self.case.handle, self.prims.view(), self.Q.view(), self.state.view(),
+                                                self.box, backend.stream), "pga_fromprims", self.case.L)
 
     @staticmethod
     def footprint(capacity, ns=9):
@@ -138,8 +138,8 @@ class ReactorPool:
         q[0], q[4], q[5:] = p0, T0, Y[:-1]
         self.prims.t[:, 0, 0, s] = self.backend.torch.from_numpy(q).to(self.prims.t.device)
         b = native.box((s, 0, 0), (1, 1, 1))
-        native.check(native.lib().pga_fromprims(self.case.handle, self.prims.view(), self.Q.view(), self.state.view(), b,
-                                                self.backend.stream), "pga_fromprims")
+        native.check(self.case.L.pga_fromprims(self.case.handle, self.prims.view(), self.Q.view(), self.state.view(), b,
+                                                self.backend.stream), "pga_fromprims", self.case.L)
 
     def admit(self, req):
         s = self.free.pop()
@@ -150,18 +150,18 @@ class ReactorPool:
         self.slot[s] = req
 
     def tick(self):
-        L, stream = native.lib(), self.backend.stream
+        L, stream = self.case.L, self.backend.stream
         ns = self.sim.ns
         native.check(L.pga_implicit(self.case.handle, ctypes.byref(self.opts), self.Q.view(), self.state.view(),
                                     self.rates.view(), 0, self.sweeps.data_ptr(), self.capacity, self.work.data_ptr(),
-                                    self.work.numel(), self.dt.ptr, self.box, stream), "pga_implicit")
+                                    self.work.numel(), self.dt.ptr, self.box, stream), "pga_implicit", self.case.L)
         qv = self.Q.view()
         species = native.view(self.Q.ptr + 8 * 5 * self.Q.strides[2], self.capacity, 1, 1, ns - 1, strides=self.Q.strides)
         X = (native.View * 1)(self.rates.view())
         native.check(L.pga_combine(ns - 1, species, 1, 1.0, 1, X, native.dbls([1.0]), self.dt.ptr, 1, self.box, stream),
-                     "pga_combine")
+                     "pga_combine", self.case.L)
         native.check(L.pga_fromQ(self.case.handle, qv, self.state.view(), (native.Box * 1)(self.box), 1, stream),
-                     "pga_fromQ")
+                     "pga_fromQ", self.case.L)
         self.T[self.ticks % self.history].copy_(self.state.t[1, 0, 0, :])
         self.ticks += 1
         for req in self.slot.values():

@@ -7,6 +7,8 @@ advective flux is chosen by ``adv-flux`` (recon, limiter, primary); the
 chemistry integration by ``chem-integration.kind``.
 """
 
+import hashlib
+
 import numpy as np
 
 from peregrine_amd import mechgen, native
@@ -179,15 +181,29 @@ def get_simulator(cfg):
         raise ConfigError("simulation.simulator", f"{name!r}: this build has euler and navier-stokes") from None
 
 
+def case_mechanism(mix):
+    """A reacting mixture's mechanism as the library takes it: (name, records, third-body sets,
+    scalars, signature), the name the one a library built for the case gives it."""
+    records, scalars = mix.chemistry.records(mix.species)
+    sets = mix.chemistry.thirdbodies(mix.species)
+    sig = mechgen.signature(records, sets, mix.ns)
+    return "case_" + hashlib.sha256(sig.encode()).hexdigest()[:12], records, sets, scalars, sig
+
+
 class NativeCase:
-    """The constant tables of a case on the device and the library's case handle."""
+    """The constant tables of a case on the device and the library's case handle. The handle is
+    made by the main library when it is compiled for the case, else by a library built for the
+    case (``build.build_variant``, cached by source hash, as the reference compiles each case's
+    kernels on demand: ``peregrine/backend/base/compiler.py:42-64``); ``L`` is the library every
+    call on the handle goes to."""
 
     def __init__(self, sim, backend):
         import ctypes as C
 
         mix = sim.mixture
         ns = mix.ns
-        layout, size = native.const_layout(ns)
+        self.L = native.lib()
+        layout, size = native.const_layout(ns, self.L)
         if size < 0:
             raise native.NativeError(f"no constant layout for {ns} species")
         k = mix.constants()
@@ -197,34 +213,57 @@ class NativeCase:
                 table[off : off + count] = np.asarray(k[name], float).ravel()[:count]
         self.consts = backend.tensor(table)
         self.chem = None
-        mechanism = -1
+        mech, sig = None, None
         if mix.chemistry is not None:
-            records, scalars = mix.chemistry.records(mix.species)
-            sets = mix.chemistry.thirdbodies(mix.species)
-            sig = mechgen.signature(records, sets, ns)
-            mechanism = native.lib().pga_mechanism_find(sig.encode())
-            if mechanism < 0:
-                have = ", ".join(f"{n} ({s})" for n, s, _ in native.mechanisms())
-                raise native.NativeError(
-                    f"no compiled mechanism has this case's structure {sig}; compiled: {have}. "
-                    "Add the mechanism and its T-range to peregrine_amd/data/mechanisms/compiled.yaml and rebuild."
-                )
+            name, records, sets, scalars, sig = case_mechanism(mix)
+            mech = (name, records, sets)
             self.chem = backend.tensor(mechgen.pack(records, sets, scalars, ns))
         desc = native.CaseDesc()
         desc.ns, desc.eos = ns, EOS[mix.eos.name]
         desc.recon, desc.limiter, desc.primary = RECON[sim.recon], LIMITER[sim.limiter], PRIMARY[sim.primary]
-        desc.mechanism = mechanism
         desc.trans = TRANS[mix.trans.name if mix.trans is not None else None]
         desc.diffusion = mix.diffusion.kind if mix.diffusion is not None else 0
+        self.desc = desc
+        self.handle = self._create(self.L, sig)
+        if self.handle is None:
+            if not backend.cfg["backend-mi355x"]["jit"]:
+                have = ", ".join(f"{n} ({s})" for n, s, _ in native.mechanisms())
+                raise native.NativeError(
+                    f"the library is not compiled for this case ({ns} species, eos {mix.eos.name}"
+                    + (f", mechanism {sig}; compiled: {have}" if sig else "")
+                    + "); set backend-mi355x.jit to build one for it, or add the mechanism to "
+                    "peregrine_amd/data/mechanisms/compiled.yaml and rebuild"
+                )
+            from peregrine_amd import build
+
+            self.L = native.lib_at(build.build_variant(ns, realgas=desc.eos == 2, mechanism=mech))
+            self.handle = self._create(self.L, sig)
+            if self.handle is None:
+                raise native.NativeError(f"the library built for this case does not take it ({ns} species)")
+
+    def _create(self, L, sig):
+        """The case's handle from library L, or None when L is not compiled for the case."""
+        import ctypes as C
+
+        desc = self.desc
+        desc.mechanism = -1
+        if sig is not None:
+            desc.mechanism = L.pga_mechanism_find(sig.encode())
+            if desc.mechanism < 0:
+                return None
         handle = C.c_void_p()
         chem_ptr = self.chem.data_ptr() if self.chem is not None else None
-        native.check(native.lib().pga_case_create(C.byref(desc), self.consts.data_ptr(), chem_ptr, C.byref(handle)),
-                     "pga_case_create")
-        self.handle = handle
-        self.desc = desc
+        rc = L.pga_case_create(C.byref(desc), self.consts.data_ptr(), chem_ptr, C.byref(handle))
+        if rc == native.PGA_EUNSUP:
+            return None
+        native.check(rc, "pga_case_create", L)
+        if not L.pga_stage_supported(handle):
+            L.pga_case_destroy(handle)
+            return None
+        return handle
 
     def __del__(self):
         h = getattr(self, "handle", None)
         if h is not None and h.value:
-            native.lib().pga_case_destroy(h)
+            self.L.pga_case_destroy(h)
             self.handle = None

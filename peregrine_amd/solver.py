@@ -363,7 +363,7 @@ class Solver:
         sect = self.cfg["backend-mi355x"]
         if not sect["fused-stage"] or self.integ.name not in ("rk1", "rk2", "tvd-rk3", "rk4"):
             return False
-        return bool(native.lib().pga_stage_supported(self.case.handle))
+        return bool(self.case.L.pga_stage_supported(self.case.handle))
 
     @property
     def regs(self):
@@ -411,14 +411,15 @@ class Solver:
 
     def _call(self, name, *args):
         """Launch one entry point of the library on the backend's stream; raise on failure."""
-        fn = getattr(native.lib(), name)
+        L = self.case.L
+        fn = getattr(L, name)
         if self.timers is None:
-            native.check(fn(*args, self.backend.stream), name)
+            native.check(fn(*args, self.backend.stream), name, L)
             return
         torch = self.backend.torch
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record()
-        native.check(fn(*args, self.backend.stream), name)
+        native.check(fn(*args, self.backend.stream), name, L)
         e1.record()
         self.timers.setdefault(name, []).append((e0, e1))
 
@@ -579,7 +580,7 @@ class Solver:
             if blk.sweeps is None:
                 blk.imopts = native.ImplicitOpts(opts["parts"], opts["sweeps"], opts["tol"])
                 blk.sweeps = torch.zeros(blk.ncells, dtype=torch.int32, device=self.backend.device)
-                n = int(native.lib().pga_implicit_worksize(self.case.handle, box))
+                n = int(self.case.L.pga_implicit_worksize(self.case.handle, box))
                 blk.imwork = torch.zeros(max(n, 1), dtype=torch.float64, device=self.backend.device)
             self._call("pga_implicit", self.case.handle, ctypes.byref(blk.imopts), Qv, blk.state.view(), out,
                        int(begin_dQ), blk.sweeps.data_ptr(), blk.ncells, blk.imwork.data_ptr(), blk.imwork.numel(),
@@ -590,7 +591,7 @@ class Solver:
             o = native.SubstepOpts(opts["cap"], opts["bulk"], opts["bins"], 0, opts["eqtol"])
             blk.subopts = o
             blk.substeps = torch.zeros(blk.ncells, dtype=torch.int32, device=self.backend.device)
-            n = int(native.lib().pga_substep_listsize(ctypes.byref(o), box)) if opts["bulk"] < opts["cap"] else 0
+            n = int(self.case.L.pga_substep_listsize(ctypes.byref(o), box)) if opts["bulk"] < opts["cap"] else 0
             blk.stifflist = torch.zeros(max(n, 1), dtype=torch.int32, device=self.backend.device)
             blk.nlist = n
         self._call("pga_substep", self.case.handle, ctypes.byref(blk.subopts), Qv, blk.state.view(), out,
@@ -725,13 +726,13 @@ class Solver:
     class _Batch:
         """A table of blocks on the device (pga_batch_create) and the buffer it lives in."""
 
-        def __init__(self, handle, table, keep):
-            self.handle, self.table, self.keep = handle, table, keep
+        def __init__(self, L, handle, table, keep):
+            self.L, self.handle, self.table, self.keep = L, handle, table, keep
             self.imwork = None
 
         def close(self):
             if self.handle:
-                native.lib().pga_batch_destroy(self.handle)
+                self.L.pga_batch_destroy(self.handle)
                 self.handle = None
 
         def __del__(self):
@@ -772,13 +773,13 @@ class Solver:
             choice = (tiles or {}).get(autotune.size_class(blk.ncell[0]), autotune.DEFAULT)
             r.stage_wi, r.stage_kc = choice["stage"]
             r.flux_ki = choice["fluxk"]
-        L = native.lib()
+        L = self.case.L
         nbytes = int(L.pga_batch_bytes(len(self.blocks), nslab))
         table = torch.zeros(nbytes, dtype=torch.uint8, device=self.backend.device)
         handle = ctypes.c_void_p()
         native.check(L.pga_batch_create(self.case.handle, len(self.blocks), recs, len(regs), table.data_ptr(), nbytes,
-                                        ctypes.byref(handle)), "pga_batch_create")
-        return self._Batch(handle, table, recs)
+                                        ctypes.byref(handle)), "pga_batch_create", L)
+        return self._Batch(L, handle, table, recs)
 
     def _batched_step(self):
         """The fused step with each phase one launch over every block: the halo pass (unless from_prims
@@ -813,7 +814,7 @@ class Solver:
     def _batch_implicit(self, q):
         """Implicit chemistry over every block in one launch per phase, into each block's rates."""
         torch = self.backend.torch
-        L = native.lib()
+        L = self.case.L
         if getattr(self.batch, "imwork", None) is None:
             opts = self.sim.implicit
             self.batch.imopts = native.ImplicitOpts(opts["parts"], opts["sweeps"], opts["tol"])
@@ -983,7 +984,7 @@ class Solver:
                 blk.dijk = self.backend.alloc(blk.cellshape + (3,), name="dijk", lead=self.ng)
                 blk.dijk.set(grid.dijk(blk.nodes, blk.ncell))
             box = blk.interior_box()
-            n = int(native.lib().pga_cflmax_worksize(box))
+            n = int(self.case.L.pga_cflmax_worksize(box))
             if self._cflwork is None or self._cflwork.numel() < n + 3:
                 self._cflwork = torch.empty(n + 3, dtype=torch.float64, device=self.backend.device)
             work = self._cflwork

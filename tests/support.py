@@ -301,3 +301,30 @@ def reorient_back(a, S):
         if f:
             a = np.flip(a, axis=m)
     return a
+
+
+FIVE_CPG = {
+    # five calorically perfect species: a species count the main library is not compiled for
+    "A": {"MW": 28.0, "cp0": 1040.0, "mu0": 1.8e-5, "kappa0": 0.026, "lewis": 1.0},
+    "B": {"MW": 44.0, "cp0": 846.0, "mu0": 1.5e-5, "kappa0": 0.017, "lewis": 1.3},
+    "C": {"MW": 4.0, "cp0": 5193.0, "mu0": 2.0e-5, "kappa0": 0.15, "lewis": 0.4},
+    "D": {"MW": 32.0, "cp0": 918.0, "mu0": 2.0e-5, "kappa0": 0.026, "lewis": 1.1},
+    "E": {"MW": 18.0, "cp0": 1864.0, "mu0": 1.0e-5, "kappa0": 0.018, "lewis": 0.9},
+}
+
+
+def trimmed_mechanism(directory, keep=21):
+    """The H2/O2 mechanism file with its first `keep` reactions only (the falloff reaction the 21st),
+    written into directory: a mechanism whose structure the main library is not compiled for."""
+    import yaml
+
+    from peregrine_amd.mixture.mixture import DATA
+
+    with open(DATA / "mechanisms" / "h2o2.yaml") as f:
+        doc = yaml.safe_load(f)
+    doc["reactions"] = doc["reactions"][:keep]
+    path = Path(directory) / f"h2o2-first{keep}.yaml"
+    with open(path, "w") as f:
+        yaml.safe_dump(doc, f, sort_keys=False)
+    return str(path)
+

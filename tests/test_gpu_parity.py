@@ -612,6 +612,45 @@ def test_viscous_binary_muscl_steps_match(dev, fused):
     assert err < 1e-10
 
 
+# --- cases the main library is not compiled for: a library built for each ------------------------
+
+
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("fused", [True, False], ids=["fused", "per-kernel"])
+def test_five_species_steps_match_on_a_library_built_for_them(dev, fused):
+    """Five calorically perfect species with the viscous flux, MUSCL + HLLC, tvd-rk3, 10 steps: a
+    species count the main library has no kernels for, run by a library built for it (cached from
+    the CPU suite's build, else built here) and matched to the oracle."""
+    ncell = (40, 20, 12)
+    L = (1.0, 0.5, 0.3)
+    mb, hb = _run_pair("binary", "muscl", "hllc", "koren", False, "tvd-rk3", ncell, 10, 2e-6,
+                       lambda x: sp.smooth_prims(x, 9, ns=5), L, fused=fused, trans="constant", species=sp.FIVE_CPG)
+    assert mb.case.L is not native.lib()
+    assert mb.fused == fused
+    g = hb.ng
+    got = mb.blocks[0].Q.get()[g:-g, g:-g, g:-g].transpose(3, 2, 1, 0)
+    err = _rowerr(got, hb.regs[0][hb.interior()])
+    print(f"five species 10 steps ({'fused' if fused else 'per-kernel'}): {err:.3e}")
+    assert err < 1e-10
+
+
+@pytest.mark.timeout(600)
+def test_a_mechanism_of_its_own_structure_steps_on_a_library_built_for_it(dev, tmp_path):
+    """The H2/O2 mechanism cut to its first 21 reactions (a structure the main library is not
+    compiled for), the reacting case by the fused stage, 10 steps against the oracle."""
+    n = 24
+    L = (0.01,) * 3
+    mb, hb = _run_pair("h2o2", "muscl", "hllc", "koren", True, "tvd-rk3", (n, n, n), 10, 1e-8,
+                       lambda x: sp.reacting_flow(x, n, L), L, species=sp.trimmed_mechanism(tmp_path))
+    assert mb.case.L is not native.lib()
+    assert len(mb.sim.mixture.chemistry.records(mb.sim.mixture.species)[0]) == 21
+    g = hb.ng
+    got = mb.blocks[0].Q.get()[g:-g, g:-g, g:-g].transpose(3, 2, 1, 0)
+    err = _rowerr(got, hb.regs[0][hb.interior()])
+    print(f"21-reaction mechanism 10 steps: {err:.3e}")
+    assert err < 1e-10
+
+
 @pytest.mark.parametrize("fused", [True, False], ids=["fused", "per-kernel"])
 def test_reacting_navier_stokes_with_binary_diffusion_matches(dev, fused):
     """H2/O2-N2 with kinetic-theory transport, binary (mixture-averaged) diffusion and coupled
