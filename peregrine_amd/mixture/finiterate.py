@@ -37,48 +37,64 @@ BOOST = {"elementary": lambda lc: 0.0, "three-body": lambda lc: lc + 2.0, "linde
 LFCDEG = 12
 
 
-def lfc(fcent, T):
-    """log10 Fcent from the signed exponential terms, a log-sum-exp about the largest."""
-    T = np.asarray(T, dtype=np.float64)
-    if not fcent:
-        return np.full(T.shape, np.log(np.finfo(np.float64).tiny) * LOG10E)
-    e = np.array([c0 + cT * T + cTinv / T for _, c0, cT, cTinv in fcent])
-    ref = e.max(axis=0)
-    total = sum(sign * np.exp(em - ref) for (sign, *_), em in zip(fcent, e))
-    return (ref + np.log(np.maximum(np.finfo(np.float64).tiny, total))) * LOG10E
+class TroeCentre:
+    """Troe's broadening centre F_cent(T) = (1 - a) exp(-T/T3) + a exp(-T/T1) + exp(-T2/T), held as
+    its exponential terms sign * exp(c0 + cT T + cTinv / T). The mechanism's sentinels (T3 or T1 zero,
+    T2 absent) switch a term off; over the temperatures a gas takes (100 to 20000 K) a term whose
+    exponent is below -745 throughout is dropped, and a coefficient whose part of the exponent stays
+    below 1e-14 is zeroed (peregrine/simulator/mixture/chemistry/finiterate.py:240-277). The kernels
+    read log10 F_cent from a polynomial in t = (ln T - uc) / uh fitted here."""
 
+    TMIN, TMAX = 100.0, 20000.0
 
-def fcent_terms(troe):
-    """Troe's centre as signed exponentials (sign, c0, cT, cTinv): sign e^(c0 + cT T + cTinv/T)."""
-    if troe is None:
-        return []
-    alpha, T3, T1, T2 = troe
-    tmin, tmax = 100.0, 20000.0
-    candidates = []
-    if alpha != 1.0 and T3 != 0:
-        candidates.append((np.sign(1.0 - alpha), np.log(abs(1.0 - alpha)), -1.0 / T3, 0.0))
-    if alpha != 0.0 and T1 != 0:
-        candidates.append((np.sign(alpha), np.log(abs(alpha)), -1.0 / T1, 0.0))
-    if T2:
-        candidates.append((1.0, 0.0, 0.0, -T2))
-    terms = []
-    for sign, c0, cT, cTinv in candidates:
-        # a term that underflows at every gas temperature is dropped; a coefficient below rounding is zeroed
-        if cT * tmin < -745 or cTinv / tmax < -745:
-            continue
-        cT = 0.0 if abs(cT) * tmax < 1e-14 else cT
-        cTinv = 0.0 if abs(cTinv) / tmin < 1e-14 else cTinv
-        terms.append((sign, c0, cT, cTinv))
-    return terms
+    def __init__(self, troe):
+        self.terms = []
+        if troe is None:
+            return
+        a, T3, T1, T2 = troe
+        parts = []
+        if a != 1.0 and T3 != 0:
+            parts.append((1.0 - a, -1.0 / T3, 0.0))
+        if a != 0.0 and T1 != 0:
+            parts.append((a, -1.0 / T1, 0.0))
+        if T2:
+            parts.append((1.0, 0.0, -T2))
+        for coeff, cT, cTinv in parts:
+            if cT * self.TMIN < -745 or cTinv / self.TMAX < -745:
+                continue
+            if abs(cT) * self.TMAX < 1e-14:
+                cT = 0.0
+            if abs(cTinv) / self.TMIN < 1e-14:
+                cTinv = 0.0
+            self.terms.append((float(np.sign(coeff)), float(np.log(abs(coeff))), cT, cTinv))
 
+    def __len__(self):
+        return len(self.terms)
 
-def lfc_poly(fcent, uc, uh, degree=LFCDEG):
-    if not fcent:
-        return []
-    k = np.arange(8 * degree + 64)
-    t = np.cos(np.pi * (k + 0.5) / k.size)
-    coef = chebyshev.chebfit(t, lfc(fcent, np.exp(uc + uh * t)), degree)
-    return [float(c) for c in chebyshev.cheb2poly(coef)]
+    def log10(self, T):
+        """log10 F_cent at T by a signed log-sum-exp about the largest exponent; with no terms,
+        log10 of the smallest normal double (Cantera's floor)."""
+        T = np.asarray(T, dtype=np.float64)
+        tiny = np.finfo(np.float64).tiny
+        if not self.terms:
+            return np.full(T.shape, np.log(tiny) * LOG10E)
+        exps = [c0 + cT * T + cTinv / T for _, c0, cT, cTinv in self.terms]
+        top = np.max(exps, axis=0)
+        acc = 0
+        for (sign, *_), x in zip(self.terms, exps):
+            acc = acc + sign * np.exp(x - top)
+        return (top + np.log(np.maximum(tiny, acc))) * LOG10E
+
+    def poly(self, uc, uh, degree=LFCDEG):
+        """log10 F_cent's power-basis coefficients in t (rising), from the least-squares Chebyshev
+        series of the given degree through 8 degree + 64 Chebyshev points of the first kind on
+        [-1, 1]: degree 12 is within 1e-3 of the centre, the precision Troe's parameters carry."""
+        if not self.terms:
+            return []
+        nodes = 8 * degree + 64
+        t = np.cos(np.pi * (np.arange(nodes) + 0.5) / nodes)
+        series = chebyshev.chebfit(t, self.log10(np.exp(uc + uh * t)), degree)
+        return [float(c) for c in chebyshev.cheb2poly(series)]
 
 
 class FiniteRate(BaseModel):
@@ -114,7 +130,7 @@ class FiniteRate(BaseModel):
                 nu[sp] = nu.get(sp, 0.0) + v
             net = [(index[sp], v) for sp, v in nu.items() if v != 0]
             kind = KINDS[r["type"]]
-            fcent = fcent_terms(r.get("troe"))
+            fcent = TroeCentre(r.get("troe"))
             rec = SimpleNamespace(
                 name=r["type"],
                 equation=r["equation"],
@@ -128,7 +144,7 @@ class FiniteRate(BaseModel):
                 prb=float(b0 - b),
                 prEaR=float(EaR0 - EaR),
                 fcent=fcent,
-                lfcpoly=lfc_poly(fcent, uc, uh),
+                lfcpoly=fcent.poly(uc, uh),
                 fwd=[(index[sp], v) for sp, v in r["fwd"].items() if v != 0],
                 react=[(index[sp], v) for sp, v in r["reactants"].items() if v != 0],
                 prod=[(index[sp], v) for sp, v in r["products"].items() if v != 0],
@@ -193,7 +209,7 @@ class FiniteRate(BaseModel):
         xk = r.logA + r.b * logT - r.EaR * Tinv - sum(nu * gRT[sp] for sp, nu in r.react)
         low, high = xk.min(), xk.max()
         if r.troe:
-            l10 = lfc(r.fcent, T)
+            l10 = r.fcent.log10(T)
             low, high = low + LN10 * min(l10.min(), 0.0), high + LN10 * max(l10.max(), 0.0)
         if r.falloff:
             xpr = r.logprA + r.prb * logT - r.prEaR * Tinv

@@ -152,34 +152,30 @@ class TempDepCp:
         return S
 
     def _sample(self, sp):
+        """cp/R samples over the case's T-range: the thermo source's own points, carried past its
+        ends where the range reaches further (caloric/tdepcp/model.py:35-64 of the reference's
+        mixture) and then cut to the range. Below the first point the samples continue at its
+        spacing down to Tlow on the straight line from the frozen 0 K value (translation, plus
+        rotation where the shape is known) to the first sample; above the last, at its spacing up to
+        Thigh, holding the last sample."""
         Tlow, Thigh = self.cfgsect["T-range"]
         T, cpR = ThermoSource.find(sp).cp_r(sp)
         if Tlow < T[0]:
-            T, cpR = self._pad_low(sp, T, cpR, Tlow)
+            below = _outward(T[0], T[1] - T[0], Tlow)[::-1]
+            frozen = 2.5 + HALF_ROTATIONAL_DOF[sp["geometry"]] if "geometry" in sp else cpR[0]
+            T, cpR = np.concatenate([below, T]), np.concatenate([frozen + (cpR[0] - frozen) * below / T[0], cpR])
         if Thigh > T[-1]:
-            T, cpR = self._pad_high(T, cpR, Thigh)
-        inside = (T >= Tlow) & (T <= Thigh)
-        return T[inside], cpR[inside]
+            above = _outward(T[-1], T[-1] - T[-2], Thigh)
+            T, cpR = np.concatenate([T, above]), np.concatenate([cpR, np.full_like(above, cpR[-1])])
+        keep = (T >= Tlow) & (T <= Thigh)
+        return T[keep], cpR[keep]
 
-    @staticmethod
-    def _pad_low(sp, T, cpR, Tlow):
-        # towards 0 K cp/R ramps to its frozen value: translation plus rotation when the shape is known
-        frozen = 2.5 + HALF_ROTATIONAL_DOF[sp["geometry"]] if "geometry" in sp else cpR[0]
-        dT = T[1] - T[0]
-        Tx = np.arange(T[0] - dT, Tlow, -dT)
-        if Tx.size == 0 or Tx[-1] > Tlow:
-            Tx = np.append(Tx, Tlow)
-        Tx = Tx[::-1]
-        ramp = frozen + (cpR[0] - frozen) * Tx / T[0]
-        return np.concatenate([Tx, T]), np.concatenate([ramp, cpR])
 
-    @staticmethod
-    def _pad_high(T, cpR, Thigh):
-        dT = T[-1] - T[-2]
-        Tx = np.arange(T[-1] + dT, Thigh, dT)
-        if Tx.size == 0 or Tx[-1] < Thigh:
-            Tx = np.append(Tx, Thigh)
-        return np.concatenate([T, Tx]), np.concatenate([cpR, np.full_like(Tx, cpR[-1])])
+def _outward(edge, spacing, bound):
+    """Points from edge outward (away from the samples) at the given spacing, short of bound, then
+    bound itself."""
+    step = abs(spacing) if bound > edge else -abs(spacing)
+    return np.append(np.arange(edge + step, bound, step), bound)
 
 
 class ConstantCp:

@@ -13,7 +13,6 @@ import numpy as np
 
 from peregrine_amd.mixture.constants import Ru, pref
 from peregrine_amd.mixture.eos import BaseEos
-from peregrine_amd.mixture.finiterate import lfc
 
 
 def production(mixture, rho, Y, T, exact_fcent=False, u=None):
@@ -41,7 +40,7 @@ def production(mixture, rho, Y, T, exact_fcent=False, u=None):
             k *= Pr / (1.0 + Pr)
             if r.troe:
                 if exact_fcent:
-                    l10 = float(lfc(r.fcent, np.array([T]))[0])
+                    l10 = float(r.fcent.log10(np.array([T]))[0])
                 else:
                     t = (np.log(T) - sc["troeuc"]) * sc["troeuhinv"]
                     l10 = np.polyval(np.flip(r.lfcpoly), t)
