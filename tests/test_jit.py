@@ -73,3 +73,24 @@ def test_the_main_library_reports_a_species_count_it_lacks():
         assert L.pga_case_create(ctypes.byref(desc), ctypes.cast(dummy, ctypes.c_void_p), None, ctypes.byref(h)) == 0
         assert bool(L.pga_stage_supported(h)) == has
         L.pga_case_destroy(h)
+
+
+def test_a_real_gas_library_at_three_species():
+    """The cubic real gas at three species (the main library has it at one and two): its library,
+    which the three-species device test takes."""
+    path = build.build_variant(3, realgas=True)
+    L = native.lib_at(path)
+    assert "((n) == 3)" in build.variant_texts(3, realgas=True)["variant.hpp"]
+    assert path != build.build_variant(3)
+    import ctypes
+
+    dummy = (ctypes.c_double * 1)()
+    desc = native.CaseDesc()
+    desc.ns, desc.eos, desc.mechanism, desc.trans = 3, 2, -1, -1
+    h = ctypes.c_void_p()
+    assert native.lib().pga_case_create(ctypes.byref(desc), ctypes.cast(dummy, ctypes.c_void_p), None,
+                                        ctypes.byref(h)) == native.PGA_EUNSUP
+    assert L.pga_case_create(ctypes.byref(desc), ctypes.cast(dummy, ctypes.c_void_p), None, ctypes.byref(h)) == 0
+    assert L.pga_stage_supported(h)
+    L.pga_case_destroy(h)
+
