@@ -226,8 +226,10 @@ __device__ __forceinline__ void flux_at(const double *__restrict__ K, int axis, 
   const int64_t d = axis == 0 ? 1 : axis == 1 ? Q.sj : Q.sk;
   FaceState<NS> L, R;
   face_sides<NS, EOS, RECON, LIM>(g, Q, st, o, d, L, R);
+  // the face's area vector is read by this face alone
   const int64_t of = S.off(i, j, k);
-  const double Sv[3] = {S.at(of, 0), S.at(of, 1), S.at(of, 2)};
+  const double Sv[3] = {__builtin_nontemporal_load(&S.at(of, 0)), __builtin_nontemporal_load(&S.at(of, 1)),
+                        __builtin_nontemporal_load(&S.at(of, 2))};
   flux_function<NS, PRIMARY>(L, R, Sv, Fv);
 }
 
@@ -239,7 +241,7 @@ __device__ __forceinline__ void flux_face(const double *__restrict__ K, int axis
   flux_at<NS, EOS, RECON, PRIMARY, LIM>(K, axis, Q, st, S, i, j, k, Fv);
   const int64_t oF = F.off(i, j, k);
 #pragma unroll
-  for (int l = 0; l < NS + 4; l++) F.at(oF, l) = Fv[l];
+  for (int l = 0; l < NS + 4; l++) __builtin_nontemporal_store(Fv[l], &F.at(oF, l));
 }
 
 }  // namespace pga

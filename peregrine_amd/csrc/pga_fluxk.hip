@@ -76,9 +76,11 @@ __global__ __launch_bounds__(256) void k_faces(FacesArgs a) {
     const View &xf = axis == 0 ? a.xf0 : axis == 1 ? a.xf1 : a.xf2;
     diff_face<NS>(a.Q, a.st, a.tr, a.hsp, a.gr, a.xcc, xf, S, a.alpha, axis, i, j, k, Fv);
   }
+  // the face array streams past the caches: the stencil's Q and state keep L2 (5% on the 128^3 step's
+  // faces); the stage reads the array back from HBM either way
   const int64_t oF = F.off(i, j, k);
 #pragma unroll
-  for (int l = 0; l < NS + 4; l++) F.at(oF, l) = Fv[l];
+  for (int l = 0; l < NS + 4; l++) __builtin_nontemporal_store(Fv[l], &F.at(oF, l));
 }
 
 template <int NS, int EOS>
