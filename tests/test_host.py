@@ -144,6 +144,35 @@ def test_case_create_checks_its_arguments():
     assert b"species" in lib.pga_last_error() or lib.pga_last_error() is not None
 
 
+def test_transport_grads_checks_its_arguments():
+    """pga_transport_grads refuses before any launch: more than 8 boxes, a case without transport, a
+    gradient box whose stencil leaves Q, a box past its arrays; an empty call launches nothing."""
+    lib = native.lib()
+    desc = native.CaseDesc()
+    desc.ns, desc.eos, desc.mechanism, desc.trans = 1, 1, -1, 0
+    h = ctypes.c_void_p()
+    assert lib.pga_case_create(ctypes.byref(desc), 8, None, ctypes.byref(h)) == 0
+    shape = (10, 10, 10)
+    # views over no memory: every refusal here comes before a kernel could touch one
+    Q, st, tr, hsp = (native.view(8, *shape, nc) for nc in (5, 8, 3, 1))
+    Jinv, gr = native.view(8, *shape, 9), native.view(8, *shape, 12)
+    inner = native.box((1, 1, 1), (8, 8, 8))
+    boxes = (native.Box * 9)(*([inner] * 9))
+    assert lib.pga_transport_grads(h, Q, st, tr, hsp, boxes, 9, Jinv, gr, inner, None) == native.PGA_EINVAL
+    past = (native.Box * 1)(native.box((5, 5, 5), (8, 8, 8)))
+    assert lib.pga_transport_grads(h, Q, st, tr, hsp, past, 1, Jinv, gr, inner, None) == native.PGA_ERANGE
+    edge = native.box((0, 0, 0), (8, 8, 8))  # the gradients' stencil needs one layer round the box
+    assert lib.pga_transport_grads(h, Q, st, tr, hsp, boxes, 0, Jinv, gr, edge, None) == native.PGA_ERANGE
+    assert b"stencil" in lib.pga_last_error()
+    empty = native.box((1, 1, 1), (0, 8, 8))
+    assert lib.pga_transport_grads(h, Q, st, tr, hsp, boxes, 0, Jinv, gr, empty, None) == 0
+    lib.pga_case_destroy(h)
+    desc.trans = -1
+    assert lib.pga_case_create(ctypes.byref(desc), 8, None, ctypes.byref(h)) == 0
+    assert lib.pga_transport_grads(h, Q, st, tr, hsp, boxes, 1, Jinv, gr, inner, None) == native.PGA_EUNSUP
+    lib.pga_case_destroy(h)
+
+
 def test_integrator_weights_give_the_amplification_polynomials():
     from peregrine_amd.integrators import RK1, RK2, RK3, RK4, RK34, FixedController
 
