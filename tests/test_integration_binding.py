@@ -1,4 +1,4 @@
-"""The reference-side binding written out in INTEGRATION.md, run as written (no GPU): its ctypes
+"""The reference-side binding written out in the integration notes, run as written (no GPU): its ctypes
 structs lay out as the ABI's, its argument lists match the ABI's, and its view() turns a
 reference-layout ArrInfo (peregrine/backend/abi.py:38, an Allocation's `info` as
 peregrine/backend/allocation.py:20-27 fills it) into the pga_view the package itself makes for the
@@ -38,7 +38,7 @@ def _fstrides(shape):
 
 @pytest.fixture(scope="module")
 def binding():
-    """The pga.py blocks of INTEGRATION.md executed in one namespace, with the library loaded from
+    """The integration notes' pga.py blocks executed in one namespace, with the library loaded from
     this tree."""
     native.lib()
     text = (ROOT / "INTEGRATION.md").read_text()
