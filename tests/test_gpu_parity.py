@@ -451,9 +451,12 @@ def test_tgv_rk4_100_steps_match(dev, gas):
     assert err < 1e-10
 
 
+@pytest.mark.parametrize("trans", [None, "constant"], ids=["euler", "navier-stokes"])
 @pytest.mark.parametrize("fused", [True, False], ids=["fused", "per-kernel"])
-def test_blocks_of_a_lattice_step_as_one_block(dev, fused):
-    """2 x 2 x 1 blocks against one: the halo exchange between blocks is the periodic fill's equal."""
+def test_blocks_of_a_lattice_step_as_one_block(dev, fused, trans):
+    """2 x 2 x 1 blocks against one: the halo exchange between blocks is the periodic fill's equal;
+    with the viscous terms, the gradients' exchange too (the fused step's transport and gradients
+    merged in one launch per block)."""
     import peregrine_amd as pg
 
     n = 16
@@ -461,7 +464,9 @@ def test_blocks_of_a_lattice_step_as_one_block(dev, fused):
     results = []
     field = None
     for mbdims in ((1, 1, 1), (2, 2, 1)):
-        cfg = sp.config("air-tpg", recon="muscl", primary="hllc", limiter="koren", dt=2e-6)
+        cfg = sp.config("air-tpg", recon="muscl", primary="hllc", limiter="koren", dt=2e-6, trans=trans)
+        if trans:
+            cfg["mixture"]["species"] = {"Air": {**sp.AIR_NASA7["Air"], "mu0": 1.8e-5, "kappa0": 0.026}}
         cfg["backend-mi355x"]["fused-stage"] = fused
         backend = pg.get_backend("mi355x", cfg)
         blkdims = [n // m + 1 for m in mbdims]
