@@ -26,10 +26,10 @@ __global__ __launch_bounds__(256) void k_flux(const double *__restrict__ K, int 
                                               View F, Box3 b) {
   int i, j, k;
   if (axis == 2) {
-    const unsigned t = xcd_block(), nbi = (b.e0 + 63) / 64, rest = t / nbi;
+    const unsigned t = xcd_block(), nbi = (b.e0 + 63) / 64, rest = t / nbi, nkt = (b.e2 + KTILE - 1) / KTILE;
     i = b.s0 + (int)(t % nbi) * 64 + (threadIdx.x & 63);
-    j = b.s1 + (int)(rest % b.e1);
-    k = b.s2 + (int)(rest / b.e1) * KTILE + (threadIdx.x >> 6);
+    k = b.s2 + (int)(rest % nkt) * KTILE + (threadIdx.x >> 6);
+    j = b.s1 + (int)(rest / nkt);
     if (i >= b.s0 + b.e0 || k >= b.s2 + b.e2) return;
   } else if (!flat_cell_of(b, i, j, k)) {
     return;
@@ -61,11 +61,13 @@ __global__ __launch_bounds__(256) void k_faces(FacesArgs a) {
     if (!flat_at(a.b1, g - a.n0, i, j, k)) return;
   } else {
     axis = 2;
+    // a column's k tiles one after another: the next tile's workgroup finds three of its seven
+    // planes of cells in L2
     const Box3 &b = a.b2;
-    const unsigned t = g - a.n0 - a.n1, nbi = (b.e0 + 63) / 64, rest = t / nbi;
+    const unsigned t = g - a.n0 - a.n1, nbi = (b.e0 + 63) / 64, rest = t / nbi, nkt = (b.e2 + KTILE - 1) / KTILE;
     i = b.s0 + (int)(t % nbi) * 64 + (threadIdx.x & 63);
-    j = b.s1 + (int)(rest % b.e1);
-    k = b.s2 + (int)(rest / b.e1) * KTILE + (threadIdx.x >> 6);
+    k = b.s2 + (int)(rest % nkt) * KTILE + (threadIdx.x >> 6);
+    j = b.s1 + (int)(rest / nkt);
     if (i >= b.s0 + b.e0 || k >= b.s2 + b.e2) return;
   }
   const View &S = axis == 0 ? a.S0 : axis == 1 ? a.S1 : a.S2;
