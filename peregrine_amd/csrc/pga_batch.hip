@@ -124,11 +124,12 @@ __global__ __launch_bounds__(FLAT) void kb_flux(BArgs a) {
   int i, j, k;
   if (a.axis == 2) {
     fb.e2 += 1;
+    // a column's k tiles one after another, as k_faces takes them
     const int wi = B.ki, kt = FLAT / wi;
-    const int64_t nbi = (fb.e0 + wi - 1) / wi, rest = t / nbi;
+    const int64_t nbi = (fb.e0 + wi - 1) / wi, rest = t / nbi, nkt = (fb.e2 + kt - 1) / kt;
     i = fb.s0 + (int)(t % nbi) * wi + (threadIdx.x & (wi - 1));
-    j = fb.s1 + (int)(rest % fb.e1);
-    k = fb.s2 + (int)(rest / fb.e1) * kt + threadIdx.x / wi;
+    k = fb.s2 + (int)(rest % nkt) * kt + threadIdx.x / wi;
+    j = fb.s1 + (int)(rest / nkt);
     if (i >= fb.s0 + fb.e0 || k >= fb.s2 + fb.e2) return;
   } else {
     if (a.axis == 0)
